@@ -1,0 +1,159 @@
+"""ReaxFF + QEq MD throughput of the MI355X engine on the 24,000-atom water workload (one GPU).
+
+One step = one NVT velocity-Verlet step (Nose-Hoover chain, 300 K, dt 0.25 fs) with the stock fix's charge update every step
+(history extrapolation + both CG solves at tol 1e-6) and all 13 ReaxFF energy terms and forces, in the engine's mixed precision
+(FP64 bond-order chain, thresholds, accumulations and charge solve; FP32 term arithmetic). Neighbour lists use a 2 A skin and are
+rebuilt when an atom moved more than 1 A. Inputs are resident on the device during the timed region.
+
+Prints one JSON line: the throughput (atom-steps/s), the roofline record of the dominant kernel measured live with HIP events on
+the engine's stream, and a CPU baseline (the FP64 oracle, one core, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "metal-reax_amd"))
+
+HBM_PEAK_GBS = 8000.0
+PUBLISHED_24K = 1.13e6   # reference, Apple M5 Max GPU FP32, 24,000-atom CHO water: 21.2 ms/step (docs/DEVELOPMENT_LOG.md:281)
+
+
+def model_bytes_per_atom_step(n_f, R, k, B, n_h, f_don):
+    """Algorithmic HBM bytes per atom-step of the list-based model (x, q, type, f, energies; far list read once + amortised rebuild;
+    H written once and read by k fused two-vector passes; per-bond state written and read once; H-bond candidates)."""
+    return 100 + 4 * n_f * (1 + 1.0 / R) + 4 * n_f + k * 8 * n_f + 240 * B + 8 * n_h * f_don
+
+
+def cpu_baseline(ffield_text, sysm, budget_s=25.0):
+    """The FP64 oracle (serial, one core): charge solve + all terms + forces of the 24,000-atom system, repeated while the
+    time budget lasts (at least once)."""
+    from oracle.oracle import Oracle
+    O = Oracle(ffield_text)
+    steps = 0
+    t0 = time.perf_counter()
+    s_prev = t_prev = None
+    while True:
+        ro = O.qeq(sysm.x, sysm.types - 1, sysm.tags, sysm.box12(), sysm.periodic, tol=1e-6, imax=200, s0=s_prev, t0=t_prev)
+        O.compute(sysm.x, sysm.types - 1, sysm.tags, sysm.box12(), sysm.periodic, q=ro["q"])
+        s_prev, t_prev = ro["s"], ro["t"]
+        steps += 1
+        if time.perf_counter() - t0 > budget_s * 0.5 or steps >= 20:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": sysm.n * steps / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d step(s) of charge solve (tol 1e-6, warm start) + all 13 terms and forces of the same 24,000-atom water system, "
+                      "FP64 oracle (oracle/reaxff_oracle.cpp), serial, %.1f s" % (steps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed", "fp32"))
+    ap.add_argument("--nside", type=int, default=20, help="water molecules per box edge (20 -> 24,000 atoms)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported (systems shard across GPUs without "
+              "communication, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import reaxamd
+    from reaxamd import systems
+    ffield_text = systems.load_ffield()
+    w = systems.water_box(args.nside, seed=1)
+    v = systems.maxwell_boltzmann(w.masses(), 300.0, seed=2)
+
+    e = reaxamd.Engine(0)
+    e.pair_style(["NULL", "precision", args.precision])
+    e.pair_coeff(ffield_text, list(systems.ELEMENTS))
+    e.set_box(w.lo, w.hi)
+    e.set_atoms(w.types, w.tags, w.x)
+    e.qeq_setup(0.0, 10.0, e.extract("chi"), e.extract("eta"), e.extract("gamma"))
+    masses = np.array([0.0] + [systems.MASS[el] for el in systems.ELEMENTS])
+    e.md_setup(v, masses, dt=0.25, t_target=300.0, t_damp=25.0, skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=1e-6)
+    print("bench: %d atoms, warmup %d steps" % (w.n, args.warmup), file=sys.stderr, flush=True)
+    if args.warmup:
+        e.md_run(args.warmup)
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None
+    e.profile(True)
+    sync()
+    t0 = time.perf_counter()
+    thermo = e.md_run(args.steps)
+    sync()
+    elapsed = time.perf_counter() - t0
+    prof = e.profile_read()
+    e.profile(False)
+    st = e.stats()
+    print("bench: timed %d steps in %.3f s; thermo %s" % (args.steps, elapsed, thermo), file=sys.stderr, flush=True)
+
+    n = w.n
+    value = n * args.steps / elapsed
+    # workload statistics for the byte model
+    n_f = st["nb_pairs"] / n / 2.0 if st["nb_pairs"] else st["h_entries"] / n / 2.0
+    B = st["bond_slots"] / n
+    spmv_ms, spmv_cnt = prof["qeq_spmv"]
+    k = spmv_cnt / args.steps
+    R = args.steps / max(1.0, thermo["rebuilds"])
+    n_h = st["hbond_entries"] / 2.0 / max(1, (w.types == 2).sum())   # donor lists + the mirrored acceptor lists
+    bytes_step = model_bytes_per_atom_step(n_f, R, k, B, n_h, (w.types == 2).mean())
+
+    # dominant kernel by measured device time
+    phase_ms = {kk: vv[0] for kk, vv in prof.items()}
+    dominant = max(phase_ms, key=phase_ms.get)
+    # algorithmic bytes per launch of each kernel family (model terms attributed to the kernel that moves them)
+    alg = {
+        "qeq_spmv": 8 * n_f * n,                                   # one fused two-vector pass over the half H (value + index)
+        "qeq_matrix": (4 * n_f + 4 * n_f) * n,                     # far list read + H written
+        "nonbonded": (4 * n_f + 64) * n,                           # far list read + x, q, type, f
+        "bonded_terms": 240 * B * n,
+        "neighbor": 4 * n_f * n,
+    }
+    dom = dominant if dominant in alg else "qeq_spmv"
+    dom_ms, dom_cnt = prof[dom]
+    avg_s = dom_ms / max(1, dom_cnt) * 1e-3
+    achieved = alg[dom] / avg_s / 1e9
+    out = {
+        "metric": "ReaxFF+QEq atom-steps/s at 24k atoms",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_24K,
+        "dtype": {"fp64": "f64", "mixed": "f64+f32", "fp32": "f32"}[args.precision],
+        "data": "synthetic: seeded 8000-H2O lattice box at 1.0 g/cm^3, Maxwell-Boltzmann 300 K; stand-in C/H/O ffield",
+        "config": {"workload": "water_24000_nvt_qeq", "atoms": n, "box_A": float(w.hi[0]), "precision": args.precision,
+                   "dt_fs": 0.25, "qeq_tol": 1e-6, "skin_A": 2.0},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg[dom], "avg_launch_us": avg_s * 1e6, "launches": dom_cnt},
+        "step_model": {"bytes_per_atom_step": bytes_step, "n_f": n_f, "R": R, "k": k, "B": B, "n_h": n_h,
+                       "roofline_atom_steps_per_s": HBM_PEAK_GBS * 1e9 / bytes_step,
+                       "frac_of_step_roofline": value / (HBM_PEAK_GBS * 1e9 / bytes_step)},
+        "phase_ms_per_step": {kk: vv[0] / args.steps for kk, vv in prof.items()},
+        "thermo": thermo,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline (FP64 oracle, one core)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(ffield_text, w)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

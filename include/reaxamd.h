@@ -1,0 +1,142 @@
+/* C ABI of the MI355X ReaxFF engine (libreaxamd.so): the boundary a LAMMPS plugin (pair_style reaxff/metal, fix qeq/reaxff/metal)
+ * or any other host binds. Plain C types only; every call returns 0 on success or a negative status, with the message available
+ * from rxa_last_error(); nothing throws across this boundary. One context = one force field + one atom set + one HIP stream on one
+ * device; a context is not thread-safe.
+ *
+ * Each entry point names the reference interface it replaces (yeatanir/Metal-Reax; paths relative to that repository).
+ * The reference has no C ABI: its LAMMPS adapter calls the C++ engine directly (plugin/adapter/pair_reaxff_metal.cpp,
+ * fix_qeq_reaxff_metal.cpp) and the device through reaxmetal::mtl::Context (include/reaxmetal/metal_backend.hpp:57-95).
+ *
+ * Atoms are the OWNED atoms of a periodic cell (the adapter reduces LAMMPS' ghosts to owner + lattice shift, as the reference's
+ * host_atom_set does at plugin/adapter/pair_reaxff_metal.cpp:244-291); every periodic image is handled inside the engine, and the
+ * returned gradient is already folded onto the owned atoms (ghost forces are zero).
+ */
+#ifndef REAXAMD_H
+#define REAXAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rxa_context rxa_context;
+
+/* status codes */
+#define RXA_OK 0
+#define RXA_EINVAL -1     /* bad argument / unsupported input */
+#define RXA_EFFIELD -2    /* force field, control file or pair_style syntax error */
+#define RXA_EDEVICE -3    /* HIP error */
+#define RXA_ESTATE -4     /* call out of order (e.g. compute before atoms) */
+#define RXA_ENOCONV -5    /* charge equilibration did not converge (strict) */
+
+/* energy slots of rxa_compute: e_bond e_ov e_un e_lp e_ang e_pen e_coa e_hb e_tor e_con e_vdW e_ele e_pol
+ * (reference include/reaxmetal/energy_terms.hpp:17-32; rxa_pvector maps them to LAMMPS' 14-slot pvector, energy_terms.cpp:48-63) */
+#define RXA_NTERMS 13
+
+/* ---- lifecycle ---------------------------------------------------------------------------------------------------
+ * replaces: PairReaxFFMetal::metal_context() lazy creation (plugin/adapter/pair_reaxff_metal.cpp:370-374),
+ *           mtl::Context::Context() / ~Context() (include/reaxmetal/metal_backend.hpp:59-60) */
+int rxa_create(rxa_context** ctx, int device);
+void rxa_destroy(rxa_context* ctx);
+const char* rxa_last_error(const rxa_context* ctx);
+/* ABI version of this header (major * 100 + minor) */
+int rxa_abi_version(void);
+
+/* ---- pair_style / pair_coeff --------------------------------------------------------------------------------------
+ * rxa_pair_style replaces PairReaxFFMetal::settings -> parse_pair_style_args (plugin/adapter/pair_reaxff_metal.cpp:86-101,
+ *   src/io/pair_settings.cpp:41-91): args[0] = control file name or "NULL", then keyword/value pairs (checkqeq, enobonds, lgvdw,
+ *   shellcheck, backend cpu64|gpu (metal accepted), bonded cpu64|gpu, precision fp64|fp32|mixed, safezone, mincap, minhbonds,
+ *   list/blocking, tabulate). control_text = contents of the control file (NULL when args[0] is "NULL").
+ * rxa_pair_coeff replaces PairReaxFFMetal::coeff (pair_reaxff_metal.cpp:103-164) -> read_force_field_file
+ *   (src/io/ffield_parser.cpp:91-460): run-time ffield reader; elements[t] is the element of LAMMPS type t+1 or "NULL".
+ *   map_out (ntypes entries, may be NULL) receives the ffield index of each LAMMPS type (-1 for NULL).
+ * rxa_extract replaces PairReaxFFMetal::extract("chi"|"eta"|"gamma") (pair_reaxff_metal.cpp:212-227): out[0..ntypes], index = LAMMPS
+ *   type, out[0] unused; eta is 2 x the file value, as LAMMPS stores it. */
+int rxa_pair_style(rxa_context* ctx, int nargs, const char* const* args, const char* control_text);
+int rxa_pair_coeff(rxa_context* ctx, const char* ffield_text, int ntypes, const char* const* elements, int* map_out);
+int rxa_extract(rxa_context* ctx, const char* name, double* out);
+/* out[0] nonb_cut, out[1] bond_cut, out[2] hbond_cut, out[3] required ghost shell max(nonb, hbond, 2 bond)
+ * (replaces PairReaxFFMetal::cutoffs, pair_reaxff_metal.cpp:230-236, and NeighborCutoffs::required_shell, src/neighbor/neighbor.cpp:15) */
+int rxa_cutoffs(rxa_context* ctx, double* out);
+/* canonical table dump (same text as the reference's src/io/table_dump.cpp:20-84); returns the length, copies up to cap-1 chars */
+long rxa_table_dump(rxa_context* ctx, int portable, char* buf, long cap);
+
+/* ---- atoms (replaces host_box / host_atom_set / make_device_list_input, pair_reaxff_metal.cpp:238-291, src/neighbor/neighbor.cpp:329-354)
+ * LAMMPS box: lo, hi, tilt (xy, xz, yz), periodic flags. types are LAMMPS types (1..ntypes), tags the atom ids. */
+int rxa_set_box(rxa_context* ctx, const double lo[3], const double hi[3], const double tilt[3], const int periodic[3]);
+int rxa_set_atoms(rxa_context* ctx, int64_t n, const int* types, const int64_t* tags, const double* x);
+int rxa_set_positions(rxa_context* ctx, const double* x); /* 3n, owned atoms, same order */
+int rxa_set_charges(rxa_context* ctx, const double* q);   /* n */
+int rxa_get_charges(rxa_context* ctx, double* q);         /* n */
+
+/* ---- forces (replaces PairReaxFFMetal::compute, pair_reaxff_metal.cpp:412-524, i.e. mtl::Context::bonded + nonbonded,
+ * metal_backend.hpp:78-80, and compute_bonded_core / compute_nonbonded_core, include/reaxmetal/bonded.hpp:56, nonbonded.hpp:35)
+ * energies[RXA_NTERMS] (may be NULL), grad[3n] = dE/dx of the owned atoms (the adapter SUBTRACTS it from atom->f, as the reference
+ * does at pair_reaxff_metal.cpp:501-504; may be NULL), virial[6] = xx yy zz xy xz yz of the non-bonded pairs (may be NULL). */
+#define RXA_COMPUTE_REBUILD 1   /* force a neighbour-list rebuild */
+int rxa_compute(rxa_context* ctx, int flags, double* energies, double* grad, double* virial);
+/* the 13 terms -> LAMMPS' 14-slot pvector (eb ea elp emol ev epen ecoa ehb et eco ew ep efi eqeq), energy_terms.cpp:48-63 */
+void rxa_pvector(const double* energies, double* pvector14);
+/* per owned atom decision census, 7 * n ints: bonds, angle_sets(-1: not collected), torsions(-1), hbonds(-1), nonbonded, sbo_region(-1),
+ * lp_trunc (replaces BondedOptions::census / DecisionCensus, include/reaxmetal/census.hpp:12-20) */
+int rxa_census(rxa_context* ctx, int* out);
+/* bond table of the owned atoms: start[n+1], nbr[], bo[] (nbr = owned index of the partner), total_bo[n], nlp[n]; returns the
+ * number of bonds; call with nbr == NULL to size (replaces BondResult::table for fix reaxff/bonds|species, bonded.hpp:29-35,
+ * publish_bonds at pair_reaxff_metal.cpp:330-368) */
+long rxa_bond_table(rxa_context* ctx, int32_t* start, int32_t* nbr, double* bo, double* total_bo, double* nlp);
+
+/* ---- charge equilibration (fix qeq/reaxff/metal; replaces mtl::Context::qeq_setup / qeq_matvec / qeq_solve, metal_backend.hpp:81-86,
+ * FixQEqReaxFFMetal::compute_H / sparse_matvec / CG / calculate_Q, plugin/adapter/fix_qeq_reaxff_metal.cpp:65-205)
+ * chi, eta, gamma indexed by LAMMPS type (index 0 unused), as the fix holds them (eta = 2 x ffield value). */
+int rxa_qeq_setup(rxa_context* ctx, double swa, double swb, const double* chi, const double* eta, const double* gamma);
+/* y = (diag(eta) + H) x over the owned atoms (the fix's sparse_matvec; images folded) */
+int rxa_qeq_matvec(rxa_context* ctx, const double* x, double* y);
+/* both systems at once (b, x0, x: 2n = system s then system t); iters[2] = the stock loop index at exit, rel[2] = sqrt(r.M^-1 r)/|b|,
+ * status[2]: 1 converged, 3 iteration limit (the reference's CgOutcome, metal_backend.hpp:84-85) */
+int rxa_qeq_solve(rxa_context* ctx, const double* b, const double* x0, int imax, double tol, double* x, int* iters, double* rel, int* status);
+/* the whole per-step charge update of the stock fix: history extrapolation, both solves, q = s - (sum s / sum t) t, history shift;
+ * the charges stay on the device for rxa_compute and are copied to q_out (may be NULL). strict != 0 turns a non-converged solve
+ * into RXA_ENOCONV (the fix's 'strict' keyword). */
+int rxa_qeq_charges(rxa_context* ctx, int imax, double tol, int strict, double* q_out, int* iters, double* rel);
+int rxa_qeq_reset_history(rxa_context* ctx);
+
+/* ---- standalone MD loop (bench.py and the serving harness; LAMMPS integrates in plugin mode)
+ * masses by LAMMPS type (index 0 unused); NVT Nose-Hoover chain (3) at t_target with damping t_damp (fs); dt in fs; skin in Angstrom;
+ * qeq_every 0 keeps charges fixed. */
+int rxa_md_setup(rxa_context* ctx, const double* v, const double* mass_by_type, double dt, double t_target, double t_damp, double skin,
+                 int qeq_every, int qeq_imax, double qeq_tol);
+/* runs `steps` steps; thermo (may be NULL) receives: potential energy, kinetic energy, temperature, neighbour rebuilds, mean QEq
+ * iterations (s + t)/2, last iters_s, last iters_t, max displacement since the last rebuild */
+int rxa_md_run(rxa_context* ctx, int steps, double* thermo);
+int rxa_get_state(rxa_context* ctx, double* x, double* v, double* q);
+
+/* capacities and counters: out[0] n, [1] max neighbours (skin list), [2] max bonds, [3] max H-bond partners, [4] skin-list entries,
+ * [5] directed bond slots in use, [6] H-bond partner entries, [7] H-matrix entries, [8] neighbour rebuilds, [9] precision (0 fp64,
+ * 1 mixed, 2 fp32), [10] non-bonded pairs within nonb_cut (directed), [11] H-bond candidates of donors */
+int rxa_stats(rxa_context* ctx, int64_t* out);
+
+/* ---- context-free helpers (no device is touched; usable on a host without a GPU) ----------------------------------
+ * rxa_ffield_table_dump: run-time reader + canonical dump of a force field text (0 on success, length in *len); err receives the
+ * reader's message on failure. rxa_check_pair_style: parses pair_style arguments; summary receives "key=value" lines of the
+ * effective settings (precision, backend, bonded, cutoffs, notices). Both replace the reference's standalone parser entry points
+ * (src/io/ffield_parser.cpp read_force_field_file + table_dump.cpp canonical_table_dump; src/io/pair_settings.cpp:41-91). */
+int rxa_ffield_table_dump(const char* ffield_text, int lgvdw, int portable, char* buf, long cap, long* len, char* err, long errcap);
+int rxa_check_pair_style(int nargs, const char* const* args, const char* control_text, char* summary, long cap);
+
+/* ---- device timing of the engine's own stream (HIP events around each phase; off by default)
+ * phases: 0 neighbour rebuild, 1 bond lists, 2 bond-order correction, 3 bonded terms, 4 non-bonded, 5 bond-derivative gathers +
+ * energy reduction, 6 charge matrix, 7 charge SpMV (one launch per CG iteration), 8 other CG work, 9 integrator.
+ * rxa_profile(enable) also clears the totals; rxa_profile_read fills ms[10] (summed milliseconds) and counts[10] (timed launches). */
+#define RXA_NPHASES 10
+int rxa_profile(rxa_context* ctx, int enable);
+int rxa_profile_read(rxa_context* ctx, double* ms, int64_t* counts);
+
+/* ---- device-resident inputs (bench / serving): pointers to device memory owned by the caller (HIP, same device) */
+int rxa_set_positions_device(rxa_context* ctx, const double* dx);
+int rxa_stream(rxa_context* ctx, void** hip_stream_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

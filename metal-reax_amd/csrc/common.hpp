@@ -1,0 +1,184 @@
+// Shared device definitions of the MI355X ReaxFF engine: parameter blob, engine state, canonical pair geometry, wave reductions.
+//
+// Data model ("explicit images"): only the n owned atoms of a periodic cell exist on the device. A neighbour, bond or H-bond partner
+// is (owner index j, integer lattice shift s); its position is x_j + s.L. Every per-atom quantity of an image (Delta', Delta, CdDelta,
+// S, bond list) is that of its owner, so every force lands on an owned atom and no ghost fold-back is needed. This is the
+// same physics as the reference's ghost-native expanded set (owner + lattice shift, reference src/core/system.hpp:43-58) for any
+// ghost shell that is wide enough, which the reference requires anyway (required_shell, src/neighbor/neighbor.cpp:15).
+//
+// Canonical pair geometry: a pair (i, j, s) is always measured from its canonical end - the smaller tag, the smaller index for
+// equal tags, and for a self image (i == j) the shift that is positive in (s2, s1, s0) order, which is the coordinate tie-break of
+// the reference for LAMMPS-form cells (reference src/neighbor/neighbor.cpp:295-310, src/cpu/bonded.cpp:226-232). Both ends of a pair
+// therefore see bitwise-identical distances (dvec exactly negated), so every cutoff decision (bond_cut, bo_cut, hbond_cut, nonb_cut)
+// is taken identically from both sides, and the counted end is the end the reference counts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rxa {
+
+constexpr int kTerms = 13;   // e_bond, e_ov, e_un, e_lp, e_ang, e_pen, e_coa, e_hb, e_tor, e_con, e_vdW, e_ele, e_pol
+enum TermIndex { T_BOND, T_OV, T_UN, T_LP, T_ANG, T_PEN, T_COA, T_HB, T_TOR, T_CON, T_VDW, T_ELE, T_POL };
+
+struct DevSpecies {
+  double r_s, r_pi, r_pi_pi, valency, valency_e, valency_boc, valency_val, nlp_opt;
+  double p_lp2, p_ovun2, p_ovun5, p_val3, p_val5, chi, eta, gamma;
+  int p_hbond, c2, heavy, pad;
+};
+
+struct DevPair {
+  double p_bo1, p_bo2, p_bo3, p_bo4, p_bo5, p_bo6, r_s, r_p, r_pp, p_boc3, p_boc4, p_boc5;
+  double p_be1, p_be2, De_s, De_p, De_pp, p_ovun1;
+  double D, alpha, r_vdW, gamma_w, rcore, ecore, acore, lgcij, lgre, gamma, v13cor, ovc;
+  int tbs, s_ok, pi_ok, pp_ok;
+};
+
+struct DevAngle { double theta_00, p_val1, p_val2, p_val4, p_val7, p_pen1, p_coa1, pad; };
+struct DevTorsion { double V1, V2, V3, p_tor1, p_cot1, pad[3]; };
+struct DevHBond { double r0, p1, p2, p3; };
+
+// global scalars (general parameters by role, cutoffs, taper coefficients), passed by value to every kernel
+struct Globals {
+  double p_boc1, p_boc2, p_lp1, p_lp3, gp3, gp4, gp7, gp10;
+  double p_ovun3, p_ovun4, p_ovun6, p_ovun7, p_ovun8;
+  double p_val6, p_val8, p_val9, p_val10, p_pen2, p_pen3, p_pen4, p_coa2, p_coa3, p_coa4;
+  double p_tor2, p_tor3, p_tor4, p_cot2, p_vdW1;
+  double bo_cut, nonb_low, nonb_cut, thb_cut, thb_cutsq, bond_cut, hbond_cut;
+  double tap[8];      // non-bonded taper (swa = gp[11], swb = gp[12])
+  double qtap[8];     // charge-equilibration taper (the fix's swa, swb)
+  double q_swa, q_swb;
+  int ntypes, vdw_type, lg, enobonds, c2_active, pad[3];
+};
+
+// pointers to the packed type tables
+struct Tables {
+  const DevSpecies* sp;     // ntypes
+  const DevPair* pair;      // ntypes^2
+  const int2* angle_idx;    // ntypes^3: (offset, count) into angle
+  const DevAngle* angle;
+  const int* tors_idx;      // ntypes^4: index into tors or -1
+  const DevTorsion* tors;
+  const DevHBond* hb;       // ntypes^3, r0 <= 0: none
+  const double* qeq_chi;    // ntypes (fix parameters)
+  const double* qeq_eta;
+  const double* qeq_shld;   // ntypes^2: (gamma_i gamma_j)^-1.5
+};
+
+struct Cell {   // LAMMPS-form periodic cell: a = (lx,0,0), b = (xy,ly,0), c = (xz,yz,lz), origin
+  double o[3];
+  double v[3][3];
+  double cr[3][3];   // cr[d] = v[d+1] x v[d+2]; fractional coordinate f_d = ((r - o) . cr[d]) / det
+  double det;        // v0 . (v1 x v2) > 0
+  double h[3];       // distance between the two faces normal to direction d
+  int per[3];
+};
+
+#pragma clang fp contract(off)
+__host__ __device__ inline double frac_coord(const Cell& c, int d, double x, double y, double z) {
+  const double dx = x - c.o[0], dy = y - c.o[1], dz = z - c.o[2];
+  return (dx * c.cr[d][0] + dy * c.cr[d][1] + dz * c.cr[d][2]) / c.det;
+}
+#pragma clang fp contract(on)
+
+// device state of one engine (owned atoms only); arrays of bond / H-bond slots are slot-major: index = p * n + i
+struct State {
+  int n, maxnb, maxb, maxh;
+  double *x, *y, *z;          // owned positions (wrapped into the cell at every rebuild)
+  int* type;
+  long long* tag;
+  double* q;
+  // skin neighbour list (atom-major: i * maxnb + k), full (both directions)
+  int *nb_j, *nb_s, *nb_cnt;
+  int* flags;                 // [0] neighbour overflow, [1] bond overflow, [2] H-bond overflow, [3] missing mirror bond
+  // bond slots
+  int *b_cnt, *b_j, *b_s, *b_sym;
+  double *b_d, *b_dx, *b_dy, *b_dz;
+  double *bop_s, *bop_pi, *bop_pi2, *dbop_c, *dlnpi_c, *dlnpi2_c;
+  double *bo, *bo_s, *bo_pi, *bo_pi2;
+  double* C[11];              // C1dbo C2dbo C3dbo C1dbopi C2dbopi C3dbopi C4dbopi C1dbopi2 C2dbopi2 C3dbopi2 C4dbopi2 (this end's view)
+  double *Cdbo, *Cdbopi, *Cdbopi2, *fpx, *fpy, *fpz, *cdp;
+  // H-bond partner slots: donors (p_hbond 1) list acceptors, acceptors (p_hbond 2) list donors, within hbond_cut
+  int *h_cnt, *h_j, *h_s;
+  // per atom
+  double *Dp, *Dpb, *tbo, *Delta, *Delta_e, *Delta_boc, *Delta_val, *vlpex, *nlp, *Delta_lp, *dDelta_lp, *Delta_lp_temp;
+  double *dDps_x, *dDps_y, *dDps_z;
+  double *CdD_self, *CdD, *S, *Dx, *Dy, *Dz;
+  double *fs_x, *fs_y, *fs_z;     // explicit-geometry gradient of bonded terms computed at this atom
+  double *gx, *gy, *gz;           // final gradient dE/dx (force = -g)
+  double* eatom;                  // kTerms * n per-atom energies
+  double* vatom;                  // 6 * n per-atom virial (xx yy zz xy xz yz)
+  double *gnb_x, *gnb_y, *gnb_z;  // non-bonded gradient
+};
+
+// ---- shift codes ------------------------------------------------------------------------------------------------
+__host__ __device__ inline int shift_code(int s0, int s1, int s2) { return (s0 & 0xFF) | ((s1 & 0xFF) << 8) | ((s2 & 0xFF) << 16); }
+__host__ __device__ inline void shift_decode(int c, int& s0, int& s1, int& s2) {
+  s0 = (int)(signed char)(c & 0xFF);
+  s1 = (int)(signed char)((c >> 8) & 0xFF);
+  s2 = (int)(signed char)((c >> 16) & 0xFF);
+}
+__host__ __device__ inline int shift_neg(int c) { int a, b, d; shift_decode(c, a, b, d); return shift_code(-a, -b, -d); }
+__host__ __device__ inline int shift_add(int c1, int c2) {
+  int a, b, d, e, f, g;
+  shift_decode(c1, a, b, d);
+  shift_decode(c2, e, f, g);
+  return shift_code(a + e, b + f, d + g);
+}
+__host__ __device__ inline bool shift_positive(int c) {
+  int s0, s1, s2;
+  shift_decode(c, s0, s1, s2);
+  return s2 > 0 || (s2 == 0 && (s1 > 0 || (s1 == 0 && s0 > 0)));
+}
+
+// the end of pair (i, j, s) whose view is canonical
+__device__ inline bool canonical_first(int i, int j, int s, const long long* tag) {
+  const long long ti = tag[i], tj = tag[j];
+  if (ti != tj) return ti < tj;
+  if (i != j) return i < j;
+  return shift_positive(s);
+}
+
+#pragma clang fp contract(off)
+// lattice shift vector, accumulated as the image expander does (sum_k s_k v_k, k = 0, 1, 2)
+__device__ inline void shift_vec(const Cell& c, int code, double& sx, double& sy, double& sz) {
+  int s[3];
+  shift_decode(code, s[0], s[1], s[2]);
+  sx = 0.0; sy = 0.0; sz = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    sx += (double)s[k] * c.v[k][0];
+    sy += (double)s[k] * c.v[k][1];
+    sz += (double)s[k] * c.v[k][2];
+  }
+}
+// dvec = x_(j,s) - x_i measured from the canonical end; returns d^2 (no FMA contraction: identical to the host arithmetic)
+__device__ inline double pair_dvec(const State& st, const Cell& c, int i, int j, int s, double& dx, double& dy, double& dz) {
+  double sx, sy, sz;
+  if (canonical_first(i, j, s, st.tag)) {
+    shift_vec(c, s, sx, sy, sz);
+    dx = (st.x[j] + sx) - st.x[i];
+    dy = (st.y[j] + sy) - st.y[i];
+    dz = (st.z[j] + sz) - st.z[i];
+  } else {
+    shift_vec(c, shift_neg(s), sx, sy, sz);
+    dx = -((st.x[i] + sx) - st.x[j]);
+    dy = -((st.y[i] + sy) - st.y[j]);
+    dz = -((st.z[i] + sz) - st.z[j]);
+  }
+  return dx * dx + dy * dy + dz * dz;
+}
+#pragma clang fp contract(on)
+
+// ---- wave (64-lane) reductions in a fixed butterfly order: deterministic run to run --------------------------------
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline float wave_sum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+// number of set bits of `mask` below this lane
+__device__ inline int lanes_below(unsigned long long mask) { return __popcll(mask & ((1ull << lane_id()) - 1ull)); }
+
+}  // namespace rxa

@@ -1,0 +1,1178 @@
+// Host orchestration of the MI355X ReaxFF engine and its C ABI (include/reaxamd.h).
+//
+// One Engine per context: force-field tables packed once at pair_coeff time, device buffers sized for the atom set and regrown
+// (grow-and-retry, the reference's policy at src/neighbor/neighbor.cpp:362-374 and src/neighbor/bonded_device.cpp:123-139) when a
+// kernel reports that a list overflowed, and one HIP stream on which every kernel of a step is queued in order. The host reads back
+// only the overflow flags, energies and (when asked) forces.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/reaxamd.h"
+#include "common.hpp"
+#include "ffield.hpp"
+#include "launch.hpp"
+
+namespace rxa {
+
+struct ApiError : std::runtime_error {
+  int code;
+  ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                      \
+  do {                                                                                                 \
+    hipError_t e_ = (x);                                                                               \
+    if (e_ != hipSuccess) throw ApiError(RXA_EDEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+// device allocation owned by the engine
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+// Optional per-phase device timing: a pair of HIP events around each timed launch sequence on the engine's stream, resolved at the
+// next host synchronisation. Off by default (no events are recorded).
+enum Phase { PH_NBR, PH_BONDS, PH_BOCORR, PH_TERMS, PH_NONB, PH_DBOND, PH_QEQ_H, PH_QEQ_SPMV, PH_QEQ_CG, PH_MD, PH_COUNT };
+struct PhaseTimer {
+  bool on = false;
+  hipStream_t s = nullptr;
+  std::vector<hipEvent_t> pool;
+  struct Pending { int ph; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  double ms[PH_COUNT] = {0};
+  long long cnt[PH_COUNT] = {0};
+  hipEvent_t take() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t begin() {
+    if (!on) return nullptr;
+    hipEvent_t a = take();
+    HIPCHK(hipEventRecord(a, s));
+    return a;
+  }
+  void end(int ph, hipEvent_t a) {
+    if (!on || !a) return;
+    hipEvent_t b = take();
+    HIPCHK(hipEventRecord(b, s));
+    pending.push_back({ph, a, b});
+  }
+  void resolve() {   // call after a stream synchronisation
+    for (auto& p : pending) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, p.a, p.b));
+      ms[p.ph] += t;
+      cnt[p.ph] += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void reset() { for (int k = 0; k < PH_COUNT; ++k) { ms[k] = 0; cnt[k] = 0; } }
+  ~PhaseTimer() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+    for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  }
+};
+
+struct Engine {
+  int device = 0;
+  PhaseTimer timer;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // force field
+  bool have_ff = false, have_style = false;
+  PairSettings ps;
+  ForceField ff;
+  std::vector<int> lmp2ff;   // LAMMPS type (1-based) -> ffield type, -1 NULL; index 0 unused
+  Precision prec = Precision::FP64;
+  Globals g{};
+  DBuf<DevSpecies> d_sp;
+  DBuf<DevPair> d_pair;
+  DBuf<int2> d_angle_idx;
+  DBuf<DevAngle> d_angle;
+  DBuf<int> d_tors_idx;
+  DBuf<DevTorsion> d_tors;
+  DBuf<DevHBond> d_hb;
+  DBuf<double> d_qchi, d_qeta, d_qshld;
+  Tables tab{};
+
+  // atoms
+  Cell cell{};
+  bool have_box = false;
+  int n = 0;
+  State st{};
+  int maxnb = 0, maxb = 12, maxh = 64;
+  double skin = 2.0;
+  DBuf<double> a_x, a_y, a_z, a_q;
+  DBuf<int> a_type;
+  DBuf<long long> a_tag;
+  DBuf<double> a_atom;   // all per-atom double arrays
+  DBuf<int> a_iatom;     // per-atom int arrays (nb_cnt, b_cnt, h_cnt) + flags
+  DBuf<int> a_nbj, a_nbs;
+  DBuf<int> a_bint;      // b_j, b_s, b_sym
+  DBuf<double> a_bdbl;   // bond slot doubles
+  DBuf<int> a_hint;      // h_j, h_s
+  BinGrid grid;
+  BinBuffers bins;
+  DBuf<unsigned> b_keys, b_skeys, b_akey;
+  DBuf<int> b_idx, b_sidx, b_cstart, b_cend;
+  DBuf<char> b_temp;
+  bool cell_mode = false;
+  int sm[3] = {0, 0, 0};
+  bool nbr_valid = false;
+  long long rebuilds = 0;
+  DBuf<double> red;      // energy reduction output + scratch
+  std::vector<double> host_types_mass;
+
+  // charge equilibration
+  QeqBuffers qb{};
+  DBuf<int> q_int;
+  DBuf<double> q_dbl;
+  DBuf<float> q_f32;
+  bool qeq_ready = false, qeq_h_valid = false;
+  int qeq_imax_cap = 0;
+  int hist_count = 0;
+  double q_swa = 0, q_swb = 0;
+  std::vector<double> q_chi_t, q_eta_t, q_gam_t;   // per ffield type
+
+  // MD
+  MdBuffers md{};
+  DBuf<double> md_buf;
+  bool md_ready = false;
+  double md_dt = 0.25;
+  int md_qeq_every = 1, md_qeq_imax = 200;
+  double md_qeq_tol = 1e-6;
+  long long md_steps = 0;
+  double qeq_iter_sum = 0;
+  long long qeq_calls = 0;
+  int last_iters[2] = {0, 0};
+  double last_rel[2] = {0, 0};
+
+  Engine(int dev) : device(dev) {
+    HIPCHK(hipSetDevice(dev));
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    timer.s = stream;
+  }
+  ~Engine() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void sync() {
+    HIPCHK(hipStreamSynchronize(stream));
+    timer.resolve();
+  }
+
+  // ---------------------------------------------------------------------------------------------------------------
+  void pack_tables() {
+    const int nt = ff.ntypes();
+    std::vector<DevSpecies> sp(nt);
+    for (int i = 0; i < nt; ++i) {
+      const Species& s = ff.sp[i];
+      DevSpecies d{};
+      d.r_s = s.r_s; d.r_pi = s.r_pi; d.r_pi_pi = s.r_pi_pi; d.valency = s.valency; d.valency_e = s.valency_e;
+      d.valency_boc = s.valency_boc; d.valency_val = s.valency_val; d.nlp_opt = s.nlp_opt; d.p_lp2 = s.p_lp2;
+      d.p_ovun2 = s.p_ovun2; d.p_ovun5 = s.p_ovun5; d.p_val3 = s.p_val3; d.p_val5 = s.p_val5; d.chi = s.chi; d.eta = s.eta; d.gamma = s.gamma;
+      d.p_hbond = s.p_hbond; d.c2 = s.c2_species; d.heavy = s.heavy;
+      sp[i] = d;
+    }
+    std::vector<DevPair> pr((size_t)nt * nt);
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j) {
+        const PairParams& p = ff.P(i, j);
+        DevPair d{};
+        d.p_bo1 = p.p_bo1; d.p_bo2 = p.p_bo2; d.p_bo3 = p.p_bo3; d.p_bo4 = p.p_bo4; d.p_bo5 = p.p_bo5; d.p_bo6 = p.p_bo6;
+        d.r_s = p.r_s; d.r_p = p.r_p; d.r_pp = p.r_pp; d.p_boc3 = p.p_boc3; d.p_boc4 = p.p_boc4; d.p_boc5 = p.p_boc5;
+        d.p_be1 = p.p_be1; d.p_be2 = p.p_be2; d.De_s = p.De_s; d.De_p = p.De_p; d.De_pp = p.De_pp; d.p_ovun1 = p.p_ovun1;
+        d.D = p.D; d.alpha = p.alpha; d.r_vdW = p.r_vdW; d.gamma_w = p.gamma_w; d.rcore = p.rcore; d.ecore = p.ecore; d.acore = p.acore;
+        d.lgcij = p.lgcij; d.lgre = p.lgre; d.gamma = p.gamma; d.v13cor = p.v13cor; d.ovc = p.ovc;
+        d.tbs = p.triple_stab;
+        d.s_ok = ff.sp[i].r_s > 0.0 && ff.sp[j].r_s > 0.0;
+        d.pi_ok = ff.sp[i].r_pi > 0.0 && ff.sp[j].r_pi > 0.0;
+        d.pp_ok = ff.sp[i].r_pi_pi > 0.0 && ff.sp[j].r_pi_pi > 0.0;
+        pr[(size_t)i * nt + j] = d;
+      }
+    std::vector<int2> aidx((size_t)nt * nt * nt, int2{0, 0});
+    std::vector<DevAngle> ang;
+    for (const auto& kv : ff.angles) {
+      const size_t key = ((size_t)kv.first[0] * nt + kv.first[1]) * nt + kv.first[2];
+      aidx[key] = int2{(int)ang.size(), (int)kv.second.size()};
+      for (const AngleSet& a : kv.second) ang.push_back(DevAngle{a.theta_00, a.p_val1, a.p_val2, a.p_val4, a.p_val7, a.p_pen1, a.p_coa1, 0});
+    }
+    std::vector<int> tidx((size_t)nt * nt * nt * nt, -1);
+    std::vector<DevTorsion> tors;
+    for (const auto& kv : ff.torsions) {
+      const size_t key = (((size_t)kv.first[0] * nt + kv.first[1]) * nt + kv.first[2]) * nt + kv.first[3];
+      tidx[key] = (int)tors.size();
+      tors.push_back(DevTorsion{kv.second.V1, kv.second.V2, kv.second.V3, kv.second.p_tor1, kv.second.p_cot1, {0, 0, 0}});
+    }
+    std::vector<DevHBond> hb((size_t)nt * nt * nt, DevHBond{-1, 0, 0, 0});
+    for (const auto& kv : ff.hbonds) hb[((size_t)kv.first[0] * nt + kv.first[1]) * nt + kv.first[2]] = DevHBond{kv.second.r0, kv.second.p1, kv.second.p2, kv.second.p3};
+    auto up = [&](auto& buf, const auto& vec) {
+      buf.alloc(vec.size());
+      if (!vec.empty()) HIPCHK(hipMemcpy(buf.p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice));
+    };
+    up(d_sp, sp); up(d_pair, pr); up(d_angle_idx, aidx); up(d_angle, ang); up(d_tors_idx, tidx); up(d_tors, tors); up(d_hb, hb);
+    d_qchi.alloc(nt); d_qeta.alloc(nt); d_qshld.alloc((size_t)nt * nt);
+    tab.sp = d_sp.p; tab.pair = d_pair.p; tab.angle_idx = d_angle_idx.p; tab.angle = d_angle.p; tab.tors_idx = d_tors_idx.p; tab.tors = d_tors.p;
+    tab.hb = d_hb.p; tab.qeq_chi = d_qchi.p; tab.qeq_eta = d_qeta.p; tab.qeq_shld = d_qshld.p;
+  }
+
+  static void taper(double swa, double swb, double* c) {
+    const double d1 = swb - swa, d7 = std::pow(d1, 7.0);
+    const double a2 = swa * swa, a3 = swa * swa * swa, b2 = swb * swb, b3 = swb * swb * swb;
+    c[7] = 20.0 / d7;
+    c[6] = -70.0 * (swa + swb) / d7;
+    c[5] = 84.0 * (a2 + 3.0 * swa * swb + b2) / d7;
+    c[4] = -35.0 * (a3 + 9.0 * a2 * swb + 9.0 * swa * b2 + b3) / d7;
+    c[3] = 140.0 * (a3 * swb + 3.0 * a2 * b2 + swa * b3) / d7;
+    c[2] = -210.0 * (a3 * b2 + a2 * b3) / d7;
+    c[1] = 140.0 * a3 * b3 / d7;
+    c[0] = (-35.0 * a3 * b2 * b2 + 21.0 * a2 * b3 * b2 - 7.0 * swa * b3 * b3 + b3 * b3 * swb) / d7;
+  }
+
+  void make_globals() {
+    const auto& gp = ff.gp;
+    const Control& c = ps.control;
+    g = Globals{};
+    g.p_boc1 = gp[0]; g.p_boc2 = gp[1]; g.p_lp1 = gp[15]; g.p_lp3 = gp[5]; g.gp3 = gp[3]; g.gp4 = gp[4]; g.gp7 = gp[7]; g.gp10 = gp[10];
+    g.p_ovun3 = gp[32]; g.p_ovun4 = gp[31]; g.p_ovun6 = gp[6]; g.p_ovun7 = gp[8]; g.p_ovun8 = gp[9];
+    g.p_val6 = gp[14]; g.p_val8 = gp[33]; g.p_val9 = gp[16]; g.p_val10 = gp[17];
+    if (gp.size() <= 38) throw ApiError(RXA_EFFIELD, "the force field needs general parameter 39 (p_coa3)");
+    g.p_pen2 = gp[19]; g.p_pen3 = gp[20]; g.p_pen4 = gp[21]; g.p_coa2 = gp[2]; g.p_coa3 = gp[38]; g.p_coa4 = gp[30];
+    g.p_tor2 = gp[23]; g.p_tor3 = gp[24]; g.p_tor4 = gp[25]; g.p_cot2 = gp[27]; g.p_vdW1 = gp[28];
+    g.bo_cut = ff.bo_cut; g.nonb_low = ff.nonb_low; g.nonb_cut = ff.nonb_cut;
+    g.thb_cut = c.thb_cut; g.thb_cutsq = c.thb_cutsq; g.bond_cut = c.bond_cut; g.hbond_cut = c.hbond_cut;
+    taper(ff.nonb_low, ff.nonb_cut, g.tap);
+    g.ntypes = ff.ntypes(); g.vdw_type = ff.vdw_type; g.lg = ps.lgvdw; g.enobonds = ps.enobonds; g.c2_active = ff.c2_active;
+    if (qeq_ready) { taper(q_swa, q_swb, g.qtap); g.q_swa = q_swa; g.q_swb = q_swb; }
+    const std::string p = ps.effective_precision();
+    prec = p == "fp64" ? Precision::FP64 : (p == "fp32" ? Precision::FP32 : Precision::MIXED);
+  }
+
+  double list_cut() const { return std::max({g.nonb_cut, g.hbond_cut, g.bond_cut, qeq_ready ? q_swb : 0.0}); }
+
+  // ---------------------------------------------------------------------------------------------------------------
+  void set_box(const double lo[3], const double hi[3], const double tilt[3], const int per[3]) {
+    Cell c{};
+    for (int d = 0; d < 3; ++d) c.o[d] = lo[d];
+    const double v[3][3] = {{hi[0] - lo[0], 0, 0}, {tilt[0], hi[1] - lo[1], 0}, {tilt[1], tilt[2], hi[2] - lo[2]}};
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) c.v[a][b] = v[a][b];
+    auto cross = [](const double* a, const double* b, double* r) { r[0] = a[1] * b[2] - a[2] * b[1]; r[1] = a[2] * b[0] - a[0] * b[2]; r[2] = a[0] * b[1] - a[1] * b[0]; };
+    cross(c.v[1], c.v[2], c.cr[0]);
+    cross(c.v[2], c.v[0], c.cr[1]);
+    cross(c.v[0], c.v[1], c.cr[2]);
+    c.det = c.v[0][0] * c.cr[0][0] + c.v[0][1] * c.cr[0][1] + c.v[0][2] * c.cr[0][2];
+    if (!(c.det > 0) || !std::isfinite(c.det)) throw ApiError(RXA_EINVAL, "box is degenerate or left-handed");
+    for (int d = 0; d < 3; ++d) {
+      const double nr = std::sqrt(c.cr[d][0] * c.cr[d][0] + c.cr[d][1] * c.cr[d][1] + c.cr[d][2] * c.cr[d][2]);
+      c.h[d] = std::fabs(c.det) / nr;
+      c.per[d] = per[d] != 0;
+    }
+    cell = c;
+    have_box = true;
+    nbr_valid = false;
+  }
+
+  double* atom_arr(int k) { return a_atom.p + (size_t)k * n; }
+
+  void alloc_atoms() {
+    a_x.alloc(n); a_y.alloc(n); a_z.alloc(n); a_q.alloc(n); a_type.alloc(n); a_tag.alloc(n);
+    const int NA = 38 + kTerms + 6;   // number of per-atom double arrays below
+    a_atom.alloc((size_t)NA * n);
+    HIPCHK(hipMemset(a_atom.p, 0, (size_t)NA * n * sizeof(double)));
+    a_iatom.alloc(3 * (size_t)n + 8);
+    HIPCHK(hipMemset(a_iatom.p, 0, (3 * (size_t)n + 8) * sizeof(int)));
+    State& s = st;
+    s.n = n;
+    s.x = a_x.p; s.y = a_y.p; s.z = a_z.p; s.q = a_q.p; s.type = a_type.p; s.tag = a_tag.p;
+    int k = 0;
+    s.Dp = atom_arr(k++); s.Dpb = atom_arr(k++); s.tbo = atom_arr(k++); s.Delta = atom_arr(k++); s.Delta_e = atom_arr(k++);
+    s.Delta_boc = atom_arr(k++); s.Delta_val = atom_arr(k++); s.vlpex = atom_arr(k++); s.nlp = atom_arr(k++); s.Delta_lp = atom_arr(k++);
+    s.dDelta_lp = atom_arr(k++); s.Delta_lp_temp = atom_arr(k++); s.dDps_x = atom_arr(k++); s.dDps_y = atom_arr(k++); s.dDps_z = atom_arr(k++);
+    s.CdD_self = atom_arr(k++); s.CdD = atom_arr(k++); s.S = atom_arr(k++); s.Dx = atom_arr(k++); s.Dy = atom_arr(k++); s.Dz = atom_arr(k++);
+    s.fs_x = atom_arr(k++); s.fs_y = atom_arr(k++); s.fs_z = atom_arr(k++); s.gx = atom_arr(k++); s.gy = atom_arr(k++); s.gz = atom_arr(k++);
+    s.gnb_x = atom_arr(k++); s.gnb_y = atom_arr(k++); s.gnb_z = atom_arr(k++);
+    k = 38;
+    s.eatom = atom_arr(k);
+    s.vatom = atom_arr(k + kTerms);
+    s.nb_cnt = a_iatom.p; s.b_cnt = a_iatom.p + n; s.h_cnt = a_iatom.p + 2 * (size_t)n; s.flags = a_iatom.p + 3 * (size_t)n;
+    red.alloc(19 + 240 * 19 + 8);
+    // binning scratch
+    b_keys.alloc(n); b_skeys.alloc(n); b_akey.alloc(n); b_idx.alloc(n); b_sidx.alloc(n);
+    bins.keys = b_keys.p; bins.skeys = b_skeys.p; bins.atom_key = b_akey.p; bins.idx = b_idx.p; bins.sidx = b_sidx.p;
+    bins.temp_bytes = bin_temp_bytes(n);
+    b_temp.alloc(bins.temp_bytes + 16);
+    bins.temp = b_temp.p;
+  }
+
+  void alloc_neighbors(int cap) {
+    maxnb = cap;
+    a_nbj.alloc((size_t)n * maxnb);
+    a_nbs.alloc((size_t)n * maxnb);
+    st.maxnb = maxnb; st.nb_j = a_nbj.p; st.nb_s = a_nbs.p;
+    qeq_h_valid = false;
+    if (qeq_ready) alloc_qeq_matrix();
+  }
+
+  void alloc_bonds(int cap_b, int cap_h) {
+    maxb = cap_b;
+    maxh = cap_h;
+    const size_t nb = (size_t)n * maxb;
+    a_bint.alloc(3 * nb);
+    a_bdbl.alloc(29 * nb);
+    a_hint.alloc(2 * (size_t)n * maxh);
+    st.maxb = maxb; st.maxh = maxh;
+    st.b_j = a_bint.p; st.b_s = a_bint.p + nb; st.b_sym = a_bint.p + 2 * nb;
+    double* b = a_bdbl.p;
+    int k = 0;
+    auto nxt = [&]() { return b + (size_t)(k++) * nb; };
+    st.b_d = nxt(); st.b_dx = nxt(); st.b_dy = nxt(); st.b_dz = nxt();
+    st.bop_s = nxt(); st.bop_pi = nxt(); st.bop_pi2 = nxt(); st.dbop_c = nxt(); st.dlnpi_c = nxt(); st.dlnpi2_c = nxt();
+    st.bo = nxt(); st.bo_s = nxt(); st.bo_pi = nxt(); st.bo_pi2 = nxt();
+    for (int c = 0; c < 8; ++c) st.C[c] = nxt();
+    for (int c = 8; c < 11; ++c) st.C[c] = nullptr;
+    st.Cdbo = nxt(); st.Cdbopi = nxt(); st.Cdbopi2 = nxt(); st.fpx = nxt(); st.fpy = nxt(); st.fpz = nxt();
+    st.cdp = nxt();
+    if (k != 29) throw ApiError(RXA_EINVAL, "internal: bond slot layout");
+    st.h_j = a_hint.p; st.h_s = a_hint.p + (size_t)n * maxh;
+  }
+
+  int estimate_neighbors() const {
+    const double vol = cell.det;
+    double dens = n / vol;
+    if (!(cell.per[0] && cell.per[1] && cell.per[2])) dens = std::max(dens, 0.12);
+    const double rs = list_cut() + skin;
+    const double est = dens * 4.0 / 3.0 * M_PI * rs * rs * rs;
+    int cap = (int)(est * 1.25) + 32;
+    return std::min(cap, std::max(64, 27 * n + 32));
+  }
+
+  void set_atoms(long long nn, const int* types, const int64_t* tags, const double* x) {
+    if (!have_ff) throw ApiError(RXA_ESTATE, "pair_coeff (the force field) must be set before the atoms");
+    if (!have_box) throw ApiError(RXA_ESTATE, "the box must be set before the atoms");
+    if (nn <= 0 || nn > (1LL << 30)) throw ApiError(RXA_EINVAL, "atom count out of range");
+    const bool realloc = (int)nn != n || !a_x.p;
+    n = (int)nn;
+    if (realloc) {
+      alloc_atoms();
+      alloc_neighbors(estimate_neighbors());
+      alloc_bonds(maxb, maxh);
+      qeq_h_valid = false;
+      hist_count = 0;
+      if (qeq_ready) alloc_qeq();
+      md_ready = false;
+    }
+    std::vector<int> ft(n);
+    std::vector<long long> tg(n);
+    std::vector<int> used;
+    for (int i = 0; i < n; ++i) {
+      const int t = types[i];
+      if (t < 1 || t >= (int)lmp2ff.size()) throw ApiError(RXA_EINVAL, "atom type " + std::to_string(t) + " outside the pair_coeff map");
+      ft[i] = lmp2ff[t];
+      tg[i] = tags[i];
+    }
+    {
+      std::vector<int> seen(ff.ntypes(), 0);
+      for (int v : ft) if (v >= 0 && !seen[v]) { seen[v] = 1; used.push_back(v); }
+      ff.require_bond_blocks(used);
+    }
+    {
+      std::vector<long long> s = tg;
+      std::sort(s.begin(), s.end());
+      for (size_t k = 1; k < s.size(); ++k)
+        if (s[k] == s[k - 1]) throw ApiError(RXA_EINVAL, "duplicate atom id " + std::to_string(s[k]));
+    }
+    HIPCHK(hipMemcpyAsync(a_type.p, ft.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(a_tag.p, tg.data(), n * sizeof(long long), hipMemcpyHostToDevice, stream));
+    host_types_mass.assign(n, 0.0);
+    set_positions(x);
+    sync();
+  }
+
+  void set_positions(const double* x) {
+    std::vector<double> xs(n), ys(n), zs(n);
+    for (int i = 0; i < n; ++i) {
+      xs[i] = x[3 * i]; ys[i] = x[3 * i + 1]; zs[i] = x[3 * i + 2];
+      if (!std::isfinite(xs[i]) || !std::isfinite(ys[i]) || !std::isfinite(zs[i])) throw ApiError(RXA_EINVAL, "non-finite coordinate of atom " + std::to_string(i));
+    }
+    HIPCHK(hipMemcpyAsync(a_x.p, xs.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(a_y.p, ys.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(a_z.p, zs.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
+    sync();
+    nbr_valid = false;
+    qeq_h_valid = false;
+  }
+
+  // ---------------------------------------------------------------------------------------------------------------
+  void build_neighbors() {
+    const double rs = list_cut() + skin;
+    launch_wrap(st, cell, stream);
+    cell_mode = true;
+    int nc[3];
+    for (int d = 0; d < 3; ++d) {
+      nc[d] = (int)std::floor(cell.h[d] / rs);
+      if (!cell.per[d] || nc[d] < 3) cell_mode = false;
+    }
+    if (cell_mode && (long long)nc[0] * nc[1] * nc[2] > (1LL << 24)) cell_mode = false;
+    if (!cell_mode && n > 20000) throw ApiError(RXA_EINVAL, "a cell narrower than 3 list cut-offs along some direction is limited to 20000 atoms");
+    if (cell_mode) {
+      for (int d = 0; d < 3; ++d) grid.nc[d] = nc[d];
+      if ((size_t)grid.ncells() > b_cstart.n) { b_cstart.alloc(grid.ncells()); b_cend.alloc(grid.ncells()); }
+      bins.cstart = b_cstart.p;
+      bins.cend = b_cend.p;
+    } else {
+      for (int d = 0; d < 3; ++d) sm[d] = cell.per[d] ? (int)std::ceil(rs / cell.h[d]) + 1 : 0;
+      for (int d = 0; d < 3; ++d)
+        if (sm[d] > 100) throw ApiError(RXA_EINVAL, "periodic cell too thin for the neighbour cut-off");
+    }
+    for (int attempt = 0; attempt < 6; ++attempt) {
+      hipEvent_t t0 = timer.begin();
+      launch_fill_int(st.flags, 4, 0, stream);
+      if (cell_mode) {
+        launch_bin(st, cell, grid, bins, stream);
+        launch_nbr_cells(st, cell, grid, bins, rs, stream);
+      } else {
+        launch_nbr_brute(st, cell, sm, rs, stream);
+      }
+      timer.end(PH_NBR, t0);
+      int flags[4];
+      HIPCHK(hipMemcpyAsync(flags, st.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
+      sync();
+      if (flags[0] <= maxnb) {
+        nbr_valid = true;
+        ++rebuilds;
+        qeq_h_valid = false;
+        return;
+      }
+      alloc_neighbors((flags[0] + 31) / 32 * 32 + 32);
+    }
+    throw ApiError(RXA_EINVAL, "neighbour list capacity did not converge");
+  }
+
+  void bond_lists() {
+    for (int attempt = 0; attempt < 6; ++attempt) {
+      hipEvent_t t0 = timer.begin();
+      launch_fill_int(st.flags, 4, 0, stream);
+      launch_bond_build(prec, st, cell, g, tab, stream);
+      launch_bond_sym(st, stream);
+      timer.end(PH_BONDS, t0);
+      int flags[4];
+      HIPCHK(hipMemcpyAsync(flags, st.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
+      sync();
+      if (flags[1] <= maxb && flags[2] <= maxh) {
+        if (flags[3]) throw ApiError(RXA_EDEVICE, "internal: a bond has no mirror slot");
+        return;
+      }
+      alloc_bonds(std::max(maxb, (flags[1] + 3) / 4 * 4 + 4), std::max(maxh, (flags[2] + 15) / 16 * 16 + 16));
+    }
+    throw ApiError(RXA_EINVAL, "bond list capacity did not converge");
+  }
+
+  // forces and energies of the current positions and charges (queued on the stream)
+  void forces(bool rebuild) {
+    if (!nbr_valid || rebuild) build_neighbors();
+    bond_lists();
+    hipEvent_t t0 = timer.begin();
+    launch_bo_correct(prec, st, g, tab, stream);
+    timer.end(PH_BOCORR, t0);
+    t0 = timer.begin();
+    launch_bonded_terms(prec, st, cell, g, tab, stream);
+    timer.end(PH_TERMS, t0);
+    t0 = timer.begin();
+    launch_nonbonded(prec, st, cell, g, tab, stream);
+    timer.end(PH_NONB, t0);
+    t0 = timer.begin();
+    launch_bond_forces(prec, st, stream);
+    launch_energy_reduce(st, red.p, stream);
+    timer.end(PH_DBOND, t0);
+  }
+
+  void compute(int flags, double* energies, double* grad, double* virial) {
+    if (!n) throw ApiError(RXA_ESTATE, "no atoms");
+    forces((flags & RXA_COMPUTE_REBUILD) != 0);
+    double out[kTerms + 6];
+    HIPCHK(hipMemcpyAsync(out, red.p, sizeof(out), hipMemcpyDeviceToHost, stream));
+    std::vector<double> gx, gy, gz;
+    if (grad) {
+      gx.resize(n); gy.resize(n); gz.resize(n);
+      HIPCHK(hipMemcpyAsync(gx.data(), st.gx, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(gy.data(), st.gy, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(gz.data(), st.gz, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    sync();
+    if (energies) for (int k = 0; k < kTerms; ++k) energies[k] = out[k];
+    if (virial) for (int k = 0; k < 6; ++k) virial[k] = out[kTerms + k];
+    if (grad)
+      for (int i = 0; i < n; ++i) { grad[3 * i] = gx[i]; grad[3 * i + 1] = gy[i]; grad[3 * i + 2] = gz[i]; }
+  }
+
+  // ---------------------------------------------------------------------------------------------------------------
+  void alloc_qeq_matrix() {
+    q_int.alloc((size_t)n * maxnb + n);
+    qb.cnt = q_int.p;
+    qb.col = q_int.p + n;
+    q_f32.alloc((size_t)n * maxnb);
+    qb.val32 = q_f32.p;
+    // val shares the double pool
+    alloc_qeq_doubles();
+  }
+  void alloc_qeq_doubles() {
+    const int imax_cap = std::max(qeq_imax_cap, 2048);
+    const size_t sz = (size_t)n * maxnb + 3 * (size_t)n + 6 * 2 * (size_t)n + 4 * kCgBlocks + 8 + 2 * (imax_cap + 1) + 8 * (size_t)n;
+    std::vector<double> keep;
+    const bool had_hist = q_dbl.p && hist_count > 0;
+    if (had_hist) {
+      keep.resize(8 * (size_t)n);
+      HIPCHK(hipMemcpy(keep.data(), qb.s_hist, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(keep.data() + 4 * (size_t)n, qb.t_hist, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    q_dbl.alloc(sz);
+    HIPCHK(hipMemset(q_dbl.p, 0, sz * sizeof(double)));
+    double* p = q_dbl.p;
+    qb.val = p; p += (size_t)n * maxnb;
+    qb.eta = p; p += n;
+    qb.hinv = p; p += n;
+    qb.chi = p; p += n;
+    qb.b = p; p += 2 * (size_t)n;
+    qb.x = p; p += 2 * (size_t)n;
+    qb.r = p; p += 2 * (size_t)n;
+    qb.d = p; p += 2 * (size_t)n;
+    qb.p = p; p += 2 * (size_t)n;
+    qb.q = p; p += 2 * (size_t)n;
+    qb.partial = p; p += 4 * kCgBlocks;
+    qb.scal = p; p += 8 + 2 * (imax_cap + 1);
+    qb.s_hist = p; p += 4 * (size_t)n;
+    qb.t_hist = p; p += 4 * (size_t)n;
+    qeq_imax_cap = imax_cap;
+    static_assert(sizeof(int) <= sizeof(double), "");
+    iscal_buf.alloc(2 * (size_t)(imax_cap + 2));
+    qb.iscal = iscal_buf.p;
+    if (had_hist) {
+      HIPCHK(hipMemcpy(qb.s_hist, keep.data(), 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(qb.t_hist, keep.data() + 4 * (size_t)n, 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      hist_count = 0;
+    }
+  }
+  DBuf<int> iscal_buf;
+  void alloc_qeq() { alloc_qeq_matrix(); }
+
+  void qeq_setup(double swa, double swb, const double* chi, const double* eta, const double* gamma) {
+    if (!have_ff) throw ApiError(RXA_ESTATE, "pair_coeff must precede the charge fix setup");
+    if (!(swb > swa) || !(swb > 0)) throw ApiError(RXA_EINVAL, "charge equilibration taper: need swb > swa");
+    if (swb > ff.nonb_cut + 1e-12 && ps.shellcheck)
+      throw ApiError(RXA_EINVAL, "fix qeq/reaxff/metal: taper radius exceeds nonb_cut (the stock fix would silently truncate the matrix)");
+    const int nt = ff.ntypes();
+    q_chi_t.assign(nt, 0.0); q_eta_t.assign(nt, 1.0); q_gam_t.assign(nt, 1.0);
+    std::vector<int> have(nt, 0);
+    for (size_t t = 1; t < lmp2ff.size(); ++t) {
+      const int f = lmp2ff[t];
+      if (f < 0) continue;
+      if (have[f] && (q_chi_t[f] != chi[t] || q_eta_t[f] != eta[t] || q_gam_t[f] != gamma[t]))
+        throw ApiError(RXA_EINVAL, "two LAMMPS types of one element have different charge parameters");
+      q_chi_t[f] = chi[t]; q_eta_t[f] = eta[t]; q_gam_t[f] = gamma[t];
+      have[f] = 1;
+    }
+    std::vector<double> shld((size_t)nt * nt);
+    for (int a = 0; a < nt; ++a) for (int b = 0; b < nt; ++b) shld[(size_t)a * nt + b] = std::pow(q_gam_t[a] * q_gam_t[b], -1.5);
+    HIPCHK(hipMemcpy(d_qchi.p, q_chi_t.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_qeta.p, q_eta_t.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_qshld.p, shld.data(), shld.size() * sizeof(double), hipMemcpyHostToDevice));
+    const bool first = !qeq_ready;
+    q_swa = swa;
+    q_swb = swb;
+    qeq_ready = true;
+    make_globals();
+    if (n && (first || !qb.val)) { alloc_qeq(); }
+    nbr_valid = false;
+    qeq_h_valid = false;
+  }
+
+  void qeq_matrix() {
+    if (!qeq_ready) throw ApiError(RXA_ESTATE, "charge equilibration is not set up");
+    if (!nbr_valid) build_neighbors();
+    if (!qeq_h_valid) {
+      hipEvent_t t0 = timer.begin();
+      launch_qeq_build(st, cell, g, tab, qb, stream);
+      timer.end(PH_QEQ_H, t0);
+      qeq_h_valid = true;
+    }
+  }
+
+  // runs CG iterations until both systems stopped; batch launches, one readback per batch
+  void ensure_cg_cap(int imax) {
+    if (imax + 2 > qeq_imax_cap) { qeq_imax_cap = imax + 2; alloc_qeq_doubles(); }
+  }
+  void qeq_iterate(int imax, double tol, bool fp32) {
+    int it = 1;
+    const int batch = 8;
+    std::vector<int> act(2 * (imax + 2));
+    while (true) {
+      int last = it;
+      for (int b = 0; b < batch && it < imax; ++b, ++it) {
+        hipEvent_t t0 = timer.begin();
+        launch_qeq_spmv(st, qb, it, fp32, stream);
+        timer.end(PH_QEQ_SPMV, t0);
+        t0 = timer.begin();
+        launch_qeq_update(st, qb, it, imax, tol, stream);
+        timer.end(PH_QEQ_CG, t0);
+      }
+      last = it - 1;
+      int a[2];
+      HIPCHK(hipMemcpyAsync(a, qb.iscal + 2 * last, sizeof(a), hipMemcpyDeviceToHost, stream));
+      sync();
+      if ((!a[0] && !a[1]) || it >= imax) break;
+    }
+    HIPCHK(hipMemcpy(act.data(), qb.iscal, 2 * (size_t)it * sizeof(int), hipMemcpyDeviceToHost));
+    for (int s = 0; s < 2; ++s) {
+      int k = 0;
+      while (k < it && act[2 * k + s]) ++k;
+      last_iters[s] = k + 1;   // the stock loop index at exit
+    }
+    std::vector<double> sc(8 + 2 * (size_t)(it + 1));
+    HIPCHK(hipMemcpy(sc.data(), qb.scal, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s = 0; s < 2; ++s) last_rel[s] = std::sqrt(sc[8 + 2 * (last_iters[s] - 1) + s]) / sc[s];
+  }
+
+  void qeq_charges(int imax, double tol, int strict) {
+    if (!n) throw ApiError(RXA_ESTATE, "no atoms");
+    if (imax < 1) throw ApiError(RXA_EINVAL, "charge equilibration: maxiter must be >= 1");
+    ensure_cg_cap(imax);
+    qeq_matrix();
+    launch_qeq_init(st, qb, imax, tol, hist_count > 0 ? 1 : 0, stream);
+    qeq_iterate(imax, tol, false);
+    launch_qeq_finish(st, qb, stream);
+    hist_count = std::min(hist_count + 1, 4);
+    qeq_iter_sum += 0.5 * (last_iters[0] + last_iters[1]);
+    ++qeq_calls;
+    if (strict)
+      for (int s = 0; s < 2; ++s)
+        if (!(last_rel[s] <= tol))
+          throw ApiError(RXA_ENOCONV, "fix qeq/reaxff/metal strict: CG did not converge after " + std::to_string(last_iters[s]) + " iterations");
+  }
+
+  // ---------------------------------------------------------------------------------------------------------------
+  void md_setup(const double* v, const double* mass_by_type, double dt, double tt, double tdamp, double sk, int qeq_every, int qimax, double qtol) {
+    if (!n) throw ApiError(RXA_ESTATE, "no atoms");
+    md_buf.alloc(8 * (size_t)n + 64 + 256);
+    double* p = md_buf.p;
+    md.vx = p; p += n; md.vy = p; p += n; md.vz = p; p += n; md.mass = p; p += n;
+    md.x0 = p; p += n; md.y0 = p; p += n; md.z0 = p; p += n;
+    p += n;
+    md.nh = p; p += 64;
+    md.partial = p;
+    md.disp = p + 250;
+    std::vector<double> vx(n), vy(n), vz(n), m(n);
+    std::vector<int> ty(n);
+    HIPCHK(hipMemcpy(ty.data(), st.type, n * sizeof(int), hipMemcpyDeviceToHost));
+    // masses by LAMMPS type: map through the host copy of the LAMMPS types kept at set_atoms
+    for (int i = 0; i < n; ++i) {
+      vx[i] = v[3 * i]; vy[i] = v[3 * i + 1]; vz[i] = v[3 * i + 2];
+      m[i] = mass_by_type[lmp_types[i]];
+      if (!(m[i] > 0)) throw ApiError(RXA_EINVAL, "non-positive mass");
+    }
+    HIPCHK(hipMemcpy(md.vx, vx.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(md.vy, vy.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(md.vz, vz.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(md.mass, m.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> nh(64, 0.0);
+    nh[13] = tt; nh[14] = 1.0 / tdamp; nh[15] = dt; nh[16] = 3.0 * n - 3.0; nh[17] = 0.0019872067; nh[18] = 48.88821291 * 48.88821291;
+    nh[19] = 1.0;
+    HIPCHK(hipMemcpy(md.nh, nh.data(), nh.size() * sizeof(double), hipMemcpyHostToDevice));
+    md_dt = dt;
+    skin = sk;
+    md_qeq_every = qeq_every; md_qeq_imax = qimax; md_qeq_tol = qtol;
+    md_ready = true;
+    md_steps = 0;
+    // initial forces (and charges)
+    build_neighbors();
+    launch_md_ref_positions(st, md, stream);
+    if (md_qeq_every > 0) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
+    forces(false);
+    sync();
+  }
+  std::vector<int> lmp_types;
+
+  void md_run(int steps, double* thermo) {
+    if (!md_ready) throw ApiError(RXA_ESTATE, "rxa_md_setup first");
+    const double ftm2v = 1.0 / 48.88821291 / 48.88821291;
+    const double dtf = 0.5 * md_dt * ftm2v;
+    long long reb0 = rebuilds;
+    double disp = 0;
+    for (int s = 0; s < steps; ++s) {
+      hipEvent_t t0 = timer.begin();
+      launch_md_thermostat(st, md, stream);
+      launch_md_half_kick(st, md, dtf, stream);
+      launch_md_drift(st, md, md_dt, stream);
+      launch_md_displacement(st, md, stream);
+      timer.end(PH_MD, t0);
+      HIPCHK(hipMemcpyAsync(&disp, md.disp, sizeof(double), hipMemcpyDeviceToHost, stream));
+      sync();
+      const bool rebuild = disp > 0.5 * skin;
+      if (rebuild) {
+        build_neighbors();
+        launch_md_ref_positions(st, md, stream);
+      }
+      qeq_h_valid = false;
+      ++md_steps;
+      if (md_qeq_every > 0 && md_steps % md_qeq_every == 0) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
+      forces(false);
+      t0 = timer.begin();
+      launch_md_half_kick(st, md, dtf, stream);
+      launch_md_thermostat(st, md, stream);
+      timer.end(PH_MD, t0);
+    }
+    sync();
+    if (thermo) {
+      double e[kTerms];
+      HIPCHK(hipMemcpy(e, red.p, sizeof(e), hipMemcpyDeviceToHost));
+      double nh[22];
+      HIPCHK(hipMemcpy(nh, md.nh, sizeof(nh), hipMemcpyDeviceToHost));
+      double pe = 0;
+      for (double v : e) pe += v;
+      thermo[0] = pe;
+      thermo[1] = nh[21];
+      thermo[2] = nh[20];
+      thermo[3] = (double)(rebuilds - reb0);
+      thermo[4] = qeq_calls ? qeq_iter_sum / qeq_calls : 0.0;
+      thermo[5] = last_iters[0];
+      thermo[6] = last_iters[1];
+      thermo[7] = disp;
+    }
+  }
+};
+
+}  // namespace rxa
+
+// =====================================================================================================================
+// C ABI
+// =====================================================================================================================
+using rxa::ApiError;
+using rxa::Engine;
+
+struct rxa_context {
+  std::unique_ptr<Engine> e;
+  std::string err;
+};
+
+namespace {
+template <class F>
+int guard(rxa_context* c, F&& f) {
+  if (!c) return RXA_EINVAL;
+  try {
+    f(*c->e);
+    c->err.clear();
+    return RXA_OK;
+  } catch (const ApiError& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const rxa::FfieldError& e) {
+    c->err = e.what();
+    return RXA_EFFIELD;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return RXA_EINVAL;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int rxa_abi_version(void) { return 100; }
+
+int rxa_create(rxa_context** out, int device) {
+  if (!out) return RXA_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return RXA_EDEVICE;
+  if (device < 0 || device >= count) return RXA_EINVAL;
+  try {
+    auto* c = new rxa_context;
+    c->e = std::make_unique<Engine>(device);
+    *out = c;
+    return RXA_OK;
+  } catch (...) {
+    return RXA_EDEVICE;
+  }
+}
+
+void rxa_destroy(rxa_context* c) { delete c; }
+
+const char* rxa_last_error(const rxa_context* c) { return c ? c->err.c_str() : "null context"; }
+
+int rxa_pair_style(rxa_context* c, int nargs, const char* const* args, const char* control_text) {
+  return guard(c, [&](Engine& e) {
+    std::vector<std::string> a;
+    for (int i = 0; i < nargs; ++i) a.emplace_back(args[i]);
+    e.ps = rxa::parse_pair_style(a, control_text ? control_text : "");
+    e.have_style = true;
+    if (e.have_ff) e.make_globals();
+  });
+}
+
+int rxa_pair_coeff(rxa_context* c, const char* text, int ntypes, const char* const* elements, int* map_out) {
+  return guard(c, [&](Engine& e) {
+    if (!e.have_style) throw ApiError(RXA_ESTATE, "pair_style must precede pair_coeff");
+    if (!text) throw ApiError(RXA_EINVAL, "no force field text");
+    e.ff = rxa::read_ffield(text, e.ps.lgvdw);
+    if (e.ps.lgvdw && e.ff.vdw_type == 1)
+      throw ApiError(RXA_EFFIELD, "lgvdw with a shielding-only force field: the lg parameters would be read but never used");
+    e.lmp2ff.assign(ntypes + 1, -1);
+    for (int t = 0; t < ntypes; ++t) {
+      const std::string el = elements[t];
+      if (el == "NULL") continue;
+      const int m = e.ff.match_element(el);
+      if (m == -1) throw ApiError(RXA_EFFIELD, "element " + el + " not found in the force field");
+      if (m == -2) throw ApiError(RXA_EFFIELD, "element " + el + " is not unique in the force field");
+      e.lmp2ff[t + 1] = m;
+    }
+    if (map_out) for (int t = 0; t < ntypes; ++t) map_out[t] = e.lmp2ff[t + 1];
+    e.pack_tables();
+    e.have_ff = true;
+    e.make_globals();
+  });
+}
+
+int rxa_extract(rxa_context* c, const char* name, double* out) {
+  return guard(c, [&](Engine& e) {
+    if (!e.have_ff) throw ApiError(RXA_ESTATE, "no force field");
+    const std::string nm = name ? name : "";
+    out[0] = 0;
+    for (size_t t = 1; t < e.lmp2ff.size(); ++t) {
+      const int f = e.lmp2ff[t];
+      double v = 0;
+      if (f >= 0) {
+        if (nm == "chi") v = e.ff.sp[f].chi;
+        else if (nm == "eta") v = e.ff.sp[f].eta;
+        else if (nm == "gamma") v = e.ff.sp[f].gamma;
+        else if (nm == "bcut_acks2") v = e.ff.sp[f].bcut_acks2;
+        else throw ApiError(RXA_EINVAL, "extract: unknown name " + nm);
+      }
+      out[t] = v;
+    }
+  });
+}
+
+int rxa_cutoffs(rxa_context* c, double* out) {
+  return guard(c, [&](Engine& e) {
+    if (!e.have_ff) throw ApiError(RXA_ESTATE, "no force field");
+    out[0] = e.ff.nonb_cut;
+    out[1] = e.ps.control.bond_cut;
+    out[2] = e.ps.control.hbond_cut;
+    out[3] = std::max({out[0], out[2], 2.0 * out[1]});
+  });
+}
+
+long rxa_table_dump(rxa_context* c, int portable, char* buf, long cap) {
+  if (!c || !c->e->have_ff) return -1;
+  const std::string d = rxa::table_dump(c->e->ff, portable != 0);
+  if (buf && cap > 0) {
+    const long m = std::min<long>(cap - 1, (long)d.size());
+    std::memcpy(buf, d.data(), (size_t)m);
+    buf[m] = 0;
+  }
+  return (long)d.size();
+}
+
+int rxa_set_box(rxa_context* c, const double lo[3], const double hi[3], const double tilt[3], const int periodic[3]) {
+  return guard(c, [&](Engine& e) { e.set_box(lo, hi, tilt, periodic); });
+}
+
+int rxa_set_atoms(rxa_context* c, int64_t n, const int* types, const int64_t* tags, const double* x) {
+  return guard(c, [&](Engine& e) {
+    e.set_atoms(n, types, tags, x);
+    e.lmp_types.assign(types, types + n);
+  });
+}
+
+int rxa_set_positions(rxa_context* c, const double* x) {
+  return guard(c, [&](Engine& e) {
+    if (!e.n) throw ApiError(RXA_ESTATE, "no atoms");
+    e.set_positions(x);
+  });
+}
+
+int rxa_set_positions_device(rxa_context* c, const double* dx) {
+  return guard(c, [&](Engine& e) {
+    if (!e.n) throw ApiError(RXA_ESTATE, "no atoms");
+    const size_t n = (size_t)e.n;
+    HIPCHK(hipMemcpyAsync(e.st.x, dx, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+    HIPCHK(hipMemcpyAsync(e.st.y, dx + n, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+    HIPCHK(hipMemcpyAsync(e.st.z, dx + 2 * n, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+    e.nbr_valid = false;
+    e.qeq_h_valid = false;
+  });
+}
+
+int rxa_stream(rxa_context* c, void** out) {
+  return guard(c, [&](Engine& e) { *out = (void*)e.stream; });
+}
+
+int rxa_set_charges(rxa_context* c, const double* q) {
+  return guard(c, [&](Engine& e) {
+    if (!e.n) throw ApiError(RXA_ESTATE, "no atoms");
+    HIPCHK(hipMemcpy(e.st.q, q, e.n * sizeof(double), hipMemcpyHostToDevice));
+  });
+}
+
+int rxa_get_charges(rxa_context* c, double* q) {
+  return guard(c, [&](Engine& e) {
+    e.sync();
+    HIPCHK(hipMemcpy(q, e.st.q, e.n * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int rxa_compute(rxa_context* c, int flags, double* energies, double* grad, double* virial) {
+  return guard(c, [&](Engine& e) { e.compute(flags, energies, grad, virial); });
+}
+
+void rxa_pvector(const double* e, double* pv) {
+  // eb ea elp emol ev epen ecoa ehb et eco ew ep efi eqeq
+  pv[0] = e[0]; pv[1] = e[1] + e[2]; pv[2] = e[3]; pv[3] = 0; pv[4] = e[4]; pv[5] = e[5]; pv[6] = e[6];
+  pv[7] = e[7]; pv[8] = e[8]; pv[9] = e[9]; pv[10] = e[10]; pv[11] = e[11]; pv[12] = 0; pv[13] = e[12];
+}
+
+int rxa_census(rxa_context* c, int* out) {
+  return guard(c, [&](Engine& e) {
+    const int n = e.n;
+    std::vector<int> b(n);
+    std::vector<double> de(n);
+    e.sync();
+    HIPCHK(hipMemcpy(b.data(), e.st.b_cnt, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(de.data(), e.st.Delta_e, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 7 * n; ++i) out[i] = -1;
+    for (int i = 0; i < n; ++i) {
+      out[i] = b[i];
+      out[6 * n + i] = (int)(de[i] / 2.0);
+    }
+  });
+}
+
+long rxa_bond_table(rxa_context* c, int32_t* start, int32_t* nbr, double* bo, double* total_bo, double* nlp) {
+  if (!c) return -1;
+  long total = -1;
+  const int rc = guard(c, [&](Engine& e) {
+    const int n = e.n, mb = e.maxb;
+    e.sync();
+    std::vector<int> cnt(n), bj((size_t)n * mb);
+    std::vector<double> bv((size_t)n * mb), tb(n), np(n);
+    HIPCHK(hipMemcpy(cnt.data(), e.st.b_cnt, n * sizeof(int), hipMemcpyDeviceToHost));
+    total = 0;
+    for (int i = 0; i < n; ++i) total += std::min(cnt[i], mb);
+    if (!nbr) return;
+    HIPCHK(hipMemcpy(bj.data(), e.st.b_j, bj.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bv.data(), e.st.bo, bv.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tb.data(), e.st.tbo, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(np.data(), e.st.nlp, n * sizeof(double), hipMemcpyDeviceToHost));
+    long k = 0;
+    for (int i = 0; i < n; ++i) {
+      if (start) start[i] = (int32_t)k;
+      for (int p = 0; p < std::min(cnt[i], mb); ++p, ++k) {
+        nbr[k] = bj[(size_t)p * n + i];
+        if (bo) bo[k] = bv[(size_t)p * n + i];
+      }
+      if (total_bo) total_bo[i] = tb[i];
+      if (nlp) nlp[i] = np[i];
+    }
+    if (start) start[n] = (int32_t)k;
+  });
+  return rc == RXA_OK ? total : rc;
+}
+
+int rxa_qeq_setup(rxa_context* c, double swa, double swb, const double* chi, const double* eta, const double* gamma) {
+  return guard(c, [&](Engine& e) { e.qeq_setup(swa, swb, chi, eta, gamma); });
+}
+
+int rxa_qeq_matvec(rxa_context* c, const double* x, double* y) {
+  return guard(c, [&](Engine& e) {
+    e.qeq_matrix();
+    const size_t n = (size_t)e.n;
+    HIPCHK(hipMemcpyAsync(e.qb.p, x, n * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    rxa::launch_qeq_matvec1(e.st, e.qb, e.qb.p, e.qb.r, e.stream);
+    HIPCHK(hipMemcpyAsync(y, e.qb.r, n * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    e.sync();
+  });
+}
+
+int rxa_qeq_solve(rxa_context* c, const double* b, const double* x0, int imax, double tol, double* x, int* iters, double* rel, int* status) {
+  return guard(c, [&](Engine& e) {
+    if (imax < 1) throw ApiError(RXA_EINVAL, "maxiter must be >= 1");
+    e.ensure_cg_cap(imax);
+    e.qeq_matrix();
+    const int n = e.n;
+    std::vector<double> bi(2 * (size_t)n), xi(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) { bi[2 * i] = b[i]; bi[2 * i + 1] = b[n + i]; xi[2 * i] = x0[i]; xi[2 * i + 1] = x0[n + i]; }
+    HIPCHK(hipMemcpyAsync(e.qb.b, bi.data(), bi.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    HIPCHK(hipMemcpyAsync(e.qb.x, xi.data(), xi.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    rxa::launch_qeq_init(e.st, e.qb, imax, tol, 2, e.stream);
+    e.qeq_iterate(imax, tol, false);
+    std::vector<double> xo(2 * (size_t)n);
+    HIPCHK(hipMemcpy(xo.data(), e.qb.x, xo.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) { x[i] = xo[2 * i]; x[n + i] = xo[2 * i + 1]; }
+    for (int s = 0; s < 2; ++s) {
+      iters[s] = e.last_iters[s];
+      rel[s] = e.last_rel[s];
+      status[s] = e.last_rel[s] <= tol ? 1 : 3;
+    }
+  });
+}
+
+int rxa_qeq_charges(rxa_context* c, int imax, double tol, int strict, double* q_out, int* iters, double* rel) {
+  return guard(c, [&](Engine& e) {
+    e.qeq_charges(imax, tol, strict);
+    if (q_out) {
+      e.sync();
+      HIPCHK(hipMemcpy(q_out, e.st.q, e.n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (iters) { iters[0] = e.last_iters[0]; iters[1] = e.last_iters[1]; }
+    if (rel) { rel[0] = e.last_rel[0]; rel[1] = e.last_rel[1]; }
+  });
+}
+
+int rxa_qeq_reset_history(rxa_context* c) {
+  return guard(c, [&](Engine& e) { e.hist_count = 0; });
+}
+
+int rxa_md_setup(rxa_context* c, const double* v, const double* mass_by_type, double dt, double t_target, double t_damp, double skin,
+                 int qeq_every, int qeq_imax, double qeq_tol) {
+  return guard(c, [&](Engine& e) { e.md_setup(v, mass_by_type, dt, t_target, t_damp, skin, qeq_every, qeq_imax, qeq_tol); });
+}
+
+int rxa_md_run(rxa_context* c, int steps, double* thermo) {
+  return guard(c, [&](Engine& e) { e.md_run(steps, thermo); });
+}
+
+int rxa_get_state(rxa_context* c, double* x, double* v, double* q) {
+  return guard(c, [&](Engine& e) {
+    e.sync();
+    const int n = e.n;
+    std::vector<double> a(n), b(n), d(n);
+    if (x) {
+      HIPCHK(hipMemcpy(a.data(), e.st.x, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(b.data(), e.st.y, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(d.data(), e.st.z, n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) { x[3 * i] = a[i]; x[3 * i + 1] = b[i]; x[3 * i + 2] = d[i]; }
+    }
+    if (v) {
+      if (!e.md_ready) throw ApiError(RXA_ESTATE, "no velocities (rxa_md_setup)");
+      HIPCHK(hipMemcpy(a.data(), e.md.vx, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(b.data(), e.md.vy, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(d.data(), e.md.vz, n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) { v[3 * i] = a[i]; v[3 * i + 1] = b[i]; v[3 * i + 2] = d[i]; }
+    }
+    if (q) HIPCHK(hipMemcpy(q, e.st.q, n * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+static void copy_out(const std::string& s, char* buf, long cap) {
+  if (!buf || cap <= 0) return;
+  const long m = std::min<long>(cap - 1, (long)s.size());
+  std::memcpy(buf, s.data(), (size_t)m);
+  buf[m] = 0;
+}
+
+int rxa_ffield_table_dump(const char* text, int lgvdw, int portable, char* buf, long cap, long* len, char* err, long errcap) {
+  try {
+    if (!text) throw std::invalid_argument("no force field text");
+    const rxa::ForceField ff = rxa::read_ffield(text, lgvdw != 0);
+    const std::string d = rxa::table_dump(ff, portable != 0);
+    if (len) *len = (long)d.size();
+    copy_out(d, buf, cap);
+    return RXA_OK;
+  } catch (const rxa::FfieldError& e) {
+    copy_out(e.what(), err, errcap);
+    return RXA_EFFIELD;
+  } catch (const std::exception& e) {
+    copy_out(e.what(), err, errcap);
+    return RXA_EINVAL;
+  }
+}
+
+int rxa_check_pair_style(int nargs, const char* const* args, const char* control_text, char* summary, long cap) {
+  try {
+    std::vector<std::string> a;
+    for (int i = 0; i < nargs; ++i) a.emplace_back(args[i]);
+    const rxa::PairSettings p = rxa::parse_pair_style(a, control_text ? control_text : "");
+    std::string o;
+    char b[128];
+    o += "precision=" + p.effective_precision() + "\n";
+    o += "backend=" + p.backend + "\nbonded=" + p.bonded + "\n";
+    o += std::string("checkqeq=") + (p.checkqeq ? "yes" : "no") + "\nenobonds=" + (p.enobonds ? "yes" : "no") + "\nlgvdw=" + (p.lgvdw ? "yes" : "no") + "\n";
+    std::snprintf(b, sizeof b, "bond_cut=%.17g\nhbond_cut=%.17g\nthb_cut=%.17g\nthb_cutsq=%.17g\n", p.control.bond_cut, p.control.hbond_cut,
+                  p.control.thb_cut, p.control.thb_cutsq);
+    o += b;
+    for (const auto& nt : p.notices) o += "notice=" + nt + "\n";
+    copy_out(o, summary, cap);
+    return RXA_OK;
+  } catch (const std::exception& e) {
+    copy_out(e.what(), summary, cap);
+    return RXA_EFFIELD;
+  }
+}
+
+int rxa_profile(rxa_context* c, int enable) {
+  return guard(c, [&](Engine& e) {
+    e.sync();
+    e.timer.on = enable != 0;
+    e.timer.reset();
+  });
+}
+
+int rxa_profile_read(rxa_context* c, double* ms, int64_t* counts) {
+  return guard(c, [&](Engine& e) {
+    e.sync();
+    for (int k = 0; k < rxa::PH_COUNT; ++k) {
+      ms[k] = e.timer.ms[k];
+      counts[k] = e.timer.cnt[k];
+    }
+  });
+}
+
+int rxa_stats(rxa_context* c, int64_t* out) {
+  return guard(c, [&](Engine& e) {
+    for (int k = 0; k < 12; ++k) out[k] = 0;
+    out[0] = e.n; out[1] = e.maxnb; out[2] = e.maxb; out[3] = e.maxh; out[8] = e.rebuilds; out[9] = (int)e.prec;
+    if (!e.n) return;
+    e.sync();
+    const int n = e.n;
+    std::vector<int> cnt(3 * (size_t)n);
+    HIPCHK(hipMemcpy(cnt.data(), e.st.nb_cnt, 3 * (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) { out[4] += cnt[i]; out[5] += cnt[n + i]; out[6] += cnt[2 * (size_t)n + i]; }
+    if (e.qeq_ready && e.qb.cnt && e.qeq_h_valid) {
+      std::vector<int> qc(n);
+      HIPCHK(hipMemcpy(qc.data(), e.qb.cnt, n * sizeof(int), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) out[7] += qc[i];
+    }
+  });
+}
+
+}  // extern "C"

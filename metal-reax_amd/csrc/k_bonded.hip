@@ -1,0 +1,876 @@
+// Bond-order chain and the ten bonded ReaxFF terms on the device.
+//
+// Pipeline per step (one launch each, in stream order):
+//   bond_build  (wave per atom)   bond list from the skin list: d <= bond_cut and BO' >= bo_cut, uncorrected BO', dBO'/dx, Delta'
+//                                 [reference src/cpu/bonded.cpp:109-162, terms.hpp:50-56]; H-bond partner lists [bonded.cpp:120-124]
+//   bond_sym    (thread per atom) index of the mirror slot of every bond
+//   bo_correct  (thread per atom) corrected BO and its 11 derivative coefficients, always from the pair's canonical end so both ends
+//                                 hold bitwise-identical values [bonded.cpp:164-204, terms.hpp:66-164]
+//   atom_terms  (thread per atom) bond energy + triple-bond stabilisation, lone pair + C2, over/under coordination [bonded.cpp:219-318]
+//   valence     (thread per atom) valence angle, penalty, 3-body conjugation at the centre [bonded.cpp:321-474]
+//   torsion     (thread per atom) torsion + 4-body conjugation, evaluated at BOTH atoms of the central bond; each end keeps the force
+//                                 and derivative contributions of its own half (its own atom, its outer partner, its outer bond) and
+//                                 the end the reference counts (tag order) adds the energy and the central-bond terms [bonded.cpp:476-652]
+//   hbond       (thread per atom) donor hydrogens evaluate their H-bonds; acceptors re-evaluate the same triples for their own force
+//                                 [bonded.cpp:654-715]
+//   cdelta, dbond, gather         the reference's Add_dBond_to_Forces [bonded.cpp:719-766] rewritten as gathers: every BO-derivative
+//                                 contribution is a central pair force along a bond, so each atom sums its own force from its own
+//                                 bond slots and the mirror slots of its partners.
+// No atomics on any force or energy path: every array element has exactly one writer, and every sum has a fixed order, so results
+// are bitwise repeatable. Contributions an atom makes to a bonded partner (angle/torsion/H-bond forces, CdDelta) go into the
+// partner's slot of the writer's own bond list (fpx/fpy/fpz/cdp) and are collected by the partner through the mirror index.
+//
+// Precision: TB is the type of the bond-order chain (BO', corrections, Delta quantities and their threshold tests); TR the type of
+// the term arithmetic. FP64: <double, double>; mixed: <double, float>; FP32: <float, float>. Energies, CdDelta and forces are
+// accumulated in double in every mode.
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace rxa {
+namespace {
+
+template <class T> __device__ inline T rexp(T x);
+template <> __device__ inline double rexp(double x) { return exp(x); }
+template <> __device__ inline float rexp(float x) { return expf(x); }
+template <class T> __device__ inline T rpow(T x, T y);
+template <> __device__ inline double rpow(double x, double y) { return pow(x, y); }
+template <> __device__ inline float rpow(float x, float y) { return powf(x, y); }
+template <class T> __device__ inline T rlog(T x);
+template <> __device__ inline double rlog(double x) { return log(x); }
+template <> __device__ inline float rlog(float x) { return logf(x); }
+template <class T> __device__ inline T rsqrt_(T x);
+template <> __device__ inline double rsqrt_(double x) { return sqrt(x); }
+template <> __device__ inline float rsqrt_(float x) { return sqrtf(x); }
+template <class T> __device__ inline T racos(T x);
+template <> __device__ inline double racos(double x) { return acos(x); }
+template <> __device__ inline float racos(float x) { return acosf(x); }
+template <class T> __device__ inline T rsin(T x);
+template <> __device__ inline double rsin(double x) { return sin(x); }
+template <> __device__ inline float rsin(float x) { return sinf(x); }
+template <class T> __device__ inline T rcos(T x);
+template <> __device__ inline double rcos(double x) { return cos(x); }
+template <> __device__ inline float rcos(float x) { return cosf(x); }
+template <class T> __device__ inline T ratan2(T y, T x);
+template <> __device__ inline double ratan2(double y, double x) { return atan2(y, x); }
+template <> __device__ inline float ratan2(float y, float x) { return atan2f(y, x); }
+// exp() capped from above (float: e^40, double: e^700); only saturating ratios use it (reference terms.hpp:30-34)
+template <class T> __device__ inline T safe_exp(T x) {
+  const T cap = sizeof(T) == 4 ? T(40) : T(700);
+  return rexp<T>(x > cap ? cap : x);
+}
+
+__device__ inline size_t sl(int p, int i, int n) { return (size_t)p * (size_t)n + (size_t)i; }
+
+template <class T> struct V3 { T x, y, z; };
+template <class T> __device__ inline T dot3(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> __device__ inline V3<T> cross3(const V3<T>& a, const V3<T>& b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// angle between two bond vectors from the same centre, and d cos(theta) / d x of (end a, centre, end b) (reference bonded.cpp:58-75)
+template <class T>
+__device__ inline void angle_and_dcos(const V3<T>& a, T da, const V3<T>& b, T db, T& theta, T& cost, V3<T>& dca, V3<T>& dcj, V3<T>& dcb) {
+  cost = dot3(a, b) / (da * db);
+  if (cost > T(1)) cost = T(1);
+  if (cost < T(-1)) cost = T(-1);
+  theta = racos<T>(cost);
+  const T da2 = da * da, db2 = db * db, inv = T(1) / (da * db), inv3 = inv * inv * inv, cd = dot3(a, b) * inv3;
+  dca = {b.x * inv - cd * db2 * a.x, b.y * inv - cd * db2 * a.y, b.z * inv - cd * db2 * a.z};
+  dcj = {-(b.x + a.x) * inv + cd * (db2 * a.x + da2 * b.x), -(b.y + a.y) * inv + cd * (db2 * a.y + da2 * b.y),
+         -(b.z + a.z) * inv + cd * (db2 * a.z + da2 * b.z)};
+  dcb = {a.x * inv - cd * da2 * b.x, a.y * inv - cd * da2 * b.y, a.z * inv - cd * da2 * b.z};
+}
+
+// ================================================================================================================
+// bond list + uncorrected bond orders
+// ================================================================================================================
+template <class TB>
+__global__ void __launch_bounds__(256) k_bond_build(State st, Cell c, Globals g, Tables t) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  const int lane = lane_id(), n = st.n;
+  const int ti = st.type[i];
+  int nb = 0, nh = 0;
+  TB sum_bo = 0, ddx = 0, ddy = 0, ddz = 0;
+  if (ti >= 0) {
+    const DevSpecies si = t.sp[ti];
+    const int role = g.hbond_cut > 0 ? si.p_hbond : 0;
+    const size_t base = (size_t)i * st.maxnb;
+    const int cnt = min(st.nb_cnt[i], st.maxnb);
+    const TB bo_cut = (TB)g.bo_cut;
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const int k = k0 + lane;
+      bool bkeep = false, hkeep = false;
+      int j = 0, s = 0;
+      double dx = 0, dy = 0, dz = 0, d = 0;
+      TB bs = 0, bp = 0, bpp = 0, cln = 0, clp = 0, clpp = 0;
+      if (k < cnt) {
+        j = st.nb_j[base + k];
+        s = st.nb_s[base + k];
+        const int tj = st.type[j];
+        if (tj >= 0) {
+          d = sqrt(pair_dvec(st, c, i, j, s, dx, dy, dz));
+          const DevSpecies sj = t.sp[tj];
+          if (role && d <= g.hbond_cut) hkeep = (role == 1 && sj.p_hbond == 2) || (role == 2 && sj.p_hbond == 1);
+          if (d <= g.bond_cut) {
+            const DevPair P = t.pair[ti * g.ntypes + tj];
+            const TB dd = (TB)d;
+            TB C12 = 0, C34 = 0, C56 = 0;
+            if (P.s_ok) { C12 = (TB)P.p_bo1 * rpow<TB>(dd / (TB)P.r_s, (TB)P.p_bo2); bs = (TB(1) + bo_cut) * rexp<TB>(C12); }
+            if (P.pi_ok) { C34 = (TB)P.p_bo3 * rpow<TB>(dd / (TB)P.r_p, (TB)P.p_bo4); bp = rexp<TB>(C34); }
+            if (P.pp_ok) { C56 = (TB)P.p_bo5 * rpow<TB>(dd / (TB)P.r_pp, (TB)P.p_bo6); bpp = rexp<TB>(C56); }
+            bkeep = bs + bp + bpp >= bo_cut;
+            const TB rr2 = TB(1) / (dd * dd);
+            cln = (TB)P.p_bo2 * C12 * rr2;
+            clp = (TB)P.p_bo4 * C34 * rr2;
+            clpp = (TB)P.p_bo6 * C56 * rr2;
+          }
+        }
+      }
+      const unsigned long long mb = __ballot(bkeep), mh = __ballot(hkeep);
+      if (bkeep) {
+        const int p = nb + lanes_below(mb);
+        if (p < st.maxb) {
+          const size_t q = sl(p, i, n);
+          st.b_j[q] = j; st.b_s[q] = s;
+          st.b_d[q] = d; st.b_dx[q] = dx; st.b_dy[q] = dy; st.b_dz[q] = dz;
+          const TB tot = (bs + bp + bpp) - bo_cut;
+          st.bop_s[q] = (double)(bs - bo_cut);
+          st.bop_pi[q] = (double)bp;
+          st.bop_pi2[q] = (double)bpp;
+          st.bo[q] = (double)tot;   // uncorrected total until bo_correct overwrites it
+          const TB dc = -(bs * cln + bp * clp + bpp * clpp);
+          st.dbop_c[q] = (double)dc;
+          st.dlnpi_c[q] = (double)(-bp * clp);
+          st.dlnpi2_c[q] = (double)(-bpp * clpp);
+          sum_bo += tot;
+          ddx += dc * (TB)dx; ddy += dc * (TB)dy; ddz += dc * (TB)dz;
+        }
+      }
+      if (hkeep) {
+        const int p = nh + lanes_below(mh);
+        if (p < st.maxh) { st.h_j[sl(p, i, n)] = j; st.h_s[sl(p, i, n)] = s; }
+      }
+      nb += __popcll(mb);
+      nh += __popcll(mh);
+    }
+  }
+  // fixed-order wave reductions
+  for (int off = 32; off > 0; off >>= 1) {
+    sum_bo += __shfl_xor(sum_bo, off, 64);
+    ddx += __shfl_xor(ddx, off, 64);
+    ddy += __shfl_xor(ddy, off, 64);
+    ddz += __shfl_xor(ddz, off, 64);
+  }
+  if (lane == 0) {
+    st.b_cnt[i] = nb;
+    st.h_cnt[i] = nh;
+    if (nb > st.maxb) atomicMax(&st.flags[1], nb);
+    if (nh > st.maxh) atomicMax(&st.flags[2], nh);
+    const double val = ti >= 0 ? t.sp[ti].valency : 0.0, valv = ti >= 0 ? t.sp[ti].valency_val : 0.0;
+    st.Dp[i] = (double)(sum_bo - (TB)val);
+    st.Dpb[i] = (double)(sum_bo - (TB)valv);
+    st.dDps_x[i] = (double)ddx; st.dDps_y[i] = (double)ddy; st.dDps_z[i] = (double)ddz;
+  }
+}
+
+__global__ void __launch_bounds__(128) k_bond_sym(State st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
+  for (int p = 0; p < nb; ++p) {
+    const int j = st.b_j[sl(p, i, n)], want = shift_neg(st.b_s[sl(p, i, n)]);
+    const int nbj = min(st.b_cnt[j], st.maxb);
+    int found = -1;
+    for (int q = 0; q < nbj; ++q)
+      if (st.b_j[sl(q, j, n)] == i && st.b_s[sl(q, j, n)] == want) { found = q; break; }
+    st.b_sym[sl(p, i, n)] = found < 0 ? 0 : found;
+    if (found < 0) st.flags[3] = 1;
+  }
+}
+
+// ================================================================================================================
+// corrected bond orders (always from the canonical end) and atom quantities
+// ================================================================================================================
+template <class TB>
+__global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int n = st.n, ti = st.type[i];
+  const int nb = min(st.b_cnt[i], st.maxb);
+  TB tbo = 0;
+  for (int p = 0; p < nb; ++p) {
+    const size_t q = sl(p, i, n);
+    const int j = st.b_j[q];
+    const bool first = canonical_first(i, j, st.b_s[q], st.tag);
+    const int a = first ? i : j, b = first ? j : i;
+    const int ta = st.type[a], tb = st.type[b];
+    const DevPair P = t.pair[ta * g.ntypes + tb];
+    const TB vi = (TB)t.sp[ta].valency, vj = (TB)t.sp[tb].valency;
+    const TB Dpi = (TB)st.Dp[a], Dpj = (TB)st.Dp[b], Dbi = (TB)st.Dpb[a], Dbj = (TB)st.Dpb[b];
+    const TB BO = (TB)st.bo[q], BOs = (TB)st.bop_s[q], BOpi = (TB)st.bop_pi[q], BOpi2 = (TB)st.bop_pi2[q];
+    TB oBO, oBOs, oBOpi, oBOpi2, C1dbo = 0, C2dbo = 0, C3dbo = 0, C1pi = 0, C2pi = 0, C3pi = 0, C4pi = 0, C1pi2 = 0, C2pi2 = 0, C3pi2 = 0, C4pi2 = 0;
+    if ((TB)P.ovc < TB(0.001) && (TB)P.v13cor < TB(0.001)) {
+      oBO = BO; oBOs = BOs; oBOpi = BOpi; oBOpi2 = BOpi2;
+      C1dbo = 1; C1pi = 1; C1pi2 = 1;
+    } else {
+      const TB boc1 = (TB)g.p_boc1, boc2 = (TB)g.p_boc2;
+      TB f1 = 1, Cf1ij = 0, Cf1ji = 0;
+      if ((TB)P.ovc >= TB(0.001)) {
+        const TB e1i = safe_exp<TB>(-boc1 * Dpi), e2i = safe_exp<TB>(-boc2 * Dpi), e1j = safe_exp<TB>(-boc1 * Dpj), e2j = safe_exp<TB>(-boc2 * Dpj);
+        const TB f2 = e1i + e1j, f3 = -TB(1) / boc2 * rlog<TB>(TB(0.5) * (e2i + e2j));
+        f1 = TB(0.5) * ((vi + f2) / (vi + f2 + f3) + (vj + f2) / (vj + f2 + f3));
+        const TB ui = vi + (f2 + f3), uj = vj + (f2 + f3), S2 = e2i + e2j;
+        Cf1ij = TB(0.5) * (-boc1 * e1i / ui - ((vi + f2) / (ui * ui)) * (-boc1 * e1i + e2i / S2) + -boc1 * e1i / uj -
+                           ((vj + f2) / (uj * uj)) * (-boc1 * e1i + e2i / S2));
+        const TB A = TB(0.5) * f3 * (TB(1) / (ui * ui) + TB(1) / (uj * uj));
+        const TB B = -TB(0.5) * ((ui - f3) / (ui * ui) + (uj - f3) / (uj * uj));
+        Cf1ji = -A * boc1 * e1j + B * e2j / S2;
+      }
+      TB f4 = 1, f5 = 1, Cf45ij = 0, Cf45ji = 0;
+      const TB boc3 = (TB)P.p_boc3, boc4 = (TB)P.p_boc4, boc5 = (TB)P.p_boc5;
+      if ((TB)P.v13cor >= TB(0.001)) {
+        const TB ef4 = safe_exp<TB>(-(boc4 * (BO * BO) - Dbi) * boc3 + boc5);
+        const TB ef5 = safe_exp<TB>(-(boc4 * (BO * BO) - Dbj) * boc3 + boc5);
+        f4 = TB(1) / (TB(1) + ef4);
+        f5 = TB(1) / (TB(1) + ef5);
+        Cf45ij = -f4 * ef4;
+        Cf45ji = -f5 * ef5;
+      }
+      const TB A0 = f1 * (f4 * f5);
+      const TB A1 = -2 * boc3 * boc4 * BO * (Cf45ij + Cf45ji);
+      const TB A2ij = Cf1ij / f1 + boc3 * Cf45ij, A2ji = Cf1ji / f1 + boc3 * Cf45ji;
+      const TB A3ij = A2ij + Cf1ij / f1, A3ji = A2ji + Cf1ji / f1;
+      oBO = BO * A0;
+      oBOpi = BOpi * A0 * f1;
+      oBOpi2 = BOpi2 * A0 * f1;
+      oBOs = oBO - (oBOpi + oBOpi2);
+      C1dbo = A0 + oBO * A1; C2dbo = oBO * A2ij; C3dbo = oBO * A2ji;
+      C1pi = f1 * f1 * f4 * f5; C2pi = oBOpi * A1; C3pi = oBOpi * A3ij; C4pi = oBOpi * A3ji;
+      C1pi2 = f1 * f1 * f4 * f5; C2pi2 = oBOpi2 * A1; C3pi2 = oBOpi2 * A3ij; C4pi2 = oBOpi2 * A3ji;
+    }
+    if (oBO < TB(1e-10)) oBO = 0;
+    if (oBOs < TB(1e-10)) oBOs = 0;
+    if (oBOpi < TB(1e-10)) oBOpi = 0;
+    if (oBOpi2 < TB(1e-10)) oBOpi2 = 0;
+    st.bo[q] = (double)oBO; st.bo_s[q] = (double)oBOs; st.bo_pi[q] = (double)oBOpi; st.bo_pi2[q] = (double)oBOpi2;
+    st.C[0][q] = (double)C1dbo; st.C[1][q] = (double)C2pi; st.C[2][q] = (double)C2pi2; st.C[3][q] = (double)C1pi; st.C[4][q] = (double)C1pi2;
+    st.C[5][q] = (double)(first ? C2dbo : C3dbo);
+    st.C[6][q] = (double)(first ? C3pi : C4pi);
+    st.C[7][q] = (double)(first ? C3pi2 : C4pi2);
+    st.Cdbo[q] = 0; st.Cdbopi[q] = 0; st.Cdbopi2[q] = 0; st.fpx[q] = 0; st.fpy[q] = 0; st.fpz[q] = 0; st.cdp[q] = 0;
+    tbo += oBO;
+  }
+  st.tbo[i] = (double)tbo;
+  st.CdD_self[i] = 0; st.fs_x[i] = 0; st.fs_y[i] = 0; st.fs_z[i] = 0;
+  if (ti < 0) return;
+  const DevSpecies s = t.sp[ti];
+  // atom quantities (reference terms.hpp:141-164); trunc toward zero
+  const TB Delta = tbo - (TB)s.valency, De = tbo - (TB)s.valency_e;
+  const TB ht = (TB)(int)(De / TB(2));
+  const TB vlpex = De - TB(2) * ht;
+  const TB ex = rexp<TB>(-(TB)g.p_lp1 * ((TB(2) + vlpex) * (TB(2) + vlpex)));
+  const TB nlp = ex - ht;
+  const TB Dlp = (TB)s.nlp_opt - nlp;
+  st.Delta[i] = (double)Delta;
+  st.Delta_e[i] = (double)De;
+  st.Delta_boc[i] = (double)(tbo - (TB)s.valency_boc);
+  st.Delta_val[i] = (double)(tbo - (TB)s.valency_val);
+  st.vlpex[i] = (double)vlpex;
+  st.nlp[i] = (double)nlp;
+  st.Delta_lp[i] = (double)Dlp;
+  st.dDelta_lp[i] = (double)(TB(2) * (TB)g.p_lp1 * ex * (TB(2) + vlpex));
+  st.Delta_lp_temp[i] = s.heavy ? (double)((TB)s.nlp_opt - TB(0.5) * ((TB)s.valency_e - (TB)s.valency)) : (double)Dlp;
+}
+
+// ================================================================================================================
+// bond energy, triple-bond stabilisation, lone pair (+ C2), over/under coordination
+// ================================================================================================================
+template <class TR>
+__global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int n = st.n, ti = st.type[i];
+  double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, cdself = 0;
+  if (ti >= 0) {
+    const DevSpecies si = t.sp[ti];
+    const int nb = min(st.b_cnt[i], st.maxb);
+    // bond energy of the bonds whose canonical end is this atom (the reference's tag order, bonded.cpp:222-252)
+    for (int p = 0; p < nb; ++p) {
+      const size_t q = sl(p, i, n);
+      const int j = st.b_j[q];
+      if (!canonical_first(i, j, st.b_s[q], st.tag)) continue;
+      const int tj = st.type[j];
+      const DevPair P = t.pair[ti * g.ntypes + tj];
+      const TR BOs = (TR)st.bo_s[q], BOpi = (TR)st.bo_pi[q], BOpi2 = (TR)st.bo_pi2[q];
+      const TR pw = BOs == TR(0) ? TR(0) : rpow<TR>(BOs, (TR)P.p_be2);
+      const TR ex = rexp<TR>((TR)P.p_be1 * (TR(1) - pw));
+      const TR CEbo = -(TR)P.De_s * ex * (TR(1) - (TR)P.p_be1 * (TR)P.p_be2 * pw);
+      e_bond += (double)(-(TR)P.De_s * BOs * ex - (TR)P.De_p * BOpi - (TR)P.De_pp * BOpi2);
+      double cdbo = (double)CEbo;
+      st.Cdbopi[q] += -(double)(CEbo + (TR)P.De_p);
+      st.Cdbopi2[q] += -(double)(CEbo + (TR)P.De_pp);
+      const TR BO = (TR)st.bo[q];
+      if (BO >= TR(1.00) && P.tbs) {
+        const TR gp3 = (TR)g.gp3, gp4 = (TR)g.gp4, gp7 = (TR)g.gp7, gp10 = (TR)g.gp10;
+        const TR exphu = rexp<TR>(-gp7 * ((BO - TR(2.50)) * (BO - TR(2.50))));
+        const TR ea = rexp<TR>(-gp3 * ((TR)st.tbo[i] - BO)), eb = rexp<TR>(-gp3 * ((TR)st.tbo[j] - BO));
+        const TR eov = rexp<TR>(gp4 * ((TR)st.Delta[i] + (TR)st.Delta[j]));
+        const TR hov = TR(1) / (TR(1) + TR(25.0) * eov);
+        e_bond += (double)(gp10 * exphu * hov * (ea + eb));
+        cdbo += (double)(gp10 * exphu * hov * (ea + eb) * (gp3 - TR(2.0) * gp7 * (BO - TR(2.50))));
+        cdself += (double)(-gp10 * exphu * hov * (gp3 * ea + TR(25.0) * gp4 * eov * hov * (ea + eb)));
+        st.cdp[q] += (double)(-gp10 * exphu * hov * (gp3 * eb + TR(25.0) * gp4 * eov * hov * (ea + eb)));
+      }
+      st.Cdbo[q] += cdbo;
+    }
+    // lone pair
+    if (nb > 0 || g.enobonds) {
+      const TR Dlp = (TR)st.Delta_lp[i], p_lp2 = (TR)si.p_lp2;
+      const TR e2 = safe_exp<TR>(TR(-75) * Dlp), inv = TR(1) / (TR(1) + e2);
+      e_lp += (double)(p_lp2 * Dlp * inv);
+      cdself += (double)((p_lp2 * inv + TR(75) * p_lp2 * Dlp * e2 * (inv * inv)) * (TR)st.dDelta_lp[i]);
+    }
+    // C2 correction, every directed C-C bond (reference bonded.cpp:268-280)
+    if (g.c2_active && si.c2) {
+      const TR p_lp3 = (TR)g.p_lp3, Di = (TR)st.Delta[i];
+      for (int p = 0; p < nb; ++p) {
+        const size_t q = sl(p, i, n);
+        if (!t.sp[st.type[st.b_j[q]]].c2) continue;
+        const TR vov3 = (TR)st.bo[q] - Di - TR(0.040) * rpow<TR>(Di, TR(4.));
+        if (vov3 > TR(3.)) {
+          e_lp += (double)(p_lp3 * ((vov3 - TR(3.0)) * (vov3 - TR(3.0))));
+          st.Cdbo[q] += (double)(TR(2.) * p_lp3 * (vov3 - TR(3.)));
+          cdself += (double)(TR(2.) * p_lp3 * (vov3 - TR(3.)) * (-TR(1.) - TR(0.16) * rpow<TR>(Di, TR(3.))));
+        }
+      }
+    }
+    // over / under coordination (reference bonded.cpp:282-318, terms.hpp:231-260)
+    {
+      const TR dfvl = si.heavy ? TR(0) : TR(1);
+      TR sum1 = 0, sum2 = 0;
+      for (int p = 0; p < nb; ++p) {
+        const size_t q = sl(p, i, n);
+        const int j = st.b_j[q];
+        const DevPair P = t.pair[ti * g.ntypes + st.type[j]];
+        sum1 += (TR)P.p_ovun1 * (TR)P.De_s * (TR)st.bo[q];
+        sum2 += ((TR)st.Delta[j] - dfvl * (TR)st.Delta_lp_temp[j]) * ((TR)st.bo_pi[q] + (TR)st.bo_pi2[q]);
+      }
+      const bool under = nb > 0 || g.enobonds;
+      const TR ovun3 = (TR)g.p_ovun3, ovun4 = (TR)g.p_ovun4, ovun6 = (TR)g.p_ovun6, ovun7 = (TR)g.p_ovun7, ovun8 = (TR)g.p_ovun8;
+      const TR p_ovun2 = (TR)si.p_ovun2, p_ovun5 = (TR)si.p_ovun5;
+      const TR Dlpt = (TR)st.Delta_lp_temp[i], dDlp = (TR)st.dDelta_lp[i];
+      const TR eo1 = ovun3 * safe_exp<TR>(ovun4 * sum2), ieo1 = TR(1) / (TR(1) + eo1);
+      const TR Dlc = (TR)st.Delta[i] - (dfvl * Dlpt) * ieo1;
+      const TR eo2 = safe_exp<TR>(p_ovun2 * Dlc), ieo2 = TR(1) / (TR(1) + eo2);
+      const TR DlpVi = TR(1) / (Dlc + (TR)si.valency + TR(1e-8));
+      const TR CEover1 = Dlc * DlpVi * ieo2;
+      e_ov += (double)(sum1 * CEover1);
+      const TR CEover2 = sum1 * DlpVi * ieo2 * (TR(1) - Dlc * (DlpVi + p_ovun2 * eo2 * ieo2));
+      const TR CEover3 = CEover2 * (TR(1) - dfvl * dDlp * ieo1);
+      const TR CEover4 = CEover2 * (dfvl * Dlpt) * ovun4 * eo1 * (ieo1 * ieo1);
+      const TR eo2n = safe_exp<TR>(-p_ovun2 * Dlc), eo6 = safe_exp<TR>(ovun6 * Dlc), eo8 = ovun7 * safe_exp<TR>(ovun8 * sum2);
+      const TR ieo2n = TR(1) / (TR(1) + eo2n), ieo8 = TR(1) / (TR(1) + eo8);
+      const TR eun = under ? -p_ovun5 * (TR(1) - eo6) * ieo2n * ieo8 : TR(0);
+      const TR CEunder1 = ieo2n * (p_ovun5 * ovun6 * eo6 * ieo8 + p_ovun2 * eun * eo2n);
+      const TR CEunder2 = -eun * ovun8 * eo8 * ieo8;
+      const TR CEunder3 = CEunder1 * (TR(1) - dfvl * dDlp * ieo1);
+      const TR CEunder4 = CEunder1 * (dfvl * Dlpt) * ovun4 * eo1 * (ieo1 * ieo1) + CEunder2;
+      if (under) e_un += (double)eun;
+      cdself += (double)CEover3;
+      if (under) cdself += (double)CEunder3;
+      for (int p = 0; p < nb; ++p) {
+        const size_t q = sl(p, i, n);
+        const int j = st.b_j[q];
+        const DevPair P = t.pair[ti * g.ntypes + st.type[j]];
+        st.Cdbo[q] += (double)(CEover1 * (TR)P.p_ovun1 * (TR)P.De_s);
+        const TR pis = (TR)st.bo_pi[q] + (TR)st.bo_pi2[q];
+        const TR dj = (TR)st.Delta[j] - dfvl * (TR)st.Delta_lp_temp[j];
+        const TR w = TR(1) - dfvl * (TR)st.dDelta_lp[j];
+        st.cdp[q] += (double)(CEover4 * w * pis) + (double)(CEunder4 * w * pis);
+        st.Cdbopi[q] += (double)(CEover4 * dj) + (double)(CEunder4 * dj);
+        st.Cdbopi2[q] += (double)(CEover4 * dj) + (double)(CEunder4 * dj);
+      }
+    }
+  }
+  st.CdD_self[i] += cdself;
+  st.eatom[T_BOND * n + i] = e_bond;
+  st.eatom[T_LP * n + i] = e_lp;
+  st.eatom[T_OV * n + i] = e_ov;
+  st.eatom[T_UN * n + i] = e_un;
+}
+
+// ================================================================================================================
+// valence angle, penalty, 3-body conjugation (centre j)
+// ================================================================================================================
+template <class TR>
+__global__ void __launch_bounds__(128) k_valence(State st, Globals g, Tables t) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= st.n) return;
+  const int n = st.n, tj = st.type[j];
+  double e_ang = 0, e_pen = 0, e_coa = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0;
+  const int nb = min(st.b_cnt[j], st.maxb);
+  if (tj >= 0 && nb >= 2) {
+    const DevSpecies sj = t.sp[tj];
+    const TR thb_cut = (TR)g.thb_cut, thb_cutsq = (TR)g.thb_cutsq;
+    const TR p_val3 = (TR)sj.p_val3, p_val5 = (TR)sj.p_val5;
+    const TR p_val6 = (TR)g.p_val6, p_val8 = (TR)g.p_val8, p_val9 = (TR)g.p_val9, p_val10 = (TR)g.p_val10;
+    TR SBOp = 0, prod = 1;
+    for (int p = 0; p < nb; ++p) {
+      const size_t q = sl(p, j, n);
+      const TR bo = (TR)st.bo[q];
+      SBOp += (TR)st.bo_pi[q] + (TR)st.bo_pi2[q];
+      TR w = bo * bo;
+      w *= w;
+      w *= w;
+      prod *= rexp<TR>(-w);
+    }
+    const TR dDlp = (TR)st.dDelta_lp[j];
+    TR vlpadj, dSBO2;
+    if ((TR)st.vlpex[j] >= TR(0)) { vlpadj = 0; dSBO2 = prod - TR(1); }
+    else { vlpadj = (TR)st.nlp[j]; dSBO2 = (prod - TR(1)) * (TR(1) - p_val8 * dDlp); }
+    const TR Dboc = (TR)st.Delta_boc[j];
+    const TR SBO = SBOp + (TR(1) - prod) * (-Dboc - p_val8 * vlpadj);
+    const TR dSBO1 = -TR(8) * prod * (Dboc + p_val8 * vlpadj);
+    TR SBO2, CSBO2;
+    if (SBO <= TR(0)) { SBO2 = 0; CSBO2 = 0; }
+    else if (SBO <= TR(1)) { SBO2 = rpow<TR>(SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(SBO, p_val9 - TR(1)); }
+    else if (SBO < TR(2)) { SBO2 = TR(2) - rpow<TR>(TR(2) - SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(TR(2) - SBO, p_val9 - TR(1)); }
+    else { SBO2 = 2; CSBO2 = 0; }
+    const TR ev6 = rexp<TR>(p_val6 * Dboc);
+    const TR Delta = (TR)st.Delta[j], Dval = (TR)st.Delta_val[j];
+    const TR p_pen2 = (TR)g.p_pen2, p_pen3 = (TR)g.p_pen3, p_pen4 = (TR)g.p_pen4;
+    const TR p_coa2 = (TR)g.p_coa2, p_coa3 = (TR)g.p_coa3, p_coa4 = (TR)g.p_coa4;
+    const TR ep3 = rexp<TR>(-p_pen3 * Delta), ep4 = rexp<TR>(p_pen4 * Delta), trm = TR(1) + ep3 + ep4;
+    const TR f9 = (TR(2) + ep3) / trm;
+    const TR Cf9 = (-p_pen3 * ep3 * trm - (TR(2) + ep3) * (-p_pen3 * ep3 + p_pen4 * ep4)) / (trm * trm);
+    const TR ec2 = rexp<TR>(p_coa2 * Dval);
+    TR sumCE5 = 0, sumCE6 = 0;
+    for (int pi = 0; pi < nb; ++pi) {
+      const size_t qi = sl(pi, j, n);
+      const TR BOij = (TR)st.bo[qi];
+      const TR BOA_ij = BOij - thb_cut;
+      if (!(BOA_ij > TR(0))) continue;
+      const int i = st.b_j[qi], ti = st.type[i];
+      const V3<TR> vij{(TR)st.b_dx[qi], (TR)st.b_dy[qi], (TR)st.b_dz[qi]};
+      const TR dij = (TR)st.b_d[qi];
+      for (int pk = pi + 1; pk < nb; ++pk) {
+        const size_t qk = sl(pk, j, n);
+        const TR BOjk = (TR)st.bo[qk];
+        const TR BOA_jk = BOjk - thb_cut;
+        if (!(BOA_jk > TR(0) && BOij > thb_cut && BOjk > thb_cut && BOij * BOjk > thb_cutsq)) continue;
+        const int k = st.b_j[qk], tk = st.type[k];
+        const int2 ai = t.angle_idx[(ti * g.ntypes + tj) * g.ntypes + tk];
+        if (ai.y == 0) continue;
+        const V3<TR> vjk{(TR)st.b_dx[qk], (TR)st.b_dy[qk], (TR)st.b_dz[qk]};
+        const TR djk = (TR)st.b_d[qk];
+        TR th, ct;
+        V3<TR> dci, dcj, dck;
+        angle_and_dcos<TR>(vij, dij, vjk, djk, th, ct, dci, dcj, dck);
+        TR sth = rsin<TR>(th);
+        if (sth < TR(1.0e-5)) sth = TR(1.0e-5);
+        const TR tboi = (TR)st.tbo[i], tbok = (TR)st.tbo[k];
+        for (int a = 0; a < ai.y; ++a) {
+          const DevAngle A = t.angle[ai.x + a];
+          const TR p_val1 = (TR)A.p_val1;
+          if (!(fabs((double)p_val1) > 0.001)) continue;
+          const TR p_val2 = (TR)A.p_val2, p_val4 = (TR)A.p_val4, p_val7 = (TR)A.p_val7, th00 = (TR)A.theta_00;
+          const TR e3ij = rexp<TR>(-p_val3 * rpow<TR>(BOA_ij, p_val4)), f7ij = TR(1) - e3ij;
+          const TR Cf7ij = p_val3 * p_val4 * rpow<TR>(BOA_ij, p_val4 - TR(1)) * e3ij;
+          const TR e3jk = rexp<TR>(-p_val3 * rpow<TR>(BOA_jk, p_val4)), f7jk = TR(1) - e3jk;
+          const TR Cf7jk = p_val3 * p_val4 * rpow<TR>(BOA_jk, p_val4 - TR(1)) * e3jk;
+          const TR ev7 = rexp<TR>(-p_val7 * Dboc), trm8 = TR(1) + ev6 + ev7;
+          const TR f8 = p_val5 - ((p_val5 - TR(1)) * (TR(2) + ev6) / trm8);
+          const TR Cf8 = ((TR(1) - p_val5) / (trm8 * trm8)) * (p_val6 * ev6 * trm8 - (TR(2) + ev6) * (p_val6 * ev6 - p_val7 * ev7));
+          const TR th0 = (TR(180.0) - th00 * (TR(1) - rexp<TR>(-p_val10 * (TR(2) - SBO2)))) * TR(3.14159265) / TR(180.0);
+          const TR e2t = rexp<TR>(-p_val2 * ((th0 - th) * (th0 - th)));
+          const TR e12t = p_val1 >= TR(0) ? p_val1 * (TR(1) - e2t) : p_val1 * -e2t;
+          const TR CE1 = Cf7ij * f7jk * f8 * e12t, CE2 = Cf7jk * f7ij * f8 * e12t, CE3 = Cf8 * f7ij * f7jk * e12t;
+          const TR CE4 = -TR(2) * p_val1 * p_val2 * f7ij * f7jk * f8 * e2t * (th0 - th);
+          const TR Cth0 = p_val10 * (th00 * TR(3.14159265) / TR(180.0)) * rexp<TR>(-p_val10 * (TR(2) - SBO2));
+          const TR CE5 = -CE4 * Cth0 * CSBO2, CE6 = CE5 * dSBO1, CE7 = CE5 * dSBO2, CE8 = -CE4 / sth;
+          e_ang += (double)(f7ij * f7jk * f8 * e12t);
+          const TR p_pen1 = (TR)A.p_pen1;
+          const TR ep2ij = rexp<TR>(-p_pen2 * ((BOA_ij - TR(2)) * (BOA_ij - TR(2)))), ep2jk = rexp<TR>(-p_pen2 * ((BOA_jk - TR(2)) * (BOA_jk - TR(2))));
+          const TR epen = p_pen1 * f9 * ep2ij * ep2jk;
+          e_pen += (double)epen;
+          const TR CEp1 = epen * Cf9 / f9, tp = -TR(2) * p_pen2 * epen, CEp2 = tp * (BOA_ij - TR(2)), CEp3 = tp * (BOA_jk - TR(2));
+          const TR p_coa1 = (TR)A.p_coa1;
+          const TR dti = tboi - BOA_ij, dtk = tbok - BOA_jk;
+          const TR ecoa = p_coa1 / (TR(1) + ec2) * rexp<TR>(-p_coa3 * (dti * dti)) * rexp<TR>(-p_coa3 * (dtk * dtk)) *
+                          rexp<TR>(-p_coa4 * ((BOA_ij - TR(1.5)) * (BOA_ij - TR(1.5)))) * rexp<TR>(-p_coa4 * ((BOA_jk - TR(1.5)) * (BOA_jk - TR(1.5))));
+          e_coa += (double)ecoa;
+          const TR CEc1 = -TR(2) * p_coa4 * (BOA_ij - TR(1.5)) * ecoa, CEc2 = -TR(2) * p_coa4 * (BOA_jk - TR(1.5)) * ecoa;
+          const TR CEc3 = -p_coa2 * ec2 * ecoa / (TR(1) + ec2), CEc4 = -TR(2) * p_coa3 * dti * ecoa, CEc5 = -TR(2) * p_coa3 * dtk * ecoa;
+          st.Cdbo[qi] += (double)(CE1 + CEp2 + (CEc1 - CEc4));
+          st.Cdbo[qk] += (double)(CE2 + CEp3 + (CEc2 - CEc5));
+          cdself += (double)((CE3 + CE7) + CEp1 + CEc3);
+          st.cdp[qi] += (double)CEc4;
+          st.cdp[qk] += (double)CEc5;
+          sumCE5 += CE5;
+          sumCE6 += CE6;
+          fjx += (double)(CE8 * dcj.x); fjy += (double)(CE8 * dcj.y); fjz += (double)(CE8 * dcj.z);
+          st.fpx[qi] += (double)(CE8 * dci.x); st.fpy[qi] += (double)(CE8 * dci.y); st.fpz[qi] += (double)(CE8 * dci.z);
+          st.fpx[qk] += (double)(CE8 * dck.x); st.fpy[qk] += (double)(CE8 * dck.y); st.fpz[qk] += (double)(CE8 * dck.z);
+        }
+      }
+    }
+    if (sumCE5 != TR(0) || sumCE6 != TR(0)) {
+      for (int p = 0; p < nb; ++p) {
+        const size_t q = sl(p, j, n);
+        const TR bo = (TR)st.bo[q], c3 = bo * bo * bo;
+        st.Cdbo[q] += (double)(sumCE6 * (c3 * c3 * bo));
+        st.Cdbopi[q] += (double)sumCE5;
+        st.Cdbopi2[q] += (double)sumCE5;
+      }
+    }
+  }
+  st.CdD_self[j] += cdself;
+  st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
+  st.eatom[T_ANG * n + j] = e_ang;
+  st.eatom[T_PEN * n + j] = e_pen;
+  st.eatom[T_COA * n + j] = e_coa;
+}
+
+// ================================================================================================================
+// torsion + 4-body conjugation, evaluated at each atom of the central bond
+// ================================================================================================================
+template <class TR>
+__device__ inline TR tan_inv(TR s, TR c) {
+  const TR M = TR(1e-10);
+  if (s >= TR(0) && s <= M) return c / M;
+  if (s <= TR(0) && s >= -M) return c / -M;
+  return c / s;
+}
+
+template <class TR>
+__global__ void __launch_bounds__(128) k_torsion(State st, Cell cell, Globals g, Tables t) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= st.n) return;
+  const int n = st.n, tj = st.type[j];
+  double e_tor = 0, e_con = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0;
+  const int nbj = min(st.b_cnt[j], st.maxb);
+  if (tj >= 0 && nbj >= 2) {
+    const TR thb_cut = (TR)g.thb_cut;
+    const TR p_tor2 = (TR)g.p_tor2, p_tor3 = (TR)g.p_tor3, p_tor4 = (TR)g.p_tor4, p_cot2 = (TR)g.p_cot2;
+    const TR MINS = TR(1e-10);
+    const TR Dj = (TR)st.Delta_boc[j];
+    for (int pk = 0; pk < nbj; ++pk) {
+      const size_t qk = sl(pk, j, n);
+      const TR BOjk = (TR)st.bo[qk];
+      if (!(BOjk > thb_cut)) continue;
+      const int k = st.b_j[qk], sk = st.b_s[qk], tk = st.type[k];
+      const int nbk = min(st.b_cnt[k], st.maxb);
+      if (nbk < 2) continue;
+      const bool counted = canonical_first(j, k, sk, st.tag);
+      const int symk = st.b_sym[qk];   // slot of j in k's list
+      const TR BOA_jk = BOjk - thb_cut;
+      const TR rjk = (TR)st.b_d[qk];
+      const V3<TR> djk{(TR)st.b_dx[qk], (TR)st.b_dy[qk], (TR)st.b_dz[qk]};
+      const V3<TR> dkj{-djk.x, -djk.y, -djk.z};
+      const TR Dk = (TR)st.Delta_boc[k];
+      const TR et2jk = rexp<TR>(-p_tor2 * BOA_jk), ec2jk = rexp<TR>(-p_cot2 * ((BOA_jk - TR(1.5)) * (BOA_jk - TR(1.5))));
+      const TR et3 = rexp<TR>(-p_tor3 * (Dj + Dk)), et4 = rexp<TR>(p_tor4 * (Dj + Dk)), it34 = TR(1) / (TR(1) + et3 + et4);
+      const TR f11 = (TR(2) + et3) * it34;
+      const TR BOpijk = (TR)st.bo_pi[qk];
+      const TR dfn11 = (-p_tor3 * et3 + (p_tor3 * et3 - p_tor4 * et4) * (TR(2) + et3) * it34) * it34;
+      double cdbo_jk = 0, cdbopi_jk = 0;
+      for (int pi = 0; pi < nbj; ++pi) {
+        if (pi == pk) continue;
+        const size_t qi = sl(pi, j, n);
+        const TR BOij = (TR)st.bo[qi];
+        if (!(BOij > thb_cut)) continue;
+        const int i = st.b_j[qi], si_ = st.b_s[qi], ti = st.type[i];
+        const TR rij = (TR)st.b_d[qi], BOA_ij = BOij - thb_cut;
+        const V3<TR> dji{(TR)st.b_dx[qi], (TR)st.b_dy[qi], (TR)st.b_dz[qi]};
+        TR thijk, cijk_;
+        V3<TR> dci, dcj, dck;   // d cos(theta_ijk) / d x_i, x_j, x_k
+        angle_and_dcos<TR>(dji, rij, djk, rjk, thijk, cijk_, dci, dcj, dck);
+        const TR sijk = rsin<TR>(thijk), cijk = rcos<TR>(thijk), tijk = tan_inv<TR>(sijk, cijk);
+        const TR et2ij = rexp<TR>(-p_tor2 * BOA_ij), ec2ij = rexp<TR>(-p_cot2 * ((BOA_ij - TR(1.5)) * (BOA_ij - TR(1.5))));
+        double cdbo_ij = 0, fix = 0, fiy = 0, fiz = 0;
+        for (int pl = 0; pl < nbk; ++pl) {
+          if (pl == symk) continue;
+          const size_t ql = sl(pl, k, n);
+          const TR BOkl = (TR)st.bo[ql];
+          if (!(BOkl > thb_cut)) continue;
+          const int l = st.b_j[ql], sl_abs = shift_add(sk, st.b_s[ql]), tl = st.type[l];
+          if (l == i && sl_abs == si_) continue;   // i and l are the same image
+          const int ti4 = t.tors_idx[((ti * g.ntypes + tj) * g.ntypes + tk) * g.ntypes + tl];
+          if (ti4 < 0) continue;
+          if (!(BOij * BOjk * BOkl > thb_cut)) continue;
+          const DevTorsion P = t.tors[ti4];
+          const TR rkl = (TR)st.b_d[ql], BOA_kl = BOkl - thb_cut;
+          const V3<TR> dkl{(TR)st.b_dx[ql], (TR)st.b_dy[ql], (TR)st.b_dz[ql]};
+          TR thjkl, cjkl_;
+          V3<TR> ecj, eck, ecl;   // d cos(theta_jkl) / d x_j, x_k, x_l
+          angle_and_dcos<TR>(dkj, rjk, dkl, rkl, thjkl, cjkl_, ecj, eck, ecl);
+          const TR sjkl = rsin<TR>(thjkl), cjkl = rcos<TR>(thjkl), tjkl = tan_inv<TR>(sjkl, cjkl);
+          // x_i - x_l in j's frame
+          const V3<TR> dli{dji.x - djk.x - dkl.x, dji.y - djk.y - dkl.y, dji.z - djk.z - dkl.z};
+          const TR rli = rsqrt_<TR>(dot3(dli, dli));
+          TR s1 = sijk, c1 = cijk, s2 = sjkl, c2 = cjkl;
+          if (s1 >= TR(0) && s1 <= MINS) s1 = MINS; else if (s1 <= TR(0) && s1 >= -MINS) s1 = -MINS;
+          if (s2 >= TR(0) && s2 <= MINS) s2 = MINS; else if (s2 <= TR(0) && s2 >= -MINS) s2 = -MINS;
+          // bond vectors in the reference's orientation: dij (j->i), djk (j->k), dkl (k->l)
+          const V3<TR>& bij = dji;
+          const TR ucos = -dot3(bij, djk) * dot3(djk, dkl) + (rjk * rjk) * dot3(bij, dkl);
+          const TR usin = -rjk * dot3(bij, cross3(djk, dkl));
+          const TR omega = ratan2<TR>(usin, ucos);
+          const TR htra = rij + c1 * (rkl * c2 - rjk), htrb = rjk - rij * c1 - rkl * c2;
+          const TR hthd = rij * s1 * (rjk - rkl * c2), hthe = rkl * s2 * (rjk - rij * c1);
+          const TR hnra = rkl * s1 * s2, hnhd = rij * rkl * c1 * s2, hnhe = rij * rkl * s1 * c2;
+          const TR tel = (rij * rij) + (rjk * rjk) + (rkl * rkl) - (rli * rli) - TR(2) * (rij * rjk * c1 - rij * rkl * c1 * c2 + rjk * rkl * c2);
+          const TR poem = TR(2) * rij * rkl * s1 * s2;
+          TR arg = tel / poem;
+          if (arg > TR(1)) arg = TR(1);
+          if (arg < TR(-1)) arg = TR(-1);
+          const TR ga = (htra - arg * hnra) / rij, gd = -(hthd - arg * hnhd) / s1, ge = -(hthe - arg * hnhe) / s2;
+          const TR sc = TR(2) / poem;
+          // d omega-like derivative vectors of i and j (the reference's dcos_omega_di / _dj)
+          const V3<TR> doi{sc * (ga * bij.x - dli.x + gd * dci.x), sc * (ga * bij.y - dli.y + gd * dci.y), sc * (ga * bij.z - dli.z + gd * dci.z)};
+          const V3<TR> doj{sc * (-ga * bij.x - htrb / rjk * djk.x + gd * dcj.x + ge * ecj.x), sc * (-ga * bij.y - htrb / rjk * djk.y + gd * dcj.y + ge * ecj.y),
+                           sc * (-ga * bij.z - htrb / rjk * djk.z + gd * dcj.z + ge * ecj.z)};
+          const TR cw = rcos<TR>(omega), c2w = rcos<TR>(TR(2) * omega), c3w = rcos<TR>(TR(3) * omega);
+          const TR x = TR(2) - BOpijk - f11;
+          const TR et1 = rexp<TR>((TR)P.p_tor1 * (x * x));
+          const TR et2kl = rexp<TR>(-p_tor2 * BOA_kl), ec2kl = rexp<TR>(-p_cot2 * ((BOA_kl - TR(1.5)) * (BOA_kl - TR(1.5))));
+          const TR fn10 = (TR(1) - et2ij) * (TR(1) - et2jk) * (TR(1) - et2kl);
+          const TR V1 = (TR)P.V1, V2 = (TR)P.V2, V3_ = (TR)P.V3, p_tor1 = (TR)P.p_tor1, p_cot1 = (TR)P.p_cot1;
+          const TR CV = TR(0.5) * (V1 * (TR(1) + cw) + V2 * et1 * (TR(1) - c2w) + V3_ * (TR(1) + c3w));
+          const TR CT1 = sijk * sjkl * CV;
+          const TR CT2 = -fn10 * TR(2) * p_tor1 * V2 * et1 * x * (TR(1) - (cw * cw)) * sijk * sjkl;
+          const TR CT3 = CT2 * dfn11;
+          const TR CT4 = CT1 * p_tor2 * et2ij * (TR(1) - et2jk) * (TR(1) - et2kl);
+          const TR CT5 = CT1 * p_tor2 * (TR(1) - et2ij) * et2jk * (TR(1) - et2kl);
+          const TR cmn = -fn10 * CV, CT7 = cmn * sjkl * tijk, CT8 = cmn * sijk * tjkl;
+          const TR CT9 = fn10 * sijk * sjkl * (TR(0.5) * V1 - TR(2) * V2 * et1 * cw + TR(1.5) * V3_ * (c2w + TR(2) * (cw * cw)));
+          const TR fn12 = ec2ij * ec2jk * ec2kl;
+          const TR gg = TR(1) + ((cw * cw) - TR(1)) * sijk * sjkl;
+          const TR Cc = -TR(2) * fn12 * p_cot1 * p_cot2 * gg;
+          const TR CC1 = Cc * (BOA_ij - TR(1.5)), CC2 = Cc * (BOA_jk - TR(1.5));
+          const TR CC4 = -p_cot1 * fn12 * ((cw * cw) - TR(1)) * sjkl * tijk;
+          const TR CC5 = -p_cot1 * fn12 * ((cw * cw) - TR(1)) * sijk * tjkl;
+          const TR CC6 = TR(2) * p_cot1 * fn12 * cw * sijk * sjkl;
+          // this end's half: j itself, its outer partner i, its outer bond ij, its CdDelta
+          cdself += (double)CT3;
+          cdbo_ij += (double)(CT4 + CC1);
+          const TR a7 = CT7 + CC4, a8 = CT8 + CC5, a9 = CT9 + CC6;
+          fix += (double)(a7 * dci.x + a9 * doi.x); fiy += (double)(a7 * dci.y + a9 * doi.y); fiz += (double)(a7 * dci.z + a9 * doi.z);
+          fjx += (double)(a7 * dcj.x + a8 * ecj.x + a9 * doj.x);
+          fjy += (double)(a7 * dcj.y + a8 * ecj.y + a9 * doj.y);
+          fjz += (double)(a7 * dcj.z + a8 * ecj.z + a9 * doj.z);
+          if (counted) {   // energy and the central bond once per torsion
+            e_tor += (double)(fn10 * sijk * sjkl * CV);
+            e_con += (double)(p_cot1 * fn12 * gg);
+            cdbopi_jk += (double)CT2;
+            cdbo_jk += (double)(CT5 + CC2);
+          }
+        }
+        if (cdbo_ij != 0.0 || fix != 0.0 || fiy != 0.0 || fiz != 0.0) {
+          st.Cdbo[qi] += cdbo_ij;
+          st.fpx[qi] += fix; st.fpy[qi] += fiy; st.fpz[qi] += fiz;
+        }
+      }
+      if (counted) { st.Cdbo[qk] += cdbo_jk; st.Cdbopi[qk] += cdbopi_jk; }
+    }
+  }
+  (void)cell;
+  st.CdD_self[j] += cdself;
+  st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
+  st.eatom[T_TOR * n + j] = e_tor;
+  st.eatom[T_CON * n + j] = e_con;
+}
+
+// ================================================================================================================
+// hydrogen bonds: donors (own terms, partner-i forces), acceptors (their own force from the same triples)
+// ================================================================================================================
+template <class TR>
+__device__ inline void hbond_term(const DevHBond& P, TR BOij, const V3<TR>& dji, TR rij, const V3<TR>& djk, TR rjk, TR& e, TR& CH1, TR& CH2,
+                                  TR& CH3, V3<TR>& dti, V3<TR>& dtj, V3<TR>& dtk) {
+  TR th, ct;
+  angle_and_dcos<TR>(dji, rij, djk, rjk, th, ct, dti, dtj, dtk);
+  const TR s2 = rsin<TR>(th / TR(2));
+  TR s4 = s2 * s2;
+  s4 *= s4;
+  const TR cx = TR(1) - ct;
+  const TR r0 = (TR)P.r0, p1 = (TR)P.p1, p2 = (TR)P.p2, p3 = (TR)P.p3;
+  const TR eh2 = rexp<TR>(-p2 * BOij), eh3 = rexp<TR>(-p3 * (r0 / rjk + rjk / r0 - TR(2)));
+  e = p1 * (TR(1) - eh2) * eh3 * s4;
+  CH1 = p1 * p2 * eh2 * eh3 * s4;
+  CH2 = -p1 / TR(2) * (TR(1) - eh2) * eh3 * cx;
+  CH3 = -p3 * (-r0 / (rjk * rjk) + TR(1) / r0) * e;
+}
+
+template <class TR>
+__global__ void __launch_bounds__(128) k_hbond(State st, Cell c, Globals g, Tables t) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= st.n) return;
+  const int n = st.n, ta = st.type[a];
+  double e_hb = 0, fax = 0, fay = 0, faz = 0;
+  if (ta >= 0 && g.hbond_cut > 0) {
+    const int role = t.sp[ta].p_hbond;
+    const int nh = min(st.h_cnt[a], st.maxh);
+    if (role == 1) {   // donor hydrogen j = a
+      const int j = a, tj = ta, nb = min(st.b_cnt[j], st.maxb);
+      for (int h = 0; h < nh; ++h) {
+        const int k = st.h_j[sl(h, j, n)], sk = st.h_s[sl(h, j, n)], tk = st.type[k];
+        double dx, dy, dz;
+        const double rjk_d = sqrt(pair_dvec(st, c, j, k, sk, dx, dy, dz));
+        const V3<TR> djk{(TR)dx, (TR)dy, (TR)dz};
+        const TR rjk = (TR)rjk_d;
+        for (int p = 0; p < nb; ++p) {
+          const size_t q = sl(p, j, n);
+          const int i = st.b_j[q], ti = st.type[i];
+          if (t.sp[ti].p_hbond != 2 || !((TR)st.bo[q] >= TR(1e-2))) continue;
+          if (st.tag[i] == st.tag[k]) continue;   // the same atom, possibly another image (reference quirk Q-32)
+          const DevHBond P = t.hb[(ti * g.ntypes + tj) * g.ntypes + tk];
+          if (!(P.r0 > 0.0)) continue;
+          const V3<TR> dji{(TR)st.b_dx[q], (TR)st.b_dy[q], (TR)st.b_dz[q]};
+          TR e, CH1, CH2, CH3;
+          V3<TR> dti, dtj, dtk;
+          hbond_term<TR>(P, (TR)st.bo[q], dji, (TR)st.b_d[q], djk, rjk, e, CH1, CH2, CH3, dti, dtj, dtk);
+          e_hb += (double)e;
+          st.Cdbo[q] += (double)CH1;
+          st.fpx[q] += (double)(CH2 * dti.x); st.fpy[q] += (double)(CH2 * dti.y); st.fpz[q] += (double)(CH2 * dti.z);
+          fax += (double)(CH2 * dtj.x - CH3 / rjk * djk.x);
+          fay += (double)(CH2 * dtj.y - CH3 / rjk * djk.y);
+          faz += (double)(CH2 * dtj.z - CH3 / rjk * djk.z);
+        }
+      }
+    } else if (role == 2) {   // acceptor k = a: re-evaluate the triples (i, j, k) of every donor j in range for the force on k
+      const int k = a, tk = ta;
+      for (int h = 0; h < nh; ++h) {
+        const int j = st.h_j[sl(h, k, n)], sj = st.h_s[sl(h, k, n)], tj = st.type[j];
+        double dx, dy, dz;
+        const double rkj_d = sqrt(pair_dvec(st, c, k, j, sj, dx, dy, dz));
+        const V3<TR> djk{(TR)(-dx), (TR)(-dy), (TR)(-dz)};   // from the donor j to k
+        const TR rjk = (TR)rkj_d;
+        const int nb = min(st.b_cnt[j], st.maxb);
+        for (int p = 0; p < nb; ++p) {
+          const size_t q = sl(p, j, n);
+          const int i = st.b_j[q], ti = st.type[i];
+          if (t.sp[ti].p_hbond != 2 || !((TR)st.bo[q] >= TR(1e-2))) continue;
+          if (st.tag[i] == st.tag[k]) continue;
+          const DevHBond P = t.hb[(ti * g.ntypes + tj) * g.ntypes + tk];
+          if (!(P.r0 > 0.0)) continue;
+          const V3<TR> dji{(TR)st.b_dx[q], (TR)st.b_dy[q], (TR)st.b_dz[q]};
+          TR e, CH1, CH2, CH3;
+          V3<TR> dti, dtj, dtk;
+          hbond_term<TR>(P, (TR)st.bo[q], dji, (TR)st.b_d[q], djk, rjk, e, CH1, CH2, CH3, dti, dtj, dtk);
+          fax += (double)(CH2 * dtk.x + CH3 / rjk * djk.x);
+          fay += (double)(CH2 * dtk.y + CH3 / rjk * djk.y);
+          faz += (double)(CH2 * dtk.z + CH3 / rjk * djk.z);
+        }
+      }
+    }
+  }
+  st.fs_x[a] += fax; st.fs_y[a] += fay; st.fs_z[a] += faz;
+  st.eatom[T_HB * n + a] = e_hb;
+}
+
+// ================================================================================================================
+// bond-order derivative pass as gathers
+// ================================================================================================================
+__global__ void __launch_bounds__(128) k_cdelta(State st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
+  double c = st.CdD_self[i];
+  for (int p = 0; p < nb; ++p) {
+    const size_t q = sl(p, i, n);
+    c += st.cdp[sl(st.b_sym[q], st.b_j[q], n)];
+  }
+  st.CdD[i] = c;
+}
+
+__global__ void __launch_bounds__(128) k_dbond(State st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
+  double S = 0, Dx = 0, Dy = 0, Dz = 0;
+  const double cdi = st.CdD[i];
+  for (int p = 0; p < nb; ++p) {
+    const size_t q = sl(p, i, n);
+    const int j = st.b_j[q];
+    const size_t m = sl(st.b_sym[q], j, n);
+    const double cbo = st.Cdbo[q] + st.Cdbo[m], cpi = st.Cdbopi[q] + st.Cdbopi[m], cpi2 = st.Cdbopi2[q] + st.Cdbopi2[m];
+    const double cdel = cdi + st.CdD[j];
+    const double C1dbo = st.C[0][q], C2pi = st.C[1][q], C2pi2 = st.C[2][q], C1pi = st.C[3][q], C1pi2 = st.C[4][q];
+    const double T1 = (C1dbo * cbo + C1dbo * cdel) + C2pi * cpi + C2pi2 * cpi2;
+    S += (st.C[5][q] * cbo + st.C[5][q] * cdel) + st.C[6][q] * cpi + st.C[7][q] * cpi2;
+    const double sc = T1 * st.dbop_c[q] + (C1pi * cpi) * st.dlnpi_c[q] + (C1pi2 * cpi2) * st.dlnpi2_c[q];
+    Dx += sc * st.b_dx[q]; Dy += sc * st.b_dy[q]; Dz += sc * st.b_dz[q];
+  }
+  st.S[i] = S;
+  st.Dx[i] = Dx; st.Dy[i] = Dy; st.Dz[i] = Dz;
+}
+
+// final gradient: bonded explicit-geometry terms (own + partner slots), BO-derivative pair forces, non-bonded
+__global__ void __launch_bounds__(128) k_gather(State st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
+  const double Si = st.S[i];
+  double gx = st.fs_x[i] + st.Dx[i] + Si * st.dDps_x[i];
+  double gy = st.fs_y[i] + st.Dy[i] + Si * st.dDps_y[i];
+  double gz = st.fs_z[i] + st.Dz[i] + Si * st.dDps_z[i];
+  for (int p = 0; p < nb; ++p) {
+    const size_t q = sl(p, i, n);
+    const int j = st.b_j[q];
+    const size_t m = sl(st.b_sym[q], j, n);
+    gx += st.fpx[m]; gy += st.fpy[m]; gz += st.fpz[m];
+    const double w = st.S[j] * st.dbop_c[q];
+    gx += w * st.b_dx[q]; gy += w * st.b_dy[q]; gz += w * st.b_dz[q];
+  }
+  st.gx[i] = gx + st.gnb_x[i];
+  st.gy[i] = gy + st.gnb_y[i];
+  st.gz[i] = gz + st.gnb_z[i];
+}
+
+inline int nblk(long long items, int per) { return (int)((items + per - 1) / per); }
+
+template <class TB>
+void bond_build_t(const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
+  hipLaunchKernelGGL(k_bond_build<TB>, dim3(nblk((long long)st.n * 64, 256)), dim3(256), 0, s, st, c, g, t);
+}
+
+}  // namespace
+
+void launch_bond_build(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
+  if (p == Precision::FP32) bond_build_t<float>(st, c, g, t, s);
+  else bond_build_t<double>(st, c, g, t, s);
+}
+
+void launch_bond_sym(const State& st, hipStream_t s) {
+  hipLaunchKernelGGL(k_bond_sym, dim3(nblk(st.n, 128)), dim3(128), 0, s, st);
+}
+
+void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
+  if (p == Precision::FP32) hipLaunchKernelGGL(k_bo_correct<float>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+  else hipLaunchKernelGGL(k_bo_correct<double>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+}
+
+void launch_bonded_terms(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
+  const dim3 grid(nblk(st.n, 128)), blk(128);
+  if (p == Precision::FP64) {
+    hipLaunchKernelGGL(k_atom_terms<double>, grid, blk, 0, s, st, g, t);
+    hipLaunchKernelGGL(k_valence<double>, grid, blk, 0, s, st, g, t);
+    hipLaunchKernelGGL(k_torsion<double>, grid, blk, 0, s, st, c, g, t);
+    hipLaunchKernelGGL(k_hbond<double>, grid, blk, 0, s, st, c, g, t);
+  } else {
+    hipLaunchKernelGGL(k_atom_terms<float>, grid, blk, 0, s, st, g, t);
+    hipLaunchKernelGGL(k_valence<float>, grid, blk, 0, s, st, g, t);
+    hipLaunchKernelGGL(k_torsion<float>, grid, blk, 0, s, st, c, g, t);
+    hipLaunchKernelGGL(k_hbond<float>, grid, blk, 0, s, st, c, g, t);
+  }
+}
+
+void launch_bond_forces(Precision, const State& st, hipStream_t s) {
+  const dim3 grid(nblk(st.n, 128)), blk(128);
+  hipLaunchKernelGGL(k_cdelta, grid, blk, 0, s, st);
+  hipLaunchKernelGGL(k_dbond, grid, blk, 0, s, st);
+  hipLaunchKernelGGL(k_gather, grid, blk, 0, s, st);
+}
+
+}  // namespace rxa

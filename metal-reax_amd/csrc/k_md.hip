@@ -1,0 +1,204 @@
+// Fixed-order reductions (energies, virial, kinetic energy, displacement) and the standalone NVT integrator of the engine:
+// velocity Verlet with a Nose-Hoover chain thermostat, the scheme of LAMMPS fix nvt (FixNH::nhc_temp_integrate, tchain 3, tloop 1),
+// in LAMMPS "real" units. Inside LAMMPS the plugin never integrates; this is the MD loop of the standalone engine and of bench.py.
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace rxa {
+namespace {
+
+constexpr int kRT = 256;
+constexpr int kRBlocks = 240;
+constexpr int kRed = kTerms + 6;
+
+__device__ inline void block_sum(double* v, int K, double* sh) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < K; ++k) sh[k * kRT + tid] = v[k];
+  __syncthreads();
+  for (int s = kRT / 2; s > 0; s >>= 1) {
+    if (tid < s)
+      for (int k = 0; k < K; ++k) sh[k * kRT + tid] += sh[k * kRT + tid + s];
+    __syncthreads();
+  }
+  for (int k = 0; k < K; ++k) v[k] = sh[k * kRT];
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kRT) k_energy_partial(State st, double* part) {
+  __shared__ double sh[kRed * kRT];
+  double v[kRed];
+  for (int k = 0; k < kRed; ++k) v[k] = 0;
+  const int n = st.n;
+  for (int i = blockIdx.x * kRT + threadIdx.x; i < n; i += gridDim.x * kRT) {
+    for (int k = 0; k < kTerms; ++k) v[k] += st.eatom[k * n + i];
+    for (int k = 0; k < 6; ++k) v[kTerms + k] += st.vatom[k * n + i];
+  }
+  block_sum(v, kRed, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kRed; ++k) part[blockIdx.x * kRed + k] = v[k];
+}
+
+__global__ void __launch_bounds__(kRT) k_energy_final(const double* part, double* out) {
+  __shared__ double sh[kRed * kRT];
+  double v[kRed];
+  for (int k = 0; k < kRed; ++k) v[k] = threadIdx.x < kRBlocks ? part[threadIdx.x * kRed + k] : 0.0;
+  block_sum(v, kRed, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kRed; ++k) out[k] = v[k];
+}
+
+// ---- integrator -----------------------------------------------------------------------------------------------
+// nh layout: [0..2] eta, [3..6] eta_dot (last stays 0), [7..9] eta_dotdot, [10..12] eta_mass, [13] t_target, [14] t_freq,
+// [15] dt, [16] dof, [17] boltz, [18] mvv2e, [19] last scale factor, [20] current temperature, [21] kinetic energy
+__global__ void k_half_kick(State st, MdBuffers md, double dtf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const double f = dtf / md.mass[i];
+  md.vx[i] -= f * st.gx[i];   // force = -gradient
+  md.vy[i] -= f * st.gy[i];
+  md.vz[i] -= f * st.gz[i];
+}
+
+__global__ void k_drift(State st, MdBuffers md, double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  st.x[i] += dt * md.vx[i];
+  st.y[i] += dt * md.vy[i];
+  st.z[i] += dt * md.vz[i];
+}
+
+__global__ void __launch_bounds__(kRT) k_ke_partial(State st, MdBuffers md) {
+  __shared__ double sh[kRT];
+  double v = 0;
+  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT)
+    v += md.mass[i] * (md.vx[i] * md.vx[i] + md.vy[i] * md.vy[i] + md.vz[i] * md.vz[i]);
+  block_sum(&v, 1, sh);
+  if (threadIdx.x == 0) md.partial[blockIdx.x] = v;
+}
+
+// one Nose-Hoover chain half step (LAMMPS FixNH::nhc_temp_integrate, mtchain 3, nc_tchain 1, no drag); writes the velocity factor
+__global__ void __launch_bounds__(kRT) k_nhc(MdBuffers md) {
+  __shared__ double sh[kRT];
+  double v = threadIdx.x < kRBlocks ? md.partial[threadIdx.x] : 0.0;
+  block_sum(&v, 1, sh);
+  if (threadIdx.x != 0) return;
+  double* nh = md.nh;
+  const double t_target = nh[13], t_freq = nh[14], dt = nh[15], dof = nh[16], boltz = nh[17], mvv2e = nh[18];
+  const double ke2 = v * mvv2e;   // sum m v^2 in energy units = 2 KE
+  double t_current = ke2 / (dof * boltz);
+  double* eta = nh;
+  double* eta_dot = nh + 3;
+  double* eta_dotdot = nh + 7;
+  double* eta_mass = nh + 10;
+  const int M = 3;
+  const double dt4 = 0.25 * dt, dt8 = 0.125 * dt, dthalf = 0.5 * dt;
+  const double ke_target = dof * boltz * t_target;
+  eta_mass[0] = dof * boltz * t_target / (t_freq * t_freq);
+  for (int c = 1; c < M; ++c) eta_mass[c] = boltz * t_target / (t_freq * t_freq);
+  double kecurrent = dof * boltz * t_current;
+  eta_dotdot[0] = eta_mass[0] > 0 ? (kecurrent - ke_target) / eta_mass[0] : 0.0;
+  for (int c = M - 1; c > 0; --c) {
+    const double e = exp(-dt8 * eta_dot[c + 1]);
+    eta_dot[c] *= e;
+    eta_dot[c] += eta_dotdot[c] * dt4;
+    eta_dot[c] *= e;
+  }
+  double e = exp(-dt8 * eta_dot[1]);
+  eta_dot[0] *= e;
+  eta_dot[0] += eta_dotdot[0] * dt4;
+  eta_dot[0] *= e;
+  const double factor = exp(-dthalf * eta_dot[0]);
+  t_current *= factor * factor;
+  kecurrent = dof * boltz * t_current;
+  eta_dotdot[0] = eta_mass[0] > 0 ? (kecurrent - ke_target) / eta_mass[0] : 0.0;
+  for (int c = 0; c < M; ++c) eta[c] += dthalf * eta_dot[c];
+  eta_dot[0] *= e;
+  eta_dot[0] += eta_dotdot[0] * dt4;
+  eta_dot[0] *= e;
+  for (int c = 1; c < M; ++c) {
+    e = exp(-dt8 * eta_dot[c + 1]);
+    eta_dot[c] *= e;
+    eta_dotdot[c] = (eta_mass[c - 1] * eta_dot[c - 1] * eta_dot[c - 1] - boltz * t_target) / eta_mass[c];
+    eta_dot[c] += eta_dotdot[c] * dt4;
+    eta_dot[c] *= e;
+  }
+  nh[19] = factor;
+  nh[20] = t_current;
+  nh[21] = 0.5 * kecurrent;
+}
+
+__global__ void k_scale_v(State st, MdBuffers md) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const double f = md.nh[19];
+  md.vx[i] *= f;
+  md.vy[i] *= f;
+  md.vz[i] *= f;
+}
+
+__global__ void k_ref_positions(State st, MdBuffers md) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  md.x0[i] = st.x[i];
+  md.y0[i] = st.y[i];
+  md.z0[i] = st.z[i];
+}
+
+__global__ void __launch_bounds__(kRT) k_disp_partial(State st, MdBuffers md) {
+  __shared__ double sh[kRT];
+  double m = 0;
+  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT) {
+    const double dx = st.x[i] - md.x0[i], dy = st.y[i] - md.y0[i], dz = st.z[i] - md.z0[i];
+    m = fmax(m, dx * dx + dy * dy + dz * dz);
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kRT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) md.partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kRT) k_disp_final(MdBuffers md) {
+  __shared__ double sh[kRT];
+  sh[threadIdx.x] = threadIdx.x < kRBlocks ? md.partial[threadIdx.x] : 0.0;
+  __syncthreads();
+  for (int s = kRT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) md.disp[0] = sqrt(sh[0]);
+}
+
+inline int nblk(int n, int per) { return (n + per - 1) / per; }
+
+}  // namespace
+
+void launch_energy_reduce(const State& st, double* out, hipStream_t s) {
+  // out[kTerms + 6] receives the sums; out + kRed .. holds kRBlocks * kRed partials (scratch)
+  double* part = out + kRed;
+  hipLaunchKernelGGL(k_energy_partial, dim3(kRBlocks), dim3(kRT), 0, s, st, part);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kRT), 0, s, part, out);
+}
+
+void launch_md_half_kick(const State& st, MdBuffers& md, double dtf, hipStream_t s) {
+  hipLaunchKernelGGL(k_half_kick, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md, dtf);
+}
+void launch_md_drift(const State& st, MdBuffers& md, double dt, hipStream_t s) {
+  hipLaunchKernelGGL(k_drift, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md, dt);
+}
+void launch_md_thermostat(const State& st, MdBuffers& md, hipStream_t s) {
+  hipLaunchKernelGGL(k_ke_partial, dim3(kRBlocks), dim3(kRT), 0, s, st, md);
+  hipLaunchKernelGGL(k_nhc, dim3(1), dim3(kRT), 0, s, md);
+  hipLaunchKernelGGL(k_scale_v, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md);
+}
+void launch_md_ref_positions(const State& st, MdBuffers& md, hipStream_t s) {
+  hipLaunchKernelGGL(k_ref_positions, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md);
+}
+void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s) {
+  hipLaunchKernelGGL(k_disp_partial, dim3(kRBlocks), dim3(kRT), 0, s, st, md);
+  hipLaunchKernelGGL(k_disp_final, dim3(1), dim3(kRT), 0, s, md);
+}
+
+}  // namespace rxa

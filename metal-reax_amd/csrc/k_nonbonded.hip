@@ -1,0 +1,196 @@
+// Tapered van der Waals + shielded Coulomb + polarization, and the charge-equilibration matrix H.
+//
+// One 64-lane wave per owned atom walks that atom's full skin list; every pair (i, j, s) with r <= nonb_cut is evaluated from both
+// ends, each end keeping only its own gradient (-CE dvec on i) and half of the pair energy, so no atom writes another atom's force
+// (atomic-free, bitwise repeatable: fixed lane assignment + fixed butterfly reduction). The distance is the canonical one, so both
+// ends agree on the cut-off decision and on r. Reference: src/cpu/nonbonded.cpp:18-82 (pair loop, owner-computes counting: a
+// physical pair counted once = half from each end here), include/reaxmetal/terms.hpp:262-325 (formulas), terms_host.hpp:17-39
+// (taper). FP64 mode uses the reference's Horner taper and pow(x, 0.33333333333333); FP32 terms use the algebraically equal
+// t-form of the taper (no cancellation of O(10) coefficients in float) and the float cube root.
+//
+// H (stock fix qeq/reaxff compute_H / calculate_H, LAMMPS fix_qeq_reaxff.cpp): H_ij = 14.4 Tap_q(r) / (r^3 + (g_i g_j)^-1.5)^(1/3) for
+// r^2 <= swb^2, stored as full rows (both directions, every image), FP64 plus an FP32 copy for the inner solves of the mixed mode.
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace rxa {
+namespace {
+
+template <class TR>
+__device__ inline void taper_eval(const Globals& g, TR r, TR& Tap, TR& dTap) {
+  if (sizeof(TR) == 8) {
+    const double* c = g.tap;
+    const double rr = (double)r;
+    double T = c[7] * rr + c[6];
+    T = T * rr + c[5]; T = T * rr + c[4]; T = T * rr + c[3]; T = T * rr + c[2]; T = T * rr + c[1]; T = T * rr + c[0];
+    double dT = 7 * c[7] * rr + 6 * c[6];
+    dT = dT * rr + 5 * c[5]; dT = dT * rr + 4 * c[4]; dT = dT * rr + 3 * c[3]; dT = dT * rr + 2 * c[2];
+    dT += c[1] / rr;
+    Tap = (TR)T;
+    dTap = (TR)dT;
+  } else {
+    const TR swa = (TR)g.nonb_low, d = (TR)g.nonb_cut - (TR)g.nonb_low;
+    const TR x = (r - swa) / d, x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+    Tap = TR(1) - x4 * (TR(35) - TR(84) * x + TR(70) * x2 - TR(20) * x3);
+    dTap = (x3 * (TR(-140) + TR(420) * x - TR(420) * x2 + TR(140) * x3)) / (d * r);
+  }
+}
+
+template <class TR> __device__ inline TR cbrt_ref(TR x);
+template <> __device__ inline double cbrt_ref(double x) { return pow(x, 0.33333333333333); }
+template <> __device__ inline float cbrt_ref(float x) { return cbrtf(x); }
+template <class TR> __device__ inline TR tpow(TR a, TR b);
+template <> __device__ inline double tpow(double a, double b) { return pow(a, b); }
+template <> __device__ inline float tpow(float a, float b) { return powf(a, b); }
+template <class TR> __device__ inline TR texp(TR a);
+template <> __device__ inline double texp(double a) { return exp(a); }
+template <> __device__ inline float texp(float a) { return expf(a); }
+
+template <class TR>
+__global__ void __launch_bounds__(256) k_nonbonded(State st, Cell c, Globals g, Tables t) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  const int lane = lane_id();
+  const int ti = st.type[i];
+  double gx = 0, gy = 0, gz = 0, ev = 0, ee = 0;
+  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+  if (ti >= 0) {
+    const size_t base = (size_t)i * st.maxnb;
+    const int cnt = min(st.nb_cnt[i], st.maxnb);
+    const TR qi = (TR)st.q[i];
+    const TR p_vdW1 = (TR)g.p_vdW1, p1i = TR(1) / p_vdW1;
+    const int vt = g.vdw_type;
+    for (int k = lane; k < cnt; k += 64) {
+      const int j = st.nb_j[base + k], tj = st.type[j];
+      if (tj < 0) continue;
+      double dx, dy, dz;
+      const double r2 = pair_dvec(st, c, i, j, st.nb_s[base + k], dx, dy, dz);
+      const double rd = sqrt(r2);
+      if (!(rd <= g.nonb_cut)) continue;
+      const DevPair P = t.pair[ti * g.ntypes + tj];
+      const TR r = (TR)rd;
+      TR Tap, dTap;
+      taper_eval<TR>(g, r, Tap, dTap);
+      TR evdw = 0, CEvd;
+      const TR alpha = (TR)P.alpha, rvdw = (TR)P.r_vdW, D = (TR)P.D;
+      if (vt == 1 || vt == 3) {
+        const TR pr = tpow<TR>(r, p_vdW1), pg = tpow<TR>(TR(1) / (TR)P.gamma_w, p_vdW1), S = pr + pg;
+        const TR fn13 = tpow<TR>(S, p1i);
+        const TR e1 = texp<TR>(alpha * (TR(1) - fn13 / rvdw)), e2 = texp<TR>(TR(0.5) * alpha * (TR(1) - fn13 / rvdw));
+        const TR e0 = D * (e1 - TR(2) * e2);
+        evdw = Tap * e0;
+        const TR dfn13 = fn13 / S * pr / (r * r);
+        CEvd = dTap * e0 - Tap * D * (alpha / rvdw) * (e1 - e2) * dfn13;
+      } else {
+        const TR e1 = texp<TR>(alpha * (TR(1) - r / rvdw)), e2 = texp<TR>(TR(0.5) * alpha * (TR(1) - r / rvdw));
+        const TR e0 = D * (e1 - TR(2) * e2);
+        evdw = Tap * e0;
+        CEvd = dTap * e0 - Tap * D * (alpha / rvdw) * (e1 - e2) / r;
+      }
+      if (vt == 2 || vt == 3) {
+        const TR acore = (TR)P.acore, rcore = (TR)P.rcore;
+        const TR ec = (TR)P.ecore * texp<TR>(acore * (TR(1) - (r / rcore)));
+        evdw += Tap * ec;
+        CEvd += dTap * ec + Tap * (-(acore / rcore) * ec) / r;
+        if (g.lg) {
+          const TR r6 = (r * r * r) * (r * r * r), r5 = r6 / r, lre = (TR)P.lgre, re6 = (lre * lre * lre) * (lre * lre * lre);
+          const TR elg = -((TR)P.lgcij / (r6 + re6));
+          evdw += Tap * elg;
+          CEvd += dTap * elg + Tap * (-TR(6) * elg * r5 / (r6 + re6)) / r;
+        }
+      }
+      const TR qj = (TR)st.q[j];
+      const TR d1 = r * r * r + (TR)P.gamma, d3 = cbrt_ref<TR>(d1);
+      const TR qq = TR(332.06371) * qi * qj;
+      const TR eel = qq * Tap / d3;
+      const TR CE = CEvd + qq * (dTap - Tap * r / d1) / d3;
+      const double ce = (double)CE;
+      gx += -ce * dx; gy += -ce * dy; gz += -ce * dz;
+      ev += 0.5 * (double)evdw;
+      ee += 0.5 * (double)eel;
+      // half of the pair virial  r_ij (x) f_j = -CE dvec (x) dvec
+      const double hv = -0.5 * ce;
+      v0 += hv * dx * dx; v1 += hv * dy * dy; v2 += hv * dz * dz; v3 += hv * dx * dy; v4 += hv * dx * dz; v5 += hv * dy * dz;
+    }
+  }
+  gx = wave_sum(gx); gy = wave_sum(gy); gz = wave_sum(gz); ev = wave_sum(ev); ee = wave_sum(ee);
+  v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2); v3 = wave_sum(v3); v4 = wave_sum(v4); v5 = wave_sum(v5);
+  if (lane == 0) {
+    const int n = st.n;
+    st.gnb_x[i] = gx; st.gnb_y[i] = gy; st.gnb_z[i] = gz;
+    st.eatom[T_VDW * n + i] = ev;
+    st.eatom[T_ELE * n + i] = ee;
+    double ep = 0;
+    if (ti >= 0) {
+      const DevSpecies s = t.sp[ti];
+      const TR q = (TR)st.q[i];
+      ep = (double)(TR(23.02) * ((TR)s.chi * q + ((TR)s.eta / TR(2.)) * (q * q)));
+    }
+    st.eatom[T_POL * n + i] = ep;
+    st.vatom[0 * n + i] = v0; st.vatom[1 * n + i] = v1; st.vatom[2 * n + i] = v2;
+    st.vatom[3 * n + i] = v3; st.vatom[4 * n + i] = v4; st.vatom[5 * n + i] = v5;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_qeq_build(State st, Cell c, Globals g, Tables t, QeqBuffers qb) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  const int lane = lane_id();
+  const int ti = st.type[i];
+  const size_t base = (size_t)i * st.maxnb;
+  const int cnt = min(st.nb_cnt[i], st.maxnb);
+  const double swb2 = g.q_swb * g.q_swb;
+  const double* T = g.qtap;
+  int m = 0;
+  for (int k0 = 0; k0 < cnt; k0 += 64) {
+    const int k = k0 + lane;
+    bool keep = false;
+    int j = 0;
+    double h = 0;
+    if (k < cnt && ti >= 0) {
+      j = st.nb_j[base + k];
+      const int tj = st.type[j];
+      double dx, dy, dz;
+      const double r2 = pair_dvec(st, c, i, j, st.nb_s[base + k], dx, dy, dz);
+      if (tj >= 0 && r2 <= swb2) {
+        const double r = sqrt(r2);
+        double Tap = T[7] * r + T[6];
+        Tap = Tap * r + T[5]; Tap = Tap * r + T[4]; Tap = Tap * r + T[3]; Tap = Tap * r + T[2]; Tap = Tap * r + T[1]; Tap = Tap * r + T[0];
+        const double den = pow(r * r * r + t.qeq_shld[ti * g.ntypes + tj], 1.0 / 3.0);
+        h = Tap * 14.4 / den;
+        keep = true;
+      }
+    }
+    const unsigned long long msk = __ballot(keep);
+    if (keep) {
+      const int slot = m + lanes_below(msk);
+      qb.col[base + slot] = j;
+      qb.val[base + slot] = h;
+      qb.val32[base + slot] = (float)h;
+    }
+    m += __popcll(msk);
+  }
+  if (lane == 0) {
+    qb.cnt[i] = m;
+    const double eta = ti >= 0 ? t.qeq_eta[ti] : 1.0;
+    qb.eta[i] = eta;
+    qb.hinv[i] = 1.0 / eta;
+    qb.chi[i] = ti >= 0 ? t.qeq_chi[ti] : 0.0;
+  }
+}
+
+inline int nblk(long long items, int per) { return (int)((items + per - 1) / per); }
+
+}  // namespace
+
+void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
+  const dim3 grid(nblk((long long)st.n * 64, 256)), blk(256);
+  if (p == Precision::FP64) hipLaunchKernelGGL(k_nonbonded<double>, grid, blk, 0, s, st, c, g, t);
+  else hipLaunchKernelGGL(k_nonbonded<float>, grid, blk, 0, s, st, c, g, t);
+}
+
+void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s) {
+  hipLaunchKernelGGL(k_qeq_build, dim3(nblk((long long)st.n * 64, 256)), dim3(256), 0, s, st, c, g, t, qb);
+}
+
+}  // namespace rxa

@@ -1,0 +1,78 @@
+// Host-side launchers of the device kernels (one translation unit per kernel family; engine.hip orchestrates them).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace rxa {
+
+enum class Precision { FP64 = 0, MIXED = 1, FP32 = 2 };
+
+struct BinGrid {
+  int nc[3] = {1, 1, 1};
+  int ncells() const { return nc[0] * nc[1] * nc[2]; }
+};
+struct BinBuffers {
+  unsigned *keys = nullptr, *skeys = nullptr, *atom_key = nullptr;
+  int *idx = nullptr, *sidx = nullptr, *cstart = nullptr, *cend = nullptr;
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+};
+
+// ---- neighbour search (k_neighbor.hip)
+void launch_wrap(const State& st, const Cell& c, hipStream_t s);
+void launch_fill_int(int* p, int n, int v, hipStream_t s);
+size_t bin_temp_bytes(int n);
+void launch_bin(const State& st, const Cell& c, const BinGrid& g, BinBuffers& b, hipStream_t s);
+void launch_nbr_cells(const State& st, const Cell& c, const BinGrid& g, const BinBuffers& b, double rs, hipStream_t s);
+void launch_nbr_brute(const State& st, const Cell& c, const int sm[3], double rs, hipStream_t s);
+
+// ---- bond-order chain and bonded terms (k_bonded.hip)
+void launch_bond_build(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
+void launch_bond_sym(const State& st, hipStream_t s);
+void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s);
+void launch_bonded_terms(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
+void launch_bond_forces(Precision p, const State& st, hipStream_t s);
+
+// ---- non-bonded terms and the charge-equilibration matrix (k_nonbonded.hip)
+void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
+
+struct QeqBuffers {
+  int* cnt = nullptr;        // n: entries per row
+  int* col = nullptr;        // n * maxnb: owned column
+  double* val = nullptr;     // n * maxnb: H_ij (FP64)
+  float* val32 = nullptr;    // n * maxnb: H_ij (FP32 copy, inner solves of the mixed mode)
+  double* eta = nullptr;     // n: diagonal (fix eta, 2 x ffield)
+  double* hinv = nullptr;    // n: 1 / eta
+  double* chi = nullptr;     // n
+  // CG state, two systems (s: b = -chi, t: b = -1) interleaved per atom: [2*i + sys]
+  double *b, *x, *r, *d, *p, *q;
+  double* partial;           // 2 * kCgBlocks * 2
+  double* scal;              // device scalars (see k_qeq.hip)
+  int* iscal;                // device integer scalars
+  double *s_hist, *t_hist;   // 4 * n each, [k * n + i], k = 0 most recent
+};
+constexpr int kCgBlocks = 256;
+void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s);
+void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, double* y, hipStream_t s);
+void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s);
+void launch_qeq_spmv(const State& st, QeqBuffers& qb, int it, bool fp32_matvec, hipStream_t s);        // q = A d (+ d.q partials)
+void launch_qeq_update(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);   // alpha, x, r, p, beta, d
+void launch_qeq_finish(const State& st, QeqBuffers& qb, hipStream_t s);
+
+// ---- reductions and integration (k_md.hip)
+void launch_energy_reduce(const State& st, double* out /*kTerms + 6 + 1*/, hipStream_t s);
+struct MdBuffers {
+  double *vx, *vy, *vz, *mass;   // per atom (mass from type)
+  double* nh;                    // thermostat state (see k_md.hip)
+  double* disp;                  // max displacement since the last rebuild
+  double *x0, *y0, *z0;          // positions at the last rebuild
+  double* partial;               // reduction scratch
+};
+void launch_md_half_kick(const State& st, MdBuffers& md, double dtf, hipStream_t s);
+void launch_md_drift(const State& st, MdBuffers& md, double dt, hipStream_t s);
+void launch_md_thermostat(const State& st, MdBuffers& md, hipStream_t s);
+void launch_md_ref_positions(const State& st, MdBuffers& md, hipStream_t s);
+void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s);
+
+}  // namespace rxa

@@ -56,6 +56,11 @@ def main():
             "id": name, "elements": case["elements"], "origin": origin, "vectors": vectors, "periodic": case["periodic"],
             "positions": gold["positions"], "types": gold["types"], "charges": gold["charges"],
             "lammps_tallies": {k: tal.get(k, 0) for k in ("vdw.oo", "vdw.og", "vdw.self")},
+            # the reference's conditioning class: well-conditioned under its extended definition (no near-linear angle X1, no
+            # near-threshold product X2, no pre-registered W-flag); only these carry the tight FP64 bar
+            "well_conditioned": bool(gold.get("conditioning", {}).get("well_conditioned_extended", False)),
+            "conditioning_flags": list(gold.get("conditioning", {}).get("extended_flags", [])) +
+                                  list(gold.get("conditioning", {}).get("fails_pre_registered", [])),
         }
         if name.startswith("cho_"):
             path = "/tmp/_golden_case.txt"

@@ -1,0 +1,83 @@
+"""Charge equilibration on the device vs the oracle's restatement of stock fix qeq/reaxff.
+
+Tolerances: matvec 1e-12 relative (FP64 both sides, summation order differs); charges 1e-9 e at CG tolerance 1e-11 (the
+reference's C1 charge bar is 9e-11 e against LAMMPS; 1e-9 leaves room for the two solves' different rounding of the residual
+recursion); iteration counts within 1 of the stock loop's.
+"""
+import numpy as np
+import pytest
+
+from helpers import case_inputs, load_case, new_engine
+
+from oracle.oracle import Oracle
+
+
+def charged_cases(golden):
+    return [c for c in golden["cases"] if c["id"].startswith("cho_") and np.abs(np.array(c["charges"])).max() > 1e-6]
+
+
+def setup_qeq(e, swb=10.0):
+    chi, eta, gam = e.extract("chi"), e.extract("eta"), e.extract("gamma")
+    e.qeq_setup(0.0, swb, chi, eta, gam)
+    return chi, eta, gam
+
+
+@pytest.mark.gpu
+def test_qeq_matvec_matches_oracle(golden, ffield_text):
+    O = Oracle(ffield_text)
+    rng = np.random.default_rng(5)
+    for case in charged_cases(golden):
+        e = new_engine(ffield_text, case["elements"], "fp64")
+        load_case(e, case)
+        setup_qeq(e)
+        x, types, tags, lo, hi, tilt, per, q, box12 = case_inputs(case)
+        v = rng.normal(size=len(types))
+        y = e.qeq_matvec(v)
+        yo = O.qeq_matvec(x, types - 1, tags, box12, per, v)
+        assert np.abs(y - yo).max() <= 1e-12 * max(1.0, np.abs(yo).max()), case["id"]
+        e.close()
+
+
+@pytest.mark.gpu
+def test_qeq_charges_match_oracle(golden, ffield_text):
+    O = Oracle(ffield_text)
+    for case in charged_cases(golden):
+        e = new_engine(ffield_text, case["elements"], "fp64")
+        load_case(e, case)
+        setup_qeq(e)
+        x, types, tags, lo, hi, tilt, per, q, box12 = case_inputs(case)
+        qd, it, rel = e.qeq_charges(imax=500, tol=1e-11)
+        ro = O.qeq(x, types - 1, tags, box12, per, swa=0.0, swb=10.0, tol=1e-11, imax=500)
+        print(case["id"], it, (ro["iters_s"], ro["iters_t"]), np.abs(qd - ro["q"]).max(), flush=True)
+        assert np.abs(qd - ro["q"]).max() <= 1e-9, case["id"]
+        assert abs(int(it[0]) - ro["iters_s"]) <= 1 and abs(int(it[1]) - ro["iters_t"]) <= 1, case["id"]
+        assert abs(qd.sum()) < 1e-9
+        e.close()
+
+
+@pytest.mark.gpu
+def test_qeq_history_extrapolation(golden, ffield_text):
+    """Five solves at one geometry: the device's history extrapolation (stock init_matvec: s = 4(s0+s2) - (6 s1 + s3),
+    t = t2 + 3 (t0 - t1), zero history at start) gives the same guesses, iteration counts and charges as the oracle fed with the
+    same extrapolated guesses; after four entries the extrapolation is exact, so the fifth solve stops at once."""
+    case = next(c for c in golden["cases"] if c["id"] == "cho_water_box_8")
+    e = new_engine(ffield_text, case["elements"], "fp64")
+    load_case(e, case)
+    setup_qeq(e)
+    x, types, tags, lo, hi, tilt, per, q, box12 = case_inputs(case)
+    O = Oracle(ffield_text)
+    sh, th = [], []
+    for step in range(5):
+        qd, it, rel = e.qeq_charges(imax=200, tol=1e-8)
+        z = np.zeros(len(types))
+        hs = sh + [z] * (4 - len(sh))
+        ht = th + [z] * (4 - len(th))
+        s0 = 4 * (hs[0] + hs[2]) - (6 * hs[1] + hs[3]) if sh else None
+        t0 = ht[2] + 3 * (ht[0] - ht[1]) if th else None
+        ro = O.qeq(x, types - 1, tags, box12, per, tol=1e-8, imax=200, s0=s0, t0=t0)
+        sh, th = [ro["s"]] + sh[:3], [ro["t"]] + th[:3]
+        print(step, it, (ro["iters_s"], ro["iters_t"]), flush=True)
+        assert abs(int(it[0]) - ro["iters_s"]) <= 1 and abs(int(it[1]) - ro["iters_t"]) <= 1
+        assert np.abs(qd - ro["q"]).max() < 1e-7
+    assert it[0] <= 2 and it[1] <= 2
+    e.close()
