@@ -102,12 +102,12 @@ def main():
     n = w.n
     value = n * args.steps / elapsed
     # workload statistics for the byte model
-    n_f = st["nb_pairs"] / n / 2.0 if st["nb_pairs"] else st["h_entries"] / n / 2.0
+    n_f = st["h_entries"] / n / 2.0                      # half of the full charge-matrix rows (pairs within 10 A)
     B = st["bond_slots"] / n
     spmv_ms, spmv_cnt = prof["qeq_spmv"]
     k = spmv_cnt / args.steps
     R = args.steps / max(1.0, thermo["rebuilds"])
-    n_h = st["hbond_entries"] / 2.0 / max(1, (w.types == 2).sum())   # donor lists + the mirrored acceptor lists
+    n_h = st["hbond_entries"] / max(1, (w.types == 2).sum())        # acceptors within hbond_cut per donor hydrogen
     bytes_step = model_bytes_per_atom_step(n_f, R, k, B, n_h, (w.types == 2).mean())
 
     # dominant kernel by measured device time

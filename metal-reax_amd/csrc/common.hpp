@@ -33,6 +33,9 @@ struct DevPair {
   int tbs, s_ok, pi_ok, pp_ok;
 };
 
+// van der Waals / Coulomb parameters of a type pair for the single-precision non-bonded kernel (pgw = (1 / gamma_w)^p_vdW1)
+struct NbPairF { float D, alpha, rvdw, pgw, ecore, acore, rcore, gamma, lgcij, lgre, pad0, pad1; };
+
 struct DevAngle { double theta_00, p_val1, p_val2, p_val4, p_val7, p_pen1, p_coa1, pad; };
 struct DevTorsion { double V1, V2, V3, p_tor1, p_cot1, pad[3]; };
 struct DevHBond { double r0, p1, p2, p3; };
@@ -62,6 +65,7 @@ struct Tables {
   const double* qeq_chi;    // ntypes (fix parameters)
   const double* qeq_eta;
   const double* qeq_shld;   // ntypes^2: (gamma_i gamma_j)^-1.5
+  const NbPairF* nbf;       // ntypes^2
 };
 
 struct Cell {   // LAMMPS-form periodic cell: a = (lx,0,0), b = (xy,ly,0), c = (xz,yz,lz), origin
@@ -82,14 +86,24 @@ __host__ __device__ inline double frac_coord(const Cell& c, int d, double x, dou
 
 // device state of one engine (owned atoms only); arrays of bond / H-bond slots are slot-major: index = p * n + i
 struct State {
-  int n, maxnb, maxb, maxh;
+  int n, maxnb, maxb;
   double *x, *y, *z;          // owned positions (wrapped into the cell at every rebuild)
+  double4* pos;               // the same positions packed (x, y, z, type) for one 32-byte gather per neighbour
+  const double4* shv;         // lattice shift vectors of every code in [-smx, smx]^3 (see shift_index)
+  int smx0, smx1, smx2;
   int* type;
   long long* tag;
   double* q;
-  // skin neighbour list (atom-major: i * maxnb + k), full (both directions)
+  // skin neighbour list (atom-major: i * maxnb + k), full (both directions); nb_s = shift code | kCanon if i is the canonical end
   int *nb_j, *nb_s, *nb_cnt;
-  int* flags;                 // [0] neighbour overflow, [1] bond overflow, [2] H-bond overflow, [3] missing mirror bond
+  // sub-lists of the skin list written at every rebuild (atom-major, same entry format): bond candidates (r <= bond_cut + skin)
+  // and H-bond partner candidates of donors / acceptors (r <= hbond_cut + skin)
+  int maxbc, maxhc;
+  int *bc_j, *bc_s, *bc_cnt, *hc_j, *hc_s, *hc_cnt;
+  int* hc_m;                  // acceptors: for each candidate donor j, the position of this acceptor in j's candidate list
+  double* hf;                 // donors: force on the acceptor of each candidate slot (3 per slot, TR-typed)
+  int* flags;                 // [0] neighbour overflow, [1] bond overflow, [2] H-bond overflow, [3] missing mirror bond,
+                              // [4] bond-candidate overflow, [5] H-bond-candidate overflow, [6] missing H-bond mirror
   // bond slots
   int *b_cnt, *b_j, *b_s, *b_sym;
   double *b_d, *b_dx, *b_dy, *b_dz;
@@ -97,8 +111,7 @@ struct State {
   double *bo, *bo_s, *bo_pi, *bo_pi2;
   double* C[11];              // C1dbo C2dbo C3dbo C1dbopi C2dbopi C3dbopi C4dbopi C1dbopi2 C2dbopi2 C3dbopi2 C4dbopi2 (this end's view)
   double *Cdbo, *Cdbopi, *Cdbopi2, *fpx, *fpy, *fpz, *cdp;
-  // H-bond partner slots: donors (p_hbond 1) list acceptors, acceptors (p_hbond 2) list donors, within hbond_cut
-  int *h_cnt, *h_j, *h_s;
+  int* h_cnt;                 // H-bond partners within hbond_cut of this donor or acceptor at the last evaluation
   // per atom
   double *Dp, *Dpb, *tbo, *Delta, *Delta_e, *Delta_boc, *Delta_val, *vlpex, *nlp, *Delta_lp, *dDelta_lp, *Delta_lp_temp;
   double *dDps_x, *dDps_y, *dDps_z;
@@ -165,6 +178,50 @@ __device__ inline double pair_dvec(const State& st, const Cell& c, int i, int j,
     dz = -((st.z[i] + sz) - st.z[j]);
   }
   return dx * dx + dy * dy + dz * dz;
+}
+#pragma clang fp contract(on)
+
+// x^(-1/3) in double for normal x > 0: a single-precision seed from the hardware log2/exp2 (relative error ~1e-7), then two
+// Newton steps for f(y) = y^-3 - x, y <- y (4 - x y^3) / 3, whose error goes e -> -2 e^2 (1e-7 -> 1e-13 -> rounding level).
+// Replaces 1 / pow(x, 1/3) (a software pow plus a division) where the result only has to agree to a few ulp.
+__device__ inline double rcbrt_f64(double x) {
+  double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)x) * (-1.0f / 3.0f));
+  y = y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
+  y = y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
+  return y;
+}
+
+// ---- list entries ---------------------------------------------------------------------------------------------------
+constexpr int kCanon = 1 << 24;   // flag of a list entry: the listing atom is the canonical end of the pair
+__host__ __device__ inline int entry_code(int e) { return e & 0xFFFFFF; }
+
+// index of a shift code in the shift-vector table; |s_d| <= smx_d is guaranteed by the list builders
+__device__ inline int shift_index(const State& st, int code) {
+  int s0, s1, s2;
+  shift_decode(code, s0, s1, s2);
+  return ((s2 + st.smx2) * (2 * st.smx1 + 1) + (s1 + st.smx1)) * (2 * st.smx0 + 1) + (s0 + st.smx0);
+}
+
+#pragma clang fp contract(off)
+// pair_dvec for a list entry e = code | flag of neighbour j of the atom at pi, without tags or branches: the same operands in the
+// same order as pair_dvec (the table holds shift_vec of every code), so the result is bitwise identical
+__device__ inline double pair_dvec_list(const State& st, const double4& pi, int j, int e, double& dx, double& dy, double& dz) {
+  const double4 pj = st.pos[j];
+  const bool canon = (e & kCanon) != 0;
+  const int code = entry_code(e);
+  const double4 S = st.shv[shift_index(st, canon ? code : shift_neg(code))];
+  const double ax = canon ? pj.x : pi.x, ay = canon ? pj.y : pi.y, az = canon ? pj.z : pi.z;
+  const double bx = canon ? pi.x : pj.x, by = canon ? pi.y : pj.y, bz = canon ? pi.z : pj.z;
+  const double ex = (ax + S.x) - bx, ey = (ay + S.y) - by, ez = (az + S.z) - bz;
+  dx = canon ? ex : -ex;
+  dy = canon ? ey : -ey;
+  dz = canon ? ez : -ez;
+  return dx * dx + dy * dy + dz * dz;
+}
+// the same, also returning the neighbour's type (carried in pos.w)
+__device__ inline double pair_dvec_list(const State& st, const double4& pi, int j, int e, double& dx, double& dy, double& dz, int& tj) {
+  tj = (int)st.pos[j].w;
+  return pair_dvec_list(st, pi, j, e, dx, dy, dz);
 }
 #pragma clang fp contract(on)
 

@@ -123,6 +123,7 @@ struct Engine {
   DBuf<int> d_tors_idx;
   DBuf<DevTorsion> d_tors;
   DBuf<DevHBond> d_hb;
+  DBuf<NbPairF> d_nbf;
   DBuf<double> d_qchi, d_qeta, d_qshld;
   Tables tab{};
 
@@ -131,17 +132,20 @@ struct Engine {
   bool have_box = false;
   int n = 0;
   State st{};
-  int maxnb = 0, maxb = 12, maxh = 64;
+  int maxnb = 0, maxb = 12, maxbc = 0, maxhc = 0, bmax = 0;
   double skin = 2.0;
   DBuf<double> a_x, a_y, a_z, a_q;
+  DBuf<double4> a_pos, a_shv;
+  int smx[3] = {-1, -1, -1};
   DBuf<int> a_type;
   DBuf<long long> a_tag;
   DBuf<double> a_atom;   // all per-atom double arrays
-  DBuf<int> a_iatom;     // per-atom int arrays (nb_cnt, b_cnt, h_cnt) + flags
+  DBuf<int> a_iatom;     // per-atom int arrays (nb_cnt, b_cnt, h_cnt, bc_cnt, hc_cnt) + flags
   DBuf<int> a_nbj, a_nbs;
   DBuf<int> a_bint;      // b_j, b_s, b_sym
   DBuf<double> a_bdbl;   // bond slot doubles
-  DBuf<int> a_hint;      // h_j, h_s
+  DBuf<int> a_bc, a_hc;  // bond / H-bond candidate sub-lists (j, entry; H-bond: + mirror position)
+  DBuf<double> a_hf;     // per H-bond candidate slot: force on the acceptor
   BinGrid grid;
   BinBuffers bins;
   DBuf<unsigned> b_keys, b_skeys, b_akey;
@@ -240,10 +244,21 @@ struct Engine {
       buf.alloc(vec.size());
       if (!vec.empty()) HIPCHK(hipMemcpy(buf.p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice));
     };
+    std::vector<NbPairF> nbf((size_t)nt * nt);
+    for (size_t k = 0; k < nbf.size(); ++k) {
+      const DevPair& d = pr[k];
+      NbPairF f{};
+      f.D = (float)d.D; f.alpha = (float)d.alpha; f.rvdw = (float)d.r_vdW;
+      f.pgw = (float)std::pow(1.0 / d.gamma_w, ff.gp[28]);
+      f.ecore = (float)d.ecore; f.acore = (float)d.acore; f.rcore = (float)d.rcore; f.gamma = (float)d.gamma;
+      f.lgcij = (float)d.lgcij; f.lgre = (float)d.lgre;
+      nbf[k] = f;
+    }
     up(d_sp, sp); up(d_pair, pr); up(d_angle_idx, aidx); up(d_angle, ang); up(d_tors_idx, tidx); up(d_tors, tors); up(d_hb, hb);
+    up(d_nbf, nbf);
     d_qchi.alloc(nt); d_qeta.alloc(nt); d_qshld.alloc((size_t)nt * nt);
     tab.sp = d_sp.p; tab.pair = d_pair.p; tab.angle_idx = d_angle_idx.p; tab.angle = d_angle.p; tab.tors_idx = d_tors_idx.p; tab.tors = d_tors.p;
-    tab.hb = d_hb.p; tab.qeq_chi = d_qchi.p; tab.qeq_eta = d_qeta.p; tab.qeq_shld = d_qshld.p;
+    tab.hb = d_hb.p; tab.qeq_chi = d_qchi.p; tab.qeq_eta = d_qeta.p; tab.qeq_shld = d_qshld.p; tab.nbf = d_nbf.p;
   }
 
   static void taper(double swa, double swb, double* c) {
@@ -276,6 +291,7 @@ struct Engine {
     if (qeq_ready) { taper(q_swa, q_swb, g.qtap); g.q_swa = q_swa; g.q_swb = q_swb; }
     const std::string p = ps.effective_precision();
     prec = p == "fp64" ? Precision::FP64 : (p == "fp32" ? Precision::FP32 : Precision::MIXED);
+    qb.fp32 = prec == Precision::FP64 ? 0 : 1;   // the charge solve multiplies with FP32 matrix values outside FP64 mode
   }
 
   double list_cut() const { return std::max({g.nonb_cut, g.hbond_cut, g.bond_cut, qeq_ready ? q_swb : 0.0}); }
@@ -305,15 +321,15 @@ struct Engine {
   double* atom_arr(int k) { return a_atom.p + (size_t)k * n; }
 
   void alloc_atoms() {
-    a_x.alloc(n); a_y.alloc(n); a_z.alloc(n); a_q.alloc(n); a_type.alloc(n); a_tag.alloc(n);
+    a_x.alloc(n); a_y.alloc(n); a_z.alloc(n); a_q.alloc(n); a_type.alloc(n); a_tag.alloc(n); a_pos.alloc(n);
     const int NA = 38 + kTerms + 6;   // number of per-atom double arrays below
     a_atom.alloc((size_t)NA * n);
     HIPCHK(hipMemset(a_atom.p, 0, (size_t)NA * n * sizeof(double)));
-    a_iatom.alloc(3 * (size_t)n + 8);
-    HIPCHK(hipMemset(a_iatom.p, 0, (3 * (size_t)n + 8) * sizeof(int)));
+    a_iatom.alloc(5 * (size_t)n + 8);
+    HIPCHK(hipMemset(a_iatom.p, 0, (5 * (size_t)n + 8) * sizeof(int)));
     State& s = st;
     s.n = n;
-    s.x = a_x.p; s.y = a_y.p; s.z = a_z.p; s.q = a_q.p; s.type = a_type.p; s.tag = a_tag.p;
+    s.x = a_x.p; s.y = a_y.p; s.z = a_z.p; s.q = a_q.p; s.type = a_type.p; s.tag = a_tag.p; s.pos = a_pos.p;
     int k = 0;
     s.Dp = atom_arr(k++); s.Dpb = atom_arr(k++); s.tbo = atom_arr(k++); s.Delta = atom_arr(k++); s.Delta_e = atom_arr(k++);
     s.Delta_boc = atom_arr(k++); s.Delta_val = atom_arr(k++); s.vlpex = atom_arr(k++); s.nlp = atom_arr(k++); s.Delta_lp = atom_arr(k++);
@@ -324,7 +340,8 @@ struct Engine {
     k = 38;
     s.eatom = atom_arr(k);
     s.vatom = atom_arr(k + kTerms);
-    s.nb_cnt = a_iatom.p; s.b_cnt = a_iatom.p + n; s.h_cnt = a_iatom.p + 2 * (size_t)n; s.flags = a_iatom.p + 3 * (size_t)n;
+    s.nb_cnt = a_iatom.p; s.b_cnt = a_iatom.p + n; s.h_cnt = a_iatom.p + 2 * (size_t)n; s.bc_cnt = a_iatom.p + 3 * (size_t)n;
+    s.hc_cnt = a_iatom.p + 4 * (size_t)n; s.flags = a_iatom.p + 5 * (size_t)n;
     red.alloc(19 + 240 * 19 + 8);
     // binning scratch
     b_keys.alloc(n); b_skeys.alloc(n); b_akey.alloc(n); b_idx.alloc(n); b_sidx.alloc(n);
@@ -343,14 +360,24 @@ struct Engine {
     if (qeq_ready) alloc_qeq_matrix();
   }
 
-  void alloc_bonds(int cap_b, int cap_h) {
+  void alloc_sublists(int cap_bc, int cap_hc) {
+    maxbc = cap_bc;
+    maxhc = cap_hc;
+    a_bc.alloc(2 * (size_t)n * maxbc);
+    a_hc.alloc(3 * (size_t)n * maxhc);
+    a_hf.alloc(3 * (size_t)n * maxhc);
+    st.maxbc = maxbc; st.maxhc = maxhc;
+    st.bc_j = a_bc.p; st.bc_s = a_bc.p + (size_t)n * maxbc;
+    st.hc_j = a_hc.p; st.hc_s = a_hc.p + (size_t)n * maxhc; st.hc_m = a_hc.p + 2 * (size_t)n * maxhc;
+    st.hf = a_hf.p;
+  }
+
+  void alloc_bonds(int cap_b) {
     maxb = cap_b;
-    maxh = cap_h;
     const size_t nb = (size_t)n * maxb;
     a_bint.alloc(3 * nb);
     a_bdbl.alloc(29 * nb);
-    a_hint.alloc(2 * (size_t)n * maxh);
-    st.maxb = maxb; st.maxh = maxh;
+    st.maxb = maxb;
     st.b_j = a_bint.p; st.b_s = a_bint.p + nb; st.b_sym = a_bint.p + 2 * nb;
     double* b = a_bdbl.p;
     int k = 0;
@@ -363,18 +390,17 @@ struct Engine {
     st.Cdbo = nxt(); st.Cdbopi = nxt(); st.Cdbopi2 = nxt(); st.fpx = nxt(); st.fpy = nxt(); st.fpz = nxt();
     st.cdp = nxt();
     if (k != 29) throw ApiError(RXA_EINVAL, "internal: bond slot layout");
-    st.h_j = a_hint.p; st.h_s = a_hint.p + (size_t)n * maxh;
   }
 
-  int estimate_neighbors() const {
-    const double vol = cell.det;
-    double dens = n / vol;
+  // expected entries within r of an atom at the mean density (at least that of a condensed phase for open cells), with margin
+  int estimate_within(double r) const {
+    double dens = n / cell.det;
     if (!(cell.per[0] && cell.per[1] && cell.per[2])) dens = std::max(dens, 0.12);
-    const double rs = list_cut() + skin;
-    const double est = dens * 4.0 / 3.0 * M_PI * rs * rs * rs;
-    int cap = (int)(est * 1.25) + 32;
+    const double est = dens * 4.0 / 3.0 * M_PI * r * r * r;
+    const int cap = (int)(est * 1.25) + 32;
     return std::min(cap, std::max(64, 27 * n + 32));
   }
+  int estimate_neighbors() const { return estimate_within(list_cut() + skin); }
 
   void set_atoms(long long nn, const int* types, const int64_t* tags, const double* x) {
     if (!have_ff) throw ApiError(RXA_ESTATE, "pair_coeff (the force field) must be set before the atoms");
@@ -385,7 +411,8 @@ struct Engine {
     if (realloc) {
       alloc_atoms();
       alloc_neighbors(estimate_neighbors());
-      alloc_bonds(maxb, maxh);
+      alloc_sublists(estimate_within(g.bond_cut + skin), g.hbond_cut > 0 ? estimate_within(g.hbond_cut + skin) : 1);
+      alloc_bonds(maxb);
       qeq_h_valid = false;
       hist_count = 0;
       if (qeq_ready) alloc_qeq();
@@ -427,6 +454,7 @@ struct Engine {
     HIPCHK(hipMemcpyAsync(a_x.p, xs.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(a_y.p, ys.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(a_z.p, zs.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
+    launch_pos4(st, stream);
     sync();
     nbr_valid = false;
     qeq_h_valid = false;
@@ -435,7 +463,6 @@ struct Engine {
   // ---------------------------------------------------------------------------------------------------------------
   void build_neighbors() {
     const double rs = list_cut() + skin;
-    launch_wrap(st, cell, stream);
     cell_mode = true;
     int nc[3];
     for (int d = 0; d < 3; ++d) {
@@ -454,26 +481,52 @@ struct Engine {
       for (int d = 0; d < 3; ++d)
         if (sm[d] > 100) throw ApiError(RXA_EINVAL, "periodic cell too thin for the neighbour cut-off");
     }
-    for (int attempt = 0; attempt < 6; ++attempt) {
-      hipEvent_t t0 = timer.begin();
-      launch_fill_int(st.flags, 4, 0, stream);
-      if (cell_mode) {
-        launch_bin(st, cell, grid, bins, stream);
-        launch_nbr_cells(st, cell, grid, bins, rs, stream);
-      } else {
-        launch_nbr_brute(st, cell, sm, rs, stream);
+    // shift-vector table of every code the lists can hold (cell stencil: one image in each direction)
+    int want[3];
+    for (int d = 0; d < 3; ++d) want[d] = cell_mode ? 1 : sm[d];
+    if (want[0] != smx[0] || want[1] != smx[1] || want[2] != smx[2]) {
+      const long long m = (2LL * want[0] + 1) * (2LL * want[1] + 1) * (2LL * want[2] + 1);
+      if (m > (1LL << 22)) throw ApiError(RXA_EINVAL, "periodic cell too thin for the neighbour cut-off");
+      a_shv.alloc((size_t)m);
+      for (int d = 0; d < 3; ++d) smx[d] = want[d];
+      st.shv = a_shv.p; st.smx0 = smx[0]; st.smx1 = smx[1]; st.smx2 = smx[2];
+    }
+    hipEvent_t t0 = timer.begin();
+    launch_shift_table(cell, a_shv.p, smx, stream);
+    launch_wrap(st, cell, stream);
+    timer.end(PH_NBR, t0);
+    const double rb = g.bond_cut + skin, rh = g.hbond_cut > 0 ? g.hbond_cut + skin : 0.0;
+    bool need_full = true;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+      t0 = timer.begin();
+      launch_fill_int(st.flags, 8, 0, stream);
+      if (need_full) {
+        if (cell_mode) {
+          launch_bin(st, cell, grid, bins, stream);
+          launch_nbr_cells(st, cell, grid, bins, rs, stream);
+        } else {
+          launch_nbr_brute(st, cell, sm, rs, stream);
+        }
       }
+      launch_sublists(st, g, tab, rb, rh, stream);
       timer.end(PH_NBR, t0);
-      int flags[4];
+      int flags[8];
       HIPCHK(hipMemcpyAsync(flags, st.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
       sync();
-      if (flags[0] <= maxnb) {
+      if (flags[0] > maxnb) {   // the skin list overflowed: grow it and redo everything
+        alloc_neighbors((flags[0] + 31) / 32 * 32 + 32);
+        need_full = true;
+        continue;
+      }
+      need_full = false;
+      if (flags[4] <= maxbc && flags[5] <= maxhc) {
+        if (flags[6]) throw ApiError(RXA_EDEVICE, "internal: an H-bond candidate has no mirror entry");
         nbr_valid = true;
         ++rebuilds;
         qeq_h_valid = false;
         return;
       }
-      alloc_neighbors((flags[0] + 31) / 32 * 32 + 32);
+      alloc_sublists(std::max(maxbc, (flags[4] + 31) / 32 * 32 + 32), std::max(maxhc, (flags[5] + 31) / 32 * 32 + 32));
     }
     throw ApiError(RXA_EINVAL, "neighbour list capacity did not converge");
   }
@@ -481,18 +534,20 @@ struct Engine {
   void bond_lists() {
     for (int attempt = 0; attempt < 6; ++attempt) {
       hipEvent_t t0 = timer.begin();
-      launch_fill_int(st.flags, 4, 0, stream);
+      launch_fill_int(st.flags, 8, 0, stream);
       launch_bond_build(prec, st, cell, g, tab, stream);
       launch_bond_sym(st, stream);
       timer.end(PH_BONDS, t0);
-      int flags[4];
+      int flags[8];
       HIPCHK(hipMemcpyAsync(flags, st.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
       sync();
-      if (flags[1] <= maxb && flags[2] <= maxh) {
+      if (flags[1] <= maxb) {
         if (flags[3]) throw ApiError(RXA_EDEVICE, "internal: a bond has no mirror slot");
+        bmax = flags[7];
+        if (bmax > 64) throw ApiError(RXA_EINVAL, "an atom has more than 64 bonds");
         return;
       }
-      alloc_bonds(std::max(maxb, (flags[1] + 3) / 4 * 4 + 4), std::max(maxh, (flags[2] + 15) / 16 * 16 + 16));
+      alloc_bonds(std::max(maxb, (flags[1] + 3) / 4 * 4 + 4));
     }
     throw ApiError(RXA_EINVAL, "bond list capacity did not converge");
   }
@@ -505,7 +560,7 @@ struct Engine {
     launch_bo_correct(prec, st, g, tab, stream);
     timer.end(PH_BOCORR, t0);
     t0 = timer.begin();
-    launch_bonded_terms(prec, st, cell, g, tab, stream);
+    launch_bonded_terms(prec, st, cell, g, tab, bmax, stream);
     timer.end(PH_TERMS, t0);
     t0 = timer.begin();
     launch_nonbonded(prec, st, cell, g, tab, stream);
@@ -547,7 +602,11 @@ struct Engine {
   }
   void alloc_qeq_doubles() {
     const int imax_cap = std::max(qeq_imax_cap, 2048);
-    const size_t sz = (size_t)n * maxnb + 3 * (size_t)n + 6 * 2 * (size_t)n + 4 * kCgBlocks + 8 + 2 * (imax_cap + 1) + 8 * (size_t)n;
+    // sub-arrays start on 256-byte boundaries (the CG reads (s, t) pairs as 16-byte loads)
+    auto rnd = [](size_t k) { return (k + 31) & ~(size_t)31; };
+    const size_t nmv = 2 * (size_t)qeq_mv_partials(n);
+    const size_t sz = rnd((size_t)n * maxnb) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + rnd(nmv) +
+                      rnd(8 + 2 * (size_t)(imax_cap + 1)) + 2 * rnd(4 * (size_t)n);
     std::vector<double> keep;
     const bool had_hist = q_dbl.p && hist_count > 0;
     if (had_hist) {
@@ -558,20 +617,22 @@ struct Engine {
     q_dbl.alloc(sz);
     HIPCHK(hipMemset(q_dbl.p, 0, sz * sizeof(double)));
     double* p = q_dbl.p;
-    qb.val = p; p += (size_t)n * maxnb;
-    qb.eta = p; p += n;
-    qb.hinv = p; p += n;
-    qb.chi = p; p += n;
-    qb.b = p; p += 2 * (size_t)n;
-    qb.x = p; p += 2 * (size_t)n;
-    qb.r = p; p += 2 * (size_t)n;
-    qb.d = p; p += 2 * (size_t)n;
-    qb.p = p; p += 2 * (size_t)n;
-    qb.q = p; p += 2 * (size_t)n;
-    qb.partial = p; p += 4 * kCgBlocks;
-    qb.scal = p; p += 8 + 2 * (imax_cap + 1);
-    qb.s_hist = p; p += 4 * (size_t)n;
-    qb.t_hist = p; p += 4 * (size_t)n;
+    auto take = [&](size_t k) { double* r = p; p += rnd(k); return r; };
+    qb.val = take((size_t)n * maxnb);
+    qb.eta = take(n);
+    qb.hinv = take(n);
+    qb.chi = take(n);
+    qb.b = take(2 * (size_t)n);
+    qb.x = take(2 * (size_t)n);
+    qb.r = take(2 * (size_t)n);
+    qb.d = take(2 * (size_t)n);
+    qb.p = take(2 * (size_t)n);
+    qb.q = take(2 * (size_t)n);
+    qb.partial = take(4 * kCgBlocks);
+    qb.partial_mv = take(nmv);
+    qb.scal = take(8 + 2 * (size_t)(imax_cap + 1));
+    qb.s_hist = take(4 * (size_t)n);
+    qb.t_hist = take(4 * (size_t)n);
     qeq_imax_cap = imax_cap;
     static_assert(sizeof(int) <= sizeof(double), "");
     iscal_buf.alloc(2 * (size_t)(imax_cap + 2));
@@ -632,7 +693,7 @@ struct Engine {
   void ensure_cg_cap(int imax) {
     if (imax + 2 > qeq_imax_cap) { qeq_imax_cap = imax + 2; alloc_qeq_doubles(); }
   }
-  void qeq_iterate(int imax, double tol, bool fp32) {
+  void qeq_iterate(int imax, double tol) {
     int it = 1;
     const int batch = 8;
     std::vector<int> act(2 * (imax + 2));
@@ -640,7 +701,7 @@ struct Engine {
       int last = it;
       for (int b = 0; b < batch && it < imax; ++b, ++it) {
         hipEvent_t t0 = timer.begin();
-        launch_qeq_spmv(st, qb, it, fp32, stream);
+        launch_qeq_spmv(st, qb, it, stream);
         timer.end(PH_QEQ_SPMV, t0);
         t0 = timer.begin();
         launch_qeq_update(st, qb, it, imax, tol, stream);
@@ -669,7 +730,7 @@ struct Engine {
     ensure_cg_cap(imax);
     qeq_matrix();
     launch_qeq_init(st, qb, imax, tol, hist_count > 0 ? 1 : 0, stream);
-    qeq_iterate(imax, tol, false);
+    qeq_iterate(imax, tol);
     launch_qeq_finish(st, qb, stream);
     hist_count = std::min(hist_count + 1, 4);
     qeq_iter_sum += 0.5 * (last_iters[0] + last_iters[1]);
@@ -928,6 +989,7 @@ int rxa_set_positions_device(rxa_context* c, const double* dx) {
     HIPCHK(hipMemcpyAsync(e.st.x, dx, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
     HIPCHK(hipMemcpyAsync(e.st.y, dx + n, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
     HIPCHK(hipMemcpyAsync(e.st.z, dx + 2 * n, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+    rxa::launch_pos4(e.st, e.stream);
     e.nbr_valid = false;
     e.qeq_h_valid = false;
   });
@@ -1034,7 +1096,7 @@ int rxa_qeq_solve(rxa_context* c, const double* b, const double* x0, int imax, d
     HIPCHK(hipMemcpyAsync(e.qb.b, bi.data(), bi.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     HIPCHK(hipMemcpyAsync(e.qb.x, xi.data(), xi.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     rxa::launch_qeq_init(e.st, e.qb, imax, tol, 2, e.stream);
-    e.qeq_iterate(imax, tol, false);
+    e.qeq_iterate(imax, tol);
     std::vector<double> xo(2 * (size_t)n);
     HIPCHK(hipMemcpy(xo.data(), e.qb.x, xo.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) { x[i] = xo[2 * i]; x[n + i] = xo[2 * i + 1]; }
@@ -1160,13 +1222,16 @@ int rxa_profile_read(rxa_context* c, double* ms, int64_t* counts) {
 int rxa_stats(rxa_context* c, int64_t* out) {
   return guard(c, [&](Engine& e) {
     for (int k = 0; k < 12; ++k) out[k] = 0;
-    out[0] = e.n; out[1] = e.maxnb; out[2] = e.maxb; out[3] = e.maxh; out[8] = e.rebuilds; out[9] = (int)e.prec;
+    out[0] = e.n; out[1] = e.maxnb; out[2] = e.maxb; out[3] = e.maxhc; out[8] = e.rebuilds; out[9] = (int)e.prec;
     if (!e.n) return;
     e.sync();
     const int n = e.n;
-    std::vector<int> cnt(3 * (size_t)n);
-    HIPCHK(hipMemcpy(cnt.data(), e.st.nb_cnt, 3 * (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) { out[4] += cnt[i]; out[5] += cnt[n + i]; out[6] += cnt[2 * (size_t)n + i]; }
+    std::vector<int> cnt(5 * (size_t)n);
+    HIPCHK(hipMemcpy(cnt.data(), e.st.nb_cnt, 5 * (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+      out[4] += cnt[i]; out[5] += cnt[n + i]; out[6] += cnt[2 * (size_t)n + i];
+      out[10] += cnt[3 * (size_t)n + i]; out[11] += cnt[4 * (size_t)n + i];
+    }
     if (e.qeq_ready && e.qb.cnt && e.qeq_h_valid) {
       std::vector<int> qc(n);
       HIPCHK(hipMemcpy(qc.data(), e.qb.cnt, n * sizeof(int), hipMemcpyDeviceToHost));

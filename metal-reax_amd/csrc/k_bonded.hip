@@ -11,8 +11,8 @@
 //   torsion     (thread per atom) torsion + 4-body conjugation, evaluated at BOTH atoms of the central bond; each end keeps the force
 //                                 and derivative contributions of its own half (its own atom, its outer partner, its outer bond) and
 //                                 the end the reference counts (tag order) adds the energy and the central-bond terms [bonded.cpp:476-652]
-//   hbond       (thread per atom) donor hydrogens evaluate their H-bonds; acceptors re-evaluate the same triples for their own force
-//                                 [bonded.cpp:654-715]
+//   hbond       (wave per atom)   donor hydrogens evaluate their H-bond triples once and leave the acceptor's force in the
+//                                 candidate's slot; acceptors collect those slots through a mirror index [bonded.cpp:654-715]
 //   cdelta, dbond, gather         the reference's Add_dBond_to_Forces [bonded.cpp:719-766] rewritten as gathers: every BO-derivative
 //                                 contribution is a central pair force along a bond, so each atom sums its own force from its own
 //                                 bond slots and the mirror slots of its partners.
@@ -84,76 +84,105 @@ __device__ inline void angle_and_dcos(const V3<T>& a, T da, const V3<T>& b, T db
 // ================================================================================================================
 // bond list + uncorrected bond orders
 // ================================================================================================================
+// One wave per atom walks the skin list in chunks of 64. The distance pass runs on every lane; entries within bond_cut are
+// compacted (ballot order = skin-list order) into a per-wave LDS ring and the FP64 BO' evaluation runs on full waves of 64
+// candidates, so the transcendental work is not repeated for the ~90% of skin entries that are out of bond range.
+constexpr int kBuildWaves = 4;
+constexpr int kRing = 128;
+
 template <class TB>
-__global__ void __launch_bounds__(256) k_bond_build(State st, Cell c, Globals g, Tables t) {
+__device__ inline void bo_prime_batch(const State& st, const Globals& g, const Tables& t, int i, int ti, int m, const int* rj,
+                                      const int* rs, const double* rd, const double* rx, const double* ry, const double* rz,
+                                      int head, int& nb, TB& sum_bo, TB& ddx, TB& ddy, TB& ddz) {
+  const int lane = lane_id(), n = st.n;
+  const TB bo_cut = (TB)g.bo_cut;
+  bool bkeep = false;
+  int j = 0, s = 0;
+  double d = 0, dx = 0, dy = 0, dz = 0;
+  TB bs = 0, bp = 0, bpp = 0, cln = 0, clp = 0, clpp = 0;
+  if (lane < m) {
+    const int e = (head + lane) & (kRing - 1);
+    j = rj[e]; s = rs[e]; d = rd[e]; dx = rx[e]; dy = ry[e]; dz = rz[e];
+    const DevPair& P = t.pair[ti * g.ntypes + st.type[j]];
+    const TB dd = (TB)d;
+    TB C12 = 0, C34 = 0, C56 = 0;
+    if (P.s_ok) { C12 = (TB)P.p_bo1 * rpow<TB>(dd / (TB)P.r_s, (TB)P.p_bo2); bs = (TB(1) + bo_cut) * rexp<TB>(C12); }
+    if (P.pi_ok) { C34 = (TB)P.p_bo3 * rpow<TB>(dd / (TB)P.r_p, (TB)P.p_bo4); bp = rexp<TB>(C34); }
+    if (P.pp_ok) { C56 = (TB)P.p_bo5 * rpow<TB>(dd / (TB)P.r_pp, (TB)P.p_bo6); bpp = rexp<TB>(C56); }
+    bkeep = bs + bp + bpp >= bo_cut;
+    const TB rr2 = TB(1) / (dd * dd);
+    cln = (TB)P.p_bo2 * C12 * rr2;
+    clp = (TB)P.p_bo4 * C34 * rr2;
+    clpp = (TB)P.p_bo6 * C56 * rr2;
+  }
+  const unsigned long long mb = __ballot(bkeep);
+  if (bkeep) {
+    const int p = nb + lanes_below(mb);
+    if (p < st.maxb) {
+      const size_t q = sl(p, i, n);
+      st.b_j[q] = j; st.b_s[q] = s;
+      st.b_d[q] = d; st.b_dx[q] = dx; st.b_dy[q] = dy; st.b_dz[q] = dz;
+      const TB tot = (bs + bp + bpp) - bo_cut;
+      st.bop_s[q] = (double)(bs - bo_cut);
+      st.bop_pi[q] = (double)bp;
+      st.bop_pi2[q] = (double)bpp;
+      st.bo[q] = (double)tot;   // uncorrected total until bo_correct overwrites it
+      const TB dc = -(bs * cln + bp * clp + bpp * clpp);
+      st.dbop_c[q] = (double)dc;
+      st.dlnpi_c[q] = (double)(-bp * clp);
+      st.dlnpi2_c[q] = (double)(-bpp * clpp);
+      sum_bo += tot;
+      ddx += dc * (TB)dx; ddy += dc * (TB)dy; ddz += dc * (TB)dz;
+    }
+  }
+  nb += __popcll(mb);
+}
+
+template <class TB>
+__global__ void __launch_bounds__(64 * kBuildWaves) k_bond_build(State st, Cell c, Globals g, Tables t) {
+  __shared__ int rj_[kBuildWaves][kRing], rs_[kBuildWaves][kRing];
+  __shared__ double rd_[kBuildWaves][kRing], rx_[kBuildWaves][kRing], ry_[kBuildWaves][kRing], rz_[kBuildWaves][kRing];
+  const int w = threadIdx.x >> 6;
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
-  const int lane = lane_id(), n = st.n;
+  int* rj = rj_[w]; int* rs = rs_[w];
+  double *rd = rd_[w], *rx = rx_[w], *ry = ry_[w], *rz = rz_[w];
+  const int lane = lane_id();
   const int ti = st.type[i];
-  int nb = 0, nh = 0;
+  int nb = 0;
   TB sum_bo = 0, ddx = 0, ddy = 0, ddz = 0;
   if (ti >= 0) {
-    const DevSpecies si = t.sp[ti];
-    const int role = g.hbond_cut > 0 ? si.p_hbond : 0;
-    const size_t base = (size_t)i * st.maxnb;
-    const int cnt = min(st.nb_cnt[i], st.maxnb);
-    const TB bo_cut = (TB)g.bo_cut;
+    const double4 pi = st.pos[i];
+    const size_t base = (size_t)i * st.maxbc;
+    const int cnt = min(st.bc_cnt[i], st.maxbc);
+    int head = 0, fill = 0;   // LDS ring of bond candidates (wave-uniform)
     for (int k0 = 0; k0 < cnt; k0 += 64) {
       const int k = k0 + lane;
-      bool bkeep = false, hkeep = false;
+      bool near = false;
       int j = 0, s = 0;
       double dx = 0, dy = 0, dz = 0, d = 0;
-      TB bs = 0, bp = 0, bpp = 0, cln = 0, clp = 0, clpp = 0;
       if (k < cnt) {
-        j = st.nb_j[base + k];
-        s = st.nb_s[base + k];
-        const int tj = st.type[j];
-        if (tj >= 0) {
-          d = sqrt(pair_dvec(st, c, i, j, s, dx, dy, dz));
-          const DevSpecies sj = t.sp[tj];
-          if (role && d <= g.hbond_cut) hkeep = (role == 1 && sj.p_hbond == 2) || (role == 2 && sj.p_hbond == 1);
-          if (d <= g.bond_cut) {
-            const DevPair P = t.pair[ti * g.ntypes + tj];
-            const TB dd = (TB)d;
-            TB C12 = 0, C34 = 0, C56 = 0;
-            if (P.s_ok) { C12 = (TB)P.p_bo1 * rpow<TB>(dd / (TB)P.r_s, (TB)P.p_bo2); bs = (TB(1) + bo_cut) * rexp<TB>(C12); }
-            if (P.pi_ok) { C34 = (TB)P.p_bo3 * rpow<TB>(dd / (TB)P.r_p, (TB)P.p_bo4); bp = rexp<TB>(C34); }
-            if (P.pp_ok) { C56 = (TB)P.p_bo5 * rpow<TB>(dd / (TB)P.r_pp, (TB)P.p_bo6); bpp = rexp<TB>(C56); }
-            bkeep = bs + bp + bpp >= bo_cut;
-            const TB rr2 = TB(1) / (dd * dd);
-            cln = (TB)P.p_bo2 * C12 * rr2;
-            clp = (TB)P.p_bo4 * C34 * rr2;
-            clpp = (TB)P.p_bo6 * C56 * rr2;
-          }
-        }
+        j = st.bc_j[base + k];
+        const int e = st.bc_s[base + k];
+        s = entry_code(e);
+        d = sqrt(pair_dvec_list(st, pi, j, e, dx, dy, dz));
+        near = d <= g.bond_cut;
       }
-      const unsigned long long mb = __ballot(bkeep), mh = __ballot(hkeep);
-      if (bkeep) {
-        const int p = nb + lanes_below(mb);
-        if (p < st.maxb) {
-          const size_t q = sl(p, i, n);
-          st.b_j[q] = j; st.b_s[q] = s;
-          st.b_d[q] = d; st.b_dx[q] = dx; st.b_dy[q] = dy; st.b_dz[q] = dz;
-          const TB tot = (bs + bp + bpp) - bo_cut;
-          st.bop_s[q] = (double)(bs - bo_cut);
-          st.bop_pi[q] = (double)bp;
-          st.bop_pi2[q] = (double)bpp;
-          st.bo[q] = (double)tot;   // uncorrected total until bo_correct overwrites it
-          const TB dc = -(bs * cln + bp * clp + bpp * clpp);
-          st.dbop_c[q] = (double)dc;
-          st.dlnpi_c[q] = (double)(-bp * clp);
-          st.dlnpi2_c[q] = (double)(-bpp * clpp);
-          sum_bo += tot;
-          ddx += dc * (TB)dx; ddy += dc * (TB)dy; ddz += dc * (TB)dz;
-        }
+      const unsigned long long mn = __ballot(near);
+      if (near) {
+        const int e = (head + fill + lanes_below(mn)) & (kRing - 1);
+        rj[e] = j; rs[e] = s; rd[e] = d; rx[e] = dx; ry[e] = dy; rz[e] = dz;
       }
-      if (hkeep) {
-        const int p = nh + lanes_below(mh);
-        if (p < st.maxh) { st.h_j[sl(p, i, n)] = j; st.h_s[sl(p, i, n)] = s; }
+      fill += __popcll(mn);
+      __builtin_amdgcn_wave_barrier();
+      if (fill >= 64) {
+        bo_prime_batch<TB>(st, g, t, i, ti, 64, rj, rs, rd, rx, ry, rz, head, nb, sum_bo, ddx, ddy, ddz);
+        head = (head + 64) & (kRing - 1);
+        fill -= 64;
+        __builtin_amdgcn_wave_barrier();
       }
-      nb += __popcll(mb);
-      nh += __popcll(mh);
     }
+    if (fill > 0) bo_prime_batch<TB>(st, g, t, i, ti, fill, rj, rs, rd, rx, ry, rz, head, nb, sum_bo, ddx, ddy, ddz);
   }
   // fixed-order wave reductions
   for (int off = 32; off > 0; off >>= 1) {
@@ -164,9 +193,7 @@ __global__ void __launch_bounds__(256) k_bond_build(State st, Cell c, Globals g,
   }
   if (lane == 0) {
     st.b_cnt[i] = nb;
-    st.h_cnt[i] = nh;
     if (nb > st.maxb) atomicMax(&st.flags[1], nb);
-    if (nh > st.maxh) atomicMax(&st.flags[2], nh);
     const double val = ti >= 0 ? t.sp[ti].valency : 0.0, valv = ti >= 0 ? t.sp[ti].valency_val : 0.0;
     st.Dp[i] = (double)(sum_bo - (TB)val);
     st.Dpb[i] = (double)(sum_bo - (TB)valv);
@@ -176,6 +203,11 @@ __global__ void __launch_bounds__(256) k_bond_build(State st, Cell c, Globals g,
 
 __global__ void __launch_bounds__(128) k_bond_sym(State st) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  {   // the largest bond count of this step (flags[7]), one atomic per wave
+    int c = i < st.n ? st.b_cnt[i] : 0;
+    for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off, 64));
+    if (lane_id() == 0 && c > 0) atomicMax(&st.flags[7], c);
+  }
   if (i >= st.n) return;
   const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
   for (int p = 0; p < nb; ++p) {
@@ -205,7 +237,7 @@ __global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables 
     const bool first = canonical_first(i, j, st.b_s[q], st.tag);
     const int a = first ? i : j, b = first ? j : i;
     const int ta = st.type[a], tb = st.type[b];
-    const DevPair P = t.pair[ta * g.ntypes + tb];
+    const DevPair& P = t.pair[ta * g.ntypes + tb];
     const TB vi = (TB)t.sp[ta].valency, vj = (TB)t.sp[tb].valency;
     const TB Dpi = (TB)st.Dp[a], Dpj = (TB)st.Dp[b], Dbi = (TB)st.Dpb[a], Dbj = (TB)st.Dpb[b];
     const TB BO = (TB)st.bo[q], BOs = (TB)st.bop_s[q], BOpi = (TB)st.bop_pi[q], BOpi2 = (TB)st.bop_pi2[q];
@@ -301,7 +333,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
       const int j = st.b_j[q];
       if (!canonical_first(i, j, st.b_s[q], st.tag)) continue;
       const int tj = st.type[j];
-      const DevPair P = t.pair[ti * g.ntypes + tj];
+      const DevPair& P = t.pair[ti * g.ntypes + tj];
       const TR BOs = (TR)st.bo_s[q], BOpi = (TR)st.bo_pi[q], BOpi2 = (TR)st.bo_pi2[q];
       const TR pw = BOs == TR(0) ? TR(0) : rpow<TR>(BOs, (TR)P.p_be2);
       const TR ex = rexp<TR>((TR)P.p_be1 * (TR(1) - pw));
@@ -352,7 +384,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
       for (int p = 0; p < nb; ++p) {
         const size_t q = sl(p, i, n);
         const int j = st.b_j[q];
-        const DevPair P = t.pair[ti * g.ntypes + st.type[j]];
+        const DevPair& P = t.pair[ti * g.ntypes + st.type[j]];
         sum1 += (TR)P.p_ovun1 * (TR)P.De_s * (TR)st.bo[q];
         sum2 += ((TR)st.Delta[j] - dfvl * (TR)st.Delta_lp_temp[j]) * ((TR)st.bo_pi[q] + (TR)st.bo_pi2[q]);
       }
@@ -382,7 +414,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
       for (int p = 0; p < nb; ++p) {
         const size_t q = sl(p, i, n);
         const int j = st.b_j[q];
-        const DevPair P = t.pair[ti * g.ntypes + st.type[j]];
+        const DevPair& P = t.pair[ti * g.ntypes + st.type[j]];
         st.Cdbo[q] += (double)(CEover1 * (TR)P.p_ovun1 * (TR)P.De_s);
         const TR pis = (TR)st.bo_pi[q] + (TR)st.bo_pi2[q];
         const TR dj = (TR)st.Delta[j] - dfvl * (TR)st.Delta_lp_temp[j];
@@ -403,21 +435,44 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
 // ================================================================================================================
 // valence angle, penalty, 3-body conjugation (centre j)
 // ================================================================================================================
-template <class TR>
-__global__ void __launch_bounds__(128) k_valence(State st, Globals g, Tables t) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= st.n) return;
-  const int n = st.n, tj = st.type[j];
-  double e_ang = 0, e_pen = 0, e_coa = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0;
-  const int nb = min(st.b_cnt[j], st.maxb);
-  if (tj >= 0 && nb >= 2) {
+// ---- atom groups: G lanes per atom (G a power of two >= the largest bond count), lane p owns bond slot p of its atom ----------
+// Sums over a group use xor shuffles with offsets below G, so they stay inside the group and run in a fixed order.
+template <int G>
+__device__ inline double gsum(double v) {
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline int wave_max(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Valence angle, penalty and three-body conjugation at the centre j [reference bonded.cpp:321-474]. Lane p of j's group evaluates
+// the pairs (p, k) with p < k of j's bonds, k running over a wave-uniform loop: the contributions to slot p stay in the lane, the
+// contributions to slot k are summed over the group and kept by lane k, and the centre's are summed over the group at the end.
+template <class TR, int G>
+__global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = gid / G, p = gid % G, n = st.n;
+  const bool live = j < n;
+  const int tj = live ? st.type[j] : -1;
+  const int nb = live ? min(st.b_cnt[j], st.maxb) : 0;
+  const int nbw = wave_max(nb);
+  const bool act = tj >= 0 && nb >= 2;
+  double e_ang = 0, e_pen = 0, e_coa = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0, sCE5 = 0, sCE6 = 0;
+  double own_cdbo = 0, own_cdp = 0, own_fx = 0, own_fy = 0, own_fz = 0;
+  // centre quantities (every lane of the group computes them)
+  TR SBO2 = 0, CSBO2 = 0, dSBO1 = 0, dSBO2 = 0, ev6 = 0, f9 = 0, Cf9 = 0, ec2 = 0, Dboc = 0, p_val3 = 0, p_val5 = 0;
+  const TR thb_cut = (TR)g.thb_cut, thb_cutsq = (TR)g.thb_cutsq;
+  const TR p_val6 = (TR)g.p_val6, p_val8 = (TR)g.p_val8, p_val9 = (TR)g.p_val9, p_val10 = (TR)g.p_val10;
+  const TR p_pen2 = (TR)g.p_pen2, p_pen3 = (TR)g.p_pen3, p_pen4 = (TR)g.p_pen4;
+  const TR p_coa2 = (TR)g.p_coa2, p_coa3 = (TR)g.p_coa3, p_coa4 = (TR)g.p_coa4;
+  if (act) {
     const DevSpecies sj = t.sp[tj];
-    const TR thb_cut = (TR)g.thb_cut, thb_cutsq = (TR)g.thb_cutsq;
-    const TR p_val3 = (TR)sj.p_val3, p_val5 = (TR)sj.p_val5;
-    const TR p_val6 = (TR)g.p_val6, p_val8 = (TR)g.p_val8, p_val9 = (TR)g.p_val9, p_val10 = (TR)g.p_val10;
+    p_val3 = (TR)sj.p_val3; p_val5 = (TR)sj.p_val5;
     TR SBOp = 0, prod = 1;
-    for (int p = 0; p < nb; ++p) {
-      const size_t q = sl(p, j, n);
+    for (int b = 0; b < nb; ++b) {
+      const size_t q = sl(b, j, n);
       const TR bo = (TR)st.bo[q];
       SBOp += (TR)st.bo_pi[q] + (TR)st.bo_pi2[q];
       TR w = bo * bo;
@@ -426,42 +481,47 @@ __global__ void __launch_bounds__(128) k_valence(State st, Globals g, Tables t) 
       prod *= rexp<TR>(-w);
     }
     const TR dDlp = (TR)st.dDelta_lp[j];
-    TR vlpadj, dSBO2;
+    TR vlpadj;
     if ((TR)st.vlpex[j] >= TR(0)) { vlpadj = 0; dSBO2 = prod - TR(1); }
     else { vlpadj = (TR)st.nlp[j]; dSBO2 = (prod - TR(1)) * (TR(1) - p_val8 * dDlp); }
-    const TR Dboc = (TR)st.Delta_boc[j];
+    Dboc = (TR)st.Delta_boc[j];
     const TR SBO = SBOp + (TR(1) - prod) * (-Dboc - p_val8 * vlpadj);
-    const TR dSBO1 = -TR(8) * prod * (Dboc + p_val8 * vlpadj);
-    TR SBO2, CSBO2;
+    dSBO1 = -TR(8) * prod * (Dboc + p_val8 * vlpadj);
     if (SBO <= TR(0)) { SBO2 = 0; CSBO2 = 0; }
     else if (SBO <= TR(1)) { SBO2 = rpow<TR>(SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(SBO, p_val9 - TR(1)); }
     else if (SBO < TR(2)) { SBO2 = TR(2) - rpow<TR>(TR(2) - SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(TR(2) - SBO, p_val9 - TR(1)); }
     else { SBO2 = 2; CSBO2 = 0; }
-    const TR ev6 = rexp<TR>(p_val6 * Dboc);
+    ev6 = rexp<TR>(p_val6 * Dboc);
     const TR Delta = (TR)st.Delta[j], Dval = (TR)st.Delta_val[j];
-    const TR p_pen2 = (TR)g.p_pen2, p_pen3 = (TR)g.p_pen3, p_pen4 = (TR)g.p_pen4;
-    const TR p_coa2 = (TR)g.p_coa2, p_coa3 = (TR)g.p_coa3, p_coa4 = (TR)g.p_coa4;
     const TR ep3 = rexp<TR>(-p_pen3 * Delta), ep4 = rexp<TR>(p_pen4 * Delta), trm = TR(1) + ep3 + ep4;
-    const TR f9 = (TR(2) + ep3) / trm;
-    const TR Cf9 = (-p_pen3 * ep3 * trm - (TR(2) + ep3) * (-p_pen3 * ep3 + p_pen4 * ep4)) / (trm * trm);
-    const TR ec2 = rexp<TR>(p_coa2 * Dval);
-    TR sumCE5 = 0, sumCE6 = 0;
-    for (int pi = 0; pi < nb; ++pi) {
-      const size_t qi = sl(pi, j, n);
-      const TR BOij = (TR)st.bo[qi];
-      const TR BOA_ij = BOij - thb_cut;
-      if (!(BOA_ij > TR(0))) continue;
-      const int i = st.b_j[qi], ti = st.type[i];
-      const V3<TR> vij{(TR)st.b_dx[qi], (TR)st.b_dy[qi], (TR)st.b_dz[qi]};
-      const TR dij = (TR)st.b_d[qi];
-      for (int pk = pi + 1; pk < nb; ++pk) {
-        const size_t qk = sl(pk, j, n);
-        const TR BOjk = (TR)st.bo[qk];
-        const TR BOA_jk = BOjk - thb_cut;
-        if (!(BOA_jk > TR(0) && BOij > thb_cut && BOjk > thb_cut && BOij * BOjk > thb_cutsq)) continue;
-        const int k = st.b_j[qk], tk = st.type[k];
-        const int2 ai = t.angle_idx[(ti * g.ntypes + tj) * g.ntypes + tk];
-        if (ai.y == 0) continue;
+    f9 = (TR(2) + ep3) / trm;
+    Cf9 = (-p_pen3 * ep3 * trm - (TR(2) + ep3) * (-p_pen3 * ep3 + p_pen4 * ep4)) / (trm * trm);
+    ec2 = rexp<TR>(p_coa2 * Dval);
+  }
+  // this lane's bond (j, i)
+  const bool mine = act && p < nb;
+  size_t qi = 0;
+  int i = 0, ti = 0;
+  TR BOij = 0, BOA_ij = 0, dij = 1, tboi = 0;
+  V3<TR> vij{0, 0, 0};
+  if (mine) {
+    qi = sl(p, j, n);
+    BOij = (TR)st.bo[qi];
+    BOA_ij = BOij - thb_cut;
+    i = st.b_j[qi]; ti = st.type[i];
+    vij = V3<TR>{(TR)st.b_dx[qi], (TR)st.b_dy[qi], (TR)st.b_dz[qi]};
+    dij = (TR)st.b_d[qi];
+    tboi = (TR)st.tbo[i];
+  }
+  for (int pk = 1; pk < nbw; ++pk) {
+    double to_cdbo = 0, to_cdp = 0, to_fx = 0, to_fy = 0, to_fz = 0;
+    if (mine && p < pk && pk < nb && BOA_ij > TR(0)) {
+      const size_t qk = sl(pk, j, n);
+      const TR BOjk = (TR)st.bo[qk];
+      const TR BOA_jk = BOjk - thb_cut;
+      const int k = st.b_j[qk], tk = st.type[k];
+      const int2 ai = t.angle_idx[(ti * g.ntypes + tj) * g.ntypes + tk];
+      if (BOA_jk > TR(0) && BOij > thb_cut && BOjk > thb_cut && BOij * BOjk > thb_cutsq && ai.y != 0) {
         const V3<TR> vjk{(TR)st.b_dx[qk], (TR)st.b_dy[qk], (TR)st.b_dz[qk]};
         const TR djk = (TR)st.b_d[qk];
         TR th, ct;
@@ -469,7 +529,7 @@ __global__ void __launch_bounds__(128) k_valence(State st, Globals g, Tables t) 
         angle_and_dcos<TR>(vij, dij, vjk, djk, th, ct, dci, dcj, dck);
         TR sth = rsin<TR>(th);
         if (sth < TR(1.0e-5)) sth = TR(1.0e-5);
-        const TR tboi = (TR)st.tbo[i], tbok = (TR)st.tbo[k];
+        const TR tbok = (TR)st.tbo[k];
         for (int a = 0; a < ai.y; ++a) {
           const DevAngle A = t.angle[ai.x + a];
           const TR p_val1 = (TR)A.p_val1;
@@ -502,39 +562,41 @@ __global__ void __launch_bounds__(128) k_valence(State st, Globals g, Tables t) 
           e_coa += (double)ecoa;
           const TR CEc1 = -TR(2) * p_coa4 * (BOA_ij - TR(1.5)) * ecoa, CEc2 = -TR(2) * p_coa4 * (BOA_jk - TR(1.5)) * ecoa;
           const TR CEc3 = -p_coa2 * ec2 * ecoa / (TR(1) + ec2), CEc4 = -TR(2) * p_coa3 * dti * ecoa, CEc5 = -TR(2) * p_coa3 * dtk * ecoa;
-          st.Cdbo[qi] += (double)(CE1 + CEp2 + (CEc1 - CEc4));
-          st.Cdbo[qk] += (double)(CE2 + CEp3 + (CEc2 - CEc5));
+          own_cdbo += (double)(CE1 + CEp2 + (CEc1 - CEc4));
+          to_cdbo += (double)(CE2 + CEp3 + (CEc2 - CEc5));
           cdself += (double)((CE3 + CE7) + CEp1 + CEc3);
-          st.cdp[qi] += (double)CEc4;
-          st.cdp[qk] += (double)CEc5;
-          sumCE5 += CE5;
-          sumCE6 += CE6;
+          own_cdp += (double)CEc4;
+          to_cdp += (double)CEc5;
+          sCE5 += (double)CE5;
+          sCE6 += (double)CE6;
           fjx += (double)(CE8 * dcj.x); fjy += (double)(CE8 * dcj.y); fjz += (double)(CE8 * dcj.z);
-          st.fpx[qi] += (double)(CE8 * dci.x); st.fpy[qi] += (double)(CE8 * dci.y); st.fpz[qi] += (double)(CE8 * dci.z);
-          st.fpx[qk] += (double)(CE8 * dck.x); st.fpy[qk] += (double)(CE8 * dck.y); st.fpz[qk] += (double)(CE8 * dck.z);
+          own_fx += (double)(CE8 * dci.x); own_fy += (double)(CE8 * dci.y); own_fz += (double)(CE8 * dci.z);
+          to_fx += (double)(CE8 * dck.x); to_fy += (double)(CE8 * dck.y); to_fz += (double)(CE8 * dck.z);
         }
       }
     }
-    if (sumCE5 != TR(0) || sumCE6 != TR(0)) {
-      for (int p = 0; p < nb; ++p) {
-        const size_t q = sl(p, j, n);
-        const TR bo = (TR)st.bo[q], c3 = bo * bo * bo;
-        st.Cdbo[q] += (double)(sumCE6 * (c3 * c3 * bo));
-        st.Cdbopi[q] += (double)sumCE5;
-        st.Cdbopi2[q] += (double)sumCE5;
-      }
-    }
+    to_cdbo = gsum<G>(to_cdbo); to_cdp = gsum<G>(to_cdp); to_fx = gsum<G>(to_fx); to_fy = gsum<G>(to_fy); to_fz = gsum<G>(to_fz);
+    if (p == pk) { own_cdbo += to_cdbo; own_cdp += to_cdp; own_fx += to_fx; own_fy += to_fy; own_fz += to_fz; }
   }
-  st.CdD_self[j] += cdself;
-  st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
-  st.eatom[T_ANG * n + j] = e_ang;
-  st.eatom[T_PEN * n + j] = e_pen;
-  st.eatom[T_COA * n + j] = e_coa;
+  e_ang = gsum<G>(e_ang); e_pen = gsum<G>(e_pen); e_coa = gsum<G>(e_coa); cdself = gsum<G>(cdself);
+  fjx = gsum<G>(fjx); fjy = gsum<G>(fjy); fjz = gsum<G>(fjz); sCE5 = gsum<G>(sCE5); sCE6 = gsum<G>(sCE6);
+  if (mine) {
+    const TR bo = BOij, c3 = bo * bo * bo;
+    st.Cdbo[qi] += own_cdbo + (double)((TR)sCE6 * (c3 * c3 * bo));
+    st.Cdbopi[qi] += (double)(TR)sCE5;
+    st.Cdbopi2[qi] += (double)(TR)sCE5;
+    st.cdp[qi] += own_cdp;
+    st.fpx[qi] += own_fx; st.fpy[qi] += own_fy; st.fpz[qi] += own_fz;
+  }
+  if (live && p == 0) {
+    st.CdD_self[j] += cdself;
+    st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
+    st.eatom[T_ANG * n + j] = e_ang;
+    st.eatom[T_PEN * n + j] = e_pen;
+    st.eatom[T_COA * n + j] = e_coa;
+  }
 }
 
-// ================================================================================================================
-// torsion + 4-body conjugation, evaluated at each atom of the central bond
-// ================================================================================================================
 template <class TR>
 __device__ inline TR tan_inv(TR s, TR c) {
   const TR M = TR(1e-10);
@@ -543,43 +605,60 @@ __device__ inline TR tan_inv(TR s, TR c) {
   return c / s;
 }
 
-template <class TR>
-__global__ void __launch_bounds__(128) k_torsion(State st, Cell cell, Globals g, Tables t) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= st.n) return;
-  const int n = st.n, tj = st.type[j];
+// Torsion + four-body conjugation [reference bonded.cpp:476-652], evaluated at BOTH atoms of the central bond: lane p of j's group
+// takes the central bond (j, k) in slot p and walks j's other bonds i (wave-uniform loop) and k's other bonds l. Each end keeps
+// its own half (its atom, its outer partner i and outer bond (j, i)); the end the reference counts (canonical_first) also adds
+// the energy and the central-bond coefficients. The outer-bond contributions of an i are summed over the group and kept by the
+// lane owning slot i.
+template <class TR, int G>
+__global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = gid / G, pk = gid % G, n = st.n;
+  const bool live = j < n;
+  const int tj = live ? st.type[j] : -1;
+  const int nbj = live ? min(st.b_cnt[j], st.maxb) : 0;
+  const int nbw = wave_max(nbj);
   double e_tor = 0, e_con = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0;
-  const int nbj = min(st.b_cnt[j], st.maxb);
-  if (tj >= 0 && nbj >= 2) {
-    const TR thb_cut = (TR)g.thb_cut;
-    const TR p_tor2 = (TR)g.p_tor2, p_tor3 = (TR)g.p_tor3, p_tor4 = (TR)g.p_tor4, p_cot2 = (TR)g.p_cot2;
-    const TR MINS = TR(1e-10);
-    const TR Dj = (TR)st.Delta_boc[j];
-    for (int pk = 0; pk < nbj; ++pk) {
-      const size_t qk = sl(pk, j, n);
-      const TR BOjk = (TR)st.bo[qk];
-      if (!(BOjk > thb_cut)) continue;
-      const int k = st.b_j[qk], sk = st.b_s[qk], tk = st.type[k];
-      const int nbk = min(st.b_cnt[k], st.maxb);
-      if (nbk < 2) continue;
-      const bool counted = canonical_first(j, k, sk, st.tag);
-      const int symk = st.b_sym[qk];   // slot of j in k's list
-      const TR BOA_jk = BOjk - thb_cut;
-      const TR rjk = (TR)st.b_d[qk];
-      const V3<TR> djk{(TR)st.b_dx[qk], (TR)st.b_dy[qk], (TR)st.b_dz[qk]};
-      const V3<TR> dkj{-djk.x, -djk.y, -djk.z};
-      const TR Dk = (TR)st.Delta_boc[k];
-      const TR et2jk = rexp<TR>(-p_tor2 * BOA_jk), ec2jk = rexp<TR>(-p_cot2 * ((BOA_jk - TR(1.5)) * (BOA_jk - TR(1.5))));
-      const TR et3 = rexp<TR>(-p_tor3 * (Dj + Dk)), et4 = rexp<TR>(p_tor4 * (Dj + Dk)), it34 = TR(1) / (TR(1) + et3 + et4);
-      const TR f11 = (TR(2) + et3) * it34;
-      const TR BOpijk = (TR)st.bo_pi[qk];
-      const TR dfn11 = (-p_tor3 * et3 + (p_tor3 * et3 - p_tor4 * et4) * (TR(2) + et3) * it34) * it34;
-      double cdbo_jk = 0, cdbopi_jk = 0;
-      for (int pi = 0; pi < nbj; ++pi) {
-        if (pi == pk) continue;
-        const size_t qi = sl(pi, j, n);
-        const TR BOij = (TR)st.bo[qi];
-        if (!(BOij > thb_cut)) continue;
+  double own_cdbo = 0, own_fx = 0, own_fy = 0, own_fz = 0;
+  const TR thb_cut = (TR)g.thb_cut;
+  const TR p_tor2 = (TR)g.p_tor2, p_tor3 = (TR)g.p_tor3, p_tor4 = (TR)g.p_tor4, p_cot2 = (TR)g.p_cot2;
+  const TR MINS = TR(1e-10);
+  // this lane's central bond (j, k)
+  bool cen = live && tj >= 0 && nbj >= 2 && pk < nbj;
+  size_t qk = 0;
+  int k = 0, sk = 0, tk = 0, nbk = 0, symk = 0;
+  bool counted = false;
+  TR BOjk = 0, BOA_jk = 0, rjk = 1, et2jk = 0, ec2jk = 0, f11 = 0, dfn11 = 0, BOpijk = 0;
+  V3<TR> djk{0, 0, 0}, dkj{0, 0, 0};
+  if (cen) {
+    qk = sl(pk, j, n);
+    BOjk = (TR)st.bo[qk];
+    k = st.b_j[qk]; sk = st.b_s[qk]; tk = st.type[k];
+    nbk = min(st.b_cnt[k], st.maxb);
+    cen = BOjk > thb_cut && nbk >= 2;
+  }
+  if (cen) {
+    counted = canonical_first(j, k, sk, st.tag);
+    symk = st.b_sym[qk];   // slot of j in k's list
+    BOA_jk = BOjk - thb_cut;
+    rjk = (TR)st.b_d[qk];
+    djk = V3<TR>{(TR)st.b_dx[qk], (TR)st.b_dy[qk], (TR)st.b_dz[qk]};
+    dkj = V3<TR>{-djk.x, -djk.y, -djk.z};
+    const TR Dj = (TR)st.Delta_boc[j], Dk = (TR)st.Delta_boc[k];
+    et2jk = rexp<TR>(-p_tor2 * BOA_jk);
+    ec2jk = rexp<TR>(-p_cot2 * ((BOA_jk - TR(1.5)) * (BOA_jk - TR(1.5))));
+    const TR et3 = rexp<TR>(-p_tor3 * (Dj + Dk)), et4 = rexp<TR>(p_tor4 * (Dj + Dk)), it34 = TR(1) / (TR(1) + et3 + et4);
+    f11 = (TR(2) + et3) * it34;
+    BOpijk = (TR)st.bo_pi[qk];
+    dfn11 = (-p_tor3 * et3 + (p_tor3 * et3 - p_tor4 * et4) * (TR(2) + et3) * it34) * it34;
+  }
+  double cdbo_jk = 0, cdbopi_jk = 0;
+  for (int pi = 0; pi < nbw; ++pi) {
+    double cdbo_ij = 0, fix = 0, fiy = 0, fiz = 0;
+    if (cen && pi != pk && pi < nbj) {
+      const size_t qi = sl(pi, j, n);
+      const TR BOij = (TR)st.bo[qi];
+      if (BOij > thb_cut) {
         const int i = st.b_j[qi], si_ = st.b_s[qi], ti = st.type[i];
         const TR rij = (TR)st.b_d[qi], BOA_ij = BOij - thb_cut;
         const V3<TR> dji{(TR)st.b_dx[qi], (TR)st.b_dy[qi], (TR)st.b_dz[qi]};
@@ -588,7 +667,6 @@ __global__ void __launch_bounds__(128) k_torsion(State st, Cell cell, Globals g,
         angle_and_dcos<TR>(dji, rij, djk, rjk, thijk, cijk_, dci, dcj, dck);
         const TR sijk = rsin<TR>(thijk), cijk = rcos<TR>(thijk), tijk = tan_inv<TR>(sijk, cijk);
         const TR et2ij = rexp<TR>(-p_tor2 * BOA_ij), ec2ij = rexp<TR>(-p_cot2 * ((BOA_ij - TR(1.5)) * (BOA_ij - TR(1.5))));
-        double cdbo_ij = 0, fix = 0, fiy = 0, fiz = 0;
         for (int pl = 0; pl < nbk; ++pl) {
           if (pl == symk) continue;
           const size_t ql = sl(pl, k, n);
@@ -667,106 +745,157 @@ __global__ void __launch_bounds__(128) k_torsion(State st, Cell cell, Globals g,
             cdbo_jk += (double)(CT5 + CC2);
           }
         }
-        if (cdbo_ij != 0.0 || fix != 0.0 || fiy != 0.0 || fiz != 0.0) {
-          st.Cdbo[qi] += cdbo_ij;
-          st.fpx[qi] += fix; st.fpy[qi] += fiy; st.fpz[qi] += fiz;
-        }
       }
-      if (counted) { st.Cdbo[qk] += cdbo_jk; st.Cdbopi[qk] += cdbopi_jk; }
     }
+    cdbo_ij = gsum<G>(cdbo_ij); fix = gsum<G>(fix); fiy = gsum<G>(fiy); fiz = gsum<G>(fiz);
+    if (pk == pi) { own_cdbo += cdbo_ij; own_fx += fix; own_fy += fiy; own_fz += fiz; }
   }
-  (void)cell;
-  st.CdD_self[j] += cdself;
-  st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
-  st.eatom[T_TOR * n + j] = e_tor;
-  st.eatom[T_CON * n + j] = e_con;
+  e_tor = gsum<G>(e_tor); e_con = gsum<G>(e_con); cdself = gsum<G>(cdself);
+  fjx = gsum<G>(fjx); fjy = gsum<G>(fjy); fjz = gsum<G>(fjz);
+  if (live && tj >= 0 && pk < nbj) {
+    const size_t q = sl(pk, j, n);
+    st.Cdbo[q] += own_cdbo + cdbo_jk;
+    st.Cdbopi[q] += cdbopi_jk;
+    st.fpx[q] += own_fx; st.fpy[q] += own_fy; st.fpz[q] += own_fz;
+  }
+  if (live && pk == 0) {
+    st.CdD_self[j] += cdself;
+    st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
+    st.eatom[T_TOR * n + j] = e_tor;
+    st.eatom[T_CON * n + j] = e_con;
+  }
 }
 
 // ================================================================================================================
-// hydrogen bonds: donors (own terms, partner-i forces), acceptors (their own force from the same triples)
-// ================================================================================================================
 template <class TR>
-__device__ inline void hbond_term(const DevHBond& P, TR BOij, const V3<TR>& dji, TR rij, const V3<TR>& djk, TR rjk, TR& e, TR& CH1, TR& CH2,
+__device__ inline void cos_and_dcos(const V3<TR>& a, TR da, const V3<TR>& b, TR db, TR& cost, V3<TR>& dca, V3<TR>& dcj, V3<TR>& dcb) {
+  cost = dot3(a, b) / (da * db);
+  if (cost > TR(1)) cost = TR(1);
+  if (cost < TR(-1)) cost = TR(-1);
+  const TR da2 = da * da, db2 = db * db, inv = TR(1) / (da * db), inv3 = inv * inv * inv, cd = dot3(a, b) * inv3;
+  dca = {b.x * inv - cd * db2 * a.x, b.y * inv - cd * db2 * a.y, b.z * inv - cd * db2 * a.z};
+  dcj = {-(b.x + a.x) * inv + cd * (db2 * a.x + da2 * b.x), -(b.y + a.y) * inv + cd * (db2 * a.y + da2 * b.y),
+         -(b.z + a.z) * inv + cd * (db2 * a.z + da2 * b.z)};
+  dcb = {a.x * inv - cd * da2 * b.x, a.y * inv - cd * da2 * b.y, a.z * inv - cd * da2 * b.z};
+}
+
+// H-bond triple (i, j, k) [reference terms of bonded.cpp:654-715]: sin^4(theta/2) = ((1 - cos theta) / 2)^2, so no acos is
+// needed; eh2 = exp(-p2 BO_ij) is the caller's (it depends on the bond only)
+template <class TR>
+__device__ inline void hbond_term(const DevHBond& P, TR eh2, const V3<TR>& dji, TR rij, const V3<TR>& djk, TR rjk, TR& e, TR& CH1, TR& CH2,
                                   TR& CH3, V3<TR>& dti, V3<TR>& dtj, V3<TR>& dtk) {
-  TR th, ct;
-  angle_and_dcos<TR>(dji, rij, djk, rjk, th, ct, dti, dtj, dtk);
-  const TR s2 = rsin<TR>(th / TR(2));
-  TR s4 = s2 * s2;
-  s4 *= s4;
-  const TR cx = TR(1) - ct;
+  TR ct;
+  cos_and_dcos<TR>(dji, rij, djk, rjk, ct, dti, dtj, dtk);
+  const TR cx = TR(1) - ct, hs = TR(0.5) * cx, s4 = hs * hs;
   const TR r0 = (TR)P.r0, p1 = (TR)P.p1, p2 = (TR)P.p2, p3 = (TR)P.p3;
-  const TR eh2 = rexp<TR>(-p2 * BOij), eh3 = rexp<TR>(-p3 * (r0 / rjk + rjk / r0 - TR(2)));
+  const TR eh3 = rexp<TR>(-p3 * (r0 / rjk + rjk / r0 - TR(2)));
   e = p1 * (TR(1) - eh2) * eh3 * s4;
   CH1 = p1 * p2 * eh2 * eh3 * s4;
   CH2 = -p1 / TR(2) * (TR(1) - eh2) * eh3 * cx;
   CH3 = -p3 * (-r0 / (rjk * rjk) + TR(1) / r0) * e;
 }
 
+// Donor side: one wave per donor hydrogen j, lanes over its H-bond candidates (chunks of 64). The geometry of a candidate k is
+// computed once; for each bond (j, i) that can carry an H-bond (wave-uniform loop) the lane evaluates the triple (i, j, k):
+// energy and the force on j stay in the lane, the BO derivative coefficient and the force on i are reduced over the wave into
+// the bond slot, and the force on the acceptor k is written to the candidate's slot of hf for the acceptor to collect. Non-donors
+// only clear their H-bond energy.
 template <class TR>
-__global__ void __launch_bounds__(128) k_hbond(State st, Cell c, Globals g, Tables t) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= st.n) return;
-  const int n = st.n, ta = st.type[a];
-  double e_hb = 0, fax = 0, fay = 0, faz = 0;
-  if (ta >= 0 && g.hbond_cut > 0) {
-    const int role = t.sp[ta].p_hbond;
-    const int nh = min(st.h_cnt[a], st.maxh);
-    if (role == 1) {   // donor hydrogen j = a
-      const int j = a, tj = ta, nb = min(st.b_cnt[j], st.maxb);
-      for (int h = 0; h < nh; ++h) {
-        const int k = st.h_j[sl(h, j, n)], sk = st.h_s[sl(h, j, n)], tk = st.type[k];
-        double dx, dy, dz;
-        const double rjk_d = sqrt(pair_dvec(st, c, j, k, sk, dx, dy, dz));
-        const V3<TR> djk{(TR)dx, (TR)dy, (TR)dz};
-        const TR rjk = (TR)rjk_d;
-        for (int p = 0; p < nb; ++p) {
-          const size_t q = sl(p, j, n);
-          const int i = st.b_j[q], ti = st.type[i];
-          if (t.sp[ti].p_hbond != 2 || !((TR)st.bo[q] >= TR(1e-2))) continue;
-          if (st.tag[i] == st.tag[k]) continue;   // the same atom, possibly another image (reference quirk Q-32)
-          const DevHBond P = t.hb[(ti * g.ntypes + tj) * g.ntypes + tk];
-          if (!(P.r0 > 0.0)) continue;
-          const V3<TR> dji{(TR)st.b_dx[q], (TR)st.b_dy[q], (TR)st.b_dz[q]};
+__global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables t) {
+  const int j = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (j >= st.n) return;
+  const int lane = lane_id(), n = st.n, tj = st.type[j];
+  const int role = (tj >= 0 && g.hbond_cut > 0) ? t.sp[tj].p_hbond : 0;
+  if (role != 1) {
+    if (lane == 0) { st.eatom[T_HB * n + j] = 0.0; st.h_cnt[j] = 0; }
+    return;
+  }
+  const int nc = min(st.hc_cnt[j], st.maxhc);
+  const size_t hbase = (size_t)j * st.maxhc;
+  const double4 pj = st.pos[j];
+  const double hcut = g.hbond_cut;
+  const int nb = min(st.b_cnt[j], st.maxb);
+  TR* hf = reinterpret_cast<TR*>(st.hf);
+  double e_hb = 0, fjx = 0, fjy = 0, fjz = 0;
+  int inrange = 0;
+  for (int h0 = 0; h0 < nc; h0 += 64) {
+    const int h = h0 + lane;
+    int k = 0, tk = -1;
+    bool ok = false;
+    double dx = 0, dy = 0, dz = 0, r = 1;
+    if (h < nc) {
+      k = st.hc_j[hbase + h];
+      r = sqrt(pair_dvec_list(st, pj, k, st.hc_s[hbase + h], dx, dy, dz, tk));
+      ok = r <= hcut;
+    }
+    inrange += ok ? 1 : 0;
+    const long long tagk = ok ? st.tag[k] : 0;
+    const V3<TR> djk{(TR)dx, (TR)dy, (TR)dz};
+    const TR rjk = (TR)r;
+    double fkx = 0, fky = 0, fkz = 0;
+    for (int p = 0; p < nb; ++p) {
+      const size_t q = sl(p, j, n);
+      const int i = st.b_j[q], ti = st.type[i];
+      if (t.sp[ti].p_hbond != 2 || !((TR)st.bo[q] >= TR(1e-2))) continue;   // wave-uniform
+      const V3<TR> dji{(TR)st.b_dx[q], (TR)st.b_dy[q], (TR)st.b_dz[q]};
+      const TR rij = (TR)st.b_d[q];
+      double c1 = 0, fx = 0, fy = 0, fz = 0;
+      // the same atom as i, possibly another image, is not an acceptor of this triple (reference quirk Q-32)
+      if (ok && tagk != st.tag[i]) {
+        const DevHBond P = t.hb[(ti * g.ntypes + tj) * g.ntypes + tk];
+        if (P.r0 > 0.0) {
+          const TR e2 = rexp<TR>(-(TR)P.p2 * (TR)st.bo[q]);
           TR e, CH1, CH2, CH3;
           V3<TR> dti, dtj, dtk;
-          hbond_term<TR>(P, (TR)st.bo[q], dji, (TR)st.b_d[q], djk, rjk, e, CH1, CH2, CH3, dti, dtj, dtk);
+          hbond_term<TR>(P, e2, dji, rij, djk, rjk, e, CH1, CH2, CH3, dti, dtj, dtk);
           e_hb += (double)e;
-          st.Cdbo[q] += (double)CH1;
-          st.fpx[q] += (double)(CH2 * dti.x); st.fpy[q] += (double)(CH2 * dti.y); st.fpz[q] += (double)(CH2 * dti.z);
-          fax += (double)(CH2 * dtj.x - CH3 / rjk * djk.x);
-          fay += (double)(CH2 * dtj.y - CH3 / rjk * djk.y);
-          faz += (double)(CH2 * dtj.z - CH3 / rjk * djk.z);
+          c1 = (double)CH1;
+          fx = (double)(CH2 * dti.x); fy = (double)(CH2 * dti.y); fz = (double)(CH2 * dti.z);
+          const TR c3 = CH3 / rjk;
+          fjx += (double)(CH2 * dtj.x - c3 * djk.x);
+          fjy += (double)(CH2 * dtj.y - c3 * djk.y);
+          fjz += (double)(CH2 * dtj.z - c3 * djk.z);
+          fkx += (double)(CH2 * dtk.x + c3 * djk.x);
+          fky += (double)(CH2 * dtk.y + c3 * djk.y);
+          fkz += (double)(CH2 * dtk.z + c3 * djk.z);
         }
       }
-    } else if (role == 2) {   // acceptor k = a: re-evaluate the triples (i, j, k) of every donor j in range for the force on k
-      const int k = a, tk = ta;
-      for (int h = 0; h < nh; ++h) {
-        const int j = st.h_j[sl(h, k, n)], sj = st.h_s[sl(h, k, n)], tj = st.type[j];
-        double dx, dy, dz;
-        const double rkj_d = sqrt(pair_dvec(st, c, k, j, sj, dx, dy, dz));
-        const V3<TR> djk{(TR)(-dx), (TR)(-dy), (TR)(-dz)};   // from the donor j to k
-        const TR rjk = (TR)rkj_d;
-        const int nb = min(st.b_cnt[j], st.maxb);
-        for (int p = 0; p < nb; ++p) {
-          const size_t q = sl(p, j, n);
-          const int i = st.b_j[q], ti = st.type[i];
-          if (t.sp[ti].p_hbond != 2 || !((TR)st.bo[q] >= TR(1e-2))) continue;
-          if (st.tag[i] == st.tag[k]) continue;
-          const DevHBond P = t.hb[(ti * g.ntypes + tj) * g.ntypes + tk];
-          if (!(P.r0 > 0.0)) continue;
-          const V3<TR> dji{(TR)st.b_dx[q], (TR)st.b_dy[q], (TR)st.b_dz[q]};
-          TR e, CH1, CH2, CH3;
-          V3<TR> dti, dtj, dtk;
-          hbond_term<TR>(P, (TR)st.bo[q], dji, (TR)st.b_d[q], djk, rjk, e, CH1, CH2, CH3, dti, dtj, dtk);
-          fax += (double)(CH2 * dtk.x + CH3 / rjk * djk.x);
-          fay += (double)(CH2 * dtk.y + CH3 / rjk * djk.y);
-          faz += (double)(CH2 * dtk.z + CH3 / rjk * djk.z);
-        }
-      }
+      c1 = wave_sum(c1); fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+      if (lane == 0) { st.Cdbo[q] += c1; st.fpx[q] += fx; st.fpy[q] += fy; st.fpz[q] += fz; }
+    }
+    if (h < nc) {
+      const size_t o = 3 * (hbase + h);
+      hf[o] = (TR)fkx; hf[o + 1] = (TR)fky; hf[o + 2] = (TR)fkz;
     }
   }
-  st.fs_x[a] += fax; st.fs_y[a] += fay; st.fs_z[a] += faz;
-  st.eatom[T_HB * n + a] = e_hb;
+  e_hb = wave_sum(e_hb); fjx = wave_sum(fjx); fjy = wave_sum(fjy); fjz = wave_sum(fjz);
+  for (int off = 32; off > 0; off >>= 1) inrange += __shfl_xor(inrange, off, 64);
+  if (lane == 0) {
+    st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
+    st.eatom[T_HB * n + j] = e_hb;
+    st.h_cnt[j] = inrange;
+  }
+}
+
+// Acceptor side: one wave per acceptor k sums the forces its donors left in its slots of their candidate lists (hc_m holds,
+// for each of k's candidates j, the position of k in j's list; fixed between neighbour rebuilds), in a fixed order.
+template <class TR>
+__global__ void __launch_bounds__(256) k_hbond_acceptor(State st, Globals g, Tables t) {
+  const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (k >= st.n) return;
+  const int lane = lane_id(), tk = st.type[k];
+  const int role = (tk >= 0 && g.hbond_cut > 0) ? t.sp[tk].p_hbond : 0;
+  if (role != 2) return;
+  const int nc = min(st.hc_cnt[k], st.maxhc);
+  const size_t hbase = (size_t)k * st.maxhc;
+  const TR* hf = reinterpret_cast<const TR*>(st.hf);
+  double fx = 0, fy = 0, fz = 0;
+  for (int h = lane; h < nc; h += 64) {
+    const size_t o = 3 * ((size_t)st.hc_j[hbase + h] * st.maxhc + st.hc_m[hbase + h]);
+    fx += (double)hf[o]; fy += (double)hf[o + 1]; fz += (double)hf[o + 2];
+  }
+  fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+  if (lane == 0) { st.fs_x[k] += fx; st.fs_y[k] += fy; st.fs_z[k] += fz; }
 }
 
 // ================================================================================================================
@@ -832,7 +961,7 @@ inline int nblk(long long items, int per) { return (int)((items + per - 1) / per
 
 template <class TB>
 void bond_build_t(const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
-  hipLaunchKernelGGL(k_bond_build<TB>, dim3(nblk((long long)st.n * 64, 256)), dim3(256), 0, s, st, c, g, t);
+  hipLaunchKernelGGL(k_bond_build<TB>, dim3(nblk(st.n, kBuildWaves)), dim3(64 * kBuildWaves), 0, s, st, c, g, t);
 }
 
 }  // namespace
@@ -851,19 +980,29 @@ void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tab
   else hipLaunchKernelGGL(k_bo_correct<double>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
 }
 
-void launch_bonded_terms(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
-  const dim3 grid(nblk(st.n, 128)), blk(128);
-  if (p == Precision::FP64) {
-    hipLaunchKernelGGL(k_atom_terms<double>, grid, blk, 0, s, st, g, t);
-    hipLaunchKernelGGL(k_valence<double>, grid, blk, 0, s, st, g, t);
-    hipLaunchKernelGGL(k_torsion<double>, grid, blk, 0, s, st, c, g, t);
-    hipLaunchKernelGGL(k_hbond<double>, grid, blk, 0, s, st, c, g, t);
-  } else {
-    hipLaunchKernelGGL(k_atom_terms<float>, grid, blk, 0, s, st, g, t);
-    hipLaunchKernelGGL(k_valence<float>, grid, blk, 0, s, st, g, t);
-    hipLaunchKernelGGL(k_torsion<float>, grid, blk, 0, s, st, c, g, t);
-    hipLaunchKernelGGL(k_hbond<float>, grid, blk, 0, s, st, c, g, t);
-  }
+template <class TR, int G>
+void launch_terms_g(const State& st, const Globals& g, const Tables& t, hipStream_t s) {
+  const dim3 grid(nblk((long long)st.n * G, 256)), blk(256), wgrid(nblk(st.n, 4));
+  hipLaunchKernelGGL(k_atom_terms<TR>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+  hipLaunchKernelGGL((k_valence<TR, G>), grid, blk, 0, s, st, g, t);
+  hipLaunchKernelGGL((k_torsion<TR, G>), grid, blk, 0, s, st, g, t);
+  hipLaunchKernelGGL(k_hbond_donor<TR>, wgrid, blk, 0, s, st, g, t);
+  hipLaunchKernelGGL(k_hbond_acceptor<TR>, wgrid, blk, 0, s, st, g, t);
+}
+
+template <class TR>
+void launch_terms_t(const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
+  if (bmax <= 4) launch_terms_g<TR, 4>(st, g, t, s);
+  else if (bmax <= 8) launch_terms_g<TR, 8>(st, g, t, s);
+  else if (bmax <= 16) launch_terms_g<TR, 16>(st, g, t, s);
+  else if (bmax <= 32) launch_terms_g<TR, 32>(st, g, t, s);
+  else launch_terms_g<TR, 64>(st, g, t, s);
+}
+
+void launch_bonded_terms(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
+  (void)c;
+  if (p == Precision::FP64) launch_terms_t<double>(st, g, t, bmax, s);
+  else launch_terms_t<float>(st, g, t, bmax, s);
 }
 
 void launch_bond_forces(Precision, const State& st, hipStream_t s) {

@@ -62,9 +62,11 @@ __global__ void k_half_kick(State st, MdBuffers md, double dtf) {
 __global__ void k_drift(State st, MdBuffers md, double dt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= st.n) return;
-  st.x[i] += dt * md.vx[i];
-  st.y[i] += dt * md.vy[i];
-  st.z[i] += dt * md.vz[i];
+  const double x = st.x[i] + dt * md.vx[i], y = st.y[i] + dt * md.vy[i], z = st.z[i] + dt * md.vz[i];
+  st.x[i] = x;
+  st.y[i] = y;
+  st.z[i] = z;
+  st.pos[i] = make_double4(x, y, z, (double)st.type[i]);
 }
 
 __global__ void __launch_bounds__(kRT) k_ke_partial(State st, MdBuffers md) {

@@ -29,8 +29,26 @@ __global__ void k_wrap(State st, Cell c) {
   st.x[i] = r[0];
   st.y[i] = r[1];
   st.z[i] = r[2];
+  st.pos[i] = make_double4(r[0], r[1], r[2], (double)st.type[i]);
+}
+
+// lattice shift vectors of all codes in [-smx, smx]^3, accumulated exactly as shift_vec does
+__global__ void k_shift_table(Cell c, double4* shv, int smx0, int smx1, int smx2) {
+  const int d0 = 2 * smx0 + 1, d1 = 2 * smx1 + 1, d2 = 2 * smx2 + 1;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d0 * d1 * d2) return;
+  const int s0 = k % d0 - smx0, s1 = (k / d0) % d1 - smx1, s2 = k / (d0 * d1) - smx2;
+  double sx, sy, sz;
+  shift_vec(c, shift_code(s0, s1, s2), sx, sy, sz);
+  shv[k] = make_double4(sx, sy, sz, 0.0);
 }
 #pragma clang fp contract(on)
+
+__global__ void k_pos4(State st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  st.pos[i] = make_double4(st.x[i], st.y[i], st.z[i], (double)st.type[i]);
+}
 
 __global__ void k_cell_keys(State st, Cell c, int nc0, int nc1, int nc2, unsigned* keys, int* idx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -65,6 +83,7 @@ __global__ void __launch_bounds__(256) k_nbr_cells(State st, Cell c, int nc0, in
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
   const int lane = lane_id();
+  const double4 pi = st.pos[i];
   const unsigned key = atom_key[i];
   const int c0 = (int)(key % (unsigned)nc0), c1 = (int)((key / (unsigned)nc0) % (unsigned)nc1), c2 = (int)(key / (unsigned)(nc0 * nc1));
   int count = 0;
@@ -82,18 +101,19 @@ __global__ void __launch_bounds__(256) k_nbr_cells(State st, Cell c, int nc0, in
         for (int p0 = cstart[cell]; p0 < pe; p0 += 64) {
           const int p = p0 + lane;
           bool keep = false;
-          int j = 0;
+          int j = 0, e = 0;
           if (p < pe) {
             j = sidx[p];
             if (!(j == i && sc == 0)) {
+              e = canonical_first(i, j, sc, st.tag) ? (sc | kCanon) : sc;
               double ddx, ddy, ddz;
-              keep = pair_dvec(st, c, i, j, sc, ddx, ddy, ddz) <= rs2;
+              keep = pair_dvec_list(st, pi, j, e, ddx, ddy, ddz) <= rs2;
             }
           }
           const unsigned long long m = __ballot(keep);
           if (keep) {
             const int slot = count + lanes_below(m);
-            if (slot < st.maxnb) { st.nb_j[base + slot] = j; st.nb_s[base + slot] = sc; }
+            if (slot < st.maxnb) { st.nb_j[base + slot] = j; st.nb_s[base + slot] = e; }
           }
           count += __popcll(m);
         }
@@ -108,6 +128,7 @@ __global__ void __launch_bounds__(256) k_nbr_brute(State st, Cell c, int sm0, in
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
   const int lane = lane_id();
+  const double4 pi = st.pos[i];
   int count = 0;
   const size_t base = (size_t)i * st.maxnb;
   for (int s2 = -sm2; s2 <= sm2; ++s2)
@@ -117,14 +138,16 @@ __global__ void __launch_bounds__(256) k_nbr_brute(State st, Cell c, int sm0, in
         for (int j0 = 0; j0 < st.n; j0 += 64) {
           const int j = j0 + lane;
           bool keep = false;
+          int e = 0;
           if (j < st.n && !(j == i && sc == 0)) {
+            e = canonical_first(i, j, sc, st.tag) ? (sc | kCanon) : sc;
             double ddx, ddy, ddz;
-            keep = pair_dvec(st, c, i, j, sc, ddx, ddy, ddz) <= rs2;
+            keep = pair_dvec_list(st, pi, j, e, ddx, ddy, ddz) <= rs2;
           }
           const unsigned long long m = __ballot(keep);
           if (keep) {
             const int slot = count + lanes_below(m);
-            if (slot < st.maxnb) { st.nb_j[base + slot] = j; st.nb_s[base + slot] = sc; }
+            if (slot < st.maxnb) { st.nb_j[base + slot] = j; st.nb_s[base + slot] = e; }
           }
           count += __popcll(m);
         }
@@ -135,6 +158,80 @@ __global__ void __launch_bounds__(256) k_nbr_brute(State st, Cell c, int sm0, in
   }
 }
 
+// Bond and H-bond candidate sub-lists of the skin list (entry order kept): candidates within bond_cut + skin of every typed atom,
+// and for H-bond donors (p_hbond 1) / acceptors (2) the partners of the other role within hbond_cut + skin. Per-step kernels scan
+// these instead of the whole skin list and apply the exact cut-offs to the current distance.
+__global__ void __launch_bounds__(256) k_sublists(State st, Globals g, Tables t, double rb2, double rh2) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  const int lane = lane_id();
+  const int ti = st.type[i];
+  int nbc = 0, nhc = 0;
+  if (ti >= 0) {
+    const double4 pi = st.pos[i];
+    const int role = g.hbond_cut > 0 ? t.sp[ti].p_hbond : 0;
+    const size_t base = (size_t)i * st.maxnb, bb = (size_t)i * st.maxbc, hb = (size_t)i * st.maxhc;
+    const int cnt = min(st.nb_cnt[i], st.maxnb);
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const int k = k0 + lane;
+      bool bk = false, hk = false;
+      int j = 0, e = 0;
+      if (k < cnt) {
+        j = st.nb_j[base + k];
+        e = st.nb_s[base + k];
+        const int tj = st.type[j];
+        if (tj >= 0) {
+          double dx, dy, dz;
+          const double d2 = pair_dvec_list(st, pi, j, e, dx, dy, dz);
+          bk = d2 <= rb2;
+          if (role && d2 <= rh2) {
+            const int rj = t.sp[tj].p_hbond;
+            hk = (role == 1 && rj == 2) || (role == 2 && rj == 1);
+          }
+        }
+      }
+      const unsigned long long mb = __ballot(bk), mh = __ballot(hk);
+      if (bk) {
+        const int p = nbc + lanes_below(mb);
+        if (p < st.maxbc) { st.bc_j[bb + p] = j; st.bc_s[bb + p] = e; }
+      }
+      if (hk) {
+        const int p = nhc + lanes_below(mh);
+        if (p < st.maxhc) { st.hc_j[hb + p] = j; st.hc_s[hb + p] = e; }
+      }
+      nbc += __popcll(mb);
+      nhc += __popcll(mh);
+    }
+  }
+  if (lane == 0) {
+    st.bc_cnt[i] = nbc;
+    st.hc_cnt[i] = nhc;
+    if (nbc > st.maxbc) atomicMax(&st.flags[4], nbc);
+    if (nhc > st.maxhc) atomicMax(&st.flags[5], nhc);
+  }
+}
+
+// For every candidate donor j of an acceptor k, the position of k in j's candidate list (the pair is listed from both ends
+// with the same canonical distance and cut-off, so it is always found unless a list overflowed).
+__global__ void __launch_bounds__(256) k_hc_mirror(State st, Globals g, Tables t) {
+  const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (k >= st.n) return;
+  const int tk = st.type[k];
+  if (tk < 0 || !(g.hbond_cut > 0) || t.sp[tk].p_hbond != 2) return;
+  const int nc = min(st.hc_cnt[k], st.maxhc);
+  const size_t kb = (size_t)k * st.maxhc;
+  for (int h = lane_id(); h < nc; h += 64) {
+    const int j = st.hc_j[kb + h], want = shift_neg(entry_code(st.hc_s[kb + h]));
+    const size_t jb = (size_t)j * st.maxhc;
+    const int ncj = min(st.hc_cnt[j], st.maxhc);
+    int found = -1;
+    for (int q = 0; q < ncj; ++q)
+      if (st.hc_j[jb + q] == k && entry_code(st.hc_s[jb + q]) == want) { found = q; break; }
+    st.hc_m[kb + h] = found < 0 ? 0 : found;
+    if (found < 0) st.flags[6] = 1;
+  }
+}
+
 inline int blocks_for(long long items, int per_block) { return (int)((items + per_block - 1) / per_block); }
 
 }  // namespace
@@ -142,6 +239,22 @@ inline int blocks_for(long long items, int per_block) { return (int)((items + pe
 void launch_wrap(const State& st, const Cell& c, hipStream_t s) {
   if (st.n == 0) return;
   hipLaunchKernelGGL(k_wrap, dim3(blocks_for(st.n, 256)), dim3(256), 0, s, st, c);
+}
+
+void launch_pos4(const State& st, hipStream_t s) {
+  if (st.n == 0) return;
+  hipLaunchKernelGGL(k_pos4, dim3(blocks_for(st.n, 256)), dim3(256), 0, s, st);
+}
+
+void launch_shift_table(const Cell& c, double4* shv, const int smx[3], hipStream_t s) {
+  const int m = (2 * smx[0] + 1) * (2 * smx[1] + 1) * (2 * smx[2] + 1);
+  hipLaunchKernelGGL(k_shift_table, dim3(blocks_for(m, 256)), dim3(256), 0, s, c, shv, smx[0], smx[1], smx[2]);
+}
+
+void launch_sublists(const State& st, const Globals& g, const Tables& t, double rb, double rh, hipStream_t s) {
+  const double m = 1.0 + 1e-12;
+  hipLaunchKernelGGL(k_sublists, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t, rb * rb * m, rh * rh * m);
+  if (rh > 0) hipLaunchKernelGGL(k_hc_mirror, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t);
 }
 
 void launch_fill_int(int* p, int n, int v, hipStream_t s) {

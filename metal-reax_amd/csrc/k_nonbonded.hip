@@ -38,16 +38,13 @@ __device__ inline void taper_eval(const Globals& g, TR r, TR& Tap, TR& dTap) {
 
 template <class TR> __device__ inline TR cbrt_ref(TR x);
 template <> __device__ inline double cbrt_ref(double x) { return pow(x, 0.33333333333333); }
-template <> __device__ inline float cbrt_ref(float x) { return cbrtf(x); }
 template <class TR> __device__ inline TR tpow(TR a, TR b);
 template <> __device__ inline double tpow(double a, double b) { return pow(a, b); }
-template <> __device__ inline float tpow(float a, float b) { return powf(a, b); }
 template <class TR> __device__ inline TR texp(TR a);
 template <> __device__ inline double texp(double a) { return exp(a); }
-template <> __device__ inline float texp(float a) { return expf(a); }
 
 template <class TR>
-__global__ void __launch_bounds__(256) k_nonbonded(State st, Cell c, Globals g, Tables t) {
+__global__ void __launch_bounds__(256) k_nonbonded(State st, Globals g, Tables t) {
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
   const int lane = lane_id();
@@ -60,14 +57,16 @@ __global__ void __launch_bounds__(256) k_nonbonded(State st, Cell c, Globals g, 
     const TR qi = (TR)st.q[i];
     const TR p_vdW1 = (TR)g.p_vdW1, p1i = TR(1) / p_vdW1;
     const int vt = g.vdw_type;
+    const double4 pi = st.pos[i];
     for (int k = lane; k < cnt; k += 64) {
-      const int j = st.nb_j[base + k], tj = st.type[j];
-      if (tj < 0) continue;
+      const int j = st.nb_j[base + k];
       double dx, dy, dz;
-      const double r2 = pair_dvec(st, c, i, j, st.nb_s[base + k], dx, dy, dz);
+      int tj;
+      const double r2 = pair_dvec_list(st, pi, j, st.nb_s[base + k], dx, dy, dz, tj);
+      if (tj < 0) continue;
       const double rd = sqrt(r2);
       if (!(rd <= g.nonb_cut)) continue;
-      const DevPair P = t.pair[ti * g.ntypes + tj];
+      const DevPair& P = t.pair[ti * g.ntypes + tj];
       const TR r = (TR)rd;
       TR Tap, dTap;
       taper_eval<TR>(g, r, Tap, dTap);
@@ -132,7 +131,102 @@ __global__ void __launch_bounds__(256) k_nonbonded(State st, Cell c, Globals g, 
   }
 }
 
-__global__ void __launch_bounds__(256) k_qeq_build(State st, Cell c, Globals g, Tables t, QeqBuffers qb) {
+// Single-precision terms (mixed and FP32 modes): the same pair loop with the hardware base-2 exponential and logarithm
+// (v_exp_f32 / v_log_f32, about 1 ulp) for exp, pow and the cube root, e1 = e2^2 for the two Morse exponentials, a pair table
+// in float with (1/gamma_w)^p_vdW1 precomputed, and a squared-distance pre-test before the FP64 square root of the cut-off test.
+__device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
+
+__global__ void __launch_bounds__(256) k_nonbonded_f(State st, Globals g, Tables t) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  const int lane = lane_id();
+  const int ti = st.type[i];
+  double gx = 0, gy = 0, gz = 0, ev = 0, ee = 0;
+  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+  if (ti >= 0) {
+    const size_t base = (size_t)i * st.maxnb;
+    const int cnt = min(st.nb_cnt[i], st.maxnb);
+    const float qi = (float)st.q[i];
+    const float p_vdW1 = (float)g.p_vdW1, p1i = 1.0f / p_vdW1;
+    const int vt = g.vdw_type, nt = g.ntypes, lg = g.lg;
+    const double4 pi = st.pos[i];
+    const double cut = g.nonb_cut, cut2 = cut * cut * (1.0 + 1e-12);
+    const float swa = (float)g.nonb_low, tw = (float)g.nonb_cut - (float)g.nonb_low;
+    const float L2E = 1.4426950408889634f;
+    const NbPairF* __restrict__ row = t.nbf + ti * nt;
+    for (int k = lane; k < cnt; k += 64) {
+      const int j = st.nb_j[base + k];
+      double dx, dy, dz;
+      int tj;
+      const double r2 = pair_dvec_list(st, pi, j, st.nb_s[base + k], dx, dy, dz, tj);
+      if (tj < 0 || !(r2 <= cut2)) continue;
+      const double rd = sqrt(r2);
+      if (!(rd <= cut)) continue;
+      const NbPairF P = row[tj];
+      const float r = (float)rd;
+      const float x = (r - swa) / tw, x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+      const float Tap = 1.0f - x4 * (35.0f - 84.0f * x + 70.0f * x2 - 20.0f * x3);
+      const float dTap = (x3 * (-140.0f + 420.0f * x - 420.0f * x2 + 140.0f * x3)) / (tw * r);
+      float evdw, CEvd;
+      if (vt == 1 || vt == 3) {
+        const float pr = fexp2(p_vdW1 * flog2(r)), S = pr + P.pgw;
+        const float fn13 = fexp2(p1i * flog2(S));
+        const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - fn13 / P.rvdw)), e1 = e2 * e2;
+        const float e0 = P.D * (e1 - 2.0f * e2);
+        evdw = Tap * e0;
+        const float dfn13 = fn13 / S * pr / (r * r);
+        CEvd = dTap * e0 - Tap * P.D * (P.alpha / P.rvdw) * (e1 - e2) * dfn13;
+      } else {
+        const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - r / P.rvdw)), e1 = e2 * e2;
+        const float e0 = P.D * (e1 - 2.0f * e2);
+        evdw = Tap * e0;
+        CEvd = dTap * e0 - Tap * P.D * (P.alpha / P.rvdw) * (e1 - e2) / r;
+      }
+      if (vt == 2 || vt == 3) {
+        const float ec = P.ecore * fexp2(L2E * P.acore * (1.0f - r / P.rcore));
+        evdw += Tap * ec;
+        CEvd += dTap * ec + Tap * (-(P.acore / P.rcore) * ec) / r;
+        if (lg) {
+          const float r6 = (r * r * r) * (r * r * r), r5 = r6 / r, re6 = (P.lgre * P.lgre * P.lgre) * (P.lgre * P.lgre * P.lgre);
+          const float elg = -(P.lgcij / (r6 + re6));
+          evdw += Tap * elg;
+          CEvd += dTap * elg + Tap * (-6.0f * elg * r5 / (r6 + re6)) / r;
+        }
+      }
+      const float qj = (float)st.q[j];
+      const float d1 = r * r * r + P.gamma, rc = fexp2(flog2(d1) * (-1.0f / 3.0f));   // d1^(-1/3)
+      const float qq = 332.06371f * qi * qj;
+      const float eel = qq * Tap * rc;
+      const float CE = CEvd + qq * (dTap - Tap * r / d1) * rc;
+      const double ce = (double)CE;
+      gx += -ce * dx; gy += -ce * dy; gz += -ce * dz;
+      ev += 0.5 * (double)evdw;
+      ee += 0.5 * (double)eel;
+      const double hv = -0.5 * ce;
+      v0 += hv * dx * dx; v1 += hv * dy * dy; v2 += hv * dz * dz; v3 += hv * dx * dy; v4 += hv * dx * dz; v5 += hv * dy * dz;
+    }
+  }
+  gx = wave_sum(gx); gy = wave_sum(gy); gz = wave_sum(gz); ev = wave_sum(ev); ee = wave_sum(ee);
+  v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2); v3 = wave_sum(v3); v4 = wave_sum(v4); v5 = wave_sum(v5);
+  if (lane == 0) {
+    const int n = st.n;
+    st.gnb_x[i] = gx; st.gnb_y[i] = gy; st.gnb_z[i] = gz;
+    st.eatom[T_VDW * n + i] = ev;
+    st.eatom[T_ELE * n + i] = ee;
+    double ep = 0;
+    if (ti >= 0) {
+      const DevSpecies sp = t.sp[ti];
+      const float q = (float)st.q[i];
+      ep = (double)(23.02f * ((float)sp.chi * q + ((float)sp.eta / 2.0f) * (q * q)));
+    }
+    st.eatom[T_POL * n + i] = ep;
+    st.vatom[0 * n + i] = v0; st.vatom[1 * n + i] = v1; st.vatom[2 * n + i] = v2;
+    st.vatom[3 * n + i] = v3; st.vatom[4 * n + i] = v4; st.vatom[5 * n + i] = v5;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t, QeqBuffers qb) {
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
   const int lane = lane_id();
@@ -141,6 +235,7 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Cell c, Globals g, 
   const int cnt = min(st.nb_cnt[i], st.maxnb);
   const double swb2 = g.q_swb * g.q_swb;
   const double* T = g.qtap;
+  const double4 pi = st.pos[i];
   int m = 0;
   for (int k0 = 0; k0 < cnt; k0 += 64) {
     const int k = k0 + lane;
@@ -149,15 +244,14 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Cell c, Globals g, 
     double h = 0;
     if (k < cnt && ti >= 0) {
       j = st.nb_j[base + k];
-      const int tj = st.type[j];
       double dx, dy, dz;
-      const double r2 = pair_dvec(st, c, i, j, st.nb_s[base + k], dx, dy, dz);
+      int tj;
+      const double r2 = pair_dvec_list(st, pi, j, st.nb_s[base + k], dx, dy, dz, tj);
       if (tj >= 0 && r2 <= swb2) {
         const double r = sqrt(r2);
         double Tap = T[7] * r + T[6];
         Tap = Tap * r + T[5]; Tap = Tap * r + T[4]; Tap = Tap * r + T[3]; Tap = Tap * r + T[2]; Tap = Tap * r + T[1]; Tap = Tap * r + T[0];
-        const double den = pow(r * r * r + t.qeq_shld[ti * g.ntypes + tj], 1.0 / 3.0);
-        h = Tap * 14.4 / den;
+        h = (Tap * 14.4) * rcbrt_f64(r * r * r + t.qeq_shld[ti * g.ntypes + tj]);
         keep = true;
       }
     }
@@ -185,12 +279,14 @@ inline int nblk(long long items, int per) { return (int)((items + per - 1) / per
 
 void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * 64, 256)), blk(256);
-  if (p == Precision::FP64) hipLaunchKernelGGL(k_nonbonded<double>, grid, blk, 0, s, st, c, g, t);
-  else hipLaunchKernelGGL(k_nonbonded<float>, grid, blk, 0, s, st, c, g, t);
+  (void)c;
+  if (p == Precision::FP64) hipLaunchKernelGGL(k_nonbonded<double>, grid, blk, 0, s, st, g, t);
+  else hipLaunchKernelGGL(k_nonbonded_f, grid, blk, 0, s, st, g, t);
 }
 
 void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s) {
-  hipLaunchKernelGGL(k_qeq_build, dim3(nblk((long long)st.n * 64, 256)), dim3(256), 0, s, st, c, g, t, qb);
+  (void)c;
+  hipLaunchKernelGGL(k_qeq_build, dim3(nblk((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t, qb);
 }
 
 }  // namespace rxa

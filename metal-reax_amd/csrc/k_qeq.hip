@@ -15,6 +15,9 @@ namespace rxa {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kMvRowsPerWave = 2;
+constexpr int kMvRowsPerBlock = kMvRowsPerWave * (kThreads / 64);
+inline __host__ __device__ int mv_blocks(int n) { return (n + kMvRowsPerBlock - 1) / kMvRowsPerBlock; }
 // scal layout: [0,1] |b| of s and t, [2,3] u terms, then sig[it][sys] at 8 + 2 it + sys
 __device__ inline double& SIG(double* scal, int it, int sys) { return scal[8 + 2 * it + sys]; }
 
@@ -32,10 +35,13 @@ __device__ inline void block_reduce(double (&v)[K], double* sh) {
   __syncthreads();
 }
 
-// every block re-reduces the kCgBlocks partials of a previous launch (K values per block) in the same fixed order
+// every block re-reduces the `np` partials of a previous launch (K values per block) in the same fixed order: thread t sums
+// partials t, t + kThreads, ... in sequence, then the fixed LDS tree
 template <int K>
-__device__ inline void reduce_partials(const double* part, double (&v)[K], double* sh) {
-  for (int k = 0; k < K; ++k) v[k] = threadIdx.x < kCgBlocks ? part[threadIdx.x * K + k] : 0.0;
+__device__ inline void reduce_partials(const double* part, double (&v)[K], double* sh, int np = kCgBlocks) {
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  for (int b = threadIdx.x; b < np; b += kThreads)
+    for (int k = 0; k < K; ++k) v[k] += part[b * K + k];
   block_reduce<K>(v, sh);
 }
 
@@ -57,43 +63,51 @@ __global__ void __launch_bounds__(kThreads) k_cg_guess(int n, QeqBuffers qb, int
   }
 }
 
-// w = (diag(eta) + H) v for both systems; optional partial sums of v.w (only systems still active)
-__global__ void __launch_bounds__(kThreads) k_spmv2(int n, int maxnb, QeqBuffers qb, const double* v, double* w, double* part,
-                                                   const int* active, int use32) {
+// w = (diag(eta) + H) v for both systems; optional per-block partial sums of v.w (launches skip entirely once both systems
+// stopped). Each wave takes kMvRowsPerWave consecutive rows and walks a row with its 64 lanes, two entries per lane in flight;
+// the pair (v_s, v_t) of a column is one 16-byte load. The matrix values are FP64 or FP32 (TV); products and sums are FP64.
+template <class TV>
+__global__ void __launch_bounds__(kThreads) k_spmv2(int n, int maxnb, const int* __restrict__ cnt, const int* __restrict__ col,
+                                                   const TV* __restrict__ val, const double* __restrict__ eta, const double* v,
+                                                   double* w, double* part, const int* active) {
   __shared__ double sh[2 * kThreads];
   const int a0 = active ? active[0] : 1, a1 = active ? active[1] : 1;
   if (!a0 && !a1) return;
   const int lane = lane_id();
-  const int wave = (blockIdx.x * kThreads + threadIdx.x) >> 6, nwaves = (gridDim.x * kThreads) >> 6;
+  const double2* __restrict__ v2 = reinterpret_cast<const double2*>(v);
+  double2* w2 = reinterpret_cast<double2*>(w);
   double dot0 = 0, dot1 = 0;
-  for (int i = wave; i < n; i += nwaves) {
+  const int row0 = blockIdx.x * kMvRowsPerBlock + (threadIdx.x >> 6) * kMvRowsPerWave;
+  for (int r = 0; r < kMvRowsPerWave; ++r) {
+    const int i = row0 + r;
+    if (i >= n) break;
     const size_t base = (size_t)i * maxnb;
-    const int cnt = qb.cnt[i];
-    double y0 = 0, y1 = 0;
-    if (use32) {
-      for (int k = lane; k < cnt; k += 64) {
-        const int j = qb.col[base + k];
-        const double h = (double)qb.val32[base + k];
-        y0 += h * v[2 * j];
-        y1 += h * v[2 * j + 1];
-      }
-    } else {
-      for (int k = lane; k < cnt; k += 64) {
-        const int j = qb.col[base + k];
-        const double h = qb.val[base + k];
-        y0 += h * v[2 * j];
-        y1 += h * v[2 * j + 1];
-      }
+    const int c = cnt[i];
+    double y0 = 0, y1 = 0, z0 = 0, z1 = 0;
+    int k = lane;
+    for (; k + 64 < c; k += 128) {
+      const int ja = col[base + k], jb = col[base + k + 64];
+      const double ha = (double)val[base + k], hb = (double)val[base + k + 64];
+      const double2 xa = v2[ja], xb = v2[jb];
+      y0 += ha * xa.x; y1 += ha * xa.y;
+      z0 += hb * xb.x; z1 += hb * xb.y;
     }
-    y0 = wave_sum(y0);
-    y1 = wave_sum(y1);
-    y0 += qb.eta[i] * v[2 * i];
-    y1 += qb.eta[i] * v[2 * i + 1];
+    if (k < c) {
+      const int ja = col[base + k];
+      const double ha = (double)val[base + k];
+      const double2 xa = v2[ja];
+      y0 += ha * xa.x; y1 += ha * xa.y;
+    }
+    y0 = wave_sum(y0 + z0);
+    y1 = wave_sum(y1 + z1);
+    const double2 xi = v2[i];
+    const double ei = eta[i];
+    y0 += ei * xi.x;
+    y1 += ei * xi.y;
     if (lane == 0) {
-      w[2 * i] = y0;
-      w[2 * i + 1] = y1;
-      dot0 += v[2 * i] * y0;
-      dot1 += v[2 * i + 1] * y1;
+      w2[i] = make_double2(y0, y1);
+      dot0 += xi.x * y0;
+      dot1 += xi.y * y1;
     }
   }
   if (part) {
@@ -155,7 +169,7 @@ __global__ void __launch_bounds__(kThreads) k_cg_update(int n, QeqBuffers qb, in
   const int a[2] = {act[0], act[1]};
   if (!a[0] && !a[1]) return;
   double dq[2];
-  reduce_partials<2>(qb.partial, dq, sh);
+  reduce_partials<2>(qb.partial_mv, dq, sh, mv_blocks(n));
   double alpha[2];
   for (int s = 0; s < 2; ++s) alpha[s] = a[s] ? SIG(qb.scal, it - 1, s) / dq[s] : 0.0;
   double acc[2] = {0, 0};
@@ -246,15 +260,27 @@ void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, 
 void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s) {
   const int n = st.n;
   hipLaunchKernelGGL(k_cg_guess, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, use_hist);
-  hipLaunchKernelGGL(k_spmv2, dim3(kCgBlocks), dim3(kThreads), 0, s, n, st.maxnb, qb, qb.x, qb.q, (double*)nullptr, (const int*)nullptr, 0);
+  if (qb.fp32)
+    hipLaunchKernelGGL(k_spmv2<float>, dim3(mv_blocks(n)), dim3(kThreads), 0, s, n, st.maxnb, qb.cnt, qb.col, qb.val32, qb.eta, qb.x, qb.q,
+                       (double*)nullptr, (const int*)nullptr);
+  else
+    hipLaunchKernelGGL(k_spmv2<double>, dim3(mv_blocks(n)), dim3(kThreads), 0, s, n, st.maxnb, qb.cnt, qb.col, qb.val, qb.eta, qb.x, qb.q,
+                       (double*)nullptr, (const int*)nullptr);
   hipLaunchKernelGGL(k_cg_init, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb);
   hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kThreads), 0, s, qb, imax, tol);
 }
 
-void launch_qeq_spmv(const State& st, QeqBuffers& qb, int it, bool fp32_matvec, hipStream_t s) {
-  hipLaunchKernelGGL(k_spmv2, dim3(kCgBlocks), dim3(kThreads), 0, s, st.n, st.maxnb, qb, qb.d, qb.q, qb.partial, qb.iscal + 2 * (it - 1),
-                     fp32_matvec ? 1 : 0);
+void launch_qeq_spmv(const State& st, QeqBuffers& qb, int it, hipStream_t s) {
+  const int n = st.n;
+  if (qb.fp32)
+    hipLaunchKernelGGL(k_spmv2<float>, dim3(mv_blocks(n)), dim3(kThreads), 0, s, n, st.maxnb, qb.cnt, qb.col, qb.val32, qb.eta, qb.d, qb.q,
+                       qb.partial_mv, qb.iscal + 2 * (it - 1));
+  else
+    hipLaunchKernelGGL(k_spmv2<double>, dim3(mv_blocks(n)), dim3(kThreads), 0, s, n, st.maxnb, qb.cnt, qb.col, qb.val, qb.eta, qb.d, qb.q,
+                       qb.partial_mv, qb.iscal + 2 * (it - 1));
 }
+
+int qeq_mv_partials(int n) { return mv_blocks(n); }
 
 void launch_qeq_update(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s) {
   const int n = st.n;

@@ -26,12 +26,15 @@ size_t bin_temp_bytes(int n);
 void launch_bin(const State& st, const Cell& c, const BinGrid& g, BinBuffers& b, hipStream_t s);
 void launch_nbr_cells(const State& st, const Cell& c, const BinGrid& g, const BinBuffers& b, double rs, hipStream_t s);
 void launch_nbr_brute(const State& st, const Cell& c, const int sm[3], double rs, hipStream_t s);
+void launch_pos4(const State& st, hipStream_t s);                                   // pos = (x, y, z, type)
+void launch_shift_table(const Cell& c, double4* shv, const int smx[3], hipStream_t s);
+void launch_sublists(const State& st, const Globals& g, const Tables& t, double rb, double rh, hipStream_t s);
 
 // ---- bond-order chain and bonded terms (k_bonded.hip)
 void launch_bond_build(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
 void launch_bond_sym(const State& st, hipStream_t s);
 void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s);
-void launch_bonded_terms(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
+void launch_bonded_terms(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, int bmax, hipStream_t s);
 void launch_bond_forces(Precision p, const State& st, hipStream_t s);
 
 // ---- non-bonded terms and the charge-equilibration matrix (k_nonbonded.hip)
@@ -47,7 +50,9 @@ struct QeqBuffers {
   double* chi = nullptr;     // n
   // CG state, two systems (s: b = -chi, t: b = -1) interleaved per atom: [2*i + sys]
   double *b, *x, *r, *d, *p, *q;
-  double* partial;           // 2 * kCgBlocks * 2
+  double* partial;           // 4 * kCgBlocks: per-block partials of the element-wise CG kernels
+  double* partial_mv;        // 2 * qeq_mv_partials(n): per-block partials of the matrix-vector product
+  int fp32 = 0;              // the CG multiplies with val32 (mixed and FP32 modes) instead of val
   double* scal;              // device scalars (see k_qeq.hip)
   int* iscal;                // device integer scalars
   double *s_hist, *t_hist;   // 4 * n each, [k * n + i], k = 0 most recent
@@ -56,7 +61,8 @@ constexpr int kCgBlocks = 256;
 void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s);
 void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, double* y, hipStream_t s);
 void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s);
-void launch_qeq_spmv(const State& st, QeqBuffers& qb, int it, bool fp32_matvec, hipStream_t s);        // q = A d (+ d.q partials)
+void launch_qeq_spmv(const State& st, QeqBuffers& qb, int it, hipStream_t s);                          // q = A d (+ d.q partials)
+int qeq_mv_partials(int n);                                                                             // partials per product
 void launch_qeq_update(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);   // alpha, x, r, p, beta, d
 void launch_qeq_finish(const State& st, QeqBuffers& qb, hipStream_t s);
 

@@ -285,8 +285,8 @@ class Engine:
     def stats(self):
         out = np.zeros(12, dtype=np.int64)
         self._chk(lib().rxa_stats(self.h, _p(out)))
-        keys = ("n", "maxnb", "maxb", "maxh", "skin_entries", "bond_slots", "hbond_entries", "h_entries", "rebuilds", "precision",
-                "nb_pairs", "hb_candidates")
+        keys = ("n", "maxnb", "maxb", "maxhc", "skin_entries", "bond_slots", "hbond_entries", "h_entries", "rebuilds", "precision",
+                "bond_candidates", "hb_candidates")
         return dict(zip(keys, out.tolist()))
 
     PHASES = ("neighbor", "bond_lists", "bo_correct", "bonded_terms", "nonbonded", "dbond_gather", "qeq_matrix", "qeq_spmv", "qeq_cg",
