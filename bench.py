@@ -30,6 +30,19 @@ def model_bytes_per_atom_step(n_f, R, k, B, n_h, f_don):
     return 100 + 4 * n_f * (1 + 1.0 / R) + 4 * n_f + k * 8 * n_f + 240 * B + 8 * n_h * f_don
 
 
+def read_traffic():
+    """HBM bytes per SpMV launch measured by the committed rocprofv3 PMC runs (FETCH_SIZE and WRITE_SIZE of k_spmv2, each
+    scaled by the same counter's reading for a plain copy of known size), or None when no summary is present."""
+    path = os.path.join(REPO, "profiles", "k_spmv2_water24000_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(ffield_text, sysm, budget_s=25.0):
     """The FP64 oracle (serial, one core): charge solve + all terms + forces of the 24,000-atom system, repeated while the
     time budget lasts (at least once)."""
@@ -59,6 +72,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed", "fp32"))
     ap.add_argument("--nside", type=int, default=20, help="water molecules per box edge (20 -> 24,000 atoms)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (10 times) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (systems shard across GPUs without "
@@ -88,43 +103,40 @@ def main():
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync = lambda: None
-    e.profile(True)
+    # timed region: the MD loop; only the SpMV launches carry HIP events (the roofline kernel)
+    e.profile(True, phases=["qeq_spmv"])
     sync()
     t0 = time.perf_counter()
     thermo = e.md_run(args.steps)
     sync()
     elapsed = time.perf_counter() - t0
-    prof = e.profile_read()
-    e.profile(False)
+    spmv_ms, spmv_cnt = e.profile_read()["qeq_spmv"]
     st = e.stats()
     print("bench: timed %d steps in %.3f s; thermo %s" % (args.steps, elapsed, thermo), file=sys.stderr, flush=True)
+    # untimed: per-phase device time of a few more steps (every phase bracketed by events)
+    nb_steps = min(10, max(1, args.steps))
+    e.profile(True)
+    e.md_run(nb_steps)
+    prof = e.profile_read()
+    e.profile(False)
 
     n = w.n
     value = n * args.steps / elapsed
     # workload statistics for the byte model
     n_f = st["h_entries"] / n / 2.0                      # half of the full charge-matrix rows (pairs within 10 A)
     B = st["bond_slots"] / n
-    spmv_ms, spmv_cnt = prof["qeq_spmv"]
     k = spmv_cnt / args.steps
     R = args.steps / max(1.0, thermo["rebuilds"])
     n_h = st["hbond_entries"] / max(1, (w.types == 2).sum())        # acceptors within hbond_cut per donor hydrogen
     bytes_step = model_bytes_per_atom_step(n_f, R, k, B, n_h, (w.types == 2).mean())
 
-    # dominant kernel by measured device time
-    phase_ms = {kk: vv[0] for kk, vv in prof.items()}
-    dominant = max(phase_ms, key=phase_ms.get)
-    # algorithmic bytes per launch of each kernel family (model terms attributed to the kernel that moves them)
-    alg = {
-        "qeq_spmv": 8 * n_f * n,                                   # one fused two-vector pass over the half H (value + index)
-        "qeq_matrix": (4 * n_f + 4 * n_f) * n,                     # far list read + H written
-        "nonbonded": (4 * n_f + 64) * n,                           # far list read + x, q, type, f
-        "bonded_terms": 240 * B * n,
-        "neighbor": 4 * n_f * n,
-    }
-    dom = dominant if dominant in alg else "qeq_spmv"
-    dom_ms, dom_cnt = prof[dom]
-    avg_s = dom_ms / max(1, dom_cnt) * 1e-3
-    achieved = alg[dom] / avg_s / 1e9
+    # roofline of the charge-matrix SpMV (the kernel with the largest device time): algorithmic bytes of one fused two-vector
+    # pass over the half matrix (4-byte value + 4-byte column per entry) / mean launch duration from the HIP events
+    alg = 8 * n_f * n
+    avg_s = spmv_ms / max(1, spmv_cnt) * 1e-3
+    achieved = alg / avg_s / 1e9
+    traffic = read_traffic()
+    phase_ms = {kk: vv[0] / nb_steps for kk, vv in prof.items()}
     out = {
         "metric": "ReaxFF+QEq atom-steps/s at 24k atoms",
         "value": value,
@@ -140,15 +152,22 @@ def main():
         "data": "synthetic: seeded 8000-H2O lattice box at 1.0 g/cm^3, Maxwell-Boltzmann 300 K; stand-in C/H/O ffield",
         "config": {"workload": "water_24000_nvt_qeq", "atoms": n, "box_A": float(w.hi[0]), "precision": args.precision,
                    "dt_fs": 0.25, "qeq_tol": 1e-6, "skin_A": 2.0},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": alg[dom], "avg_launch_us": avg_s * 1e6, "launches": dom_cnt},
+        "roofline": {"bound": "hbm", "kernel": "k_spmv2", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg, "avg_launch_us": avg_s * 1e6, "launches": spmv_cnt},
         "step_model": {"bytes_per_atom_step": bytes_step, "n_f": n_f, "R": R, "k": k, "B": B, "n_h": n_h,
                        "roofline_atom_steps_per_s": HBM_PEAK_GBS * 1e9 / bytes_step,
                        "frac_of_step_roofline": value / (HBM_PEAK_GBS * 1e9 / bytes_step)},
-        "phase_ms_per_step": {kk: vv[0] / args.steps for kk, vv in prof.items()},
+        "phase_ms_per_step": phase_ms,
         "thermo": thermo,
     }
+    if args.calib_copy:
+        import torch
+        a = torch.empty(args.calib_copy << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (FP64 oracle, one core)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(ffield_text, w)

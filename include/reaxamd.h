@@ -111,9 +111,10 @@ int rxa_md_setup(rxa_context* ctx, const double* v, const double* mass_by_type, 
 int rxa_md_run(rxa_context* ctx, int steps, double* thermo);
 int rxa_get_state(rxa_context* ctx, double* x, double* v, double* q);
 
-/* capacities and counters: out[0] n, [1] max neighbours (skin list), [2] max bonds, [3] max H-bond partners, [4] skin-list entries,
- * [5] directed bond slots in use, [6] H-bond partner entries, [7] H-matrix entries, [8] neighbour rebuilds, [9] precision (0 fp64,
- * 1 mixed, 2 fp32), [10] non-bonded pairs within nonb_cut (directed), [11] H-bond candidates of donors */
+/* capacities and counters: out[0] n, [1] skin-list capacity per atom, [2] bond-slot capacity, [3] H-bond candidate capacity,
+ * [4] skin-list entries, [5] directed bond slots in use, [6] acceptors within hbond_cut summed over the donors (last evaluation),
+ * [7] charge-matrix entries (full rows), [8] neighbour rebuilds, [9] precision (0 fp64, 1 mixed, 2 fp32), [10] bond candidates
+ * (r <= bond_cut + skin, directed), [11] H-bond candidates (donor and acceptor lists, r <= hbond_cut + skin) */
 int rxa_stats(rxa_context* ctx, int64_t* out);
 
 /* ---- context-free helpers (no device is touched; usable on a host without a GPU) ----------------------------------
@@ -127,9 +128,11 @@ int rxa_check_pair_style(int nargs, const char* const* args, const char* control
 /* ---- device timing of the engine's own stream (HIP events around each phase; off by default)
  * phases: 0 neighbour rebuild, 1 bond lists, 2 bond-order correction, 3 bonded terms, 4 non-bonded, 5 bond-derivative gathers +
  * energy reduction, 6 charge matrix, 7 charge SpMV (one launch per CG iteration), 8 other CG work, 9 integrator.
- * rxa_profile(enable) also clears the totals; rxa_profile_read fills ms[10] (summed milliseconds) and counts[10] (timed launches). */
+ * rxa_profile(mask): 0 off, 1 every phase, otherwise a bit mask of the phases to time (bit k = phase k, e.g. 1 << 7 times only
+ * the SpMV launches); it also clears the totals. rxa_profile_read fills ms[10] (summed milliseconds) and counts[10] (timed
+ * launches). */
 #define RXA_NPHASES 10
-int rxa_profile(rxa_context* ctx, int enable);
+int rxa_profile(rxa_context* ctx, int mask);
 int rxa_profile_read(rxa_context* ctx, double* ms, int64_t* counts);
 
 /* ---- device-resident inputs (bench / serving): pointers to device memory owned by the caller (HIP, same device) */

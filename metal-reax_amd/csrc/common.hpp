@@ -119,7 +119,8 @@ struct State {
   double *fs_x, *fs_y, *fs_z;     // explicit-geometry gradient of bonded terms computed at this atom
   double *gx, *gy, *gz;           // final gradient dE/dx (force = -g)
   double* eatom;                  // kTerms * n per-atom energies
-  double* vatom;                  // 6 * n per-atom virial (xx yy zz xy xz yz)
+  double* vatom;                  // 6 * n per-atom virial (xx yy zz xy xz yz): non-bonded half pairs + vb, complete after k_gather
+  double* vb;                     // 6 * n bonded part of the per-atom virial, accumulated by the bonded kernels
   double *gnb_x, *gnb_y, *gnb_z;  // non-bonded gradient
 };
 
@@ -189,6 +190,12 @@ __device__ inline double rcbrt_f64(double x) {
   y = y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
   y = y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
   return y;
+}
+
+// ---- virial: w[6] (xx yy zz xy xz yz) += s * (d (x) g) for a relative position d and a gradient g --------------------------
+__device__ inline void vir_add(double* w, double s, double dx, double dy, double dz, double gx, double gy, double gz) {
+  w[0] += s * dx * gx; w[1] += s * dy * gy; w[2] += s * dz * gz;
+  w[3] += s * dx * gy; w[4] += s * dx * gz; w[5] += s * dy * gz;
 }
 
 // ---- list entries ---------------------------------------------------------------------------------------------------

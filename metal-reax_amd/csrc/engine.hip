@@ -9,11 +9,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <unordered_map>
 
 #include "../../include/reaxamd.h"
 #include "common.hpp"
@@ -57,6 +59,7 @@ struct DBuf {
 enum Phase { PH_NBR, PH_BONDS, PH_BOCORR, PH_TERMS, PH_NONB, PH_DBOND, PH_QEQ_H, PH_QEQ_SPMV, PH_QEQ_CG, PH_MD, PH_COUNT };
 struct PhaseTimer {
   bool on = false;
+  unsigned mask = 0;   // phases being timed (bit k = phase k)
   hipStream_t s = nullptr;
   std::vector<hipEvent_t> pool;
   struct Pending { int ph; hipEvent_t a, b; };
@@ -73,8 +76,8 @@ struct PhaseTimer {
     pool.pop_back();
     return e;
   }
-  hipEvent_t begin() {
-    if (!on) return nullptr;
+  hipEvent_t begin(int ph) {
+    if (!on || !(mask >> ph & 1u)) return nullptr;
     hipEvent_t a = take();
     HIPCHK(hipEventRecord(a, s));
     return a;
@@ -136,6 +139,13 @@ struct Engine {
   double skin = 2.0;
   DBuf<double> a_x, a_y, a_z, a_q;
   DBuf<double4> a_pos, a_shv;
+  // engine order vs the caller's order: engine atom i is the caller's atom perm_h[i] (device copy d_perm). In cell mode the atoms
+  // are put in the order of their cells at every neighbour rebuild, so neighbour gathers touch nearby memory.
+  bool sort_atoms = true;
+  std::vector<int> perm_h;
+  DBuf<int> d_perm, d_src, d_psi;
+  DBuf<double> d_psd;
+  DBuf<long long> d_psl;
   int smx[3] = {-1, -1, -1};
   DBuf<int> a_type;
   DBuf<long long> a_tag;
@@ -183,6 +193,8 @@ struct Engine {
   double last_rel[2] = {0, 0};
 
   Engine(int dev) : device(dev) {
+    const char* sort_env = std::getenv("RXA_ATOM_SORT");
+    if (sort_env && std::atoi(sort_env) == 0) sort_atoms = false;
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     timer.s = stream;
@@ -322,6 +334,8 @@ struct Engine {
 
   void alloc_atoms() {
     a_x.alloc(n); a_y.alloc(n); a_z.alloc(n); a_q.alloc(n); a_type.alloc(n); a_tag.alloc(n); a_pos.alloc(n);
+    d_perm.alloc(n); d_src.alloc(n); d_psd.alloc(24 * (size_t)n); d_psi.alloc(4 * (size_t)n); d_psl.alloc(2 * (size_t)n);
+    HIPCHK(hipMemset(a_q.p, 0, n * sizeof(double)));
     const int NA = 38 + kTerms + 6;   // number of per-atom double arrays below
     a_atom.alloc((size_t)NA * n);
     HIPCHK(hipMemset(a_atom.p, 0, (size_t)NA * n * sizeof(double)));
@@ -337,6 +351,7 @@ struct Engine {
     s.CdD_self = atom_arr(k++); s.CdD = atom_arr(k++); s.S = atom_arr(k++); s.Dx = atom_arr(k++); s.Dy = atom_arr(k++); s.Dz = atom_arr(k++);
     s.fs_x = atom_arr(k++); s.fs_y = atom_arr(k++); s.fs_z = atom_arr(k++); s.gx = atom_arr(k++); s.gy = atom_arr(k++); s.gz = atom_arr(k++);
     s.gnb_x = atom_arr(k++); s.gnb_y = atom_arr(k++); s.gnb_z = atom_arr(k++);
+    s.vb = atom_arr(k);   // 6 arrays: 30..35
     k = 38;
     s.eatom = atom_arr(k);
     s.vatom = atom_arr(k + kTerms);
@@ -402,6 +417,21 @@ struct Engine {
   }
   int estimate_neighbors() const { return estimate_within(list_cut() + skin); }
 
+  // moves every persistent per-atom array to a new order: new[i] = old[src[i]] (src on the device)
+  void permute(const int* src) {
+    PermuteSet ps;
+    ps.n = n;
+    auto D = [&](double* p) { ps.d[ps.nd++] = p; };
+    D(st.x); D(st.y); D(st.z); D(st.q);
+    if (md_ready) { D(md.vx); D(md.vy); D(md.vz); D(md.mass); }
+    if (qeq_ready && qb.s_hist)
+      for (int k = 0; k < 4; ++k) { D(qb.s_hist + (size_t)k * n); D(qb.t_hist + (size_t)k * n); }
+    ps.i[ps.ni++] = st.type;
+    ps.i[ps.ni++] = d_perm.p;
+    ps.l[ps.nl++] = st.tag;
+    launch_permute(ps, src, d_psd.p, d_psi.p, d_psl.p, stream);
+  }
+
   void set_atoms(long long nn, const int* types, const int64_t* tags, const double* x) {
     if (!have_ff) throw ApiError(RXA_ESTATE, "pair_coeff (the force field) must be set before the atoms");
     if (!have_box) throw ApiError(RXA_ESTATE, "the box must be set before the atoms");
@@ -438,6 +468,30 @@ struct Engine {
       for (size_t k = 1; k < s.size(); ++k)
         if (s[k] == s[k - 1]) throw ApiError(RXA_EINVAL, "duplicate atom id " + std::to_string(s[k]));
     }
+    // the same atoms in a new order (e.g. LAMMPS re-sorted or re-neighboured): carry charges, charge history and velocities by tag
+    if (!realloc && !perm_h.empty()) {
+      std::vector<long long> old_tag(n);
+      HIPCHK(hipMemcpy(old_tag.data(), a_tag.p, n * sizeof(long long), hipMemcpyDeviceToHost));
+      std::unordered_map<long long, int> where;
+      where.reserve(2 * (size_t)n);
+      for (int i = 0; i < n; ++i) where[old_tag[i]] = i;
+      std::vector<int> src(n);
+      bool same = true;
+      for (int i = 0; i < n && same; ++i) {
+        auto it = where.find(tg[i]);
+        if (it == where.end()) same = false; else src[i] = it->second;
+      }
+      if (same) {
+        HIPCHK(hipMemcpy(d_src.p, src.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        permute(d_src.p);
+      } else {
+        hist_count = 0;
+        md_ready = false;
+      }
+    }
+    perm_h.resize(n);
+    for (int i = 0; i < n; ++i) perm_h[i] = i;
+    HIPCHK(hipMemcpyAsync(d_perm.p, perm_h.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(a_type.p, ft.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(a_tag.p, tg.data(), n * sizeof(long long), hipMemcpyHostToDevice, stream));
     host_types_mass.assign(n, 0.0);
@@ -448,8 +502,9 @@ struct Engine {
   void set_positions(const double* x) {
     std::vector<double> xs(n), ys(n), zs(n);
     for (int i = 0; i < n; ++i) {
-      xs[i] = x[3 * i]; ys[i] = x[3 * i + 1]; zs[i] = x[3 * i + 2];
-      if (!std::isfinite(xs[i]) || !std::isfinite(ys[i]) || !std::isfinite(zs[i])) throw ApiError(RXA_EINVAL, "non-finite coordinate of atom " + std::to_string(i));
+      const int u = perm_h[i];
+      xs[i] = x[3 * u]; ys[i] = x[3 * u + 1]; zs[i] = x[3 * u + 2];
+      if (!std::isfinite(xs[i]) || !std::isfinite(ys[i]) || !std::isfinite(zs[i])) throw ApiError(RXA_EINVAL, "non-finite coordinate of atom " + std::to_string(u));
     }
     HIPCHK(hipMemcpyAsync(a_x.p, xs.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(a_y.p, ys.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -491,18 +546,25 @@ struct Engine {
       for (int d = 0; d < 3; ++d) smx[d] = want[d];
       st.shv = a_shv.p; st.smx0 = smx[0]; st.smx1 = smx[1]; st.smx2 = smx[2];
     }
-    hipEvent_t t0 = timer.begin();
+    hipEvent_t t0 = timer.begin(PH_NBR);
     launch_shift_table(cell, a_shv.p, smx, stream);
     launch_wrap(st, cell, stream);
     timer.end(PH_NBR, t0);
     const double rb = g.bond_cut + skin, rh = g.hbond_cut > 0 ? g.hbond_cut + skin : 0.0;
     bool need_full = true;
     for (int attempt = 0; attempt < 8; ++attempt) {
-      t0 = timer.begin();
+      t0 = timer.begin(PH_NBR);
       launch_fill_int(st.flags, 8, 0, stream);
+      bool sorted = false;
       if (need_full) {
         if (cell_mode) {
           launch_bin(st, cell, grid, bins, stream);
+          if (sort_atoms) {   // atoms into the order of their cells
+            permute(bins.sidx);
+            launch_bin_identity(n, bins, stream);
+            launch_pos4(st, stream);
+            sorted = true;
+          }
           launch_nbr_cells(st, cell, grid, bins, rs, stream);
         } else {
           launch_nbr_brute(st, cell, sm, rs, stream);
@@ -511,6 +573,7 @@ struct Engine {
       launch_sublists(st, g, tab, rb, rh, stream);
       timer.end(PH_NBR, t0);
       int flags[8];
+      if (sorted) HIPCHK(hipMemcpyAsync(perm_h.data(), d_perm.p, n * sizeof(int), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipMemcpyAsync(flags, st.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
       sync();
       if (flags[0] > maxnb) {   // the skin list overflowed: grow it and redo everything
@@ -533,7 +596,7 @@ struct Engine {
 
   void bond_lists() {
     for (int attempt = 0; attempt < 6; ++attempt) {
-      hipEvent_t t0 = timer.begin();
+      hipEvent_t t0 = timer.begin(PH_BONDS);
       launch_fill_int(st.flags, 8, 0, stream);
       launch_bond_build(prec, st, cell, g, tab, stream);
       launch_bond_sym(st, stream);
@@ -556,16 +619,16 @@ struct Engine {
   void forces(bool rebuild) {
     if (!nbr_valid || rebuild) build_neighbors();
     bond_lists();
-    hipEvent_t t0 = timer.begin();
+    hipEvent_t t0 = timer.begin(PH_BOCORR);
     launch_bo_correct(prec, st, g, tab, stream);
     timer.end(PH_BOCORR, t0);
-    t0 = timer.begin();
+    t0 = timer.begin(PH_TERMS);
     launch_bonded_terms(prec, st, cell, g, tab, bmax, stream);
     timer.end(PH_TERMS, t0);
-    t0 = timer.begin();
+    t0 = timer.begin(PH_NONB);
     launch_nonbonded(prec, st, cell, g, tab, stream);
     timer.end(PH_NONB, t0);
-    t0 = timer.begin();
+    t0 = timer.begin(PH_DBOND);
     launch_bond_forces(prec, st, stream);
     launch_energy_reduce(st, red.p, stream);
     timer.end(PH_DBOND, t0);
@@ -587,7 +650,10 @@ struct Engine {
     if (energies) for (int k = 0; k < kTerms; ++k) energies[k] = out[k];
     if (virial) for (int k = 0; k < 6; ++k) virial[k] = out[kTerms + k];
     if (grad)
-      for (int i = 0; i < n; ++i) { grad[3 * i] = gx[i]; grad[3 * i + 1] = gy[i]; grad[3 * i + 2] = gz[i]; }
+      for (int i = 0; i < n; ++i) {
+        const int u = perm_h[i];
+        grad[3 * u] = gx[i]; grad[3 * u + 1] = gy[i]; grad[3 * u + 2] = gz[i];
+      }
   }
 
   // ---------------------------------------------------------------------------------------------------------------
@@ -682,7 +748,7 @@ struct Engine {
     if (!qeq_ready) throw ApiError(RXA_ESTATE, "charge equilibration is not set up");
     if (!nbr_valid) build_neighbors();
     if (!qeq_h_valid) {
-      hipEvent_t t0 = timer.begin();
+      hipEvent_t t0 = timer.begin(PH_QEQ_H);
       launch_qeq_build(st, cell, g, tab, qb, stream);
       timer.end(PH_QEQ_H, t0);
       qeq_h_valid = true;
@@ -693,17 +759,20 @@ struct Engine {
   void ensure_cg_cap(int imax) {
     if (imax + 2 > qeq_imax_cap) { qeq_imax_cap = imax + 2; alloc_qeq_doubles(); }
   }
+  // The first batch launches as many iterations as the previous solve needed (warm-started solves of consecutive steps need
+  // about the same number), later batches two at a time; iterations past convergence exit at once on the device flags.
+  int cg_guess = 8;
   void qeq_iterate(int imax, double tol) {
     int it = 1;
-    const int batch = 8;
+    int batch = std::max(1, cg_guess);
     std::vector<int> act(2 * (imax + 2));
     while (true) {
       int last = it;
       for (int b = 0; b < batch && it < imax; ++b, ++it) {
-        hipEvent_t t0 = timer.begin();
+        hipEvent_t t0 = timer.begin(PH_QEQ_SPMV);
         launch_qeq_spmv(st, qb, it, stream);
         timer.end(PH_QEQ_SPMV, t0);
-        t0 = timer.begin();
+        t0 = timer.begin(PH_QEQ_CG);
         launch_qeq_update(st, qb, it, imax, tol, stream);
         timer.end(PH_QEQ_CG, t0);
       }
@@ -712,6 +781,7 @@ struct Engine {
       HIPCHK(hipMemcpyAsync(a, qb.iscal + 2 * last, sizeof(a), hipMemcpyDeviceToHost, stream));
       sync();
       if ((!a[0] && !a[1]) || it >= imax) break;
+      batch = 2;
     }
     HIPCHK(hipMemcpy(act.data(), qb.iscal, 2 * (size_t)it * sizeof(int), hipMemcpyDeviceToHost));
     for (int s = 0; s < 2; ++s) {
@@ -722,6 +792,7 @@ struct Engine {
     std::vector<double> sc(8 + 2 * (size_t)(it + 1));
     HIPCHK(hipMemcpy(sc.data(), qb.scal, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int s = 0; s < 2; ++s) last_rel[s] = std::sqrt(sc[8 + 2 * (last_iters[s] - 1) + s]) / sc[s];
+    cg_guess = std::max(last_iters[0], last_iters[1]) - 1;
   }
 
   void qeq_charges(int imax, double tol, int strict) {
@@ -741,6 +812,14 @@ struct Engine {
           throw ApiError(RXA_ENOCONV, "fix qeq/reaxff/metal strict: CG did not converge after " + std::to_string(last_iters[s]) + " iterations");
   }
 
+  // the device charges in the caller's order
+  void charges_out(double* q) {
+    std::vector<double> qi(n);
+    sync();
+    HIPCHK(hipMemcpy(qi.data(), st.q, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) q[perm_h[i]] = qi[i];
+  }
+
   // ---------------------------------------------------------------------------------------------------------------
   void md_setup(const double* v, const double* mass_by_type, double dt, double tt, double tdamp, double sk, int qeq_every, int qimax, double qtol) {
     if (!n) throw ApiError(RXA_ESTATE, "no atoms");
@@ -757,8 +836,9 @@ struct Engine {
     HIPCHK(hipMemcpy(ty.data(), st.type, n * sizeof(int), hipMemcpyDeviceToHost));
     // masses by LAMMPS type: map through the host copy of the LAMMPS types kept at set_atoms
     for (int i = 0; i < n; ++i) {
-      vx[i] = v[3 * i]; vy[i] = v[3 * i + 1]; vz[i] = v[3 * i + 2];
-      m[i] = mass_by_type[lmp_types[i]];
+      const int u = perm_h[i];
+      vx[i] = v[3 * u]; vy[i] = v[3 * u + 1]; vz[i] = v[3 * u + 2];
+      m[i] = mass_by_type[lmp_types[u]];
       if (!(m[i] > 0)) throw ApiError(RXA_EINVAL, "non-positive mass");
     }
     HIPCHK(hipMemcpy(md.vx, vx.data(), n * sizeof(double), hipMemcpyHostToDevice));
@@ -790,7 +870,7 @@ struct Engine {
     long long reb0 = rebuilds;
     double disp = 0;
     for (int s = 0; s < steps; ++s) {
-      hipEvent_t t0 = timer.begin();
+      hipEvent_t t0 = timer.begin(PH_MD);
       launch_md_thermostat(st, md, stream);
       launch_md_half_kick(st, md, dtf, stream);
       launch_md_drift(st, md, md_dt, stream);
@@ -807,7 +887,7 @@ struct Engine {
       ++md_steps;
       if (md_qeq_every > 0 && md_steps % md_qeq_every == 0) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
       forces(false);
-      t0 = timer.begin();
+      t0 = timer.begin(PH_MD);
       launch_md_half_kick(st, md, dtf, stream);
       launch_md_thermostat(st, md, stream);
       timer.end(PH_MD, t0);
@@ -985,11 +1065,7 @@ int rxa_set_positions(rxa_context* c, const double* x) {
 int rxa_set_positions_device(rxa_context* c, const double* dx) {
   return guard(c, [&](Engine& e) {
     if (!e.n) throw ApiError(RXA_ESTATE, "no atoms");
-    const size_t n = (size_t)e.n;
-    HIPCHK(hipMemcpyAsync(e.st.x, dx, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
-    HIPCHK(hipMemcpyAsync(e.st.y, dx + n, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
-    HIPCHK(hipMemcpyAsync(e.st.z, dx + 2 * n, n * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
-    rxa::launch_pos4(e.st, e.stream);
+    rxa::launch_positions_in(e.st, dx, e.d_perm.p, e.stream);
     e.nbr_valid = false;
     e.qeq_h_valid = false;
   });
@@ -1002,15 +1078,15 @@ int rxa_stream(rxa_context* c, void** out) {
 int rxa_set_charges(rxa_context* c, const double* q) {
   return guard(c, [&](Engine& e) {
     if (!e.n) throw ApiError(RXA_ESTATE, "no atoms");
-    HIPCHK(hipMemcpy(e.st.q, q, e.n * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> qi(e.n);
+    for (int i = 0; i < e.n; ++i) qi[i] = q[e.perm_h[i]];
+    e.sync();
+    HIPCHK(hipMemcpy(e.st.q, qi.data(), e.n * sizeof(double), hipMemcpyHostToDevice));
   });
 }
 
 int rxa_get_charges(rxa_context* c, double* q) {
-  return guard(c, [&](Engine& e) {
-    e.sync();
-    HIPCHK(hipMemcpy(q, e.st.q, e.n * sizeof(double), hipMemcpyDeviceToHost));
-  });
+  return guard(c, [&](Engine& e) { e.charges_out(q); });
 }
 
 int rxa_compute(rxa_context* c, int flags, double* energies, double* grad, double* virial) {
@@ -1033,8 +1109,9 @@ int rxa_census(rxa_context* c, int* out) {
     HIPCHK(hipMemcpy(de.data(), e.st.Delta_e, n * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < 7 * n; ++i) out[i] = -1;
     for (int i = 0; i < n; ++i) {
-      out[i] = b[i];
-      out[6 * n + i] = (int)(de[i] / 2.0);
+      const int u = e.perm_h[i];
+      out[u] = b[i];
+      out[6 * n + u] = (int)(de[i] / 2.0);
     }
   });
 }
@@ -1055,15 +1132,18 @@ long rxa_bond_table(rxa_context* c, int32_t* start, int32_t* nbr, double* bo, do
     HIPCHK(hipMemcpy(bv.data(), e.st.bo, bv.size() * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(tb.data(), e.st.tbo, n * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(np.data(), e.st.nlp, n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int> inv(n);
+    for (int i = 0; i < n; ++i) inv[e.perm_h[i]] = i;
     long k = 0;
-    for (int i = 0; i < n; ++i) {
-      if (start) start[i] = (int32_t)k;
+    for (int u = 0; u < n; ++u) {   // rows and partners in the caller's order
+      const int i = inv[u];
+      if (start) start[u] = (int32_t)k;
       for (int p = 0; p < std::min(cnt[i], mb); ++p, ++k) {
-        nbr[k] = bj[(size_t)p * n + i];
+        nbr[k] = e.perm_h[bj[(size_t)p * n + i]];
         if (bo) bo[k] = bv[(size_t)p * n + i];
       }
-      if (total_bo) total_bo[i] = tb[i];
-      if (nlp) nlp[i] = np[i];
+      if (total_bo) total_bo[u] = tb[i];
+      if (nlp) nlp[u] = np[i];
     }
     if (start) start[n] = (int32_t)k;
   });
@@ -1077,11 +1157,14 @@ int rxa_qeq_setup(rxa_context* c, double swa, double swb, const double* chi, con
 int rxa_qeq_matvec(rxa_context* c, const double* x, double* y) {
   return guard(c, [&](Engine& e) {
     e.qeq_matrix();
-    const size_t n = (size_t)e.n;
-    HIPCHK(hipMemcpyAsync(e.qb.p, x, n * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    const int n = e.n;
+    std::vector<double> xi(n), yo(n);
+    for (int i = 0; i < n; ++i) xi[i] = x[e.perm_h[i]];
+    HIPCHK(hipMemcpyAsync(e.qb.p, xi.data(), n * sizeof(double), hipMemcpyHostToDevice, e.stream));
     rxa::launch_qeq_matvec1(e.st, e.qb, e.qb.p, e.qb.r, e.stream);
-    HIPCHK(hipMemcpyAsync(y, e.qb.r, n * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(hipMemcpyAsync(yo.data(), e.qb.r, n * sizeof(double), hipMemcpyDeviceToHost, e.stream));
     e.sync();
+    for (int i = 0; i < n; ++i) y[e.perm_h[i]] = yo[i];
   });
 }
 
@@ -1092,14 +1175,17 @@ int rxa_qeq_solve(rxa_context* c, const double* b, const double* x0, int imax, d
     e.qeq_matrix();
     const int n = e.n;
     std::vector<double> bi(2 * (size_t)n), xi(2 * (size_t)n);
-    for (int i = 0; i < n; ++i) { bi[2 * i] = b[i]; bi[2 * i + 1] = b[n + i]; xi[2 * i] = x0[i]; xi[2 * i + 1] = x0[n + i]; }
+    for (int i = 0; i < n; ++i) {
+      const int u = e.perm_h[i];
+      bi[2 * i] = b[u]; bi[2 * i + 1] = b[n + u]; xi[2 * i] = x0[u]; xi[2 * i + 1] = x0[n + u];
+    }
     HIPCHK(hipMemcpyAsync(e.qb.b, bi.data(), bi.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     HIPCHK(hipMemcpyAsync(e.qb.x, xi.data(), xi.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     rxa::launch_qeq_init(e.st, e.qb, imax, tol, 2, e.stream);
     e.qeq_iterate(imax, tol);
     std::vector<double> xo(2 * (size_t)n);
     HIPCHK(hipMemcpy(xo.data(), e.qb.x, xo.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) { x[i] = xo[2 * i]; x[n + i] = xo[2 * i + 1]; }
+    for (int i = 0; i < n; ++i) { x[e.perm_h[i]] = xo[2 * i]; x[n + e.perm_h[i]] = xo[2 * i + 1]; }
     for (int s = 0; s < 2; ++s) {
       iters[s] = e.last_iters[s];
       rel[s] = e.last_rel[s];
@@ -1111,10 +1197,7 @@ int rxa_qeq_solve(rxa_context* c, const double* b, const double* x0, int imax, d
 int rxa_qeq_charges(rxa_context* c, int imax, double tol, int strict, double* q_out, int* iters, double* rel) {
   return guard(c, [&](Engine& e) {
     e.qeq_charges(imax, tol, strict);
-    if (q_out) {
-      e.sync();
-      HIPCHK(hipMemcpy(q_out, e.st.q, e.n * sizeof(double), hipMemcpyDeviceToHost));
-    }
+    if (q_out) e.charges_out(q_out);
     if (iters) { iters[0] = e.last_iters[0]; iters[1] = e.last_iters[1]; }
     if (rel) { rel[0] = e.last_rel[0]; rel[1] = e.last_rel[1]; }
   });
@@ -1138,20 +1221,21 @@ int rxa_get_state(rxa_context* c, double* x, double* v, double* q) {
     e.sync();
     const int n = e.n;
     std::vector<double> a(n), b(n), d(n);
+    const std::vector<int>& P = e.perm_h;
     if (x) {
       HIPCHK(hipMemcpy(a.data(), e.st.x, n * sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(b.data(), e.st.y, n * sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(d.data(), e.st.z, n * sizeof(double), hipMemcpyDeviceToHost));
-      for (int i = 0; i < n; ++i) { x[3 * i] = a[i]; x[3 * i + 1] = b[i]; x[3 * i + 2] = d[i]; }
+      for (int i = 0; i < n; ++i) { x[3 * P[i]] = a[i]; x[3 * P[i] + 1] = b[i]; x[3 * P[i] + 2] = d[i]; }
     }
     if (v) {
       if (!e.md_ready) throw ApiError(RXA_ESTATE, "no velocities (rxa_md_setup)");
       HIPCHK(hipMemcpy(a.data(), e.md.vx, n * sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(b.data(), e.md.vy, n * sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(d.data(), e.md.vz, n * sizeof(double), hipMemcpyDeviceToHost));
-      for (int i = 0; i < n; ++i) { v[3 * i] = a[i]; v[3 * i + 1] = b[i]; v[3 * i + 2] = d[i]; }
+      for (int i = 0; i < n; ++i) { v[3 * P[i]] = a[i]; v[3 * P[i] + 1] = b[i]; v[3 * P[i] + 2] = d[i]; }
     }
-    if (q) HIPCHK(hipMemcpy(q, e.st.q, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (q) e.charges_out(q);
   });
 }
 
@@ -1205,6 +1289,7 @@ int rxa_profile(rxa_context* c, int enable) {
   return guard(c, [&](Engine& e) {
     e.sync();
     e.timer.on = enable != 0;
+    e.timer.mask = enable == 1 ? ((1u << rxa::PH_COUNT) - 1u) : (unsigned)enable;
     e.timer.reset();
   });
 }

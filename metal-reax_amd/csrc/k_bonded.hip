@@ -295,6 +295,7 @@ __global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables 
   }
   st.tbo[i] = (double)tbo;
   st.CdD_self[i] = 0; st.fs_x[i] = 0; st.fs_y[i] = 0; st.fs_z[i] = 0;
+  for (int k = 0; k < 6; ++k) st.vb[(size_t)k * n + i] = 0;
   if (ti < 0) return;
   const DevSpecies s = t.sp[ti];
   // atom quantities (reference terms.hpp:141-164); trunc toward zero
@@ -580,6 +581,10 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
   }
   e_ang = gsum<G>(e_ang); e_pen = gsum<G>(e_pen); e_coa = gsum<G>(e_coa); cdself = gsum<G>(cdself);
   fjx = gsum<G>(fjx); fjy = gsum<G>(fjy); fjz = gsum<G>(fjz); sCE5 = gsum<G>(sCE5); sCE6 = gsum<G>(sCE6);
+  // virial about the centre j: - sum over j's slots of (x_slot - x_j) (x) gradient of the slot atom
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  if (mine) vir_add(w, -1.0, st.b_dx[qi], st.b_dy[qi], st.b_dz[qi], own_fx, own_fy, own_fz);
+  for (int c = 0; c < 6; ++c) w[c] = gsum<G>(w[c]);
   if (mine) {
     const TR bo = BOij, c3 = bo * bo * bo;
     st.Cdbo[qi] += own_cdbo + (double)((TR)sCE6 * (c3 * c3 * bo));
@@ -591,6 +596,7 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
   if (live && p == 0) {
     st.CdD_self[j] += cdself;
     st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
+    for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + j] += w[c];
     st.eatom[T_ANG * n + j] = e_ang;
     st.eatom[T_PEN * n + j] = e_pen;
     st.eatom[T_COA * n + j] = e_coa;
@@ -620,6 +626,7 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
   const int nbw = wave_max(nbj);
   double e_tor = 0, e_con = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0;
   double own_cdbo = 0, own_fx = 0, own_fy = 0, own_fz = 0;
+  double w[6] = {0, 0, 0, 0, 0, 0};   // this end's half of the torsion virial, about the midpoint of the central bond
   const TR thb_cut = (TR)g.thb_cut;
   const TR p_tor2 = (TR)g.p_tor2, p_tor3 = (TR)g.p_tor3, p_tor4 = (TR)g.p_tor4, p_cot2 = (TR)g.p_cot2;
   const TR MINS = TR(1e-10);
@@ -734,10 +741,16 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
           cdself += (double)CT3;
           cdbo_ij += (double)(CT4 + CC1);
           const TR a7 = CT7 + CC4, a8 = CT8 + CC5, a9 = CT9 + CC6;
-          fix += (double)(a7 * dci.x + a9 * doi.x); fiy += (double)(a7 * dci.y + a9 * doi.y); fiz += (double)(a7 * dci.z + a9 * doi.z);
-          fjx += (double)(a7 * dcj.x + a8 * ecj.x + a9 * doj.x);
-          fjy += (double)(a7 * dcj.y + a8 * ecj.y + a9 * doj.y);
-          fjz += (double)(a7 * dcj.z + a8 * ecj.z + a9 * doj.z);
+          const double gix = (double)(a7 * dci.x + a9 * doi.x), giy = (double)(a7 * dci.y + a9 * doi.y), giz = (double)(a7 * dci.z + a9 * doi.z);
+          const double gjx = (double)(a7 * dcj.x + a8 * ecj.x + a9 * doj.x), gjy = (double)(a7 * dcj.y + a8 * ecj.y + a9 * doj.y),
+                       gjz = (double)(a7 * dcj.z + a8 * ecj.z + a9 * doj.z);
+          fix += gix; fiy += giy; fiz += giz;
+          fjx += gjx; fjy += gjy; fjz += gjz;
+          {   // x_i - m = d_ji - d_jk / 2, x_j - m = -d_jk / 2 (m: midpoint of the central bond)
+            const double hx = 0.5 * (double)djk.x, hy = 0.5 * (double)djk.y, hz = 0.5 * (double)djk.z;
+            vir_add(w, -1.0, (double)dji.x - hx, (double)dji.y - hy, (double)dji.z - hz, gix, giy, giz);
+            vir_add(w, -1.0, -hx, -hy, -hz, gjx, gjy, gjz);
+          }
           if (counted) {   // energy and the central bond once per torsion
             e_tor += (double)(fn10 * sijk * sjkl * CV);
             e_con += (double)(p_cot1 * fn12 * gg);
@@ -752,6 +765,7 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
   }
   e_tor = gsum<G>(e_tor); e_con = gsum<G>(e_con); cdself = gsum<G>(cdself);
   fjx = gsum<G>(fjx); fjy = gsum<G>(fjy); fjz = gsum<G>(fjz);
+  for (int c = 0; c < 6; ++c) w[c] = gsum<G>(w[c]);
   if (live && tj >= 0 && pk < nbj) {
     const size_t q = sl(pk, j, n);
     st.Cdbo[q] += own_cdbo + cdbo_jk;
@@ -761,6 +775,7 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
   if (live && pk == 0) {
     st.CdD_self[j] += cdself;
     st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
+    for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + j] += w[c];
     st.eatom[T_TOR * n + j] = e_tor;
     st.eatom[T_CON * n + j] = e_con;
   }
@@ -817,6 +832,7 @@ __global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables
   const int nb = min(st.b_cnt[j], st.maxb);
   TR* hf = reinterpret_cast<TR*>(st.hf);
   double e_hb = 0, fjx = 0, fjy = 0, fjz = 0;
+  double w[6] = {0, 0, 0, 0, 0, 0};   // virial of the triples about the donor
   int inrange = 0;
   for (int h0 = 0; h0 < nc; h0 += 64) {
     const int h = h0 + lane;
@@ -855,9 +871,11 @@ __global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables
           fjx += (double)(CH2 * dtj.x - c3 * djk.x);
           fjy += (double)(CH2 * dtj.y - c3 * djk.y);
           fjz += (double)(CH2 * dtj.z - c3 * djk.z);
-          fkx += (double)(CH2 * dtk.x + c3 * djk.x);
-          fky += (double)(CH2 * dtk.y + c3 * djk.y);
-          fkz += (double)(CH2 * dtk.z + c3 * djk.z);
+          const double gkx = (double)(CH2 * dtk.x + c3 * djk.x), gky = (double)(CH2 * dtk.y + c3 * djk.y),
+                       gkz = (double)(CH2 * dtk.z + c3 * djk.z);
+          fkx += gkx; fky += gky; fkz += gkz;
+          vir_add(w, -1.0, dx, dy, dz, gkx, gky, gkz);
+          vir_add(w, -1.0, (double)dji.x, (double)dji.y, (double)dji.z, fx, fy, fz);
         }
       }
       c1 = wave_sum(c1); fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
@@ -869,9 +887,11 @@ __global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables
     }
   }
   e_hb = wave_sum(e_hb); fjx = wave_sum(fjx); fjy = wave_sum(fjy); fjz = wave_sum(fjz);
+  for (int c = 0; c < 6; ++c) w[c] = wave_sum(w[c]);
   for (int off = 32; off > 0; off >>= 1) inrange += __shfl_xor(inrange, off, 64);
   if (lane == 0) {
     st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
+    for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + j] += w[c];
     st.eatom[T_HB * n + j] = e_hb;
     st.h_cnt[j] = inrange;
   }
@@ -918,6 +938,7 @@ __global__ void __launch_bounds__(128) k_dbond(State st) {
   if (i >= st.n) return;
   const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
   double S = 0, Dx = 0, Dy = 0, Dz = 0;
+  double w[6] = {0, 0, 0, 0, 0, 0};
   const double cdi = st.CdD[i];
   for (int p = 0; p < nb; ++p) {
     const size_t q = sl(p, i, n);
@@ -930,7 +951,10 @@ __global__ void __launch_bounds__(128) k_dbond(State st) {
     S += (st.C[5][q] * cbo + st.C[5][q] * cdel) + st.C[6][q] * cpi + st.C[7][q] * cpi2;
     const double sc = T1 * st.dbop_c[q] + (C1pi * cpi) * st.dlnpi_c[q] + (C1pi2 * cpi2) * st.dlnpi2_c[q];
     Dx += sc * st.b_dx[q]; Dy += sc * st.b_dy[q]; Dz += sc * st.b_dz[q];
+    // a central pair force: gradient c d on this end, virial d (x) c d, half at each end
+    vir_add(w, 0.5 * sc, st.b_dx[q], st.b_dy[q], st.b_dz[q], st.b_dx[q], st.b_dy[q], st.b_dz[q]);
   }
+  for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + i] += w[c];
   st.S[i] = S;
   st.Dx[i] = Dx; st.Dy[i] = Dy; st.Dz[i] = Dz;
 }
@@ -941,6 +965,7 @@ __global__ void __launch_bounds__(128) k_gather(State st) {
   if (i >= st.n) return;
   const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
   const double Si = st.S[i];
+  double w[6] = {0, 0, 0, 0, 0, 0};
   double gx = st.fs_x[i] + st.Dx[i] + Si * st.dDps_x[i];
   double gy = st.fs_y[i] + st.Dy[i] + Si * st.dDps_y[i];
   double gz = st.fs_z[i] + st.Dz[i] + Si * st.dDps_z[i];
@@ -949,12 +974,14 @@ __global__ void __launch_bounds__(128) k_gather(State st) {
     const int j = st.b_j[q];
     const size_t m = sl(st.b_sym[q], j, n);
     gx += st.fpx[m]; gy += st.fpy[m]; gz += st.fpz[m];
-    const double w = st.S[j] * st.dbop_c[q];
-    gx += w * st.b_dx[q]; gy += w * st.b_dy[q]; gz += w * st.b_dz[q];
+    const double c = st.S[j] * st.dbop_c[q];
+    gx += c * st.b_dx[q]; gy += c * st.b_dy[q]; gz += c * st.b_dz[q];
+    vir_add(w, 0.5 * (c + Si * st.dbop_c[q]), st.b_dx[q], st.b_dy[q], st.b_dz[q], st.b_dx[q], st.b_dy[q], st.b_dz[q]);
   }
   st.gx[i] = gx + st.gnb_x[i];
   st.gy[i] = gy + st.gnb_y[i];
   st.gz[i] = gz + st.gnb_z[i];
+  for (int c = 0; c < 6; ++c) st.vatom[(size_t)c * n + i] += st.vb[(size_t)c * n + i] + w[c];
 }
 
 inline int nblk(long long items, int per) { return (int)((items + per - 1) / per); }

@@ -225,11 +225,60 @@ __global__ void __launch_bounds__(256) k_hc_mirror(State st, Globals g, Tables t
     const size_t jb = (size_t)j * st.maxhc;
     const int ncj = min(st.hc_cnt[j], st.maxhc);
     int found = -1;
-    for (int q = 0; q < ncj; ++q)
-      if (st.hc_j[jb + q] == k && entry_code(st.hc_s[jb + q]) == want) { found = q; break; }
+    // eight independent loads in flight per step of the scan
+    for (int q0 = 0; q0 < ncj && found < 0; q0 += 8) {
+      int hj[8], hs[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = min(q0 + u, ncj - 1);
+        hj[u] = st.hc_j[jb + q];
+        hs[u] = st.hc_s[jb + q];
+      }
+#pragma unroll
+      for (int u = 7; u >= 0; --u)
+        if (q0 + u < ncj && hj[u] == k && entry_code(hs[u]) == want) found = q0 + u;
+    }
     st.hc_m[kb + h] = found < 0 ? 0 : found;
     if (found < 0) st.flags[6] = 1;
   }
+}
+
+// Gather of every persistent per-atom array into a new atom order: new[i] = old[src[i]] (through a scratch area, one kernel to
+// gather, one to copy back). Used to keep atoms in cell order (spatial locality of every neighbour gather) and to carry state
+// across a change of the caller's atom order.
+__global__ void k_permute_gather(PermuteSet ps, const int* src, double* sd, int* si, long long* sl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ps.n) return;
+  const int j = src[i];
+  for (int k = 0; k < ps.nd; ++k) sd[(size_t)k * ps.n + i] = ps.d[k][j];
+  for (int k = 0; k < ps.ni; ++k) si[(size_t)k * ps.n + i] = ps.i[k][j];
+  for (int k = 0; k < ps.nl; ++k) sl[(size_t)k * ps.n + i] = ps.l[k][j];
+}
+
+__global__ void k_permute_scatter(PermuteSet ps, const double* sd, const int* si, const long long* sl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ps.n) return;
+  for (int k = 0; k < ps.nd; ++k) ps.d[k][i] = sd[(size_t)k * ps.n + i];
+  for (int k = 0; k < ps.ni; ++k) ps.i[k][i] = si[(size_t)k * ps.n + i];
+  for (int k = 0; k < ps.nl; ++k) ps.l[k][i] = sl[(size_t)k * ps.n + i];
+}
+
+// after the atoms were put in the order of their cells: atom p sits at sorted position p
+__global__ void k_bin_identity(int n, const unsigned* skeys, int* sidx, unsigned* atom_key) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  sidx[p] = p;
+  atom_key[p] = skeys[p];
+}
+
+// positions from a caller's device arrays (x[n], y[n], z[n] in the caller's order) into the engine's order
+__global__ void k_positions_in(State st, const double* ux, const int* perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int u = perm[i];
+  const double x = ux[u], y = ux[st.n + u], z = ux[2 * (size_t)st.n + u];
+  st.x[i] = x; st.y[i] = y; st.z[i] = z;
+  st.pos[i] = make_double4(x, y, z, (double)st.type[i]);
 }
 
 inline int blocks_for(long long items, int per_block) { return (int)((items + per_block - 1) / per_block); }
@@ -239,6 +288,20 @@ inline int blocks_for(long long items, int per_block) { return (int)((items + pe
 void launch_wrap(const State& st, const Cell& c, hipStream_t s) {
   if (st.n == 0) return;
   hipLaunchKernelGGL(k_wrap, dim3(blocks_for(st.n, 256)), dim3(256), 0, s, st, c);
+}
+
+void launch_permute(const PermuteSet& ps, const int* src, double* sd, int* si, long long* sl, hipStream_t s) {
+  if (ps.n == 0) return;
+  hipLaunchKernelGGL(k_permute_gather, dim3(blocks_for(ps.n, 256)), dim3(256), 0, s, ps, src, sd, si, sl);
+  hipLaunchKernelGGL(k_permute_scatter, dim3(blocks_for(ps.n, 256)), dim3(256), 0, s, ps, sd, si, sl);
+}
+
+void launch_bin_identity(int n, BinBuffers& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_identity, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, b.skeys, b.sidx, b.atom_key);
+}
+
+void launch_positions_in(const State& st, const double* ux, const int* perm, hipStream_t s) {
+  hipLaunchKernelGGL(k_positions_in, dim3(blocks_for(st.n, 256)), dim3(256), 0, s, st, ux, perm);
 }
 
 void launch_pos4(const State& st, hipStream_t s) {

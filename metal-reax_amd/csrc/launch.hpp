@@ -27,6 +27,15 @@ void launch_bin(const State& st, const Cell& c, const BinGrid& g, BinBuffers& b,
 void launch_nbr_cells(const State& st, const Cell& c, const BinGrid& g, const BinBuffers& b, double rs, hipStream_t s);
 void launch_nbr_brute(const State& st, const Cell& c, const int sm[3], double rs, hipStream_t s);
 void launch_pos4(const State& st, hipStream_t s);                                   // pos = (x, y, z, type)
+struct PermuteSet {   // per-atom arrays moved together by launch_permute
+  int n = 0, nd = 0, ni = 0, nl = 0;
+  double* d[24];
+  int* i[4];
+  long long* l[2];
+};
+void launch_permute(const PermuteSet& ps, const int* src, double* sd, int* si, long long* sl, hipStream_t s);
+void launch_bin_identity(int n, BinBuffers& b, hipStream_t s);
+void launch_positions_in(const State& st, const double* ux, const int* perm, hipStream_t s);
 void launch_shift_table(const Cell& c, double4* shv, const int smx[3], hipStream_t s);
 void launch_sublists(const State& st, const Globals& g, const Tables& t, double rb, double rh, hipStream_t s);
 

@@ -292,8 +292,14 @@ class Engine:
     PHASES = ("neighbor", "bond_lists", "bo_correct", "bonded_terms", "nonbonded", "dbond_gather", "qeq_matrix", "qeq_spmv", "qeq_cg",
               "integrate")
 
-    def profile(self, enable=True):
-        self._chk(lib().rxa_profile(self.h, int(enable)))
+    def profile(self, enable=True, phases=None):
+        """Device timing of the engine's phases (HIP events on its stream): off, every phase, or only the named phases."""
+        mask = 0
+        if enable:
+            mask = 1 if phases is None else sum(1 << self.PHASES.index(p) for p in phases)
+            if mask == 1 and phases is not None:   # only phase 0: bit 0 alone would read as "every phase"
+                mask = 1 | (1 << 31)
+        self._chk(lib().rxa_profile(self.h, int(np.int32(np.uint32(mask)))))
 
     def profile_read(self):
         ms = np.zeros(10)

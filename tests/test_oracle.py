@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from helpers import case_inputs
+from helpers import VOIGT, case_inputs, strain_virial
 from oracle.oracle import Oracle, TERMS
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -168,3 +168,17 @@ def test_oracle_matches_live_reference_build(golden, ffield_text, tmp_path):
     r = Oracle(ffield_text).compute(x, types - 1, tags, box12, per, q=q)
     for k in TERMS:
         assert abs(r["energy"][k] - ref[k]) <= 1e-13 * max(1.0, abs(ref[k]))
+
+
+def test_strain_virial_of_an_open_molecule(golden, ffield_text):
+    """The strain derivative used to pin the device virial, checked where the virial has a second closed form: for a molecule
+    with no periodic images W = -sum_i x_i (x) dE/dx_i (origin-independent because the forces sum to zero)."""
+    O = Oracle(ffield_text)
+    for cid in ("cho_methanol_pert", "cho_benzene_pert"):
+        c = next(k for k in golden["cases"] if k["id"] == cid)
+        x, types, tags, lo, hi, tilt, per, q, box12 = case_inputs(c)
+        assert not any(per)
+        g = O.compute(x, types - 1, tags, box12, per, q=q)["grad"]
+        direct = np.array([-(x[:, a] * g[:, b]).sum() for a, b in VOIGT])
+        fd = strain_virial(O, x, types - 1, tags, box12, per, q)
+        assert np.abs(fd - direct).max() <= 1e-5 * np.abs(direct).max() + 1e-5, (cid, fd, direct)
