@@ -31,9 +31,9 @@ def model_bytes_per_atom_step(n_f, R, k, B, n_h, f_don):
 
 
 def read_traffic():
-    """HBM bytes per SpMV launch measured by the committed rocprofv3 PMC runs (FETCH_SIZE and WRITE_SIZE of k_spmv2, each
+    """HBM bytes per product launch measured by the committed rocprofv3 PMC runs (FETCH_SIZE and WRITE_SIZE of k_cg_mvdir, each
     scaled by the same counter's reading for a plain copy of known size), or None when no summary is present."""
-    path = os.path.join(REPO, "profiles", "k_spmv2_water24000_pmc.md")
+    path = os.path.join(REPO, "profiles", "k_cg_mvdir_water24000_pmc.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--nside", type=int, default=20, help="water molecules per box edge (20 -> 24,000 atoms)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
-                    help="also run a plain device copy of MIB MiB (10 times) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
+                    help="also run a streaming kernel (MIB MiB in, MIB MiB out, 10 times) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (systems shard across GPUs without "
@@ -130,8 +130,9 @@ def main():
     n_h = st["hbond_entries"] / max(1, (w.types == 2).sum())        # acceptors within hbond_cut per donor hydrogen
     bytes_step = model_bytes_per_atom_step(n_f, R, k, B, n_h, (w.types == 2).mean())
 
-    # roofline of the charge-matrix SpMV (the kernel with the largest device time): algorithmic bytes of one fused two-vector
-    # pass over the half matrix (4-byte value + 4-byte column per entry) / mean launch duration from the HIP events
+    # roofline of the charge-matrix product k_cg_mvdir (the kernel with the largest device time; it also forms the CG direction):
+    # algorithmic bytes of one fused two-vector pass over the half matrix (4-byte value + 4-byte column per entry) / mean launch
+    # duration from the HIP events around each launch
     alg = 8 * n_f * n
     avg_s = spmv_ms / max(1, spmv_cnt) * 1e-3
     achieved = alg / avg_s / 1e9
@@ -152,7 +153,7 @@ def main():
         "data": "synthetic: seeded 8000-H2O lattice box at 1.0 g/cm^3, Maxwell-Boltzmann 300 K; stand-in C/H/O ffield",
         "config": {"workload": "water_24000_nvt_qeq", "atoms": n, "box_A": float(w.hi[0]), "precision": args.precision,
                    "dt_fs": 0.25, "qeq_tol": 1e-6, "skin_A": 2.0},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv2", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": "k_cg_mvdir", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg, "avg_launch_us": avg_s * 1e6, "launches": spmv_cnt},
         "step_model": {"bytes_per_atom_step": bytes_step, "n_f": n_f, "R": R, "k": k, "B": B, "n_h": n_h,
@@ -161,13 +162,8 @@ def main():
         "phase_ms_per_step": phase_ms,
         "thermo": thermo,
     }
-    if args.calib_copy:
-        import torch
-        a = torch.empty(args.calib_copy << 20, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(10):
-            b.copy_(a)
-        torch.cuda.synchronize()
+    if args.calib_copy:   # a plain streaming kernel (read MIB MiB, write MIB MiB) to calibrate the PMC byte counters
+        e.stream_copy(args.calib_copy << 20, reps=10)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (FP64 oracle, one core)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(ffield_text, w)

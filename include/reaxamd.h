@@ -138,6 +138,9 @@ int rxa_profile_read(rxa_context* ctx, double* ms, int64_t* counts);
 /* ---- device-resident inputs (bench / serving): pointers to device memory owned by the caller (HIP, same device) */
 int rxa_set_positions_device(rxa_context* ctx, const double* dx);
 int rxa_stream(rxa_context* ctx, void** hip_stream_out);
+/* profiling aid (no counterpart in the reference): `reps` launches of a plain copy kernel over `bytes` (a multiple of 16) between
+ * two scratch buffers on the context's stream, so memory counters can be calibrated against a kernel of known traffic */
+int rxa_stream_copy(rxa_context* ctx, int64_t bytes, int reps);
 
 #ifdef __cplusplus
 }

@@ -97,13 +97,14 @@ struct State {
   // skin neighbour list (atom-major: i * maxnb + k), full (both directions); nb_s = shift code | kCanon if i is the canonical end
   int *nb_j, *nb_s, *nb_cnt;
   // sub-lists of the skin list written at every rebuild (atom-major, same entry format): bond candidates (r <= bond_cut + skin)
-  // and H-bond partner candidates of donors / acceptors (r <= hbond_cut + skin)
+  // and the acceptor candidates of H-bond donors (r <= hbond_cut + skin). An acceptor's row of hc_j instead holds, in ascending
+  // order, the flat donor slots (j * maxhc + h) that list it; hc_m is the unsorted scratch of that transpose.
   int maxbc, maxhc;
   int *bc_j, *bc_s, *bc_cnt, *hc_j, *hc_s, *hc_cnt;
-  int* hc_m;                  // acceptors: for each candidate donor j, the position of this acceptor in j's candidate list
+  int* hc_m;
   double* hf;                 // donors: force on the acceptor of each candidate slot (3 per slot, TR-typed)
   int* flags;                 // [0] neighbour overflow, [1] bond overflow, [2] H-bond overflow, [3] missing mirror bond,
-                              // [4] bond-candidate overflow, [5] H-bond-candidate overflow, [6] missing H-bond mirror
+                              // [4] bond-candidate overflow, [5] H-bond-candidate overflow
   // bond slots
   int *b_cnt, *b_j, *b_s, *b_sym;
   double *b_d, *b_dx, *b_dy, *b_dz;

@@ -173,6 +173,7 @@ struct Engine {
   DBuf<int> q_int;
   DBuf<double> q_dbl;
   DBuf<float> q_f32;
+  DBuf<unsigned short> q_u16;
   bool qeq_ready = false, qeq_h_valid = false;
   int qeq_imax_cap = 0;
   int hist_count = 0;
@@ -304,6 +305,7 @@ struct Engine {
     const std::string p = ps.effective_precision();
     prec = p == "fp64" ? Precision::FP64 : (p == "fp32" ? Precision::FP32 : Precision::MIXED);
     qb.fp32 = prec == Precision::FP64 ? 0 : 1;   // the charge solve multiplies with FP32 matrix values outside FP64 mode
+    qeq_h_valid = false;
   }
 
   double list_cut() const { return std::max({g.nonb_cut, g.hbond_cut, g.bond_cut, qeq_ready ? q_swb : 0.0}); }
@@ -583,7 +585,6 @@ struct Engine {
       }
       need_full = false;
       if (flags[4] <= maxbc && flags[5] <= maxhc) {
-        if (flags[6]) throw ApiError(RXA_EDEVICE, "internal: an H-bond candidate has no mirror entry");
         nbr_valid = true;
         ++rebuilds;
         qeq_h_valid = false;
@@ -658,9 +659,11 @@ struct Engine {
 
   // ---------------------------------------------------------------------------------------------------------------
   void alloc_qeq_matrix() {
-    q_int.alloc((size_t)n * maxnb + n);
+    qb.narrow = n <= 65535 ? 1 : 0;
+    q_int.alloc(qb.narrow ? (size_t)n : (size_t)n * maxnb + n);
     qb.cnt = q_int.p;
-    qb.col = q_int.p + n;
+    qb.col = qb.narrow ? nullptr : q_int.p + n;
+    if (qb.narrow) { q_u16.alloc((size_t)n * maxnb); qb.col16 = q_u16.p; } else { q_u16.release(); qb.col16 = nullptr; }
     q_f32.alloc((size_t)n * maxnb);
     qb.val32 = q_f32.p;
     // val shares the double pool
@@ -763,22 +766,26 @@ struct Engine {
   // about the same number), later batches two at a time; iterations past convergence exit at once on the device flags.
   int cg_guess = 8;
   void qeq_iterate(int imax, double tol) {
+    // launch order: mvdir(1) [update(1) mvdir(2)] [update(2) mvdir(3)] ...; mvdir(it) decides whether iteration it runs, so after
+    // a batch ending with mvdir(it) the flags of iteration it tell the host whether to go on
     int it = 1;
     int batch = std::max(1, cg_guess);
     std::vector<int> act(2 * (imax + 2));
+    hipEvent_t t0 = timer.begin(PH_QEQ_SPMV);
+    launch_qeq_mvdir(st, qb, 1, imax, tol, stream);
+    timer.end(PH_QEQ_SPMV, t0);
     while (true) {
-      int last = it;
-      for (int b = 0; b < batch && it < imax; ++b, ++it) {
-        hipEvent_t t0 = timer.begin(PH_QEQ_SPMV);
-        launch_qeq_spmv(st, qb, it, stream);
-        timer.end(PH_QEQ_SPMV, t0);
+      for (int b = 0; b < batch && it < imax; ++b) {
         t0 = timer.begin(PH_QEQ_CG);
-        launch_qeq_update(st, qb, it, imax, tol, stream);
+        launch_qeq_update(st, qb, it, stream);
         timer.end(PH_QEQ_CG, t0);
+        ++it;
+        t0 = timer.begin(PH_QEQ_SPMV);
+        launch_qeq_mvdir(st, qb, it, imax, tol, stream);
+        timer.end(PH_QEQ_SPMV, t0);
       }
-      last = it - 1;
       int a[2];
-      HIPCHK(hipMemcpyAsync(a, qb.iscal + 2 * last, sizeof(a), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(a, qb.iscal + 2 * (it - 1), sizeof(a), hipMemcpyDeviceToHost, stream));
       sync();
       if ((!a[0] && !a[1]) || it >= imax) break;
       batch = 2;
@@ -1073,6 +1080,18 @@ int rxa_set_positions_device(rxa_context* c, const double* dx) {
 
 int rxa_stream(rxa_context* c, void** out) {
   return guard(c, [&](Engine& e) { *out = (void*)e.stream; });
+}
+
+int rxa_stream_copy(rxa_context* c, int64_t bytes, int reps) {
+  return guard(c, [&](Engine& e) {
+    if (bytes <= 0 || bytes % 16 || reps < 1) throw ApiError(RXA_EINVAL, "stream copy: bytes must be a positive multiple of 16");
+    rxa::DBuf<char> a, b;
+    a.alloc((size_t)bytes);
+    b.alloc((size_t)bytes);
+    HIPCHK(hipMemsetAsync(a.p, 0, (size_t)bytes, e.stream));
+    for (int r = 0; r < reps; ++r) rxa::launch_stream_copy(a.p, b.p, (size_t)bytes, e.stream);
+    e.sync();
+  });
 }
 
 int rxa_set_charges(rxa_context* c, const double* q) {

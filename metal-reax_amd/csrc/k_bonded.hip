@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables 
   st.CdD_self[i] = 0; st.fs_x[i] = 0; st.fs_y[i] = 0; st.fs_z[i] = 0;
   for (int k = 0; k < 6; ++k) st.vb[(size_t)k * n + i] = 0;
   if (ti < 0) return;
-  const DevSpecies s = t.sp[ti];
+  const DevSpecies& s = t.sp[ti];
   // atom quantities (reference terms.hpp:141-164); trunc toward zero
   const TB Delta = tbo - (TB)s.valency, De = tbo - (TB)s.valency_e;
   const TB ht = (TB)(int)(De / TB(2));
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
   const int n = st.n, ti = st.type[i];
   double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, cdself = 0;
   if (ti >= 0) {
-    const DevSpecies si = t.sp[ti];
+    const DevSpecies& si = t.sp[ti];
     const int nb = min(st.b_cnt[i], st.maxb);
     // bond energy of the bonds whose canonical end is this atom (the reference's tag order, bonded.cpp:222-252)
     for (int p = 0; p < nb; ++p) {
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
   const TR p_pen2 = (TR)g.p_pen2, p_pen3 = (TR)g.p_pen3, p_pen4 = (TR)g.p_pen4;
   const TR p_coa2 = (TR)g.p_coa2, p_coa3 = (TR)g.p_coa3, p_coa4 = (TR)g.p_coa4;
   if (act) {
-    const DevSpecies sj = t.sp[tj];
+    const DevSpecies& sj = t.sp[tj];
     p_val3 = (TR)sj.p_val3; p_val5 = (TR)sj.p_val5;
     TR SBOp = 0, prod = 1;
     for (int b = 0; b < nb; ++b) {
@@ -897,8 +897,8 @@ __global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables
   }
 }
 
-// Acceptor side: one wave per acceptor k sums the forces its donors left in its slots of their candidate lists (hc_m holds,
-// for each of k's candidates j, the position of k in j's list; fixed between neighbour rebuilds), in a fixed order.
+// Acceptor side: one wave per acceptor k sums the forces its donors left in k's slots of their candidate lists (k's row of hc_j
+// lists those slots in ascending order; fixed between neighbour rebuilds).
 template <class TR>
 __global__ void __launch_bounds__(256) k_hbond_acceptor(State st, Globals g, Tables t) {
   const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -911,7 +911,7 @@ __global__ void __launch_bounds__(256) k_hbond_acceptor(State st, Globals g, Tab
   const TR* hf = reinterpret_cast<const TR*>(st.hf);
   double fx = 0, fy = 0, fz = 0;
   for (int h = lane; h < nc; h += 64) {
-    const size_t o = 3 * ((size_t)st.hc_j[hbase + h] * st.maxhc + st.hc_m[hbase + h]);
+    const size_t o = 3 * (size_t)st.hc_j[hbase + h];   // flat donor slot
     fx += (double)hf[o]; fy += (double)hf[o + 1]; fz += (double)hf[o + 2];
   }
   fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);

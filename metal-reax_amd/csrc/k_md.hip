@@ -203,4 +203,14 @@ void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s) {
   hipLaunchKernelGGL(k_disp_final, dim3(1), dim3(kRT), 0, s, md);
 }
 
+// plain streaming copy (16-byte elements, grid-stride): a kernel of known traffic for calibrating memory counters
+__global__ void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t m) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (size_t)gridDim.x * blockDim.x) dst[k] = src[k];
+}
+
+void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const size_t m = bytes / 16;
+  hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, s, (const double2*)src, (double2*)dst, m);
+}
+
 }  // namespace rxa

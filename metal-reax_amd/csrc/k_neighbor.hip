@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_nbr_brute(State st, Cell c, int sm0, in
 }
 
 // Bond and H-bond candidate sub-lists of the skin list (entry order kept): candidates within bond_cut + skin of every typed atom,
-// and for H-bond donors (p_hbond 1) / acceptors (2) the partners of the other role within hbond_cut + skin. Per-step kernels scan
+// and for H-bond donors (p_hbond 1) the acceptors (p_hbond 2) within hbond_cut + skin. Per-step kernels scan
 // these instead of the whole skin list and apply the exact cut-offs to the current distance.
 __global__ void __launch_bounds__(256) k_sublists(State st, Globals g, Tables t, double rb2, double rh2) {
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -184,10 +184,7 @@ __global__ void __launch_bounds__(256) k_sublists(State st, Globals g, Tables t,
           double dx, dy, dz;
           const double d2 = pair_dvec_list(st, pi, j, e, dx, dy, dz);
           bk = d2 <= rb2;
-          if (role && d2 <= rh2) {
-            const int rj = t.sp[tj].p_hbond;
-            hk = (role == 1 && rj == 2) || (role == 2 && rj == 1);
-          }
+          if (role == 1 && d2 <= rh2) hk = t.sp[tj].p_hbond == 2;   // donors list acceptors; acceptor rows come from the transpose
         }
       }
       const unsigned long long mb = __ballot(bk), mh = __ballot(hk);
@@ -211,35 +208,36 @@ __global__ void __launch_bounds__(256) k_sublists(State st, Globals g, Tables t,
   }
 }
 
-// For every candidate donor j of an acceptor k, the position of k in j's candidate list (the pair is listed from both ends
-// with the same canonical distance and cut-off, so it is always found unless a list overflowed).
-__global__ void __launch_bounds__(256) k_hc_mirror(State st, Globals g, Tables t) {
+// Acceptor lists as the transpose of the donor lists: every donor candidate slot (flat index j * maxhc + h) is appended to its
+// acceptor's row in arrival order (an integer atomic claims the position), then each row is put in ascending slot order by
+// rank counting (the slot indices are distinct), so the acceptor's later force gather runs in a fixed order.
+__global__ void __launch_bounds__(256) k_hc_transpose(State st, Globals g, Tables t) {
+  const int j = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (j >= st.n) return;
+  const int tj = st.type[j];
+  if (tj < 0 || t.sp[tj].p_hbond != 1) return;
+  const int nc = min(st.hc_cnt[j], st.maxhc);
+  const size_t jb = (size_t)j * st.maxhc;
+  for (int h = lane_id(); h < nc; h += 64) {
+    const int k = st.hc_j[jb + h];
+    const int p = atomicAdd(&st.hc_cnt[k], 1);
+    if (p < st.maxhc) st.hc_m[(size_t)k * st.maxhc + p] = (int)(jb + h);
+    else atomicMax(&st.flags[5], p + 1);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_hc_sort(State st, Globals g, Tables t) {
   const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (k >= st.n) return;
   const int tk = st.type[k];
-  if (tk < 0 || !(g.hbond_cut > 0) || t.sp[tk].p_hbond != 2) return;
-  const int nc = min(st.hc_cnt[k], st.maxhc);
+  if (tk < 0 || t.sp[tk].p_hbond != 2) return;
+  const int m = min(st.hc_cnt[k], st.maxhc);
   const size_t kb = (size_t)k * st.maxhc;
-  for (int h = lane_id(); h < nc; h += 64) {
-    const int j = st.hc_j[kb + h], want = shift_neg(entry_code(st.hc_s[kb + h]));
-    const size_t jb = (size_t)j * st.maxhc;
-    const int ncj = min(st.hc_cnt[j], st.maxhc);
-    int found = -1;
-    // eight independent loads in flight per step of the scan
-    for (int q0 = 0; q0 < ncj && found < 0; q0 += 8) {
-      int hj[8], hs[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int q = min(q0 + u, ncj - 1);
-        hj[u] = st.hc_j[jb + q];
-        hs[u] = st.hc_s[jb + q];
-      }
-#pragma unroll
-      for (int u = 7; u >= 0; --u)
-        if (q0 + u < ncj && hj[u] == k && entry_code(hs[u]) == want) found = q0 + u;
-    }
-    st.hc_m[kb + h] = found < 0 ? 0 : found;
-    if (found < 0) st.flags[6] = 1;
+  for (int h = lane_id(); h < m; h += 64) {
+    const int key = st.hc_m[kb + h];
+    int rank = 0;
+    for (int q = 0; q < m; ++q) rank += st.hc_m[kb + q] < key ? 1 : 0;
+    st.hc_j[kb + rank] = key;
   }
 }
 
@@ -317,7 +315,10 @@ void launch_shift_table(const Cell& c, double4* shv, const int smx[3], hipStream
 void launch_sublists(const State& st, const Globals& g, const Tables& t, double rb, double rh, hipStream_t s) {
   const double m = 1.0 + 1e-12;
   hipLaunchKernelGGL(k_sublists, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t, rb * rb * m, rh * rh * m);
-  if (rh > 0) hipLaunchKernelGGL(k_hc_mirror, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t);
+  if (rh > 0) {
+    hipLaunchKernelGGL(k_hc_transpose, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t);
+    hipLaunchKernelGGL(k_hc_sort, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t);
+  }
 }
 
 void launch_fill_int(int* p, int n, int v, hipStream_t s) {

@@ -9,7 +9,8 @@
 // t-form of the taper (no cancellation of O(10) coefficients in float) and the float cube root.
 //
 // H (stock fix qeq/reaxff compute_H / calculate_H, LAMMPS fix_qeq_reaxff.cpp): H_ij = 14.4 Tap_q(r) / (r^3 + (g_i g_j)^-1.5)^(1/3) for
-// r^2 <= swb^2, stored as full rows (both directions, every image), FP64 plus an FP32 copy for the inner solves of the mixed mode.
+// r^2 <= swb^2, stored as full rows (both directions, every image): FP64 values in FP64 mode, FP32 values otherwise; 16-bit
+// column indices when the system has at most 65535 atoms.
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(256) k_nonbonded(State st, Globals g, Tables t
     st.eatom[T_ELE * n + i] = ee;
     double ep = 0;
     if (ti >= 0) {
-      const DevSpecies s = t.sp[ti];
+      const DevSpecies& s = t.sp[ti];
       const TR q = (TR)st.q[i];
       ep = (double)(TR(23.02) * ((TR)s.chi * q + ((TR)s.eta / TR(2.)) * (q * q)));
     }
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(256) k_nonbonded_f(State st, Globals g, Tables
     st.eatom[T_ELE * n + i] = ee;
     double ep = 0;
     if (ti >= 0) {
-      const DevSpecies sp = t.sp[ti];
+      const DevSpecies& sp = t.sp[ti];
       const float q = (float)st.q[i];
       ep = (double)(23.02f * ((float)sp.chi * q + ((float)sp.eta / 2.0f) * (q * q)));
     }
@@ -258,9 +259,8 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
     const unsigned long long msk = __ballot(keep);
     if (keep) {
       const int slot = m + lanes_below(msk);
-      qb.col[base + slot] = j;
-      qb.val[base + slot] = h;
-      qb.val32[base + slot] = (float)h;
+      if (qb.narrow) qb.col16[base + slot] = (unsigned short)j; else qb.col[base + slot] = j;
+      if (qb.fp32) qb.val32[base + slot] = (float)h; else qb.val[base + slot] = h;
     }
     m += __popcll(msk);
   }

@@ -51,9 +51,12 @@ void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals
 
 struct QeqBuffers {
   int* cnt = nullptr;        // n: entries per row
-  int* col = nullptr;        // n * maxnb: owned column
-  double* val = nullptr;     // n * maxnb: H_ij (FP64)
-  float* val32 = nullptr;    // n * maxnb: H_ij (FP32 copy, inner solves of the mixed mode)
+  // the rows are stored in exactly one column format and one value format:
+  int* col = nullptr;               // n * maxnb: column (owned atom index), when narrow == 0
+  unsigned short* col16 = nullptr;  // n * maxnb: the same in 16 bits, when narrow == 1 (n <= 65535)
+  double* val = nullptr;            // n * maxnb: H_ij in FP64 (FP64 mode)
+  float* val32 = nullptr;           // n * maxnb: H_ij in FP32 (mixed and FP32 modes)
+  int narrow = 0;
   double* eta = nullptr;     // n: diagonal (fix eta, 2 x ffield)
   double* hinv = nullptr;    // n: 1 / eta
   double* chi = nullptr;     // n
@@ -61,7 +64,7 @@ struct QeqBuffers {
   double *b, *x, *r, *d, *p, *q;
   double* partial;           // 4 * kCgBlocks: per-block partials of the element-wise CG kernels
   double* partial_mv;        // 2 * qeq_mv_partials(n): per-block partials of the matrix-vector product
-  int fp32 = 0;              // the CG multiplies with val32 (mixed and FP32 modes) instead of val
+  int fp32 = 0;              // the matrix values are val32 (mixed and FP32 modes) instead of val
   double* scal;              // device scalars (see k_qeq.hip)
   int* iscal;                // device integer scalars
   double *s_hist, *t_hist;   // 4 * n each, [k * n + i], k = 0 most recent
@@ -70,9 +73,9 @@ constexpr int kCgBlocks = 256;
 void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s);
 void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, double* y, hipStream_t s);
 void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s);
-void launch_qeq_spmv(const State& st, QeqBuffers& qb, int it, hipStream_t s);                          // q = A d (+ d.q partials)
+void launch_qeq_mvdir(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);   // beta, d, q = A d, d.q partials
+void launch_qeq_update(const State& st, QeqBuffers& qb, int it, hipStream_t s);                        // alpha, x, r, p, r.p partials
 int qeq_mv_partials(int n);                                                                             // partials per product
-void launch_qeq_update(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);   // alpha, x, r, p, beta, d
 void launch_qeq_finish(const State& st, QeqBuffers& qb, hipStream_t s);
 
 // ---- reductions and integration (k_md.hip)
@@ -89,5 +92,7 @@ void launch_md_drift(const State& st, MdBuffers& md, double dt, hipStream_t s);
 void launch_md_thermostat(const State& st, MdBuffers& md, hipStream_t s);
 void launch_md_ref_positions(const State& st, MdBuffers& md, hipStream_t s);
 void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s);
+
+void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);   // bytes a multiple of 16
 
 }  // namespace rxa

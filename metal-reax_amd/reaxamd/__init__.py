@@ -65,6 +65,7 @@ def lib():
             "rxa_stats": (I, [P, P]),
             "rxa_set_positions_device": (I, [P, P]),
             "rxa_stream": (I, [P, ctypes.POINTER(P)]),
+            "rxa_stream_copy": (I, [P, I64, I]),
             "rxa_profile": (I, [P, I]),
             "rxa_profile_read": (I, [P, P, P]),
             "rxa_ffield_table_dump": (I, [S, I, I, S, LONG, ctypes.POINTER(LONG), S, LONG]),
@@ -306,6 +307,10 @@ class Engine:
         cnt = np.zeros(10, dtype=np.int64)
         self._chk(lib().rxa_profile_read(self.h, _p(ms), _p(cnt)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.PHASES)}
+
+    def stream_copy(self, nbytes, reps=10):
+        """`reps` launches of a plain copy kernel over nbytes (reads nbytes, writes nbytes): memory-counter calibration."""
+        self._chk(lib().rxa_stream_copy(self.h, int(nbytes), int(reps)))
 
     def stream_handle(self):
         h = ctypes.c_void_p()
