@@ -138,78 +138,121 @@ __global__ void __launch_bounds__(256) k_nonbonded(State st, Globals g, Tables t
 __device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
 
-__global__ void __launch_bounds__(256) k_nonbonded_f(State st, Globals g, Tables t) {
+struct NbAcc { double gx = 0, gy = 0, gz = 0, ev = 0, ee = 0, v[6] = {0, 0, 0, 0, 0, 0}; };
+
+// one pair (i, j) within nonb_cut at distance rd, dvec = x_j - x_i: this end's gradient, half of the pair energy and virial
+__device__ inline void nb_pair_f(const Globals& g, const NbPairF* __restrict__ row, float qi, float qj, int tj, double rd, double dx,
+                             double dy, double dz, NbAcc& a) {
+  const float p_vdW1 = (float)g.p_vdW1, p1i = 1.0f / p_vdW1;
+  const int vt = g.vdw_type, lg = g.lg;
+  const float swa = (float)g.nonb_low, tw = (float)g.nonb_cut - (float)g.nonb_low;
+  const float L2E = 1.4426950408889634f;
+  const NbPairF P = row[tj];
+  const float r = (float)rd;
+  const float x = (r - swa) / tw, x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+  const float Tap = 1.0f - x4 * (35.0f - 84.0f * x + 70.0f * x2 - 20.0f * x3);
+  const float dTap = (x3 * (-140.0f + 420.0f * x - 420.0f * x2 + 140.0f * x3)) / (tw * r);
+  float evdw, CEvd;
+  if (vt == 1 || vt == 3) {
+    const float pr = fexp2(p_vdW1 * flog2(r)), S = pr + P.pgw;
+    const float fn13 = fexp2(p1i * flog2(S));
+    const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - fn13 / P.rvdw)), e1 = e2 * e2;
+    const float e0 = P.D * (e1 - 2.0f * e2);
+    evdw = Tap * e0;
+    const float dfn13 = fn13 / S * pr / (r * r);
+    CEvd = dTap * e0 - Tap * P.D * (P.alpha / P.rvdw) * (e1 - e2) * dfn13;
+  } else {
+    const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - r / P.rvdw)), e1 = e2 * e2;
+    const float e0 = P.D * (e1 - 2.0f * e2);
+    evdw = Tap * e0;
+    CEvd = dTap * e0 - Tap * P.D * (P.alpha / P.rvdw) * (e1 - e2) / r;
+  }
+  if (vt == 2 || vt == 3) {
+    const float ec = P.ecore * fexp2(L2E * P.acore * (1.0f - r / P.rcore));
+    evdw += Tap * ec;
+    CEvd += dTap * ec + Tap * (-(P.acore / P.rcore) * ec) / r;
+    if (lg) {
+      const float r6 = (r * r * r) * (r * r * r), r5 = r6 / r, re6 = (P.lgre * P.lgre * P.lgre) * (P.lgre * P.lgre * P.lgre);
+      const float elg = -(P.lgcij / (r6 + re6));
+      evdw += Tap * elg;
+      CEvd += dTap * elg + Tap * (-6.0f * elg * r5 / (r6 + re6)) / r;
+    }
+  }
+  const float d1 = r * r * r + P.gamma, rc = fexp2(flog2(d1) * (-1.0f / 3.0f));   // d1^(-1/3)
+  const float qq = 332.06371f * qi * qj;
+  const float eel = qq * Tap * rc;
+  const float CE = CEvd + qq * (dTap - Tap * r / d1) * rc;
+  const double ce = (double)CE;
+  a.gx += -ce * dx; a.gy += -ce * dy; a.gz += -ce * dz;
+  a.ev += 0.5 * (double)evdw;
+  a.ee += 0.5 * (double)eel;
+  const double hv = -0.5 * ce;
+  a.v[0] += hv * dx * dx; a.v[1] += hv * dy * dy; a.v[2] += hv * dz * dz; a.v[3] += hv * dx * dy; a.v[4] += hv * dx * dz; a.v[5] += hv * dy * dz;
+}
+
+// Wave per atom over the skin list: the distance test runs on all lanes, pairs within nonb_cut are compacted (list order) into
+// a per-wave LDS ring, and the term arithmetic runs on full waves of 64 pairs.
+constexpr int kNbWaves = 4;
+constexpr int kNbRing = 128;
+
+__global__ void __launch_bounds__(64 * kNbWaves) k_nonbonded_f(State st, Globals g, Tables t) {
+  __shared__ int rj_[kNbWaves][kNbRing], rt_[kNbWaves][kNbRing];
+  __shared__ double rx_[kNbWaves][kNbRing], ry_[kNbWaves][kNbRing], rz_[kNbWaves][kNbRing], rr_[kNbWaves][kNbRing];
+  const int w = threadIdx.x >> 6;
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
+  int* rj = rj_[w]; int* rt = rt_[w];
+  double *rx = rx_[w], *ry = ry_[w], *rz = rz_[w], *rr = rr_[w];
   const int lane = lane_id();
   const int ti = st.type[i];
-  double gx = 0, gy = 0, gz = 0, ev = 0, ee = 0;
-  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+  NbAcc a;
   if (ti >= 0) {
     const size_t base = (size_t)i * st.maxnb;
     const int cnt = min(st.nb_cnt[i], st.maxnb);
     const float qi = (float)st.q[i];
-    const float p_vdW1 = (float)g.p_vdW1, p1i = 1.0f / p_vdW1;
-    const int vt = g.vdw_type, nt = g.ntypes, lg = g.lg;
     const double4 pi = st.pos[i];
     const double cut = g.nonb_cut, cut2 = cut * cut * (1.0 + 1e-12);
-    const float swa = (float)g.nonb_low, tw = (float)g.nonb_cut - (float)g.nonb_low;
-    const float L2E = 1.4426950408889634f;
-    const NbPairF* __restrict__ row = t.nbf + ti * nt;
-    for (int k = lane; k < cnt; k += 64) {
-      const int j = st.nb_j[base + k];
-      double dx, dy, dz;
-      int tj;
-      const double r2 = pair_dvec_list(st, pi, j, st.nb_s[base + k], dx, dy, dz, tj);
-      if (tj < 0 || !(r2 <= cut2)) continue;
-      const double rd = sqrt(r2);
-      if (!(rd <= cut)) continue;
-      const NbPairF P = row[tj];
-      const float r = (float)rd;
-      const float x = (r - swa) / tw, x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
-      const float Tap = 1.0f - x4 * (35.0f - 84.0f * x + 70.0f * x2 - 20.0f * x3);
-      const float dTap = (x3 * (-140.0f + 420.0f * x - 420.0f * x2 + 140.0f * x3)) / (tw * r);
-      float evdw, CEvd;
-      if (vt == 1 || vt == 3) {
-        const float pr = fexp2(p_vdW1 * flog2(r)), S = pr + P.pgw;
-        const float fn13 = fexp2(p1i * flog2(S));
-        const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - fn13 / P.rvdw)), e1 = e2 * e2;
-        const float e0 = P.D * (e1 - 2.0f * e2);
-        evdw = Tap * e0;
-        const float dfn13 = fn13 / S * pr / (r * r);
-        CEvd = dTap * e0 - Tap * P.D * (P.alpha / P.rvdw) * (e1 - e2) * dfn13;
-      } else {
-        const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - r / P.rvdw)), e1 = e2 * e2;
-        const float e0 = P.D * (e1 - 2.0f * e2);
-        evdw = Tap * e0;
-        CEvd = dTap * e0 - Tap * P.D * (P.alpha / P.rvdw) * (e1 - e2) / r;
+    const NbPairF* __restrict__ row = t.nbf + ti * g.ntypes;
+    int head = 0, fill = 0;
+    auto batch = [&](int m) {
+      if (lane < m) {
+        const int e = (head + lane) & (kNbRing - 1);
+        const int j = rj[e];
+        nb_pair_f(g, row, qi, (float)st.q[j], rt[e], rr[e], rx[e], ry[e], rz[e], a);
       }
-      if (vt == 2 || vt == 3) {
-        const float ec = P.ecore * fexp2(L2E * P.acore * (1.0f - r / P.rcore));
-        evdw += Tap * ec;
-        CEvd += dTap * ec + Tap * (-(P.acore / P.rcore) * ec) / r;
-        if (lg) {
-          const float r6 = (r * r * r) * (r * r * r), r5 = r6 / r, re6 = (P.lgre * P.lgre * P.lgre) * (P.lgre * P.lgre * P.lgre);
-          const float elg = -(P.lgcij / (r6 + re6));
-          evdw += Tap * elg;
-          CEvd += dTap * elg + Tap * (-6.0f * elg * r5 / (r6 + re6)) / r;
+    };
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const int k = k0 + lane;
+      bool keep = false;
+      int j = 0, tj = -1;
+      double dx = 0, dy = 0, dz = 0, rd = 0;
+      if (k < cnt) {
+        j = st.nb_j[base + k];
+        const double r2 = pair_dvec_list(st, pi, j, st.nb_s[base + k], dx, dy, dz, tj);
+        if (tj >= 0 && r2 <= cut2) {
+          rd = sqrt(r2);
+          keep = rd <= cut;
         }
       }
-      const float qj = (float)st.q[j];
-      const float d1 = r * r * r + P.gamma, rc = fexp2(flog2(d1) * (-1.0f / 3.0f));   // d1^(-1/3)
-      const float qq = 332.06371f * qi * qj;
-      const float eel = qq * Tap * rc;
-      const float CE = CEvd + qq * (dTap - Tap * r / d1) * rc;
-      const double ce = (double)CE;
-      gx += -ce * dx; gy += -ce * dy; gz += -ce * dz;
-      ev += 0.5 * (double)evdw;
-      ee += 0.5 * (double)eel;
-      const double hv = -0.5 * ce;
-      v0 += hv * dx * dx; v1 += hv * dy * dy; v2 += hv * dz * dz; v3 += hv * dx * dy; v4 += hv * dx * dz; v5 += hv * dy * dz;
+      const unsigned long long mk = __ballot(keep);
+      if (keep) {
+        const int e = (head + fill + lanes_below(mk)) & (kNbRing - 1);
+        rj[e] = j; rt[e] = tj; rx[e] = dx; ry[e] = dy; rz[e] = dz; rr[e] = rd;
+      }
+      fill += __popcll(mk);
+      __builtin_amdgcn_wave_barrier();
+      if (fill >= 64) {
+        batch(64);
+        head = (head + 64) & (kNbRing - 1);
+        fill -= 64;
+        __builtin_amdgcn_wave_barrier();
+      }
     }
+    if (fill > 0) batch(fill);
   }
-  gx = wave_sum(gx); gy = wave_sum(gy); gz = wave_sum(gz); ev = wave_sum(ev); ee = wave_sum(ee);
-  v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2); v3 = wave_sum(v3); v4 = wave_sum(v4); v5 = wave_sum(v5);
+  double gx = wave_sum(a.gx), gy = wave_sum(a.gy), gz = wave_sum(a.gz), ev = wave_sum(a.ev), ee = wave_sum(a.ee);
+  double v0 = wave_sum(a.v[0]), v1 = wave_sum(a.v[1]), v2 = wave_sum(a.v[2]), v3 = wave_sum(a.v[3]), v4 = wave_sum(a.v[4]),
+         v5 = wave_sum(a.v[5]);
   if (lane == 0) {
     const int n = st.n;
     st.gnb_x[i] = gx; st.gnb_y[i] = gy; st.gnb_z[i] = gz;

@@ -15,6 +15,7 @@ namespace rxa {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kUpdBlocks = 64, kUpdThreads = 1024;   // CG update kernel
 constexpr int kMvThreads = 256;             // product kernels: 4 waves per block,
 constexpr int kMvRowsPerBlock = kMvThreads / 16;   // 16 lanes per row (kRowLanes below)
 inline __host__ __device__ int mv_blocks(int n) { return (n + kMvRowsPerBlock - 1) / kMvRowsPerBlock; }
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
     const int* pa = qb.iscal + 2 * (it - 2);
     const int pact[2] = {pa[0], pa[1]};
     double sn[2];
-    reduce_partials<2, kMvThreads>(qb.partial + 2 * kCgBlocks, sn, sh);
+    reduce_partials<2, kMvThreads>(qb.partial + 2 * kCgBlocks, sn, sh, kUpdBlocks);
     for (int s = 0; s < 2; ++s) {
       act[s] = pact[s] && it < imax && sqrt(sn[s]) / qb.scal[s] > tol ? 1 : 0;
       if (pact[s]) beta[s] = sn[s] / SIG(qb.scal, it - 2, s);
@@ -222,18 +223,19 @@ __global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, int ima
   }
 }
 
-// x += alpha d, r -= alpha q, p = r / eta, partial r.p   (iteration it >= 1)
-__global__ void __launch_bounds__(kThreads) k_cg_update(int n, QeqBuffers qb, int it) {
-  __shared__ double sh[2 * kThreads];
+// x += alpha d, r -= alpha q, p = r / eta, partial r.p   (iteration it >= 1); few large blocks, since every block first
+// re-reduces all partials of the product
+__global__ void __launch_bounds__(kUpdThreads) k_cg_update(int n, QeqBuffers qb, int it) {
+  __shared__ double sh[2 * kUpdThreads];
   const int* act = qb.iscal + 2 * (it - 1);
   const int a[2] = {act[0], act[1]};
   if (!a[0] && !a[1]) return;
   double dq[2];
-  reduce_partials<2>(qb.partial_mv, dq, sh, mv_blocks(n));
+  reduce_partials<2, kUpdThreads>(qb.partial_mv, dq, sh, mv_blocks(n));
   double alpha[2];
   for (int s = 0; s < 2; ++s) alpha[s] = a[s] ? SIG(qb.scal, it - 1, s) / dq[s] : 0.0;
   double acc[2] = {0, 0};
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+  for (int i = blockIdx.x * kUpdThreads + threadIdx.x; i < n; i += gridDim.x * kUpdThreads) {
     const double hi = qb.hinv[i];
     for (int s = 0; s < 2; ++s) {
       if (!a[s]) continue;
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(kThreads) k_cg_update(int n, QeqBuffers qb, in
       acc[s] += qb.r[k] * p;
     }
   }
-  block_reduce<2>(acc, sh);
+  block_reduce<2, kUpdThreads>(acc, sh);
   if (threadIdx.x == 0) { qb.partial[2 * kCgBlocks + blockIdx.x * 2] = acc[0]; qb.partial[2 * kCgBlocks + blockIdx.x * 2 + 1] = acc[1]; }
 }
 
@@ -329,7 +331,7 @@ void launch_qeq_mvdir(const State& st, QeqBuffers& qb, int it, int imax, double 
 int qeq_mv_partials(int n) { return mv_blocks(n); }
 
 void launch_qeq_update(const State& st, QeqBuffers& qb, int it, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_update, dim3(kCgBlocks), dim3(kThreads), 0, s, st.n, qb, it);
+  hipLaunchKernelGGL(k_cg_update, dim3(kUpdBlocks), dim3(kUpdThreads), 0, s, st.n, qb, it);
 }
 
 void launch_qeq_finish(const State& st, QeqBuffers& qb, hipStream_t s) {
