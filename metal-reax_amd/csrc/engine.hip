@@ -193,14 +193,28 @@ struct Engine {
   int last_iters[2] = {0, 0};
   double last_rel[2] = {0, 0};
 
+  // pinned host words the device writes asynchronously (bond-list flags, displacement), and the events that mark them ready
+  int* flags_h = nullptr;
+  double* disp_h = nullptr;
+  hipEvent_t ev_flags = nullptr, ev_disp = nullptr;
+
   Engine(int dev) : device(dev) {
     const char* sort_env = std::getenv("RXA_ATOM_SORT");
     if (sort_env && std::atoi(sort_env) == 0) sort_atoms = false;
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     timer.s = stream;
+    HIPCHK(hipHostMalloc((void**)&flags_h, 8 * sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&disp_h, sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&ev_flags, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_disp, hipEventDisableTiming));
   }
   ~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (ev_flags) (void)hipEventDestroy(ev_flags);
+    if (ev_disp) (void)hipEventDestroy(ev_disp);
+    if (flags_h) (void)hipHostFree(flags_h);
+    if (disp_h) (void)hipHostFree(disp_h);
     if (stream) (void)hipStreamDestroy(stream);
   }
   void sync() {
@@ -595,40 +609,43 @@ struct Engine {
     throw ApiError(RXA_EINVAL, "neighbour list capacity did not converge");
   }
 
-  void bond_lists() {
-    for (int attempt = 0; attempt < 6; ++attempt) {
+  // Forces and energies of the current positions and charges (queued on the stream). The bond-list capacity check is read back
+  // through pinned memory while the kernels that are valid whenever the lists fit (bond-order correction, bond / lone-pair /
+  // over-under terms, non-bonded) are already queued; only the group-size dependent kernels wait for it. An overflow drains
+  // the stream, grows the lists and redoes the step.
+  void forces(bool rebuild) {
+    if (!nbr_valid || rebuild) build_neighbors();
+    for (int attempt = 0;; ++attempt) {
       hipEvent_t t0 = timer.begin(PH_BONDS);
       launch_fill_int(st.flags, 8, 0, stream);
       launch_bond_build(prec, st, cell, g, tab, stream);
       launch_bond_sym(st, stream);
       timer.end(PH_BONDS, t0);
-      int flags[8];
-      HIPCHK(hipMemcpyAsync(flags, st.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
-      sync();
-      if (flags[1] <= maxb) {
-        if (flags[3]) throw ApiError(RXA_EDEVICE, "internal: a bond has no mirror slot");
-        bmax = flags[7];
+      HIPCHK(hipMemcpyAsync(flags_h, st.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipEventRecord(ev_flags, stream));
+      t0 = timer.begin(PH_BOCORR);
+      launch_bo_correct(prec, st, g, tab, stream);
+      timer.end(PH_BOCORR, t0);
+      t0 = timer.begin(PH_TERMS);
+      launch_atom_terms(prec, st, g, tab, stream);
+      timer.end(PH_TERMS, t0);
+      t0 = timer.begin(PH_NONB);
+      launch_nonbonded(prec, st, cell, g, tab, stream);
+      timer.end(PH_NONB, t0);
+      HIPCHK(hipEventSynchronize(ev_flags));
+      if (flags_h[1] <= maxb) {
+        if (flags_h[3]) throw ApiError(RXA_EDEVICE, "internal: a bond has no mirror slot");
+        bmax = flags_h[7];
         if (bmax > 64) throw ApiError(RXA_EINVAL, "an atom has more than 64 bonds");
-        return;
+        break;
       }
-      alloc_bonds(std::max(maxb, (flags[1] + 3) / 4 * 4 + 4));
+      if (attempt >= 5) throw ApiError(RXA_EINVAL, "bond list capacity did not converge");
+      sync();
+      alloc_bonds(std::max(maxb, (flags_h[1] + 3) / 4 * 4 + 4));
     }
-    throw ApiError(RXA_EINVAL, "bond list capacity did not converge");
-  }
-
-  // forces and energies of the current positions and charges (queued on the stream)
-  void forces(bool rebuild) {
-    if (!nbr_valid || rebuild) build_neighbors();
-    bond_lists();
-    hipEvent_t t0 = timer.begin(PH_BOCORR);
-    launch_bo_correct(prec, st, g, tab, stream);
-    timer.end(PH_BOCORR, t0);
-    t0 = timer.begin(PH_TERMS);
-    launch_bonded_terms(prec, st, cell, g, tab, bmax, stream);
+    hipEvent_t t0 = timer.begin(PH_TERMS);
+    launch_angle_terms(prec, st, g, tab, bmax, stream);
     timer.end(PH_TERMS, t0);
-    t0 = timer.begin(PH_NONB);
-    launch_nonbonded(prec, st, cell, g, tab, stream);
-    timer.end(PH_NONB, t0);
     t0 = timer.begin(PH_DBOND);
     launch_bond_forces(prec, st, stream);
     launch_energy_reduce(st, red.p, stream);
@@ -765,12 +782,15 @@ struct Engine {
   // The first batch launches as many iterations as the previous solve needed (warm-started solves of consecutive steps need
   // about the same number), later batches two at a time; iterations past convergence exit at once on the device flags.
   int cg_guess = 8;
+  std::vector<int> cg_flags_h;
+  std::vector<double> cg_scal_h;
   void qeq_iterate(int imax, double tol) {
     // launch order: mvdir(1) [update(1) mvdir(2)] [update(2) mvdir(3)] ...; mvdir(it) decides whether iteration it runs, so after
     // a batch ending with mvdir(it) the flags of iteration it tell the host whether to go on
     int it = 1;
     int batch = std::max(1, cg_guess);
-    std::vector<int> act(2 * (imax + 2));
+    // flags of iterations 1..it and the scalars, read back in one go after each batch (no further synchronisation)
+    if (cg_flags_h.size() < 2 * (size_t)(imax + 2)) { cg_flags_h.resize(2 * (size_t)(imax + 2)); cg_scal_h.resize(8 + 2 * (size_t)(imax + 2)); }
     hipEvent_t t0 = timer.begin(PH_QEQ_SPMV);
     launch_qeq_mvdir(st, qb, 1, imax, tol, stream);
     timer.end(PH_QEQ_SPMV, t0);
@@ -784,21 +804,19 @@ struct Engine {
         launch_qeq_mvdir(st, qb, it, imax, tol, stream);
         timer.end(PH_QEQ_SPMV, t0);
       }
-      int a[2];
-      HIPCHK(hipMemcpyAsync(a, qb.iscal + 2 * (it - 1), sizeof(a), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(cg_flags_h.data(), qb.iscal, 2 * (size_t)it * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(cg_scal_h.data(), qb.scal, (8 + 2 * (size_t)it) * sizeof(double), hipMemcpyDeviceToHost, stream));
       sync();
+      const int* a = cg_flags_h.data() + 2 * (it - 1);
       if ((!a[0] && !a[1]) || it >= imax) break;
       batch = 2;
     }
-    HIPCHK(hipMemcpy(act.data(), qb.iscal, 2 * (size_t)it * sizeof(int), hipMemcpyDeviceToHost));
     for (int s = 0; s < 2; ++s) {
       int k = 0;
-      while (k < it && act[2 * k + s]) ++k;
+      while (k < it && cg_flags_h[2 * k + s]) ++k;
       last_iters[s] = k + 1;   // the stock loop index at exit
+      last_rel[s] = std::sqrt(cg_scal_h[8 + 2 * (last_iters[s] - 1) + s]) / cg_scal_h[s];
     }
-    std::vector<double> sc(8 + 2 * (size_t)(it + 1));
-    HIPCHK(hipMemcpy(sc.data(), qb.scal, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int s = 0; s < 2; ++s) last_rel[s] = std::sqrt(sc[8 + 2 * (last_iters[s] - 1) + s]) / sc[s];
     cg_guess = std::max(last_iters[0], last_iters[1]) - 1;
   }
 
@@ -883,16 +901,19 @@ struct Engine {
       launch_md_drift(st, md, md_dt, stream);
       launch_md_displacement(st, md, stream);
       timer.end(PH_MD, t0);
-      HIPCHK(hipMemcpyAsync(&disp, md.disp, sizeof(double), hipMemcpyDeviceToHost, stream));
-      sync();
-      const bool rebuild = disp > 0.5 * skin;
-      if (rebuild) {
-        build_neighbors();
-        launch_md_ref_positions(st, md, stream);
-      }
+      HIPCHK(hipMemcpyAsync(disp_h, md.disp, sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipEventRecord(ev_disp, stream));
       qeq_h_valid = false;
       ++md_steps;
-      if (md_qeq_every > 0 && md_steps % md_qeq_every == 0) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
+      const bool do_qeq = md_qeq_every > 0 && md_steps % md_qeq_every == 0;
+      if (do_qeq && nbr_valid) qeq_matrix();   // speculative: the common step keeps its lists; redone after a rebuild
+      HIPCHK(hipEventSynchronize(ev_disp));
+      disp = *disp_h;
+      if (disp > 0.5 * skin) {
+        build_neighbors();   // invalidates the matrix
+        launch_md_ref_positions(st, md, stream);
+      }
+      if (do_qeq) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
       forces(false);
       t0 = timer.begin(PH_MD);
       launch_md_half_kick(st, md, dtf, stream);

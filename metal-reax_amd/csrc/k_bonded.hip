@@ -1010,7 +1010,6 @@ void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tab
 template <class TR, int G>
 void launch_terms_g(const State& st, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * G, 256)), blk(256), wgrid(nblk(st.n, 4));
-  hipLaunchKernelGGL(k_atom_terms<TR>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
   hipLaunchKernelGGL((k_valence<TR, G>), grid, blk, 0, s, st, g, t);
   hipLaunchKernelGGL((k_torsion<TR, G>), grid, blk, 0, s, st, g, t);
   hipLaunchKernelGGL(k_hbond_donor<TR>, wgrid, blk, 0, s, st, g, t);
@@ -1026,8 +1025,12 @@ void launch_terms_t(const State& st, const Globals& g, const Tables& t, int bmax
   else launch_terms_g<TR, 64>(st, g, t, s);
 }
 
-void launch_bonded_terms(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
-  (void)c;
+void launch_atom_terms(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
+  if (p == Precision::FP64) hipLaunchKernelGGL(k_atom_terms<double>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+  else hipLaunchKernelGGL(k_atom_terms<float>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+}
+
+void launch_angle_terms(Precision p, const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
   if (p == Precision::FP64) launch_terms_t<double>(st, g, t, bmax, s);
   else launch_terms_t<float>(st, g, t, bmax, s);
 }

@@ -43,7 +43,10 @@ void launch_sublists(const State& st, const Globals& g, const Tables& t, double 
 void launch_bond_build(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
 void launch_bond_sym(const State& st, hipStream_t s);
 void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s);
-void launch_bonded_terms(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, int bmax, hipStream_t s);
+// bond energy / lone pair / over-under (no group size needed), then valence, torsion, H-bonds with the group size for the
+// largest bond count bmax
+void launch_atom_terms(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s);
+void launch_angle_terms(Precision p, const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s);
 void launch_bond_forces(Precision p, const State& st, hipStream_t s);
 
 // ---- non-bonded terms and the charge-equilibration matrix (k_nonbonded.hip)
