@@ -102,8 +102,8 @@ int rxa_qeq_charges(rxa_context* ctx, int imax, double tol, int strict, double* 
 int rxa_qeq_reset_history(rxa_context* ctx);
 
 /* ---- standalone MD loop (bench.py and the serving harness; LAMMPS integrates in plugin mode)
- * masses by LAMMPS type (index 0 unused); NVT Nose-Hoover chain (3) at t_target with damping t_damp (fs); dt in fs; skin in Angstrom;
- * qeq_every 0 keeps charges fixed. */
+ * masses by LAMMPS type (index 0 unused); NVT Nose-Hoover chain (3) at t_target with damping t_damp (fs), or NVE when t_damp <= 0;
+ * dt in fs; skin in Angstrom; qeq_every 0 keeps charges fixed. */
 int rxa_md_setup(rxa_context* ctx, const double* v, const double* mass_by_type, double dt, double t_target, double t_damp, double skin,
                  int qeq_every, int qeq_imax, double qeq_tol);
 /* runs `steps` steps; thermo (may be NULL) receives: potential energy, kinetic energy, temperature, neighbour rebuilds, mean QEq

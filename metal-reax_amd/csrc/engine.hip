@@ -172,6 +172,7 @@ struct Engine {
   QeqBuffers qb{};
   DBuf<int> q_int;
   DBuf<double> q_dbl;
+  int q_dbl_n = -1;   // atom count the charge buffers were sized for
   DBuf<float> q_f32;
   DBuf<unsigned short> q_u16;
   bool qeq_ready = false, qeq_h_valid = false;
@@ -183,7 +184,7 @@ struct Engine {
   // MD
   MdBuffers md{};
   DBuf<double> md_buf;
-  bool md_ready = false;
+  bool md_ready = false, md_nve = false;
   double md_dt = 0.25;
   int md_qeq_every = 1, md_qeq_imax = 200;
   double md_qeq_tol = 1e-6;
@@ -455,6 +456,7 @@ struct Engine {
     const bool realloc = (int)nn != n || !a_x.p;
     n = (int)nn;
     if (realloc) {
+      hist_count = 0;   // a new atom set: no charge history to carry
       alloc_atoms();
       alloc_neighbors(estimate_neighbors());
       alloc_sublists(estimate_within(g.bond_cut + skin), g.hbond_cut > 0 ? estimate_within(g.hbond_cut + skin) : 1);
@@ -613,7 +615,7 @@ struct Engine {
   // through pinned memory while the kernels that are valid whenever the lists fit (bond-order correction, bond / lone-pair /
   // over-under terms, non-bonded) are already queued; only the group-size dependent kernels wait for it. An overflow drains
   // the stream, grows the lists and redoes the step.
-  void forces(bool rebuild) {
+  void forces(bool rebuild, bool energies = true) {
     if (!nbr_valid || rebuild) build_neighbors();
     for (int attempt = 0;; ++attempt) {
       hipEvent_t t0 = timer.begin(PH_BONDS);
@@ -648,7 +650,7 @@ struct Engine {
     timer.end(PH_TERMS, t0);
     t0 = timer.begin(PH_DBOND);
     launch_bond_forces(prec, st, stream);
-    launch_energy_reduce(st, red.p, stream);
+    if (energies) launch_energy_reduce(st, red.p, stream);   // the per-atom arrays hold the energies either way
     timer.end(PH_DBOND, t0);
   }
 
@@ -694,13 +696,14 @@ struct Engine {
     const size_t sz = rnd((size_t)n * maxnb) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + rnd(nmv) +
                       rnd(8 + 2 * (size_t)(imax_cap + 1)) + 2 * rnd(4 * (size_t)n);
     std::vector<double> keep;
-    const bool had_hist = q_dbl.p && hist_count > 0;
+    const bool had_hist = q_dbl.p && hist_count > 0 && q_dbl_n == n;   // history of the same atoms only
     if (had_hist) {
       keep.resize(8 * (size_t)n);
       HIPCHK(hipMemcpy(keep.data(), qb.s_hist, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(keep.data() + 4 * (size_t)n, qb.t_hist, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     }
     q_dbl.alloc(sz);
+    q_dbl_n = n;
     HIPCHK(hipMemset(q_dbl.p, 0, sz * sizeof(double)));
     double* p = q_dbl.p;
     auto take = [&](size_t k) { double* r = p; p += rnd(k); return r; };
@@ -873,6 +876,8 @@ struct Engine {
     std::vector<double> nh(64, 0.0);
     nh[13] = tt; nh[14] = 1.0 / tdamp; nh[15] = dt; nh[16] = 3.0 * n - 3.0; nh[17] = 0.0019872067; nh[18] = 48.88821291 * 48.88821291;
     nh[19] = 1.0;
+    md_nve = !(tdamp > 0);   // t_damp <= 0: no thermostat (NVE)
+    nh[22] = md_nve ? 1.0 : 0.0;
     HIPCHK(hipMemcpy(md.nh, nh.data(), nh.size() * sizeof(double), hipMemcpyHostToDevice));
     md_dt = dt;
     skin = sk;
@@ -896,7 +901,7 @@ struct Engine {
     double disp = 0;
     for (int s = 0; s < steps; ++s) {
       hipEvent_t t0 = timer.begin(PH_MD);
-      launch_md_thermostat(st, md, stream);
+      if (!md_nve) launch_md_thermostat(st, md, stream);
       launch_md_half_kick(st, md, dtf, stream);
       launch_md_drift(st, md, md_dt, stream);
       launch_md_displacement(st, md, stream);
@@ -914,10 +919,10 @@ struct Engine {
         launch_md_ref_positions(st, md, stream);
       }
       if (do_qeq) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
-      forces(false);
+      forces(false, s == steps - 1);   // energy totals are read once, after the last step
       t0 = timer.begin(PH_MD);
       launch_md_half_kick(st, md, dtf, stream);
-      launch_md_thermostat(st, md, stream);
+      if (!md_nve || s == steps - 1) launch_md_thermostat(st, md, stream);   // NVE: kinetic energy of the last step only
       timer.end(PH_MD, t0);
     }
     sync();

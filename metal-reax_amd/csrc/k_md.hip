@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kRT) k_energy_final(const double* part, double
 
 // ---- integrator -----------------------------------------------------------------------------------------------
 // nh layout: [0..2] eta, [3..6] eta_dot (last stays 0), [7..9] eta_dotdot, [10..12] eta_mass, [13] t_target, [14] t_freq,
-// [15] dt, [16] dof, [17] boltz, [18] mvv2e, [19] last scale factor, [20] current temperature, [21] kinetic energy
+// [15] dt, [16] dof, [17] boltz, [18] mvv2e, [19] last scale factor, [20] current temperature, [21] kinetic energy, [22] 1 = NVE
 __global__ void k_half_kick(State st, MdBuffers md, double dtf) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= st.n) return;
@@ -88,6 +88,12 @@ __global__ void __launch_bounds__(kRT) k_nhc(MdBuffers md) {
   const double t_target = nh[13], t_freq = nh[14], dt = nh[15], dof = nh[16], boltz = nh[17], mvv2e = nh[18];
   const double ke2 = v * mvv2e;   // sum m v^2 in energy units = 2 KE
   double t_current = ke2 / (dof * boltz);
+  if (nh[22] != 0.0) {   // NVE: temperature and kinetic energy only, velocities untouched
+    nh[19] = 1.0;
+    nh[20] = t_current;
+    nh[21] = 0.5 * ke2;
+    return;
+  }
   double* eta = nh;
   double* eta_dot = nh + 3;
   double* eta_dotdot = nh + 7;

@@ -1,0 +1,174 @@
+"""Batched and served MD on one GPU: independent systems of uneven size advanced concurrently.
+
+A `Server` owns `slots` worker threads; each worker owns one engine context (its own HIP stream and device buffers) and runs
+one request at a time: box, atoms, charge-equilibration setup, NVT (or NVE) MD for the requested steps, then the final
+energies, forces and charges. A dispatcher admits queued requests in arrival order as soon as a slot is free and the
+request's device memory (its atom count times the bytes per atom the engines report) fits in the budget, so requests that
+arrive while others run start at once and their kernels overlap on the device through the workers' streams.
+
+Every engine is deterministic (fixed-order reductions, no atomics on floating point) and a worker resets the charge history
+before each request, so a request's result does not depend on what runs beside it: alone or in the mix, the answer is the
+same bit for bit (tests/test_gpu_serving.py).
+
+Reference: the reference has no batching or serving (capabilities.cpp:112); the workloads are those of the reference's
+benchmark plan (512 systems, N log-uniform in [100, 3000], half water, half CH4:O2 = 1:2 gas).
+"""
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Engine, systems
+
+
+@dataclass
+class Request:
+    rid: int
+    system: object                  # systems.System
+    steps: int
+    t_target: float = 300.0
+    dt: float = 0.25
+    t_damp: float = 25.0
+    qeq_tol: float = 1e-6
+    seed: int = 0
+    arrival: float = 0.0            # seconds after the server's start (for Poisson traces)
+    # filled in by the server
+    start: float = 0.0
+    end: float = 0.0
+    result: dict = field(default_factory=dict)
+
+    @property
+    def atom_steps(self):
+        return self.system.n * self.steps
+
+
+# device bytes per atom of one engine context at the capacities it allocates for condensed C/H/O systems (skin list, bond and
+# H-bond candidate lists, bond slots, charge matrix, per-atom state); the server refines it from the engines' own capacities
+DEFAULT_BYTES_PER_ATOM = 40_000
+
+
+def run_one(engine, ffield_text, req, precision):
+    """One request on one engine: the plain single-system path (the same calls a user makes)."""
+    w = req.system
+    e = engine
+    e.set_box(w.lo, w.hi)
+    e.set_atoms(w.types, w.tags, w.x)
+    e.qeq_reset_history()
+    e.qeq_setup(0.0, 10.0, e.extract("chi"), e.extract("eta"), e.extract("gamma"))
+    v = systems.maxwell_boltzmann(w.masses(), req.t_target, seed=req.seed)
+    masses = np.array([0.0] + [systems.MASS[el] for el in systems.ELEMENTS])
+    e.md_setup(v, masses, dt=req.dt, t_target=req.t_target, t_damp=req.t_damp, skin=2.0, qeq_every=1, qeq_imax=200,
+               qeq_tol=req.qeq_tol)
+    thermo = e.md_run(req.steps)
+    r = e.compute()
+    x, vel, q = e.state()
+    return {"energy": r["energy"], "grad": r["grad"], "charges": q, "positions": x, "thermo": thermo}
+
+
+def new_engine(ffield_text, precision):
+    e = Engine(0)
+    e.pair_style(["NULL", "precision", precision])
+    e.pair_coeff(ffield_text, list(systems.ELEMENTS))
+    return e
+
+
+def engine_bytes_per_atom(e):
+    """Device bytes per atom implied by an engine's current capacities (lists, slots, matrix, per-atom arrays)."""
+    st = e.stats()
+    if not st["n"]:
+        return DEFAULT_BYTES_PER_ATOM
+    per = 8 * st["maxnb"] + 6 * st["maxnb"] + 8 * st["maxnb"]      # skin list (j, entry), charge rows (col + value), FP64 copy
+    per += 8 * 29 * st["maxb"] + 12 * st["maxb"]                     # bond slots
+    per += 8 * 64 + 8 * 200                                          # per-atom arrays, candidate lists (typical)
+    return per
+
+
+class Server:
+    """Slot-based concurrent executor of MD requests (see the module docstring)."""
+
+    def __init__(self, ffield_text, slots=4, precision="mixed", mem_budget_bytes=200 << 30):
+        self.ffield_text = ffield_text
+        self.precision = precision
+        self.slots = slots
+        self.budget = mem_budget_bytes
+        self.bytes_per_atom = DEFAULT_BYTES_PER_ATOM
+        self._free = queue.Queue()
+        self._lock = threading.Condition()
+        self._mem_in_use = 0
+        self._engines = [new_engine(ffield_text, precision) for _ in range(slots)]
+        for k in range(slots):
+            self._free.put(k)
+
+    def close(self):
+        for e in self._engines:
+            e.close()
+
+    def _need(self, req):
+        return req.system.n * self.bytes_per_atom
+
+    def run(self, requests, realtime=False):
+        """Runs `requests` (in arrival order). realtime=True honours Request.arrival (seconds after start); otherwise all
+        requests are queued at once. Returns the requests with start/end times (seconds after start) and results."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        t0 = time.perf_counter()
+        threads = []
+        errors = []
+
+        def work(slot, req):
+            try:
+                req.result = run_one(self._engines[slot], self.ffield_text, req, self.precision)
+                self.bytes_per_atom = max(self.bytes_per_atom, engine_bytes_per_atom(self._engines[slot]))
+            except Exception as ex:   # surfaced after the run
+                errors.append((req.rid, ex))
+            finally:
+                req.end = time.perf_counter() - t0
+                with self._lock:
+                    self._mem_in_use -= self._need(req)
+                    self._lock.notify_all()
+                self._free.put(slot)
+
+        for req in reqs:
+            if realtime:
+                wait = req.arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+            need = self._need(req)
+            if need > self.budget:
+                raise ValueError("request %d needs more device memory than the budget" % req.rid)
+            with self._lock:   # admission by device memory
+                while self._mem_in_use + need > self.budget:
+                    self._lock.wait()
+                self._mem_in_use += need
+            slot = self._free.get()   # admission by slot
+            req.start = time.perf_counter() - t0
+            th = threading.Thread(target=work, args=(slot, req), daemon=True)
+            th.start()
+            threads.append(th)
+        for th in threads:
+            th.join()
+        if errors:
+            rid, ex = errors[0]
+            raise RuntimeError("request %d failed: %s" % (rid, ex))
+        return reqs
+
+
+def batch_systems(count=512, nmin=100, nmax=3000, seed=1):
+    """The batched workload: `count` periodic systems, N log-uniform in [nmin, nmax]; even indices water (216-molecule recipe),
+    odd indices CH4:O2 = 1:2 gas at 0.3 g/cm^3 with 2.5 A between molecules."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        if k % 2 == 0:
+            side = max(2, int(round((n / 3.0) ** (1.0 / 3.0))))
+            out.append(systems.water_box(side, seed=seed * 100003 + k))
+        else:
+            out.append(systems.ch4_o2_gas(int(n), density=0.3, seed=seed * 100003 + k))
+    return out
+
+
+def latency_stats(reqs):
+    lat = np.array([r.end - r.arrival for r in reqs])
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean())}

@@ -1,0 +1,56 @@
+"""Batched / served MD (reaxamd.serving): the result of a request is the same alone and in the mix (bit for bit: every engine
+is deterministic and owns its buffers and stream), requests of uneven size run concurrently, and a Poisson-arrival trace
+reports latencies."""
+import numpy as np
+import pytest
+
+
+def small_batch(count=10, seed=3):
+    from reaxamd import serving
+    return serving.batch_systems(count, nmin=100, nmax=700, seed=seed)
+
+
+def requests(systems_, steps=20):
+    from reaxamd.serving import Request
+    return [Request(rid=k, system=s, steps=steps, t_target=2500.0, dt=0.1, seed=100 + k) for k, s in enumerate(systems_)]
+
+
+@pytest.mark.gpu
+def test_same_result_alone_and_in_the_mix(ffield_text):
+    from reaxamd.serving import Server
+    syst = small_batch()
+    print("systems", [s.n for s in syst], flush=True)
+    alone = Server(ffield_text, slots=1)
+    ra = alone.run(requests(syst))
+    alone.close()
+    mix = Server(ffield_text, slots=4)
+    rb = mix.run(requests(syst))
+    mix.close()
+    overlap = sum(1 for a in rb for b in rb if a.rid < b.rid and a.start < b.end and b.start < a.end)
+    print("overlapping request pairs in the mix:", overlap, flush=True)
+    assert overlap > 0
+    for a, b in zip(sorted(ra, key=lambda r: r.rid), sorted(rb, key=lambda r: r.rid)):
+        for k in a.result["energy"]:
+            assert a.result["energy"][k] == b.result["energy"][k], (a.rid, k)
+        assert np.array_equal(a.result["grad"], b.result["grad"]), a.rid
+        assert np.array_equal(a.result["charges"], b.result["charges"]), a.rid
+        assert np.array_equal(a.result["positions"], b.result["positions"]), a.rid
+
+
+@pytest.mark.gpu
+def test_poisson_trace_reports_latency(ffield_text):
+    from reaxamd.serving import Server, latency_stats
+    syst = small_batch(8, seed=5)
+    reqs = requests(syst, steps=10)
+    rng = np.random.default_rng(1)
+    t = 0.0
+    for r in reqs:
+        t += rng.exponential(0.02)
+        r.arrival = t
+    srv = Server(ffield_text, slots=3)
+    done = srv.run(reqs, realtime=True)
+    srv.close()
+    st = latency_stats(done)
+    print(st, flush=True)
+    assert all(r.end > r.start >= r.arrival - 1e-6 for r in done)
+    assert st["p99_s"] >= st["p50_s"] > 0
