@@ -115,6 +115,7 @@ class Server:
         t0 = time.perf_counter()
         threads = []
         errors = []
+        admitted = {}   # device bytes reserved for each running request
 
         def work(slot, req):
             try:
@@ -125,7 +126,7 @@ class Server:
             finally:
                 req.end = time.perf_counter() - t0
                 with self._lock:
-                    self._mem_in_use -= self._need(req)
+                    self._mem_in_use -= admitted[req.rid]
                     self._lock.notify_all()
                 self._free.put(slot)
 
@@ -141,6 +142,7 @@ class Server:
                 while self._mem_in_use + need > self.budget:
                     self._lock.wait()
                 self._mem_in_use += need
+                admitted[req.rid] = need
             slot = self._free.get()   # admission by slot
             req.start = time.perf_counter() - t0
             th = threading.Thread(target=work, args=(slot, req), daemon=True)
@@ -172,3 +174,12 @@ def batch_systems(count=512, nmin=100, nmax=3000, seed=1):
 def latency_stats(reqs):
     lat = np.array([r.end - r.arrival for r in reqs])
     return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean())}
+
+
+def serving_requests(count=32, nmin=200, nmax=20000, smin=10, smax=1000, seed=2, t_target=2500.0, dt=0.1):
+    """The serving workload: systems from the batch generator with N log-uniform in [nmin, nmax] and step counts uniform in
+    [smin, smax]; arrival times are set by the caller (e.g. Poisson)."""
+    rng = np.random.default_rng(seed)
+    syst = batch_systems(count, nmin=nmin, nmax=nmax, seed=seed)
+    return [Request(rid=k, system=sy, steps=int(rng.integers(smin, smax + 1)), t_target=t_target, dt=dt, seed=1000 + k)
+            for k, sy in enumerate(syst)]

@@ -95,21 +95,36 @@ def ch4_o2_gas(n_target=600, density=0.3, seed=1, dmin=2.5):
     o2 = np.array([[0, 0, -0.604], [0, 0, 0.604]])
     mols = [(ch4, [1, 2, 2, 2, 2])] * units + [(o2, [3, 3])] * (2 * units)
     order = rng.permutation(len(mols))
-    centres = []
+    rc = dmin + 1.2                      # minimum distance between molecule centres
+    nc = max(1, int(L // rc))            # cells at least rc wide: a clash can only come from the 27 surrounding cells
+    cells = {}
     xs, ts = [], []
     for k in order:
         geom, ty = mols[k]
         for _ in range(10000):
             c = rng.random(3) * L
-            if centres:
-                d = np.array(centres) - c
-                d -= L * np.round(d / L)
-                if np.min(np.einsum("ij,ij->i", d, d)) < (dmin + 1.2) ** 2:
-                    continue
-            break
+            cc = np.minimum((c / L * nc).astype(int), nc - 1)
+            clash = False
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        for o in cells.get(((cc[0] + dx) % nc, (cc[1] + dy) % nc, (cc[2] + dz) % nc), ()):
+                            d = o - c
+                            d -= L * np.round(d / L)
+                            if d @ d < rc * rc:
+                                clash = True
+                                break
+                        if clash:
+                            break
+                    if clash:
+                        break
+                if clash:
+                    break
+            if not clash:
+                break
         else:
             raise RuntimeError("gas packing failed")
-        centres.append(c)
+        cells.setdefault(tuple(cc), []).append(c)
         R = _rotations(rng, 1)[0]
         xs.append(c + geom @ R.T)
         ts += ty
