@@ -59,6 +59,15 @@ template <class T> __device__ inline T safe_exp(T x) {
   return rexp<T>(x > cap ? cap : x);
 }
 
+// d/dD of the switching ratio (2 + a) / (1 + a + b), where a' = ka a and b' = kb b and inv = 1 / (1 + a + b): the f8, f9 and
+// f11 functions of the valence, penalty and torsion terms. The quotient rule as usually written, [a' (1 + a + b) - (2 + a)(a' + b')]
+// / (1 + a + b)^2, holds a^2 terms that cancel; in FP32 they overflow once a passes ~1e19 (exp(10 * 4.5) in f11 at two
+// under-coordinated centres) and the derivative, which is ~1/a, comes out infinite. With the cancellation done by hand,
+// [-ka a - 2 kb b + (ka - kb) a b] inv inv, and a, b from safe_exp, every intermediate stays finite.
+template <class T> __device__ inline T ratio2_deriv(T a, T b, T ka, T kb, T inv) {
+  return ((-ka * a - T(2) * kb * b + (ka - kb) * (a * b)) * inv) * inv;
+}
+
 __device__ inline size_t sl(int p, int i, int n) { return (size_t)p * (size_t)n + (size_t)i; }
 
 template <class T> struct V3 { T x, y, z; };
@@ -67,13 +76,26 @@ template <class T> __device__ inline V3<T> cross3(const V3<T>& a, const V3<T>& b
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-// angle between two bond vectors from the same centre, and d cos(theta) / d x of (end a, centre, end b) (reference bonded.cpp:58-75)
+// angle between two bond vectors from the same centre, its sine, and d cos(theta) / d x of (end a, centre, end b) (reference
+// bonded.cpp:58-75). FP64 follows the reference: theta = acos of the clamped cosine, sin(theta) of that. FP32 takes
+// theta = atan2(|a x b|, a.b) and sin = |a x b| / (|a||b|) instead: acosf of a cosine within an ulp of -1 or 1 loses half the
+// digits, and sinf(acosf(-1)) is -8.7e-8 (the float nearest pi lies above pi), a negative sine that flips the sign of the
+// torsion's 1/tan guard. The cosine stays the clamped dot product in both.
 template <class T>
-__device__ inline void angle_and_dcos(const V3<T>& a, T da, const V3<T>& b, T db, T& theta, T& cost, V3<T>& dca, V3<T>& dcj, V3<T>& dcb) {
+__device__ inline void angle_and_dcos(const V3<T>& a, T da, const V3<T>& b, T db, T& theta, T& cost, T& sint, V3<T>& dca, V3<T>& dcj,
+                                      V3<T>& dcb) {
   cost = dot3(a, b) / (da * db);
   if (cost > T(1)) cost = T(1);
   if (cost < T(-1)) cost = T(-1);
-  theta = racos<T>(cost);
+  if (sizeof(T) == 8) {
+    theta = racos<T>(cost);
+    sint = rsin<T>(theta);
+  } else {
+    const V3<T> c = cross3(a, b);
+    const T sn = rsqrt_<T>(dot3(c, c));
+    theta = ratan2<T>(sn, dot3(a, b));
+    sint = sn / (da * db);
+  }
   const T da2 = da * da, db2 = db * db, inv = T(1) / (da * db), inv3 = inv * inv * inv, cd = dot3(a, b) * inv3;
   dca = {b.x * inv - cd * db2 * a.x, b.y * inv - cd * db2 * a.y, b.z * inv - cd * db2 * a.z};
   dcj = {-(b.x + a.x) * inv + cd * (db2 * a.x + da2 * b.x), -(b.y + a.y) * inv + cd * (db2 * a.y + da2 * b.y),
@@ -348,7 +370,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
         const TR gp3 = (TR)g.gp3, gp4 = (TR)g.gp4, gp7 = (TR)g.gp7, gp10 = (TR)g.gp10;
         const TR exphu = rexp<TR>(-gp7 * ((BO - TR(2.50)) * (BO - TR(2.50))));
         const TR ea = rexp<TR>(-gp3 * ((TR)st.tbo[i] - BO)), eb = rexp<TR>(-gp3 * ((TR)st.tbo[j] - BO));
-        const TR eov = rexp<TR>(gp4 * ((TR)st.Delta[i] + (TR)st.Delta[j]));
+        const TR eov = safe_exp<TR>(gp4 * ((TR)st.Delta[i] + (TR)st.Delta[j]));
         const TR hov = TR(1) / (TR(1) + TR(25.0) * eov);
         e_bond += (double)(gp10 * exphu * hov * (ea + eb));
         cdbo += (double)(gp10 * exphu * hov * (ea + eb) * (gp3 - TR(2.0) * gp7 * (BO - TR(2.50))));
@@ -492,12 +514,12 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
     else if (SBO <= TR(1)) { SBO2 = rpow<TR>(SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(SBO, p_val9 - TR(1)); }
     else if (SBO < TR(2)) { SBO2 = TR(2) - rpow<TR>(TR(2) - SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(TR(2) - SBO, p_val9 - TR(1)); }
     else { SBO2 = 2; CSBO2 = 0; }
-    ev6 = rexp<TR>(p_val6 * Dboc);
+    ev6 = safe_exp<TR>(p_val6 * Dboc);
     const TR Delta = (TR)st.Delta[j], Dval = (TR)st.Delta_val[j];
-    const TR ep3 = rexp<TR>(-p_pen3 * Delta), ep4 = rexp<TR>(p_pen4 * Delta), trm = TR(1) + ep3 + ep4;
-    f9 = (TR(2) + ep3) / trm;
-    Cf9 = (-p_pen3 * ep3 * trm - (TR(2) + ep3) * (-p_pen3 * ep3 + p_pen4 * ep4)) / (trm * trm);
-    ec2 = rexp<TR>(p_coa2 * Dval);
+    const TR ep3 = safe_exp<TR>(-p_pen3 * Delta), ep4 = safe_exp<TR>(p_pen4 * Delta), itrm = TR(1) / (TR(1) + ep3 + ep4);
+    f9 = (TR(2) + ep3) * itrm;
+    Cf9 = ratio2_deriv<TR>(ep3, ep4, -p_pen3, p_pen4, itrm);
+    ec2 = safe_exp<TR>(p_coa2 * Dval);
   }
   // this lane's bond (j, i)
   const bool mine = act && p < nb;
@@ -525,10 +547,9 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
       if (BOA_jk > TR(0) && BOij > thb_cut && BOjk > thb_cut && BOij * BOjk > thb_cutsq && ai.y != 0) {
         const V3<TR> vjk{(TR)st.b_dx[qk], (TR)st.b_dy[qk], (TR)st.b_dz[qk]};
         const TR djk = (TR)st.b_d[qk];
-        TR th, ct;
+        TR th, ct, sth;
         V3<TR> dci, dcj, dck;
-        angle_and_dcos<TR>(vij, dij, vjk, djk, th, ct, dci, dcj, dck);
-        TR sth = rsin<TR>(th);
+        angle_and_dcos<TR>(vij, dij, vjk, djk, th, ct, sth, dci, dcj, dck);
         if (sth < TR(1.0e-5)) sth = TR(1.0e-5);
         const TR tbok = (TR)st.tbo[k];
         for (int a = 0; a < ai.y; ++a) {
@@ -540,9 +561,9 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
           const TR Cf7ij = p_val3 * p_val4 * rpow<TR>(BOA_ij, p_val4 - TR(1)) * e3ij;
           const TR e3jk = rexp<TR>(-p_val3 * rpow<TR>(BOA_jk, p_val4)), f7jk = TR(1) - e3jk;
           const TR Cf7jk = p_val3 * p_val4 * rpow<TR>(BOA_jk, p_val4 - TR(1)) * e3jk;
-          const TR ev7 = rexp<TR>(-p_val7 * Dboc), trm8 = TR(1) + ev6 + ev7;
-          const TR f8 = p_val5 - ((p_val5 - TR(1)) * (TR(2) + ev6) / trm8);
-          const TR Cf8 = ((TR(1) - p_val5) / (trm8 * trm8)) * (p_val6 * ev6 * trm8 - (TR(2) + ev6) * (p_val6 * ev6 - p_val7 * ev7));
+          const TR ev7 = safe_exp<TR>(-p_val7 * Dboc), itrm8 = TR(1) / (TR(1) + ev6 + ev7);
+          const TR f8 = p_val5 - (p_val5 - TR(1)) * ((TR(2) + ev6) * itrm8);
+          const TR Cf8 = (TR(1) - p_val5) * ratio2_deriv<TR>(ev6, ev7, p_val6, -p_val7, itrm8);
           const TR th0 = (TR(180.0) - th00 * (TR(1) - rexp<TR>(-p_val10 * (TR(2) - SBO2)))) * TR(3.14159265) / TR(180.0);
           const TR e2t = rexp<TR>(-p_val2 * ((th0 - th) * (th0 - th)));
           const TR e12t = p_val1 >= TR(0) ? p_val1 * (TR(1) - e2t) : p_val1 * -e2t;
@@ -562,7 +583,7 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
                           rexp<TR>(-p_coa4 * ((BOA_ij - TR(1.5)) * (BOA_ij - TR(1.5)))) * rexp<TR>(-p_coa4 * ((BOA_jk - TR(1.5)) * (BOA_jk - TR(1.5))));
           e_coa += (double)ecoa;
           const TR CEc1 = -TR(2) * p_coa4 * (BOA_ij - TR(1.5)) * ecoa, CEc2 = -TR(2) * p_coa4 * (BOA_jk - TR(1.5)) * ecoa;
-          const TR CEc3 = -p_coa2 * ec2 * ecoa / (TR(1) + ec2), CEc4 = -TR(2) * p_coa3 * dti * ecoa, CEc5 = -TR(2) * p_coa3 * dtk * ecoa;
+          const TR CEc3 = -p_coa2 * (ec2 / (TR(1) + ec2)) * ecoa, CEc4 = -TR(2) * p_coa3 * dti * ecoa, CEc5 = -TR(2) * p_coa3 * dtk * ecoa;
           own_cdbo += (double)(CE1 + CEp2 + (CEc1 - CEc4));
           to_cdbo += (double)(CE2 + CEp3 + (CEc2 - CEc5));
           cdself += (double)((CE3 + CE7) + CEp1 + CEc3);
@@ -654,10 +675,10 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
     const TR Dj = (TR)st.Delta_boc[j], Dk = (TR)st.Delta_boc[k];
     et2jk = rexp<TR>(-p_tor2 * BOA_jk);
     ec2jk = rexp<TR>(-p_cot2 * ((BOA_jk - TR(1.5)) * (BOA_jk - TR(1.5))));
-    const TR et3 = rexp<TR>(-p_tor3 * (Dj + Dk)), et4 = rexp<TR>(p_tor4 * (Dj + Dk)), it34 = TR(1) / (TR(1) + et3 + et4);
+    const TR et3 = safe_exp<TR>(-p_tor3 * (Dj + Dk)), et4 = safe_exp<TR>(p_tor4 * (Dj + Dk)), it34 = TR(1) / (TR(1) + et3 + et4);
     f11 = (TR(2) + et3) * it34;
     BOpijk = (TR)st.bo_pi[qk];
-    dfn11 = (-p_tor3 * et3 + (p_tor3 * et3 - p_tor4 * et4) * (TR(2) + et3) * it34) * it34;
+    dfn11 = ratio2_deriv<TR>(et3, et4, -p_tor3, p_tor4, it34);
   }
   double cdbo_jk = 0, cdbopi_jk = 0;
   for (int pi = 0; pi < nbw; ++pi) {
@@ -669,10 +690,10 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
         const int i = st.b_j[qi], si_ = st.b_s[qi], ti = st.type[i];
         const TR rij = (TR)st.b_d[qi], BOA_ij = BOij - thb_cut;
         const V3<TR> dji{(TR)st.b_dx[qi], (TR)st.b_dy[qi], (TR)st.b_dz[qi]};
-        TR thijk, cijk_;
+        TR thijk, cijk_, sijk;
         V3<TR> dci, dcj, dck;   // d cos(theta_ijk) / d x_i, x_j, x_k
-        angle_and_dcos<TR>(dji, rij, djk, rjk, thijk, cijk_, dci, dcj, dck);
-        const TR sijk = rsin<TR>(thijk), cijk = rcos<TR>(thijk), tijk = tan_inv<TR>(sijk, cijk);
+        angle_and_dcos<TR>(dji, rij, djk, rjk, thijk, cijk_, sijk, dci, dcj, dck);
+        const TR cijk = sizeof(TR) == 8 ? rcos<TR>(thijk) : cijk_, tijk = tan_inv<TR>(sijk, cijk);
         const TR et2ij = rexp<TR>(-p_tor2 * BOA_ij), ec2ij = rexp<TR>(-p_cot2 * ((BOA_ij - TR(1.5)) * (BOA_ij - TR(1.5))));
         for (int pl = 0; pl < nbk; ++pl) {
           if (pl == symk) continue;
@@ -687,10 +708,10 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
           const DevTorsion P = t.tors[ti4];
           const TR rkl = (TR)st.b_d[ql], BOA_kl = BOkl - thb_cut;
           const V3<TR> dkl{(TR)st.b_dx[ql], (TR)st.b_dy[ql], (TR)st.b_dz[ql]};
-          TR thjkl, cjkl_;
+          TR thjkl, cjkl_, sjkl;
           V3<TR> ecj, eck, ecl;   // d cos(theta_jkl) / d x_j, x_k, x_l
-          angle_and_dcos<TR>(dkj, rjk, dkl, rkl, thjkl, cjkl_, ecj, eck, ecl);
-          const TR sjkl = rsin<TR>(thjkl), cjkl = rcos<TR>(thjkl), tjkl = tan_inv<TR>(sjkl, cjkl);
+          angle_and_dcos<TR>(dkj, rjk, dkl, rkl, thjkl, cjkl_, sjkl, ecj, eck, ecl);
+          const TR cjkl = sizeof(TR) == 8 ? rcos<TR>(thjkl) : cjkl_, tjkl = tan_inv<TR>(sjkl, cjkl);
           // x_i - x_l in j's frame
           const V3<TR> dli{dji.x - djk.x - dkl.x, dji.y - djk.y - dkl.y, dji.z - djk.z - dkl.z};
           const TR rli = rsqrt_<TR>(dot3(dli, dli));
