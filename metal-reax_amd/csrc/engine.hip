@@ -175,6 +175,7 @@ struct Engine {
   int q_dbl_n = -1;   // atom count the charge buffers were sized for
   DBuf<float> q_f32;
   DBuf<unsigned short> q_u16;
+  DBuf<unsigned short> q_key;
   bool qeq_ready = false, qeq_h_valid = false;
   int qeq_imax_cap = 0;
   int hist_count = 0;
@@ -198,10 +199,13 @@ struct Engine {
   int* flags_h = nullptr;
   double* disp_h = nullptr;
   hipEvent_t ev_flags = nullptr, ev_disp = nullptr;
+  bool nb_rows = true;   // non-bonded pass over the charge-matrix rows when they are current (RXA_NB_ROWS=0: skin list always)
 
   Engine(int dev) : device(dev) {
     const char* sort_env = std::getenv("RXA_ATOM_SORT");
     if (sort_env && std::atoi(sort_env) == 0) sort_atoms = false;
+    const char* rows_env = std::getenv("RXA_NB_ROWS");
+    if (rows_env && std::atoi(rows_env) == 0) nb_rows = false;
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     timer.s = stream;
@@ -503,7 +507,7 @@ struct Engine {
         HIPCHK(hipMemcpy(d_src.p, src.data(), n * sizeof(int), hipMemcpyHostToDevice));
         permute(d_src.p);
       } else {
-        hist_count = 0;
+        reset_history();
         md_ready = false;
       }
     }
@@ -632,7 +636,11 @@ struct Engine {
       launch_atom_terms(prec, st, g, tab, stream);
       timer.end(PH_TERMS, t0);
       t0 = timer.begin(PH_NONB);
-      launch_nonbonded(prec, st, cell, g, tab, stream);
+      // the charge-matrix rows of these positions already list the non-bonded pairs in skin-list order
+      const bool rows = prec != Precision::FP64 && qeq_h_valid && qb.keyed && qb.key && g.q_swb >= g.nonb_cut &&
+                        nb_rows;
+      if (rows) launch_nonbonded_rows(st, g, tab, qb, stream);
+      else launch_nonbonded(prec, st, cell, g, tab, stream);
       timer.end(PH_NONB, t0);
       HIPCHK(hipEventSynchronize(ev_flags));
       if (flags_h[1] <= maxb) {
@@ -685,6 +693,8 @@ struct Engine {
     if (qb.narrow) { q_u16.alloc((size_t)n * maxnb); qb.col16 = q_u16.p; } else { q_u16.release(); qb.col16 = nullptr; }
     q_f32.alloc((size_t)n * maxnb);
     qb.val32 = q_f32.p;
+    q_key.alloc((size_t)n * maxnb);
+    qb.key = q_key.p;
     // val shares the double pool
     alloc_qeq_doubles();
   }
@@ -736,6 +746,16 @@ struct Engine {
   DBuf<int> iscal_buf;
   void alloc_qeq() { alloc_qeq_matrix(); }
 
+  // A new atom set starts the stock extrapolation from a zero history: the first solve starts from zero and the later ones
+  // extrapolate over the solutions so far and zeros, exactly as for atoms the stock fix has just created.
+  void reset_history() {
+    hist_count = 0;
+    if (qb.s_hist && q_dbl_n == n) {
+      HIPCHK(hipMemsetAsync(qb.s_hist, 0, 4 * (size_t)n * sizeof(double), stream));
+      HIPCHK(hipMemsetAsync(qb.t_hist, 0, 4 * (size_t)n * sizeof(double), stream));
+    }
+  }
+
   void qeq_setup(double swa, double swb, const double* chi, const double* eta, const double* gamma) {
     if (!have_ff) throw ApiError(RXA_ESTATE, "pair_coeff must precede the charge fix setup");
     if (!(swb > swa) || !(swb > 0)) throw ApiError(RXA_EINVAL, "charge equilibration taper: need swb > swa");
@@ -772,6 +792,7 @@ struct Engine {
     if (!nbr_valid) build_neighbors();
     if (!qeq_h_valid) {
       hipEvent_t t0 = timer.begin(PH_QEQ_H);
+      qb.keyed = (2 * smx[0] + 1) * (2 * smx[1] + 1) * (2 * smx[2] + 1) <= 32768 ? 1 : 0;
       launch_qeq_build(st, cell, g, tab, qb, stream);
       timer.end(PH_QEQ_H, t0);
       qeq_h_valid = true;
@@ -1249,7 +1270,7 @@ int rxa_qeq_charges(rxa_context* c, int imax, double tol, int strict, double* q_
 }
 
 int rxa_qeq_reset_history(rxa_context* c) {
-  return guard(c, [&](Engine& e) { e.hist_count = 0; });
+  return guard(c, [&](Engine& e) { e.reset_history(); });
 }
 
 int rxa_md_setup(rxa_context* c, const double* v, const double* mass_by_type, double dt, double t_target, double t_damp, double skin,

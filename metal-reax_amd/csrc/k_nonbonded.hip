@@ -270,6 +270,71 @@ __global__ void __launch_bounds__(64 * kNbWaves) k_nonbonded_f(State st, Globals
   }
 }
 
+// pair_dvec_list for a charge-matrix entry: the same operands in the same order, the shift vector found through the entry's key
+#pragma clang fp contract(off)
+__device__ inline double pair_dvec_key(const State& st, const double4& pi, const double4& pj, int key, double& dx, double& dy, double& dz) {
+  const bool canon = (key & 0x8000) != 0;
+  const double4 S = st.shv[key & 0x7FFF];
+  const double ax = canon ? pj.x : pi.x, ay = canon ? pj.y : pi.y, az = canon ? pj.z : pi.z;
+  const double bx = canon ? pi.x : pj.x, by = canon ? pi.y : pj.y, bz = canon ? pi.z : pj.z;
+  const double ex = (ax + S.x) - bx, ey = (ay + S.y) - by, ez = (az + S.z) - bz;
+  dx = canon ? ex : -ex;
+  dy = canon ? ey : -ey;
+  dz = canon ? ez : -ez;
+  return dx * dx + dy * dy + dz * dz;
+}
+#pragma clang fp contract(on)
+
+// The single-precision terms over the charge-matrix rows: the rows hold, in skin-list order, exactly the pairs within the matrix
+// cutoff, so every lane carries a pair (no distance pre-pass, no compaction) and a row costs its column and key (4 bytes per
+// pair with 16-bit columns) instead of the 8-byte skin entries of all pairs out to the list radius. Lane l takes entries l,
+// l + 64, ..., the assignment of the skin-list kernel's compacted batches, so the sums are the same.
+template <class TC>
+__global__ void __launch_bounds__(256) k_nonbonded_rows(State st, Globals g, Tables t, const TC* __restrict__ col,
+                                                        const unsigned short* __restrict__ key, const int* __restrict__ rcnt) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  const int lane = lane_id();
+  const int ti = st.type[i];
+  NbAcc a;
+  if (ti >= 0) {
+    const size_t base = (size_t)i * st.maxnb;
+    const int cnt = rcnt[i];
+    const float qi = (float)st.q[i];
+    const double4 pi = st.pos[i];
+    const double cut = g.nonb_cut, cut2 = cut * cut * (1.0 + 1e-12);
+    const NbPairF* __restrict__ row = t.nbf + ti * g.ntypes;
+    for (int k = lane; k < cnt; k += 64) {
+      const int j = (int)col[base + k];
+      const double4 pj = st.pos[j];
+      double dx, dy, dz;
+      const double r2 = pair_dvec_key(st, pi, pj, (int)key[base + k], dx, dy, dz);
+      if (r2 <= cut2) {
+        const double rd = sqrt(r2);
+        if (rd <= cut) nb_pair_f(g, row, qi, (float)st.q[j], (int)pj.w, rd, dx, dy, dz, a);
+      }
+    }
+  }
+  double gx = wave_sum(a.gx), gy = wave_sum(a.gy), gz = wave_sum(a.gz), ev = wave_sum(a.ev), ee = wave_sum(a.ee);
+  double v0 = wave_sum(a.v[0]), v1 = wave_sum(a.v[1]), v2 = wave_sum(a.v[2]), v3 = wave_sum(a.v[3]), v4 = wave_sum(a.v[4]),
+         v5 = wave_sum(a.v[5]);
+  if (lane == 0) {
+    const int n = st.n;
+    st.gnb_x[i] = gx; st.gnb_y[i] = gy; st.gnb_z[i] = gz;
+    st.eatom[T_VDW * n + i] = ev;
+    st.eatom[T_ELE * n + i] = ee;
+    double ep = 0;
+    if (ti >= 0) {
+      const DevSpecies& sp = t.sp[ti];
+      const float q = (float)st.q[i];
+      ep = (double)(23.02f * ((float)sp.chi * q + ((float)sp.eta / 2.0f) * (q * q)));
+    }
+    st.eatom[T_POL * n + i] = ep;
+    st.vatom[0 * n + i] = v0; st.vatom[1 * n + i] = v1; st.vatom[2 * n + i] = v2;
+    st.vatom[3 * n + i] = v3; st.vatom[4 * n + i] = v4; st.vatom[5 * n + i] = v5;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t, QeqBuffers qb) {
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
@@ -286,11 +351,13 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
     bool keep = false;
     int j = 0;
     double h = 0;
+    int e = 0;
     if (k < cnt && ti >= 0) {
       j = st.nb_j[base + k];
+      e = st.nb_s[base + k];
       double dx, dy, dz;
       int tj;
-      const double r2 = pair_dvec_list(st, pi, j, st.nb_s[base + k], dx, dy, dz, tj);
+      const double r2 = pair_dvec_list(st, pi, j, e, dx, dy, dz, tj);
       if (tj >= 0 && r2 <= swb2) {
         const double r = sqrt(r2);
         double Tap = T[7] * r + T[6];
@@ -304,6 +371,11 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
       const int slot = m + lanes_below(msk);
       if (qb.narrow) qb.col16[base + slot] = (unsigned short)j; else qb.col[base + slot] = j;
       if (qb.fp32) qb.val32[base + slot] = (float)h; else qb.val[base + slot] = h;
+      if (qb.keyed) {
+        const bool canon = (e & kCanon) != 0;
+        const int code = entry_code(e);
+        qb.key[base + slot] = (unsigned short)((canon ? 0x8000 : 0) | shift_index(st, canon ? code : shift_neg(code)));
+      }
     }
     m += __popcll(msk);
   }
@@ -325,6 +397,12 @@ void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals
   (void)c;
   if (p == Precision::FP64) hipLaunchKernelGGL(k_nonbonded<double>, grid, blk, 0, s, st, g, t);
   else hipLaunchKernelGGL(k_nonbonded_f, grid, blk, 0, s, st, g, t);
+}
+
+void launch_nonbonded_rows(const State& st, const Globals& g, const Tables& t, const QeqBuffers& qb, hipStream_t s) {
+  const dim3 grid(nblk((long long)st.n * 64, 256)), blk(256);
+  if (qb.narrow) hipLaunchKernelGGL(k_nonbonded_rows<unsigned short>, grid, blk, 0, s, st, g, t, qb.col16, qb.key, qb.cnt);
+  else hipLaunchKernelGGL(k_nonbonded_rows<int>, grid, blk, 0, s, st, g, t, qb.col, qb.key, qb.cnt);
 }
 
 void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s) {

@@ -60,6 +60,11 @@ struct QeqBuffers {
   double* val = nullptr;            // n * maxnb: H_ij in FP64 (FP64 mode)
   float* val32 = nullptr;           // n * maxnb: H_ij in FP32 (mixed and FP32 modes)
   int narrow = 0;
+  // n * maxnb: how the pair of each entry is reached from the row's atom, for the non-bonded pass over the same rows: bit 15 set
+  // if the row's atom is the pair's canonical end, bits 0-14 the index in the shift-vector table of the lattice shift applied to
+  // the canonical end's partner. Written when keyed == 1 (shift table of at most 32768 vectors).
+  unsigned short* key = nullptr;
+  int keyed = 0;
   double* eta = nullptr;     // n: diagonal (fix eta, 2 x ffield)
   double* hinv = nullptr;    // n: 1 / eta
   double* chi = nullptr;     // n
@@ -74,6 +79,9 @@ struct QeqBuffers {
 };
 constexpr int kCgBlocks = 256;
 void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s);
+// Non-bonded terms over the charge-matrix rows instead of the skin list (mixed / FP32 modes): valid while the rows belong to the
+// current positions, carry keys, and the matrix cutoff covers nonb_cut.
+void launch_nonbonded_rows(const State& st, const Globals& g, const Tables& t, const QeqBuffers& qb, hipStream_t s);
 void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, double* y, hipStream_t s);
 void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s);
 void launch_qeq_mvdir(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);   // beta, d, q = A d, d.q partials

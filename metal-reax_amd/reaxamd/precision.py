@@ -92,7 +92,10 @@ def main():
     res = {"atoms": w.n}
     if not args.no_oracle:
         res["step_error_vs_fp64_oracle"] = step_error(ff, w, precs)
-    res["nve"] = {p: nve_drift(ff, w, p, args.nvt_steps, args.nve_steps, args.chunk) for p in precs}
+    res["nve"] = {}
+    for p in precs:
+        res["nve"][p] = nve_drift(ff, w, p, args.nvt_steps, args.nve_steps, args.chunk)
+        print(p, json.dumps(res["nve"][p]), file=sys.stderr, flush=True)
     print(json.dumps(res))
 
 

@@ -54,3 +54,25 @@ def test_poisson_trace_reports_latency(ffield_text):
     print(st, flush=True)
     assert all(r.end > r.start >= r.arrival - 1e-6 for r in done)
     assert st["p99_s"] >= st["p50_s"] > 0
+
+
+@pytest.mark.gpu
+def test_engine_reuse_matches_fresh_engine(ffield_text):
+    """An engine that ran one system and then receives another with the same atom count and ids (1..n) must give the fresh
+    engine's answer once its charge history is reset: the stock extrapolation of the new system starts from a zero history."""
+    from reaxamd.serving import Request, new_engine, run_one
+    from reaxamd import systems
+    a = systems.ch4_o2_gas(81, density=0.3, seed=11)
+    b = systems.ch4_o2_gas(81, density=0.3, seed=12)
+    assert np.array_equal(a.tags, b.tags)
+    ra_req, rb_req = Request(0, a, 15, t_target=2500.0, dt=0.1, seed=1), Request(1, b, 15, t_target=2500.0, dt=0.1, seed=2)
+    e = new_engine(ffield_text, "mixed")
+    run_one(e, ffield_text, ra_req, "mixed")
+    reused = run_one(e, ffield_text, rb_req, "mixed")
+    e.close()
+    f = new_engine(ffield_text, "mixed")
+    fresh = run_one(f, ffield_text, rb_req, "mixed")
+    f.close()
+    assert np.array_equal(reused["charges"], fresh["charges"])
+    assert np.array_equal(reused["positions"], fresh["positions"])
+    assert np.array_equal(reused["grad"], fresh["grad"])
