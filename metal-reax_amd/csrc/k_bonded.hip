@@ -831,13 +831,21 @@ __device__ inline void hbond_term(const DevHBond& P, TR eh2, const V3<TR>& dji, 
   CH3 = -p3 * (-r0 / (rjk * rjk) + TR(1) / r0) * e;
 }
 
-// Donor side: one wave per donor hydrogen j, lanes over its H-bond candidates (chunks of 64). The geometry of a candidate k is
-// computed once; for each bond (j, i) that can carry an H-bond (wave-uniform loop) the lane evaluates the triple (i, j, k):
-// energy and the force on j stay in the lane, the BO derivative coefficient and the force on i are reduced over the wave into
-// the bond slot, and the force on the acceptor k is written to the candidate's slot of hf for the acceptor to collect. Non-donors
-// only clear their H-bond energy.
+// Donor side: one wave per donor hydrogen j. A first pass over j's H-bond candidates (chunks of 64) computes each distance once,
+// clears the acceptor-force slot of every candidate beyond hbond_cut and compacts the candidates within it (list order) into a
+// per-wave LDS ring; the triple work then runs on full waves of 64 in-range candidates (typically one batch per donor instead of
+// two half-empty chunks). For each bond (j, i) that can carry an H-bond (wave-uniform loop) a lane evaluates the triple (i, j, k)
+// of its candidate: energy and the force on j stay in the lane, the BO derivative coefficient and the force on i are reduced over
+// the wave into the bond slot, and the force on the acceptor k is written to the candidate's slot of hf for the acceptor to
+// collect. Non-donors only clear their H-bond energy.
+constexpr int kHbWaves = 4;
+constexpr int kHbRing = 128;
+
 template <class TR>
-__global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables t) {
+__global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals g, Tables t) {
+  __shared__ int rk_[kHbWaves][kHbRing], rh_[kHbWaves][kHbRing], rt_[kHbWaves][kHbRing];
+  __shared__ double rx_[kHbWaves][kHbRing], ry_[kHbWaves][kHbRing], rz_[kHbWaves][kHbRing], rr_[kHbWaves][kHbRing];
+  const int wv = threadIdx.x >> 6;
   const int j = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (j >= st.n) return;
   const int lane = lane_id(), n = st.n, tj = st.type[j];
@@ -846,6 +854,8 @@ __global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables
     if (lane == 0) { st.eatom[T_HB * n + j] = 0.0; st.h_cnt[j] = 0; }
     return;
   }
+  int* rk = rk_[wv]; int* rh = rh_[wv]; int* rt = rt_[wv];
+  double *rx = rx_[wv], *ry = ry_[wv], *rz = rz_[wv], *rr = rr_[wv];
   const int nc = min(st.hc_cnt[j], st.maxhc);
   const size_t hbase = (size_t)j * st.maxhc;
   const double4 pj = st.pos[j];
@@ -854,19 +864,17 @@ __global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables
   TR* hf = reinterpret_cast<TR*>(st.hf);
   double e_hb = 0, fjx = 0, fjy = 0, fjz = 0;
   double w[6] = {0, 0, 0, 0, 0, 0};   // virial of the triples about the donor
-  int inrange = 0;
-  for (int h0 = 0; h0 < nc; h0 += 64) {
-    const int h = h0 + lane;
-    int k = 0, tk = -1;
-    bool ok = false;
+  int head = 0, fill = 0, inrange = 0;
+  // the triples of the m ring entries from head on
+  auto batch = [&](int m) {
+    const bool has = lane < m;
+    int k = 0, tk = 0, h = 0;
     double dx = 0, dy = 0, dz = 0, r = 1;
-    if (h < nc) {
-      k = st.hc_j[hbase + h];
-      r = sqrt(pair_dvec_list(st, pj, k, st.hc_s[hbase + h], dx, dy, dz, tk));
-      ok = r <= hcut;
+    if (has) {
+      const int e = (head + lane) & (kHbRing - 1);
+      k = rk[e]; tk = rt[e]; h = rh[e]; dx = rx[e]; dy = ry[e]; dz = rz[e]; r = rr[e];
     }
-    inrange += ok ? 1 : 0;
-    const long long tagk = ok ? st.tag[k] : 0;
+    const long long tagk = has ? st.tag[k] : 0;
     const V3<TR> djk{(TR)dx, (TR)dy, (TR)dz};
     const TR rjk = (TR)r;
     double fkx = 0, fky = 0, fkz = 0;
@@ -878,7 +886,7 @@ __global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables
       const TR rij = (TR)st.b_d[q];
       double c1 = 0, fx = 0, fy = 0, fz = 0;
       // the same atom as i, possibly another image, is not an acceptor of this triple (reference quirk Q-32)
-      if (ok && tagk != st.tag[i]) {
+      if (has && tagk != st.tag[i]) {
         const DevHBond P = t.hb[(ti * g.ntypes + tj) * g.ntypes + tk];
         if (P.r0 > 0.0) {
           const TR e2 = rexp<TR>(-(TR)P.p2 * (TR)st.bo[q]);
@@ -902,14 +910,43 @@ __global__ void __launch_bounds__(256) k_hbond_donor(State st, Globals g, Tables
       c1 = wave_sum(c1); fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
       if (lane == 0) { st.Cdbo[q] += c1; st.fpx[q] += fx; st.fpy[q] += fy; st.fpz[q] += fz; }
     }
-    if (h < nc) {
+    if (has) {
       const size_t o = 3 * (hbase + h);
       hf[o] = (TR)fkx; hf[o + 1] = (TR)fky; hf[o + 2] = (TR)fkz;
     }
+  };
+  for (int h0 = 0; h0 < nc; h0 += 64) {
+    const int h = h0 + lane;
+    int k = 0, tk = -1;
+    bool ok = false;
+    double dx = 0, dy = 0, dz = 0, r = 0;
+    if (h < nc) {
+      k = st.hc_j[hbase + h];
+      r = sqrt(pair_dvec_list(st, pj, k, st.hc_s[hbase + h], dx, dy, dz, tk));
+      ok = r <= hcut;
+      if (!ok) {   // beyond hbond_cut this step: no force on the acceptor from this slot
+        const size_t o = 3 * (hbase + h);
+        hf[o] = TR(0); hf[o + 1] = TR(0); hf[o + 2] = TR(0);
+      }
+    }
+    const unsigned long long mk = __ballot(ok);
+    if (ok) {
+      const int e = (head + fill + lanes_below(mk)) & (kHbRing - 1);
+      rk[e] = k; rt[e] = tk; rh[e] = h; rx[e] = dx; ry[e] = dy; rz[e] = dz; rr[e] = r;
+    }
+    fill += __popcll(mk);
+    inrange += __popcll(mk);
+    __builtin_amdgcn_wave_barrier();
+    if (fill >= 64) {
+      batch(64);
+      head = (head + 64) & (kHbRing - 1);
+      fill -= 64;
+      __builtin_amdgcn_wave_barrier();
+    }
   }
+  if (fill > 0) batch(fill);
   e_hb = wave_sum(e_hb); fjx = wave_sum(fjx); fjy = wave_sum(fjy); fjz = wave_sum(fjz);
   for (int c = 0; c < 6; ++c) w[c] = wave_sum(w[c]);
-  for (int off = 32; off > 0; off >>= 1) inrange += __shfl_xor(inrange, off, 64);
   if (lane == 0) {
     st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
     for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + j] += w[c];
@@ -1033,7 +1070,7 @@ void launch_terms_g(const State& st, const Globals& g, const Tables& t, hipStrea
   const dim3 grid(nblk((long long)st.n * G, 256)), blk(256), wgrid(nblk(st.n, 4));
   hipLaunchKernelGGL((k_valence<TR, G>), grid, blk, 0, s, st, g, t);
   hipLaunchKernelGGL((k_torsion<TR, G>), grid, blk, 0, s, st, g, t);
-  hipLaunchKernelGGL(k_hbond_donor<TR>, wgrid, blk, 0, s, st, g, t);
+  hipLaunchKernelGGL(k_hbond_donor<TR>, dim3(nblk(st.n, kHbWaves)), dim3(64 * kHbWaves), 0, s, st, g, t);
   hipLaunchKernelGGL(k_hbond_acceptor<TR>, wgrid, blk, 0, s, st, g, t);
 }
 
