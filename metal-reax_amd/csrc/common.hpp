@@ -211,10 +211,9 @@ __device__ inline int shift_index(const State& st, int code) {
 }
 
 #pragma clang fp contract(off)
-// pair_dvec for a list entry e = code | flag of neighbour j of the atom at pi, without tags or branches: the same operands in the
-// same order as pair_dvec (the table holds shift_vec of every code), so the result is bitwise identical
-__device__ inline double pair_dvec_list(const State& st, const double4& pi, int j, int e, double& dx, double& dy, double& dz) {
-  const double4 pj = st.pos[j];
+// pair_dvec for a list entry e = code | flag of neighbour j (position pj) of the atom at pi, without tags or branches: the same
+// operands in the same order as pair_dvec (the table holds shift_vec of every code), so the result is bitwise identical
+__device__ inline double pair_dvec_pos(const State& st, const double4& pi, const double4& pj, int e, double& dx, double& dy, double& dz) {
   const bool canon = (e & kCanon) != 0;
   const int code = entry_code(e);
   const double4 S = st.shv[shift_index(st, canon ? code : shift_neg(code))];
@@ -225,6 +224,9 @@ __device__ inline double pair_dvec_list(const State& st, const double4& pi, int 
   dy = canon ? ey : -ey;
   dz = canon ? ez : -ez;
   return dx * dx + dy * dy + dz * dz;
+}
+__device__ inline double pair_dvec_list(const State& st, const double4& pi, int j, int e, double& dx, double& dy, double& dz) {
+  return pair_dvec_pos(st, pi, st.pos[j], e, dx, dy, dz);
 }
 // the same, also returning the neighbour's type (carried in pos.w)
 __device__ inline double pair_dvec_list(const State& st, const double4& pi, int j, int e, double& dx, double& dy, double& dz, int& tj) {

@@ -200,14 +200,29 @@ struct Engine {
   double* disp_h = nullptr;
   hipEvent_t ev_flags = nullptr, ev_disp = nullptr;
   bool nb_rows = true;   // non-bonded pass over the charge-matrix rows when they are current (RXA_NB_ROWS=0: skin list always)
+  // The bonded chain (bond list ... BO-derivative pass) does not depend on the charges, and the charge solve and the non-bonded
+  // pass do not depend on it: it runs on a second stream (default priority) between a fork and a join event, overlapping the
+  // latency-bound CG kernels. RXA_OVERLAP=0, or timing a bonded phase, keeps everything on one stream.
+  bool overlap = true;
+  hipStream_t sb = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_bonded = nullptr, ev_cg = nullptr;
+  int* cgf_h = nullptr;      // pinned: CG activity flags read back per batch
+  double* cgs_h = nullptr;   // pinned: CG scalars read back per batch
+  size_t cg_cap_h = 0;
 
   Engine(int dev) : device(dev) {
     const char* sort_env = std::getenv("RXA_ATOM_SORT");
     if (sort_env && std::atoi(sort_env) == 0) sort_atoms = false;
     const char* rows_env = std::getenv("RXA_NB_ROWS");
     if (rows_env && std::atoi(rows_env) == 0) nb_rows = false;
+    const char* ovl_env = std::getenv("RXA_OVERLAP");
+    if (ovl_env && std::atoi(ovl_env) == 0) overlap = false;
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&sb, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_bonded, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_cg, hipEventDisableTiming));
     timer.s = stream;
     HIPCHK(hipHostMalloc((void**)&flags_h, 8 * sizeof(int), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&disp_h, sizeof(double), hipHostMallocDefault));
@@ -216,6 +231,13 @@ struct Engine {
   }
   ~Engine() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (sb) (void)hipStreamSynchronize(sb);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_bonded) (void)hipEventDestroy(ev_bonded);
+    if (ev_cg) (void)hipEventDestroy(ev_cg);
+    if (cgf_h) (void)hipHostFree(cgf_h);
+    if (cgs_h) (void)hipHostFree(cgs_h);
+    if (sb) (void)hipStreamDestroy(sb);
     if (ev_flags) (void)hipEventDestroy(ev_flags);
     if (ev_disp) (void)hipEventDestroy(ev_disp);
     if (flags_h) (void)hipHostFree(flags_h);
@@ -223,8 +245,14 @@ struct Engine {
     if (stream) (void)hipStreamDestroy(stream);
   }
   void sync() {
+    HIPCHK(hipStreamSynchronize(sb));
     HIPCHK(hipStreamSynchronize(stream));
     timer.resolve();
+  }
+  // the stream of the bonded chain
+  hipStream_t bstream() const {
+    const unsigned bonded_phases = 1u << PH_BONDS | 1u << PH_BOCORR | 1u << PH_TERMS | 1u << PH_DBOND;
+    return overlap && !(timer.on && (timer.mask & bonded_phases)) ? sb : stream;
   }
 
   // ---------------------------------------------------------------------------------------------------------------
@@ -457,6 +485,7 @@ struct Engine {
     if (!have_ff) throw ApiError(RXA_ESTATE, "pair_coeff (the force field) must be set before the atoms");
     if (!have_box) throw ApiError(RXA_ESTATE, "the box must be set before the atoms");
     if (nn <= 0 || nn > (1LL << 30)) throw ApiError(RXA_EINVAL, "atom count out of range");
+    sync();   // blocking copies below touch arrays that queued work may still use
     const bool realloc = (int)nn != n || !a_x.p;
     n = (int)nn;
     if (realloc) {
@@ -615,33 +644,33 @@ struct Engine {
     throw ApiError(RXA_EINVAL, "neighbour list capacity did not converge");
   }
 
-  // Forces and energies of the current positions and charges (queued on the stream). The bond-list capacity check is read back
-  // through pinned memory while the kernels that are valid whenever the lists fit (bond-order correction, bond / lone-pair /
-  // over-under terms, non-bonded) are already queued; only the group-size dependent kernels wait for it. An overflow drains
-  // the stream, grows the lists and redoes the step.
-  void forces(bool rebuild, bool energies = true) {
-    if (!nbr_valid || rebuild) build_neighbors();
+  // Forces and energies of the current positions and charges. The bonded chain is queued on the bonded stream after a fork
+  // from the main stream: bond list and uncorrected BO, the capacity flags (read back through pinned memory), BO correction and
+  // the per-atom terms (valid whenever the lists fit); the main stream meanwhile runs the charge solve and the non-bonded pass.
+  // Only the group-size dependent angle kernels wait for the flags. An overflow drains both streams, grows the lists and redoes
+  // the chain. The gather on the main stream joins the two.
+  void bonded_begin() {
+    const hipStream_t b = bstream();
+    if (b != stream) {
+      HIPCHK(hipEventRecord(ev_fork, stream));
+      HIPCHK(hipStreamWaitEvent(b, ev_fork, 0));
+    }
+    hipEvent_t t0 = timer.begin(PH_BONDS);
+    launch_fill_int(st.flags, 8, 0, b);
+    launch_bond_build(prec, st, cell, g, tab, b);
+    launch_bond_sym(st, b);
+    timer.end(PH_BONDS, t0);
+    HIPCHK(hipMemcpyAsync(flags_h, st.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, b));
+    HIPCHK(hipEventRecord(ev_flags, b));
+    t0 = timer.begin(PH_BOCORR);
+    launch_bo_correct(prec, st, g, tab, b);
+    timer.end(PH_BOCORR, t0);
+    t0 = timer.begin(PH_TERMS);
+    launch_atom_terms(prec, st, g, tab, b);
+    timer.end(PH_TERMS, t0);
+  }
+  void bonded_end() {
     for (int attempt = 0;; ++attempt) {
-      hipEvent_t t0 = timer.begin(PH_BONDS);
-      launch_fill_int(st.flags, 8, 0, stream);
-      launch_bond_build(prec, st, cell, g, tab, stream);
-      launch_bond_sym(st, stream);
-      timer.end(PH_BONDS, t0);
-      HIPCHK(hipMemcpyAsync(flags_h, st.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipEventRecord(ev_flags, stream));
-      t0 = timer.begin(PH_BOCORR);
-      launch_bo_correct(prec, st, g, tab, stream);
-      timer.end(PH_BOCORR, t0);
-      t0 = timer.begin(PH_TERMS);
-      launch_atom_terms(prec, st, g, tab, stream);
-      timer.end(PH_TERMS, t0);
-      t0 = timer.begin(PH_NONB);
-      // the charge-matrix rows of these positions already list the non-bonded pairs in skin-list order
-      const bool rows = prec != Precision::FP64 && qeq_h_valid && qb.keyed && qb.key && g.q_swb >= g.nonb_cut &&
-                        nb_rows;
-      if (rows) launch_nonbonded_rows(st, g, tab, qb, stream);
-      else launch_nonbonded(prec, st, cell, g, tab, stream);
-      timer.end(PH_NONB, t0);
       HIPCHK(hipEventSynchronize(ev_flags));
       if (flags_h[1] <= maxb) {
         if (flags_h[3]) throw ApiError(RXA_EDEVICE, "internal: a bond has no mirror slot");
@@ -652,14 +681,38 @@ struct Engine {
       if (attempt >= 5) throw ApiError(RXA_EINVAL, "bond list capacity did not converge");
       sync();
       alloc_bonds(std::max(maxb, (flags_h[1] + 3) / 4 * 4 + 4));
+      bonded_begin();
     }
+    const hipStream_t b = bstream();
     hipEvent_t t0 = timer.begin(PH_TERMS);
-    launch_angle_terms(prec, st, g, tab, bmax, stream);
+    launch_angle_terms(prec, st, g, tab, bmax, b);
     timer.end(PH_TERMS, t0);
     t0 = timer.begin(PH_DBOND);
-    launch_bond_forces(prec, st, stream);
+    launch_bond_derivatives(st, b);
+    timer.end(PH_DBOND, t0);
+    if (b != stream) HIPCHK(hipEventRecord(ev_bonded, b));
+  }
+  void nonbonded_launch() {
+    hipEvent_t t0 = timer.begin(PH_NONB);
+    // the charge-matrix rows of these positions already list the non-bonded pairs in skin-list order
+    const bool rows = prec != Precision::FP64 && qeq_h_valid && qb.keyed && qb.key && g.q_swb >= g.nonb_cut && nb_rows;
+    if (rows) launch_nonbonded_rows(st, g, tab, qb, stream);
+    else launch_nonbonded(prec, st, cell, g, tab, stream);
+    timer.end(PH_NONB, t0);
+  }
+  void join_gather(bool energies) {
+    if (bstream() != stream) HIPCHK(hipStreamWaitEvent(stream, ev_bonded, 0));
+    hipEvent_t t0 = timer.begin(PH_DBOND);
+    launch_gather(st, stream);
     if (energies) launch_energy_reduce(st, red.p, stream);   // the per-atom arrays hold the energies either way
     timer.end(PH_DBOND, t0);
+  }
+  void forces(bool rebuild, bool energies = true) {
+    if (!nbr_valid || rebuild) build_neighbors();
+    bonded_begin();
+    nonbonded_launch();
+    bonded_end();
+    join_gather(energies);
   }
 
   void compute(int flags, double* energies, double* grad, double* virial) {
@@ -802,63 +855,90 @@ struct Engine {
   // runs CG iterations until both systems stopped; batch launches, one readback per batch
   void ensure_cg_cap(int imax) {
     if (imax + 2 > qeq_imax_cap) { qeq_imax_cap = imax + 2; alloc_qeq_doubles(); }
+    const size_t need = 2 * (size_t)(std::max(qeq_imax_cap, imax + 2) + 2);
+    if (need > cg_cap_h) {
+      if (cgf_h) HIPCHK(hipHostFree(cgf_h));
+      if (cgs_h) HIPCHK(hipHostFree(cgs_h));
+      HIPCHK(hipHostMalloc((void**)&cgf_h, need * sizeof(int), hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void**)&cgs_h, (8 + need) * sizeof(double), hipHostMallocDefault));
+      cg_cap_h = need;
+    }
   }
-  // The first batch launches as many iterations as the previous solve needed (warm-started solves of consecutive steps need
-  // about the same number), later batches two at a time; iterations past convergence exit at once on the device flags.
   int cg_guess = 8;
-  std::vector<int> cg_flags_h;
-  std::vector<double> cg_scal_h;
-  void qeq_iterate(int imax, double tol) {
-    // launch order: mvdir(1) [update(1) mvdir(2)] [update(2) mvdir(3)] ...; mvdir(it) decides whether iteration it runs, so after
-    // a batch ending with mvdir(it) the flags of iteration it tell the host whether to go on
-    int it = 1;
-    int batch = std::max(1, cg_guess);
-    // flags of iterations 1..it and the scalars, read back in one go after each batch (no further synchronisation)
-    if (cg_flags_h.size() < 2 * (size_t)(imax + 2)) { cg_flags_h.resize(2 * (size_t)(imax + 2)); cg_scal_h.resize(8 + 2 * (size_t)(imax + 2)); }
-    hipEvent_t t0 = timer.begin(PH_QEQ_SPMV);
-    launch_qeq_mvdir(st, qb, 1, imax, tol, stream);
-    timer.end(PH_QEQ_SPMV, t0);
-    while (true) {
-      for (int b = 0; b < batch && it < imax; ++b) {
-        t0 = timer.begin(PH_QEQ_CG);
-        launch_qeq_update(st, qb, it, stream);
-        timer.end(PH_QEQ_CG, t0);
-        ++it;
-        t0 = timer.begin(PH_QEQ_SPMV);
-        launch_qeq_mvdir(st, qb, it, imax, tol, stream);
-        timer.end(PH_QEQ_SPMV, t0);
-      }
-      HIPCHK(hipMemcpyAsync(cg_flags_h.data(), qb.iscal, 2 * (size_t)it * sizeof(int), hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipMemcpyAsync(cg_scal_h.data(), qb.scal, (8 + 2 * (size_t)it) * sizeof(double), hipMemcpyDeviceToHost, stream));
-      sync();
-      const int* a = cg_flags_h.data() + 2 * (it - 1);
-      if ((!a[0] && !a[1]) || it >= imax) break;
-      batch = 2;
+  int cg_it = 0, cg_batch = 0;
+  // launches the next batch of iterations, then the asynchronous readback of the flags and scalars (pinned) and its event
+  void cg_launch_batch(int imax) {
+    for (int b = 0; b < cg_batch && cg_it < imax; ++b) {
+      hipEvent_t t0 = timer.begin(PH_QEQ_CG);
+      launch_qeq_update(st, qb, cg_it, stream);
+      timer.end(PH_QEQ_CG, t0);
+      ++cg_it;
+      t0 = timer.begin(PH_QEQ_SPMV);
+      launch_qeq_mvdir(st, qb, cg_it, imax, cg_tol, stream);
+      timer.end(PH_QEQ_SPMV, t0);
     }
-    for (int s = 0; s < 2; ++s) {
-      int k = 0;
-      while (k < it && cg_flags_h[2 * k + s]) ++k;
-      last_iters[s] = k + 1;   // the stock loop index at exit
-      last_rel[s] = std::sqrt(cg_scal_h[8 + 2 * (last_iters[s] - 1) + s]) / cg_scal_h[s];
-    }
-    cg_guess = std::max(last_iters[0], last_iters[1]) - 1;
+    HIPCHK(hipMemcpyAsync(cgf_h, qb.iscal, 2 * (size_t)cg_it * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(cgs_h, qb.scal, (8 + 2 * (size_t)cg_it) * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipEventRecord(ev_cg, stream));
   }
-
-  void qeq_charges(int imax, double tol, int strict) {
+  double cg_tol = 1e-6;
+  // Charge solve in two halves so that other work can be queued in between: qeq_begin queues the matrix (if not current), the
+  // initial residuals and the first batch of iterations (launch order mvdir(1) [update(1) mvdir(2)] ...; mvdir(it) decides
+  // whether iteration it runs); qeq_end reads each batch's flags back and queues more until both systems stopped, then the
+  // charges. The first batch is as long as the previous solve (warm-started solves of consecutive steps need about the same
+  // number of iterations), later ones two iterations; iterations past convergence exit at once on the device flags.
+  void qeq_begin(int imax, double tol) {
     if (!n) throw ApiError(RXA_ESTATE, "no atoms");
     if (imax < 1) throw ApiError(RXA_EINVAL, "charge equilibration: maxiter must be >= 1");
     ensure_cg_cap(imax);
     qeq_matrix();
     launch_qeq_init(st, qb, imax, tol, hist_count > 0 ? 1 : 0, stream);
-    qeq_iterate(imax, tol);
+    cg_start(imax, tol);
+  }
+  void cg_start(int imax, double tol) {
+    cg_tol = tol;
+    cg_it = 1;
+    cg_batch = std::max(1, cg_guess);
+    hipEvent_t t0 = timer.begin(PH_QEQ_SPMV);
+    launch_qeq_mvdir(st, qb, 1, imax, tol, stream);
+    timer.end(PH_QEQ_SPMV, t0);
+    cg_launch_batch(imax);
+  }
+  void cg_wait_end(int imax) {
+    while (true) {
+      HIPCHK(hipEventSynchronize(ev_cg));
+      const int* a = cgf_h + 2 * (cg_it - 1);
+      if ((!a[0] && !a[1]) || cg_it >= imax) break;
+      cg_batch = 2;
+      cg_launch_batch(imax);
+    }
+    for (int s = 0; s < 2; ++s) {
+      int k = 0;
+      while (k < cg_it && cgf_h[2 * k + s]) ++k;
+      last_iters[s] = k + 1;   // the stock loop index at exit
+      last_rel[s] = std::sqrt(cgs_h[8 + 2 * (last_iters[s] - 1) + s]) / cgs_h[s];
+    }
+    cg_guess = std::max(last_iters[0], last_iters[1]) - 1;
+  }
+  // CG iterations on the current initial residuals until both systems stopped (the raw solve of rxa_qeq_solve)
+  void qeq_iterate(int imax, double tol) {
+    cg_start(imax, tol);
+    cg_wait_end(imax);
+  }
+  void qeq_end(int imax, int strict) {
+    cg_wait_end(imax);
     launch_qeq_finish(st, qb, stream);
     hist_count = std::min(hist_count + 1, 4);
     qeq_iter_sum += 0.5 * (last_iters[0] + last_iters[1]);
     ++qeq_calls;
     if (strict)
       for (int s = 0; s < 2; ++s)
-        if (!(last_rel[s] <= tol))
+        if (!(last_rel[s] <= cg_tol))
           throw ApiError(RXA_ENOCONV, "fix qeq/reaxff/metal strict: CG did not converge after " + std::to_string(last_iters[s]) + " iterations");
+  }
+  void qeq_charges(int imax, double tol, int strict) {
+    qeq_begin(imax, tol);
+    qeq_end(imax, strict);
   }
 
   // the device charges in the caller's order
@@ -872,6 +952,7 @@ struct Engine {
   // ---------------------------------------------------------------------------------------------------------------
   void md_setup(const double* v, const double* mass_by_type, double dt, double tt, double tdamp, double sk, int qeq_every, int qimax, double qtol) {
     if (!n) throw ApiError(RXA_ESTATE, "no atoms");
+    sync();
     md_buf.alloc(8 * (size_t)n + 64 + 256);
     double* p = md_buf.p;
     md.vx = p; p += n; md.vy = p; p += n; md.vz = p; p += n; md.mass = p; p += n;
@@ -939,8 +1020,14 @@ struct Engine {
         build_neighbors();   // invalidates the matrix
         launch_md_ref_positions(st, md, stream);
       }
-      if (do_qeq) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
-      forces(false, s == steps - 1);   // energy totals are read once, after the last step
+      // bonded chain on the bonded stream while the main stream solves for the charges, then the non-bonded pass and the join;
+      // energy totals are read once, after the last step
+      bonded_begin();
+      if (do_qeq) qeq_begin(md_qeq_imax, md_qeq_tol);
+      bonded_end();
+      if (do_qeq) qeq_end(md_qeq_imax, 0);
+      nonbonded_launch();
+      join_gather(s == steps - 1);
       t0 = timer.begin(PH_MD);
       launch_md_half_kick(st, md, dtf, stream);
       if (!md_nve || s == steps - 1) launch_md_thermostat(st, md, stream);   // NVE: kinetic energy of the last step only
@@ -1249,6 +1336,7 @@ int rxa_qeq_solve(rxa_context* c, const double* b, const double* x0, int imax, d
     HIPCHK(hipMemcpyAsync(e.qb.x, xi.data(), xi.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
     rxa::launch_qeq_init(e.st, e.qb, imax, tol, 2, e.stream);
     e.qeq_iterate(imax, tol);
+    e.sync();
     std::vector<double> xo(2 * (size_t)n);
     HIPCHK(hipMemcpy(xo.data(), e.qb.x, xo.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) { x[e.perm_h[i]] = xo[2 * i]; x[n + e.perm_h[i]] = xo[2 * i + 1]; }

@@ -1093,11 +1093,14 @@ void launch_angle_terms(Precision p, const State& st, const Globals& g, const Ta
   else launch_terms_t<float>(st, g, t, bmax, s);
 }
 
-void launch_bond_forces(Precision, const State& st, hipStream_t s) {
+void launch_bond_derivatives(const State& st, hipStream_t s) {
   const dim3 grid(nblk(st.n, 128)), blk(128);
   hipLaunchKernelGGL(k_cdelta, grid, blk, 0, s, st);
   hipLaunchKernelGGL(k_dbond, grid, blk, 0, s, st);
-  hipLaunchKernelGGL(k_gather, grid, blk, 0, s, st);
+}
+
+void launch_gather(const State& st, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3(nblk(st.n, 128)), dim3(128), 0, s, st);
 }
 
 }  // namespace rxa

@@ -47,7 +47,9 @@ void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tab
 // largest bond count bmax
 void launch_atom_terms(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s);
 void launch_angle_terms(Precision p, const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s);
-void launch_bond_forces(Precision p, const State& st, hipStream_t s);
+// BO-derivative pass (CdDelta, then the per-atom bond sums); the final gather adds the non-bonded gradient and the virial
+void launch_bond_derivatives(const State& st, hipStream_t s);
+void launch_gather(const State& st, hipStream_t s);
 
 // ---- non-bonded terms and the charge-equilibration matrix (k_nonbonded.hip)
 void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
