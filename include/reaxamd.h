@@ -73,7 +73,7 @@ int rxa_get_charges(rxa_context* ctx, double* q);         /* n */
 /* ---- forces (replaces PairReaxFFMetal::compute, pair_reaxff_metal.cpp:412-524, i.e. mtl::Context::bonded + nonbonded,
  * metal_backend.hpp:78-80, and compute_bonded_core / compute_nonbonded_core, include/reaxmetal/bonded.hpp:56, nonbonded.hpp:35)
  * energies[RXA_NTERMS] (may be NULL), grad[3n] = dE/dx of the owned atoms (the adapter SUBTRACTS it from atom->f, as the reference
- * does at pair_reaxff_metal.cpp:501-504; may be NULL), virial[6] = xx yy zz xy xz yz of the non-bonded pairs (may be NULL). */
+ * does at pair_reaxff_metal.cpp:501-504; may be NULL), virial[6] = xx yy zz xy xz yz of all terms (may be NULL). */
 #define RXA_COMPUTE_REBUILD 1   /* force a neighbour-list rebuild */
 int rxa_compute(rxa_context* ctx, int flags, double* energies, double* grad, double* virial);
 /* the 13 terms -> LAMMPS' 14-slot pvector (eb ea elp emol ev epen ecoa ehb et eco ew ep efi eqeq), energy_terms.cpp:48-63 */
@@ -109,6 +109,21 @@ int rxa_md_setup(rxa_context* ctx, const double* v, const double* mass_by_type, 
 /* runs `steps` steps; thermo (may be NULL) receives: potential energy, kinetic energy, temperature, neighbour rebuilds, mean QEq
  * iterations (s + t)/2, last iters_s, last iters_t, max displacement since the last rebuild */
 int rxa_md_run(rxa_context* ctx, int steps, double* thermo);
+
+/* ---- several independent systems in one context (no counterpart in the reference, which runs one system per context:
+ * capabilities.cpp:112; for batched and served MD, where one launch of each kernel then covers every system)
+ * nsys systems, counts[nsys] atoms each, stored one after another in types / tags / x (3 per atom); boxes[9 nsys] = lo[3], hi[3],
+ * tilt[3] per system; periodic[3 nsys]. Atom ids are per system. Replaces rxa_set_box + rxa_set_atoms; every other call then
+ * acts on all systems at once (positions, charges, gradient: all atoms in this order; energies and thermo: totals). Each
+ * system keeps its own neighbour rebuilds, charge solve (own scalars and stopping), thermostat and energy sums, so its results
+ * do not depend on the systems packed with it. */
+int rxa_set_systems(rxa_context* ctx, int nsys, const int64_t* counts, const double* boxes, const int* periodic, const int* types,
+                    const int64_t* tags, const double* x);
+/* per-system results of the last force call / MD run: energies[RXA_NTERMS nsys], virial[6 nsys], thermo[4 nsys] = kinetic energy,
+ * temperature, stock loop index of the s and t solves of the last charge solve (each may be NULL) */
+int rxa_system_results(rxa_context* ctx, double* energies, double* virial, double* thermo);
+/* per-system thermostat target (K) and damping (fs; <= 0: NVE) after rxa_md_setup (either may be NULL) */
+int rxa_md_system_targets(rxa_context* ctx, const double* t_target, const double* t_damp);
 int rxa_get_state(rxa_context* ctx, double* x, double* v, double* q);
 
 /* capacities and counters: out[0] n, [1] skin-list capacity per atom, [2] bond-slot capacity, [3] H-bond candidate capacity,

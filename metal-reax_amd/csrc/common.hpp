@@ -88,10 +88,12 @@ __host__ __device__ inline double frac_coord(const Cell& c, int d, double x, dou
 struct State {
   int n, maxnb, maxb;
   double *x, *y, *z;          // owned positions (wrapped into the cell at every rebuild)
-  double4* pos;               // the same positions packed (x, y, z, type) for one 32-byte gather per neighbour
-  const double4* shv;         // lattice shift vectors of every code in [-smx, smx]^3 (see shift_index)
+  double4* pos;               // the same positions packed (x, y, z, w) for one 32-byte gather per neighbour; w = type + 4096 * system
+  const double4* shv;         // lattice shift vectors of every code in [-smx, smx]^3 (see shift_index), one table per system
   int smx0, smx1, smx2;
+  int shv_stride;             // entries per system table (0 with one system)
   int* type;
+  int* sysid;                 // system of each atom (several independent systems packed in one engine; all 0 otherwise)
   long long* tag;
   double* q;
   // skin neighbour list (atom-major: i * maxnb + k), full (both directions); nb_s = shift code | kCanon if i is the canonical end
@@ -201,6 +203,13 @@ __device__ inline void vir_add(double* w, double s, double dx, double dy, double
 
 // ---- list entries ---------------------------------------------------------------------------------------------------
 constexpr int kCanon = 1 << 24;   // flag of a list entry: the listing atom is the canonical end of the pair
+
+// pos.w carries the atom type (-1 for an untyped atom) and the atom's system: w = type + 4096 * system (exact in a double)
+constexpr int kSysShift = 12;
+__host__ __device__ inline double pos_w(int type, int sys) { return (double)(type + (sys << kSysShift)); }
+__device__ inline int pos_sys(double w) { return ((int)w + 1) >> kSysShift; }
+__device__ inline int pos_type(double w) { const int iw = (int)w; return iw - (((iw + 1) >> kSysShift) << kSysShift); }
+__device__ inline double pos_w(const State& st, int i) { return pos_w(st.type[i], st.sysid[i]); }
 __host__ __device__ inline int entry_code(int e) { return e & 0xFFFFFF; }
 
 // index of a shift code in the shift-vector table; |s_d| <= smx_d is guaranteed by the list builders
@@ -216,7 +225,7 @@ __device__ inline int shift_index(const State& st, int code) {
 __device__ inline double pair_dvec_pos(const State& st, const double4& pi, const double4& pj, int e, double& dx, double& dy, double& dz) {
   const bool canon = (e & kCanon) != 0;
   const int code = entry_code(e);
-  const double4 S = st.shv[shift_index(st, canon ? code : shift_neg(code))];
+  const double4 S = st.shv[pos_sys(pi.w) * st.shv_stride + shift_index(st, canon ? code : shift_neg(code))];
   const double ax = canon ? pj.x : pi.x, ay = canon ? pj.y : pi.y, az = canon ? pj.z : pi.z;
   const double bx = canon ? pi.x : pj.x, by = canon ? pi.y : pj.y, bz = canon ? pi.z : pj.z;
   const double ex = (ax + S.x) - bx, ey = (ay + S.y) - by, ez = (az + S.z) - bz;
@@ -230,7 +239,7 @@ __device__ inline double pair_dvec_list(const State& st, const double4& pi, int 
 }
 // the same, also returning the neighbour's type (carried in pos.w)
 __device__ inline double pair_dvec_list(const State& st, const double4& pi, int j, int e, double& dx, double& dy, double& dz, int& tj) {
-  tj = (int)st.pos[j].w;
+  tj = pos_type(st.pos[j].w);
   return pair_dvec_list(st, pi, j, e, dx, dy, dz);
 }
 #pragma clang fp contract(on)

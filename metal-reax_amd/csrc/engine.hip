@@ -218,6 +218,7 @@ struct Engine {
     const char* ovl_env = std::getenv("RXA_OVERLAP");
     if (ovl_env && std::atoi(ovl_env) == 0) overlap = false;
     HIPCHK(hipSetDevice(dev));
+    st.shv_stride = 0;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&sb, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
@@ -236,6 +237,9 @@ struct Engine {
     if (ev_bonded) (void)hipEventDestroy(ev_bonded);
     if (ev_cg) (void)hipEventDestroy(ev_cg);
     if (cgf_h) (void)hipHostFree(cgf_h);
+    if (pki_h) (void)hipHostFree(pki_h);
+    if (pk_h) (void)hipHostFree(pk_h);
+    if (rebany_h) (void)hipHostFree(rebany_h);
     if (cgs_h) (void)hipHostFree(cgs_h);
     if (sb) (void)hipStreamDestroy(sb);
     if (ev_flags) (void)hipEventDestroy(ev_flags);
@@ -359,6 +363,11 @@ struct Engine {
 
   // ---------------------------------------------------------------------------------------------------------------
   void set_box(const double lo[3], const double hi[3], const double tilt[3], const int per[3]) {
+    cell = make_cell(lo, hi, tilt, per);
+    have_box = true;
+    nbr_valid = false;
+  }
+  static Cell make_cell(const double lo[3], const double hi[3], const double tilt[3], const int per[3]) {
     Cell c{};
     for (int d = 0; d < 3; ++d) c.o[d] = lo[d];
     const double v[3][3] = {{hi[0] - lo[0], 0, 0}, {tilt[0], hi[1] - lo[1], 0}, {tilt[1], tilt[2], hi[2] - lo[2]}};
@@ -374,9 +383,7 @@ struct Engine {
       c.h[d] = std::fabs(c.det) / nr;
       c.per[d] = per[d] != 0;
     }
-    cell = c;
-    have_box = true;
-    nbr_valid = false;
+    return c;
   }
 
   double* atom_arr(int k) { return a_atom.p + (size_t)k * n; }
@@ -388,8 +395,8 @@ struct Engine {
     const int NA = 38 + kTerms + 6;   // number of per-atom double arrays below
     a_atom.alloc((size_t)NA * n);
     HIPCHK(hipMemset(a_atom.p, 0, (size_t)NA * n * sizeof(double)));
-    a_iatom.alloc(5 * (size_t)n + 8);
-    HIPCHK(hipMemset(a_iatom.p, 0, (5 * (size_t)n + 8) * sizeof(int)));
+    a_iatom.alloc(6 * (size_t)n + 8);
+    HIPCHK(hipMemset(a_iatom.p, 0, (6 * (size_t)n + 8) * sizeof(int)));
     State& s = st;
     s.n = n;
     s.x = a_x.p; s.y = a_y.p; s.z = a_z.p; s.q = a_q.p; s.type = a_type.p; s.tag = a_tag.p; s.pos = a_pos.p;
@@ -405,7 +412,7 @@ struct Engine {
     s.eatom = atom_arr(k);
     s.vatom = atom_arr(k + kTerms);
     s.nb_cnt = a_iatom.p; s.b_cnt = a_iatom.p + n; s.h_cnt = a_iatom.p + 2 * (size_t)n; s.bc_cnt = a_iatom.p + 3 * (size_t)n;
-    s.hc_cnt = a_iatom.p + 4 * (size_t)n; s.flags = a_iatom.p + 5 * (size_t)n;
+    s.hc_cnt = a_iatom.p + 4 * (size_t)n; s.sysid = a_iatom.p + 5 * (size_t)n; s.flags = a_iatom.p + 6 * (size_t)n;
     red.alloc(19 + 240 * 19 + 8);
     // binning scratch
     b_keys.alloc(n); b_skeys.alloc(n); b_akey.alloc(n); b_idx.alloc(n); b_sidx.alloc(n);
@@ -458,8 +465,8 @@ struct Engine {
 
   // expected entries within r of an atom at the mean density (at least that of a condensed phase for open cells), with margin
   int estimate_within(double r) const {
-    double dens = n / cell.det;
-    if (!(cell.per[0] && cell.per[1] && cell.per[2])) dens = std::max(dens, 0.12);
+    double dens = packed ? pk_dens : n / cell.det;
+    if (!packed && !(cell.per[0] && cell.per[1] && cell.per[2])) dens = std::max(dens, 0.12);
     const double est = dens * 4.0 / 3.0 * M_PI * r * r * r;
     const int cap = (int)(est * 1.25) + 32;
     return std::min(cap, std::max(64, 27 * n + 32));
@@ -481,12 +488,210 @@ struct Engine {
     launch_permute(ps, src, d_psd.p, d_psi.p, d_psl.p, stream);
   }
 
+  // ---- several independent systems packed in one engine --------------------------------------------------------------
+  // Atoms of system s occupy [sys_start[s], sys_start[s+1]) in the caller's order; each system has its own cell, shift table,
+  // thermostat, charge-solve scalars and energy sums, and its own neighbour rebuilds. Lists never cross systems.
+  bool packed = false;
+  int S = 1;
+  double pk_dens = 0;
+  std::vector<int> sys_start_h;
+  std::vector<Cell> cells_h;
+  std::vector<int> ext_h;
+  bool shv_valid = false;
+  DBuf<int> d_sys_start, d_reb, d_rebany, d_pki, d_cg_any;
+  DBuf<Cell> d_cells;
+  DBuf<int4> d_ext;
+  DBuf<double> d_pk, d_rowv, d_nh_pk, d_disp_pk, d_sys_out;
+  PackedQeq pq;
+  int* rebany_h = nullptr;   // pinned
+  int* pki_h = nullptr;      // pinned, 4 S
+  double* pk_h = nullptr;    // pinned, 10 S
+  size_t pk_cap_h = 0;
+
+  void leave_packed() {
+    packed = false;
+    S = 1;
+    st.shv_stride = 0;
+    smx[0] = smx[1] = smx[2] = -1;
+  }
+
+  void set_systems(int ns, const int64_t* counts, const double* boxes, const int* periodic, const int* types, const int64_t* tags,
+                   const double* x) {
+    if (!have_ff) throw ApiError(RXA_ESTATE, "pair_coeff (the force field) must be set before the atoms");
+    if (ns < 1 || ns > (1 << 18)) throw ApiError(RXA_EINVAL, "system count out of range");
+    sync();
+    std::vector<int> start(ns + 1, 0);
+    std::vector<Cell> cells(ns);
+    double dens = 0;
+    for (int k = 0; k < ns; ++k) {
+      if (counts[k] <= 0) throw ApiError(RXA_EINVAL, "system " + std::to_string(k) + " has no atoms");
+      if (start[k] + counts[k] > (1LL << 30)) throw ApiError(RXA_EINVAL, "atom count out of range");
+      start[k + 1] = start[k] + (int)counts[k];
+      const double* b = boxes + 9 * (size_t)k;
+      cells[k] = make_cell(b, b + 3, b + 6, periodic + 3 * (size_t)k);
+      double d = counts[k] / cells[k].det;
+      if (!(cells[k].per[0] && cells[k].per[1] && cells[k].per[2])) d = std::max(d, 0.12);
+      dens = std::max(dens, d);
+    }
+    const int nn = start[ns];
+    std::vector<int> ft(nn), sid(nn);
+    std::vector<long long> tg(nn);
+    std::vector<int> used;
+    for (int k = 0; k < ns; ++k) {
+      std::vector<long long> t2;
+      for (int i = start[k]; i < start[k + 1]; ++i) {
+        const int t = types[i];
+        if (t < 1 || t >= (int)lmp2ff.size()) throw ApiError(RXA_EINVAL, "atom type " + std::to_string(t) + " outside the pair_coeff map");
+        ft[i] = lmp2ff[t];
+        tg[i] = tags[i];
+        sid[i] = k;
+        t2.push_back(tags[i]);
+      }
+      std::sort(t2.begin(), t2.end());
+      for (size_t q = 1; q < t2.size(); ++q)
+        if (t2[q] == t2[q - 1]) throw ApiError(RXA_EINVAL, "duplicate atom id " + std::to_string(t2[q]) + " in system " + std::to_string(k));
+    }
+    {
+      std::vector<int> seen(ff.ntypes(), 0);
+      for (int v : ft) if (v >= 0 && !seen[v]) { seen[v] = 1; used.push_back(v); }
+      ff.require_bond_blocks(used);
+    }
+    packed = true;
+    S = ns;
+    pk_dens = dens;
+    sys_start_h = start;
+    cells_h = cells;
+    cell = cells[0];
+    have_box = true;
+    n = nn;
+    hist_count = 0;
+    alloc_atoms();
+    alloc_neighbors(estimate_neighbors());
+    alloc_sublists(estimate_within(g.bond_cut + skin), g.hbond_cut > 0 ? estimate_within(g.hbond_cut + skin) : 1);
+    alloc_bonds(maxb);
+    qeq_h_valid = false;
+    if (qeq_ready) alloc_qeq();
+    md_ready = false;
+    d_sys_start.alloc(S + 1); d_cells.alloc(S); d_ext.alloc(S); d_reb.alloc(S); d_rebany.alloc(1);
+    d_pk.alloc(10 * (size_t)S); d_pki.alloc(4 * (size_t)S); d_rowv.alloc(4 * (size_t)n); d_nh_pk.alloc(32 * (size_t)S);
+    d_disp_pk.alloc(S); d_sys_out.alloc((size_t)S * (kTerms + 6));
+    HIPCHK(hipMemset(d_pki.p, 0, 4 * (size_t)S * sizeof(int)));
+    HIPCHK(hipMemcpy(d_sys_start.p, start.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_cells.p, cells.data(), S * sizeof(Cell), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(st.sysid, sid.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    if ((size_t)S > pk_cap_h) {
+      if (pki_h) HIPCHK(hipHostFree(pki_h));
+      if (pk_h) HIPCHK(hipHostFree(pk_h));
+      HIPCHK(hipHostMalloc((void**)&pki_h, 4 * (size_t)S * sizeof(int), hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void**)&pk_h, 10 * (size_t)S * sizeof(double), hipHostMallocDefault));
+      pk_cap_h = S;
+    }
+    if (!rebany_h) HIPCHK(hipHostMalloc((void**)&rebany_h, sizeof(int), hipHostMallocDefault));
+    pq.S = S; pq.sys_start = d_sys_start.p; pq.sysid = st.sysid; pq.pk = d_pk.p; pq.pki = d_pki.p; pq.rowv = d_rowv.p;
+    ext_h.clear();
+    smx[0] = smx[1] = smx[2] = -1;
+    shv_valid = false;
+    perm_h.resize(n);
+    for (int i = 0; i < n; ++i) perm_h[i] = i;
+    HIPCHK(hipMemcpy(d_perm.p, perm_h.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(a_type.p, ft.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(a_tag.p, tg.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+    host_types_mass.assign(n, 0.0);
+    set_positions(x);
+    sync();
+  }
+
+  // Neighbour lists of the systems flagged in d_reb (all of them when `all`): wrap, direct search, sub-lists.
+  void build_neighbors_packed(bool all) {
+    const double rs = list_cut() + skin;
+    int want[3] = {1, 1, 1};
+    std::vector<int> ext(4 * (size_t)S, 0);
+    for (int k = 0; k < S; ++k)
+      for (int d = 0; d < 3; ++d) {
+        const Cell& c = cells_h[k];
+        const int e = c.per[d] ? (int)std::floor(rs / c.h[d] + 0.5) : 0;
+        if (e > 60) throw ApiError(RXA_EINVAL, "periodic cell of system " + std::to_string(k) + " too thin for the neighbour cut-off");
+        ext[4 * (size_t)k + d] = e;
+        want[d] = std::max(want[d], 1 + e);
+      }
+    if (ext != ext_h) {
+      HIPCHK(hipMemcpyAsync(d_ext.p, ext.data(), ext.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+      sync();
+      ext_h = ext;
+    }
+    if (want[0] != smx[0] || want[1] != smx[1] || want[2] != smx[2]) {
+      const long long T = (2LL * want[0] + 1) * (2LL * want[1] + 1) * (2LL * want[2] + 1);
+      if (T * S > (1LL << 26) || T > 32768) throw ApiError(RXA_EINVAL, "periodic cells too thin for the neighbour cut-off");
+      a_shv.alloc((size_t)(T * S));
+      for (int d = 0; d < 3; ++d) smx[d] = want[d];
+      st.shv = a_shv.p; st.smx0 = smx[0]; st.smx1 = smx[1]; st.smx2 = smx[2]; st.shv_stride = (int)T;
+      shv_valid = false;
+    }
+    hipEvent_t t0 = timer.begin(PH_NBR);
+    if (!shv_valid) { launch_pk_shift_table(d_cells.p, S, a_shv.p, smx, stream); shv_valid = true; }
+    if (all) launch_fill_int(d_reb.p, S, 1, stream);
+    launch_pk_wrap(st, d_cells.p, d_reb.p, stream);
+    timer.end(PH_NBR, t0);
+    const double rb = g.bond_cut + skin, rh = g.hbond_cut > 0 ? g.hbond_cut + skin : 0.0;
+    bool need_full = true;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+      t0 = timer.begin(PH_NBR);
+      launch_fill_int(st.flags, 8, 0, stream);
+      if (need_full) launch_pk_nbr(st, d_cells.p, d_sys_start.p, d_ext.p, d_reb.p, rs, stream);
+      launch_sublists(st, g, tab, rb, rh, stream);
+      timer.end(PH_NBR, t0);
+      int flags[8];
+      HIPCHK(hipMemcpyAsync(flags, st.flags, sizeof(flags), hipMemcpyDeviceToHost, stream));
+      sync();
+      if (flags[0] > maxnb) {   // the skin list overflowed: grow it and list every system again (without a new wrap)
+        alloc_neighbors((flags[0] + 31) / 32 * 32 + 32);
+        launch_fill_int(d_reb.p, S, 1, stream);
+        need_full = true;
+        continue;
+      }
+      need_full = false;
+      if (flags[4] <= maxbc && flags[5] <= maxhc) {
+        nbr_valid = true;
+        ++rebuilds;
+        qeq_h_valid = false;
+        return;
+      }
+      alloc_sublists(std::max(maxbc, (flags[4] + 31) / 32 * 32 + 32), std::max(maxhc, (flags[5] + 31) / 32 * 32 + 32));
+    }
+    throw ApiError(RXA_EINVAL, "neighbour list capacity did not converge");
+  }
+
+  // per-system energy terms, virial and thermostat state after a force call: out[s*(kTerms+6)], ke, temperature, iterations
+  void system_results(double* energies, double* virial, double* thermo) {
+    if (!packed) throw ApiError(RXA_ESTATE, "no packed systems (rxa_set_systems)");
+    launch_pk_energy(st, d_sys_start.p, S, d_sys_out.p, stream);
+    std::vector<double> out((size_t)S * (kTerms + 6)), nh;
+    HIPCHK(hipMemcpyAsync(out.data(), d_sys_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (md_ready) {
+      nh.resize(32 * (size_t)S);
+      HIPCHK(hipMemcpyAsync(nh.data(), d_nh_pk.p, nh.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    HIPCHK(hipMemcpyAsync(pki_h, d_pki.p, 4 * (size_t)S * sizeof(int), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int k = 0; k < S; ++k) {
+      if (energies) for (int t = 0; t < kTerms; ++t) energies[(size_t)k * kTerms + t] = out[(size_t)k * (kTerms + 6) + t];
+      if (virial) for (int t = 0; t < 6; ++t) virial[(size_t)k * 6 + t] = out[(size_t)k * (kTerms + 6) + kTerms + t];
+      if (thermo) {
+        thermo[4 * (size_t)k] = md_ready ? nh[32 * (size_t)k + 21] : 0.0;
+        thermo[4 * (size_t)k + 1] = md_ready ? nh[32 * (size_t)k + 20] : 0.0;
+        thermo[4 * (size_t)k + 2] = pki_h[4 * k + 1];
+        thermo[4 * (size_t)k + 3] = pki_h[4 * k + 3];
+      }
+    }
+  }
+
   void set_atoms(long long nn, const int* types, const int64_t* tags, const double* x) {
     if (!have_ff) throw ApiError(RXA_ESTATE, "pair_coeff (the force field) must be set before the atoms");
     if (!have_box) throw ApiError(RXA_ESTATE, "the box must be set before the atoms");
     if (nn <= 0 || nn > (1LL << 30)) throw ApiError(RXA_EINVAL, "atom count out of range");
     sync();   // blocking copies below touch arrays that queued work may still use
-    const bool realloc = (int)nn != n || !a_x.p;
+    const bool realloc = (int)nn != n || !a_x.p || packed;
+    if (packed) leave_packed();
     n = (int)nn;
     if (realloc) {
       hist_count = 0;   // a new atom set: no charge history to carry
@@ -568,6 +773,7 @@ struct Engine {
 
   // ---------------------------------------------------------------------------------------------------------------
   void build_neighbors() {
+    if (packed) { build_neighbors_packed(true); return; }
     const double rs = list_cut() + skin;
     cell_mode = true;
     int nc[3];
@@ -855,6 +1061,8 @@ struct Engine {
   // runs CG iterations until both systems stopped; batch launches, one readback per batch
   void ensure_cg_cap(int imax) {
     if (imax + 2 > qeq_imax_cap) { qeq_imax_cap = imax + 2; alloc_qeq_doubles(); }
+    if (d_cg_any.n < (size_t)qeq_imax_cap + 2) { d_cg_any.alloc((size_t)qeq_imax_cap + 2); }
+    pq.any = d_cg_any.p;
     const size_t need = 2 * (size_t)(std::max(qeq_imax_cap, imax + 2) + 2);
     if (need > cg_cap_h) {
       if (cgf_h) HIPCHK(hipHostFree(cgf_h));
@@ -868,6 +1076,17 @@ struct Engine {
   int cg_it = 0, cg_batch = 0;
   // launches the next batch of iterations, then the asynchronous readback of the flags and scalars (pinned) and its event
   void cg_launch_batch(int imax) {
+    if (packed) {   // iteration it sets any[it]: some system active for the next one
+      for (int b = 0; b < cg_batch && cg_it < imax; ++b) {
+        hipEvent_t t0 = timer.begin(PH_QEQ_SPMV);
+        launch_pk_qeq_iteration(st, qb, pq, cg_it, imax, cg_tol, stream);
+        timer.end(PH_QEQ_SPMV, t0);
+        ++cg_it;
+      }
+      HIPCHK(hipMemcpyAsync(cgf_h, d_cg_any.p, (size_t)cg_it * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipEventRecord(ev_cg, stream));
+      return;
+    }
     for (int b = 0; b < cg_batch && cg_it < imax; ++b) {
       hipEvent_t t0 = timer.begin(PH_QEQ_CG);
       launch_qeq_update(st, qb, cg_it, stream);
@@ -892,6 +1111,15 @@ struct Engine {
     if (imax < 1) throw ApiError(RXA_EINVAL, "charge equilibration: maxiter must be >= 1");
     ensure_cg_cap(imax);
     qeq_matrix();
+    if (packed) {
+      launch_fill_int(d_cg_any.p, (int)d_cg_any.n, 0, stream);
+      launch_pk_qeq_init(st, qb, pq, imax, tol, hist_count > 0 ? 1 : 0, stream);
+      cg_tol = tol;
+      cg_it = 1;
+      cg_batch = std::max(1, cg_guess);
+      cg_launch_batch(imax);
+      return;
+    }
     launch_qeq_init(st, qb, imax, tol, hist_count > 0 ? 1 : 0, stream);
     cg_start(imax, tol);
   }
@@ -926,6 +1154,31 @@ struct Engine {
     cg_wait_end(imax);
   }
   void qeq_end(int imax, int strict) {
+    if (packed) {
+      while (true) {
+        HIPCHK(hipEventSynchronize(ev_cg));
+        if (!cgf_h[cg_it - 1] || cg_it >= imax) break;
+        cg_batch = 2;
+        cg_launch_batch(imax);
+      }
+      int done = 0;   // iterations the slowest system ran
+      for (int k = 0; k < cg_it; ++k) if (cgf_h[k]) done = k + 1;
+      cg_guess = std::max(1, done);
+      launch_pk_qeq_finish(st, qb, pq, stream);
+      HIPCHK(hipMemcpyAsync(pki_h, d_pki.p, 4 * (size_t)S * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(pk_h, d_pk.p, 10 * (size_t)S * sizeof(double), hipMemcpyDeviceToHost, stream));
+      hist_count = std::min(hist_count + 1, 4);
+      ++qeq_calls;
+      pk_collect_pending = true;
+      if (strict) {
+        sync();
+        pk_collect();
+        for (int q = 0; q < 2; ++q)
+          if (!(last_rel[q] <= cg_tol))
+            throw ApiError(RXA_ENOCONV, "fix qeq/reaxff/metal strict: CG did not converge in some system after " + std::to_string(last_iters[q]) + " iterations");
+      }
+      return;
+    }
     cg_wait_end(imax);
     launch_qeq_finish(st, qb, stream);
     hist_count = std::min(hist_count + 1, 4);
@@ -939,6 +1192,23 @@ struct Engine {
   void qeq_charges(int imax, double tol, int strict) {
     qeq_begin(imax, tol);
     qeq_end(imax, strict);
+  }
+  // packed systems: the largest iteration count and relative residual over the systems of the last solve (after a sync)
+  bool pk_collect_pending = false;
+  void pk_collect() {
+    if (!pk_collect_pending) return;
+    pk_collect_pending = false;
+    for (int q = 0; q < 2; ++q) {
+      int it = 0;
+      double rel = 0;
+      for (int k = 0; k < S; ++k) {
+        it = std::max(it, pki_h[4 * k + 2 * q + 1]);
+        rel = std::max(rel, std::sqrt(pk_h[10 * k + 4 * q + 1]) / pk_h[10 * k + 4 * q]);
+      }
+      last_iters[q] = it;
+      last_rel[q] = rel;
+    }
+    qeq_iter_sum += 0.5 * (last_iters[0] + last_iters[1]);
   }
 
   // the device charges in the caller's order
@@ -981,6 +1251,15 @@ struct Engine {
     md_nve = !(tdamp > 0);   // t_damp <= 0: no thermostat (NVE)
     nh[22] = md_nve ? 1.0 : 0.0;
     HIPCHK(hipMemcpy(md.nh, nh.data(), nh.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (packed) {   // one chain per system, the same layout
+      std::vector<double> nhp(32 * (size_t)S, 0.0);
+      for (int k = 0; k < S; ++k) {
+        double* h = nhp.data() + 32 * (size_t)k;
+        for (int q = 13; q < 23; ++q) h[q] = nh[q];
+        h[16] = 3.0 * (sys_start_h[k + 1] - sys_start_h[k]) - 3.0;
+      }
+      HIPCHK(hipMemcpy(d_nh_pk.p, nhp.data(), nhp.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     md_dt = dt;
     skin = sk;
     md_qeq_every = qeq_every; md_qeq_imax = qimax; md_qeq_tol = qtol;
@@ -1003,22 +1282,36 @@ struct Engine {
     double disp = 0;
     for (int s = 0; s < steps; ++s) {
       hipEvent_t t0 = timer.begin(PH_MD);
-      if (!md_nve) launch_md_thermostat(st, md, stream);
+      if (packed) launch_pk_thermostat(st, md, d_sys_start.p, S, d_nh_pk.p, stream);
+      else if (!md_nve) launch_md_thermostat(st, md, stream);
       launch_md_half_kick(st, md, dtf, stream);
       launch_md_drift(st, md, md_dt, stream);
-      launch_md_displacement(st, md, stream);
+      if (packed) {   // each system's own displacement decides its own rebuild
+        launch_fill_int(d_rebany.p, 1, 0, stream);
+        launch_pk_disp(st, md, d_sys_start.p, S, 0.5 * skin, d_reb.p, d_rebany.p, d_disp_pk.p, stream);
+        HIPCHK(hipMemcpyAsync(rebany_h, d_rebany.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+      } else {
+        launch_md_displacement(st, md, stream);
+        HIPCHK(hipMemcpyAsync(disp_h, md.disp, sizeof(double), hipMemcpyDeviceToHost, stream));
+      }
       timer.end(PH_MD, t0);
-      HIPCHK(hipMemcpyAsync(disp_h, md.disp, sizeof(double), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipEventRecord(ev_disp, stream));
       qeq_h_valid = false;
       ++md_steps;
       const bool do_qeq = md_qeq_every > 0 && md_steps % md_qeq_every == 0;
       if (do_qeq && nbr_valid) qeq_matrix();   // speculative: the common step keeps its lists; redone after a rebuild
       HIPCHK(hipEventSynchronize(ev_disp));
-      disp = *disp_h;
-      if (disp > 0.5 * skin) {
-        build_neighbors();   // invalidates the matrix
-        launch_md_ref_positions(st, md, stream);
+      if (packed) {
+        if (*rebany_h) {
+          build_neighbors_packed(false);   // the flagged systems only; invalidates the matrix
+          launch_pk_ref(st, md, d_reb.p, stream);
+        }
+      } else {
+        disp = *disp_h;
+        if (disp > 0.5 * skin) {
+          build_neighbors();   // invalidates the matrix
+          launch_md_ref_positions(st, md, stream);
+        }
       }
       // bonded chain on the bonded stream while the main stream solves for the charges, then the non-bonded pass and the join;
       // energy totals are read once, after the last step
@@ -1030,22 +1323,32 @@ struct Engine {
       join_gather(s == steps - 1);
       t0 = timer.begin(PH_MD);
       launch_md_half_kick(st, md, dtf, stream);
-      if (!md_nve || s == steps - 1) launch_md_thermostat(st, md, stream);   // NVE: kinetic energy of the last step only
+      if (packed) launch_pk_thermostat(st, md, d_sys_start.p, S, d_nh_pk.p, stream);
+      else if (!md_nve || s == steps - 1) launch_md_thermostat(st, md, stream);   // NVE: kinetic energy of the last step only
       timer.end(PH_MD, t0);
     }
     sync();
+    if (packed) pk_collect();
     if (thermo) {
       double e[kTerms];
       HIPCHK(hipMemcpy(e, red.p, sizeof(e), hipMemcpyDeviceToHost));
       double nh[22];
       HIPCHK(hipMemcpy(nh, md.nh, sizeof(nh), hipMemcpyDeviceToHost));
+      if (packed) {   // totals over the systems: kinetic energy, temperature from the summed degrees of freedom
+        std::vector<double> nhp(32 * (size_t)S);
+        HIPCHK(hipMemcpy(nhp.data(), d_nh_pk.p, nhp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double ke = 0, dof = 0;
+        for (int k = 0; k < S; ++k) { ke += nhp[32 * (size_t)k + 21]; dof += nhp[32 * (size_t)k + 16]; }
+        nh[21] = ke;
+        nh[20] = dof > 0 ? 2.0 * ke / (dof * nhp[17]) : 0.0;
+      }
       double pe = 0;
       for (double v : e) pe += v;
       thermo[0] = pe;
       thermo[1] = nh[21];
       thermo[2] = nh[20];
       thermo[3] = (double)(rebuilds - reb0);
-      thermo[4] = qeq_calls ? qeq_iter_sum / qeq_calls : 0.0;
+      thermo[4] = packed ? 0.5 * (last_iters[0] + last_iters[1]) : (qeq_calls ? qeq_iter_sum / qeq_calls : 0.0);
       thermo[5] = last_iters[0];
       thermo[6] = last_iters[1];
       thermo[7] = disp;
@@ -1193,6 +1496,36 @@ int rxa_set_atoms(rxa_context* c, int64_t n, const int* types, const int64_t* ta
   return guard(c, [&](Engine& e) {
     e.set_atoms(n, types, tags, x);
     e.lmp_types.assign(types, types + n);
+  });
+}
+
+int rxa_set_systems(rxa_context* c, int nsys, const int64_t* counts, const double* boxes, const int* periodic, const int* types,
+                    const int64_t* tags, const double* x) {
+  return guard(c, [&](Engine& e) {
+    if (nsys < 1 || !counts || !boxes || !periodic || !types || !tags || !x) throw ApiError(RXA_EINVAL, "rxa_set_systems: null argument");
+    long long nn = 0;
+    for (int k = 0; k < nsys; ++k) nn += counts[k];
+    e.set_systems(nsys, counts, boxes, periodic, types, tags, x);
+    e.lmp_types.assign(types, types + nn);
+  });
+}
+
+int rxa_system_results(rxa_context* c, double* energies, double* virial, double* thermo) {
+  return guard(c, [&](Engine& e) { e.system_results(energies, virial, thermo); });
+}
+
+int rxa_md_system_targets(rxa_context* c, const double* t_target, const double* t_damp) {
+  return guard(c, [&](Engine& e) {
+    if (!e.packed) throw ApiError(RXA_ESTATE, "no packed systems (rxa_set_systems)");
+    if (!e.md_ready) throw ApiError(RXA_ESTATE, "rxa_md_setup first");
+    std::vector<double> nh(32 * (size_t)e.S);
+    HIPCHK(hipMemcpy(nh.data(), e.d_nh_pk.p, nh.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < e.S; ++k) {
+      double* h = nh.data() + 32 * (size_t)k;
+      if (t_target) h[13] = t_target[k];
+      if (t_damp) { h[14] = t_damp[k] > 0 ? 1.0 / t_damp[k] : 0.0; h[22] = t_damp[k] > 0 ? 0.0 : 1.0; }
+    }
+    HIPCHK(hipMemcpy(e.d_nh_pk.p, nh.data(), nh.size() * sizeof(double), hipMemcpyHostToDevice));
   });
 }
 
@@ -1351,6 +1684,7 @@ int rxa_qeq_solve(rxa_context* c, const double* b, const double* x0, int imax, d
 int rxa_qeq_charges(rxa_context* c, int imax, double tol, int strict, double* q_out, int* iters, double* rel) {
   return guard(c, [&](Engine& e) {
     e.qeq_charges(imax, tol, strict);
+    if (e.packed) { e.sync(); e.pk_collect(); }
     if (q_out) e.charges_out(q_out);
     if (iters) { iters[0] = e.last_iters[0]; iters[1] = e.last_iters[1]; }
     if (rel) { rel[0] = e.last_rel[0]; rel[1] = e.last_rel[1]; }

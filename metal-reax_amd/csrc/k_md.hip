@@ -66,7 +66,7 @@ __global__ void k_drift(State st, MdBuffers md, double dt) {
   st.x[i] = x;
   st.y[i] = y;
   st.z[i] = z;
-  st.pos[i] = make_double4(x, y, z, (double)st.type[i]);
+  st.pos[i] = make_double4(x, y, z, pos_w(st, i));
 }
 
 __global__ void __launch_bounds__(kRT) k_ke_partial(State st, MdBuffers md) {

@@ -29,7 +29,7 @@ __global__ void k_wrap(State st, Cell c) {
   st.x[i] = r[0];
   st.y[i] = r[1];
   st.z[i] = r[2];
-  st.pos[i] = make_double4(r[0], r[1], r[2], (double)st.type[i]);
+  st.pos[i] = make_double4(r[0], r[1], r[2], pos_w(st, i));
 }
 
 // lattice shift vectors of all codes in [-smx, smx]^3, accumulated exactly as shift_vec does
@@ -47,7 +47,7 @@ __global__ void k_shift_table(Cell c, double4* shv, int smx0, int smx1, int smx2
 __global__ void k_pos4(State st) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= st.n) return;
-  st.pos[i] = make_double4(st.x[i], st.y[i], st.z[i], (double)st.type[i]);
+  st.pos[i] = make_double4(st.x[i], st.y[i], st.z[i], pos_w(st, i));
 }
 
 __global__ void k_cell_keys(State st, Cell c, int nc0, int nc1, int nc2, unsigned* keys, int* idx) {
@@ -276,7 +276,7 @@ __global__ void k_positions_in(State st, const double* ux, const int* perm) {
   const int u = perm[i];
   const double x = ux[u], y = ux[st.n + u], z = ux[2 * (size_t)st.n + u];
   st.x[i] = x; st.y[i] = y; st.z[i] = z;
-  st.pos[i] = make_double4(x, y, z, (double)st.type[i]);
+  st.pos[i] = make_double4(x, y, z, pos_w(st, i));
 }
 
 inline int blocks_for(long long items, int per_block) { return (int)((items + per_block - 1) / per_block); }

@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(64 * kNbWaves) k_nonbonded_f(State st, Globals
 #pragma clang fp contract(off)
 __device__ inline double pair_dvec_key(const State& st, const double4& pi, const double4& pj, int key, double& dx, double& dy, double& dz) {
   const bool canon = (key & 0x8000) != 0;
-  const double4 S = st.shv[key & 0x7FFF];
+  const double4 S = st.shv[pos_sys(pi.w) * st.shv_stride + (key & 0x7FFF)];
   const double ax = canon ? pj.x : pi.x, ay = canon ? pj.y : pi.y, az = canon ? pj.z : pi.z;
   const double bx = canon ? pi.x : pj.x, by = canon ? pi.y : pj.y, bz = canon ? pi.z : pj.z;
   const double ex = (ax + S.x) - bx, ey = (ay + S.y) - by, ez = (az + S.z) - bz;
@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(256) k_nonbonded_rows(State st, Globals g, Tab
       const double r2 = pair_dvec_key(st, pi, pj, (int)key[base + k], dx, dy, dz);
       if (r2 <= cut2) {
         const double rd = sqrt(r2);
-        if (rd <= cut) nb_pair_f(g, row, qi, (float)st.q[j], (int)pj.w, rd, dx, dy, dz, a);
+        if (rd <= cut) nb_pair_f(g, row, qi, (float)st.q[j], pos_type(pj.w), rd, dx, dy, dz, a);
       }
     }
   }

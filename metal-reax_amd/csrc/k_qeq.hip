@@ -281,6 +281,128 @@ __global__ void __launch_bounds__(kThreads) k_cg_charges(int n, QeqBuffers qb, d
   }
 }
 
+// ---- several independent systems packed in one engine ---------------------------------------------------------------
+// The same two CG systems (s, t) per packed system, with that system's own scalars: one wave per system sums its rows' values
+// (lane = row offset within the system, then a fixed butterfly), so a system's solve does not depend on the systems beside it.
+// pk[10 s + 4 k + {0, 1, 2, 3}] = |b|, r.p, alpha, beta of solve k; pk[10 s + 8, 9] = the sums of s and t for the charges;
+// pki[4 s + 2 k + {0, 1}] = active, the stock loop index. rowv[4 i + {0, 1}] = d.q (b.b at the start), rowv[4 i + {2, 3}] = r.p.
+__global__ void k_pk_cg_init(int n, QeqBuffers qb, double* rowv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hi = qb.hinv[i];
+  for (int s = 0; s < 2; ++s) {
+    const double r = qb.b[2 * i + s] - qb.q[2 * i + s];
+    const double d = r * hi;
+    qb.r[2 * i + s] = r;
+    qb.p[2 * i + s] = d;
+    rowv[4 * i + s] = qb.b[2 * i + s] * qb.b[2 * i + s];
+    rowv[4 * i + 2 + s] = r * d;
+  }
+}
+
+// kind 0: start (|b|, r.p, activity for iteration 1); 1: alpha of iteration it; 2: r.p, beta and activity after iteration it;
+// 3: sums of s and t
+__global__ void __launch_bounds__(256) k_pk_cg_seg(int kind, int S, const int* sys_start, const double* rowv, const double* x, double* pk,
+                                                  int* pki, int it, int imax, double tol, int* any) {
+  const int sy = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (sy >= S) return;
+  const int lane = lane_id();
+  double v[4] = {0, 0, 0, 0};
+  for (int i = sys_start[sy] + lane; i < sys_start[sy + 1]; i += 64) {
+    if (kind == 3) { v[0] += x[2 * i]; v[1] += x[2 * i + 1]; }
+    else for (int k = 0; k < 4; ++k) v[k] += rowv[4 * i + k];
+  }
+  for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+  if (lane != 0) return;
+  double* P = pk + 10 * (size_t)sy;
+  int* A = pki + 4 * (size_t)sy;
+  if (kind == 3) { P[8] = v[0]; P[9] = v[1]; return; }
+  for (int k = 0; k < 2; ++k) {
+    double* q = P + 4 * k;
+    int* a = A + 2 * k;
+    if (kind == 0) {
+      q[0] = sqrt(v[k]);
+      q[1] = v[2 + k];
+      q[3] = 0.0;
+      a[0] = (1 < imax && sqrt(v[2 + k]) / q[0] > tol) ? 1 : 0;
+      a[1] = 1;
+      if (a[0]) atomicOr(any, 1);
+    } else if (kind == 1) {
+      if (a[0]) q[2] = q[1] / v[k];
+    } else if (a[0]) {
+      q[3] = v[2 + k] / q[1];
+      q[1] = v[2 + k];
+      a[1] = it + 1;
+      a[0] = (it + 1 < imax && sqrt(v[2 + k]) / q[0] > tol) ? 1 : 0;
+      if (a[0]) atomicOr(any + it, 1);
+    }
+  }
+}
+
+// w = (diag(eta) + H) p, d = p + beta d, q = w + beta q and d.q per row of the active solves (iteration it)
+template <class TV, class TC>
+__global__ void __launch_bounds__(kMvThreads) k_pk_cg_mvdir(int n, int maxnb, const TC* __restrict__ col, const TV* __restrict__ val, QeqBuffers qb,
+                                                           const int* sysid, const double* pk, const int* pki, double* rowv, int it) {
+  const int i = product_row();
+  int act[2] = {0, 0}, sy = 0;
+  if (i < n) {
+    sy = sysid[i];
+    act[0] = pki[4 * sy];
+    act[1] = pki[4 * sy + 2];
+  }
+  if (!act[0] && !act[1]) return;   // uniform over the row's lane group
+  const double2* __restrict__ p2 = reinterpret_cast<const double2*>(qb.p);
+  const double2 y = row_product<TV, TC>(i, n, maxnb, qb.cnt, col, val, p2);
+  if (lane_id() % kRowLanes != 0) return;
+  const double2 pi = p2[i];
+  const double ei = qb.eta[i];
+  const double w[2] = {y.x + ei * pi.x, y.y + ei * pi.y}, pv[2] = {pi.x, pi.y};
+  for (int s = 0; s < 2; ++s) {
+    if (!act[s]) continue;
+    const int k = 2 * i + s;
+    const double beta = pk[10 * sy + 4 * s + 3];
+    const double d = it == 1 ? pv[s] : pv[s] + beta * qb.d[k];
+    const double q = it == 1 ? w[s] : w[s] + beta * qb.q[k];
+    qb.d[k] = d;
+    qb.q[k] = q;
+    rowv[4 * i + s] = d * q;
+  }
+}
+
+// x += alpha d, r -= alpha q, p = r / eta and r.p per row of the active solves
+__global__ void k_pk_cg_update(int n, QeqBuffers qb, const int* sysid, const double* pk, const int* pki, double* rowv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sy = sysid[i];
+  const double hi = qb.hinv[i];
+  for (int s = 0; s < 2; ++s) {
+    if (!pki[4 * sy + 2 * s]) continue;
+    const int k = 2 * i + s;
+    const double alpha = pk[10 * sy + 4 * s + 2];
+    qb.x[k] += alpha * qb.d[k];
+    qb.r[k] -= alpha * qb.q[k];
+    const double p = qb.r[k] * hi;
+    qb.p[k] = p;
+    rowv[4 * i + 2 + s] = qb.r[k] * p;
+  }
+}
+
+// q = s - (sum s / sum t) t per system, then the history shift of calculate_Q
+__global__ void k_pk_cg_charges(int n, QeqBuffers qb, const int* sysid, const double* pk, double* q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sy = sysid[i];
+  const double u = pk[10 * sy + 8] / pk[10 * sy + 9];
+  const double s = qb.x[2 * i], t = qb.x[2 * i + 1];
+  q[i] = s - u * t;
+  for (int k = 3; k > 0; --k) {
+    qb.s_hist[k * n + i] = qb.s_hist[(k - 1) * n + i];
+    qb.t_hist[k * n + i] = qb.t_hist[(k - 1) * n + i];
+  }
+  qb.s_hist[i] = s;
+  qb.t_hist[i] = t;
+}
+
 }  // namespace
 
 template <class TV, class TC>
@@ -337,6 +459,47 @@ void launch_qeq_update(const State& st, QeqBuffers& qb, int it, hipStream_t s) {
 void launch_qeq_finish(const State& st, QeqBuffers& qb, hipStream_t s) {
   hipLaunchKernelGGL(k_cg_sums, dim3(kCgBlocks), dim3(kThreads), 0, s, st.n, qb);
   hipLaunchKernelGGL(k_cg_charges, dim3(kCgBlocks), dim3(kThreads), 0, s, st.n, qb, st.q);
+}
+
+// ---- packed systems ----
+namespace {
+inline int pk_blocks(long long items, int per) { return (int)((items + per - 1) / per); }
+template <class TV, class TC>
+void pk_mvdir_t(int n, int maxnb, const QeqBuffers& qb, const TC* col, const TV* val, const PackedQeq& pq, int it, hipStream_t s) {
+  hipLaunchKernelGGL((k_pk_cg_mvdir<TV, TC>), dim3(mv_blocks(n)), dim3(kMvThreads), 0, s, n, maxnb, col, val, qb, pq.sysid, pq.pk, pq.pki, pq.rowv, it);
+}
+}  // namespace
+
+void launch_pk_qeq_init(const State& st, QeqBuffers& qb, const PackedQeq& pq, int imax, double tol, int use_hist, hipStream_t s) {
+  const int n = st.n;
+  hipLaunchKernelGGL(k_cg_guess, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, use_hist);
+  if (qb.fp32) {
+    if (qb.narrow) mv2_t(n, st.maxnb, qb, qb.col16, qb.val32, qb.x, qb.q, s); else mv2_t(n, st.maxnb, qb, qb.col, qb.val32, qb.x, qb.q, s);
+  } else {
+    if (qb.narrow) mv2_t(n, st.maxnb, qb, qb.col16, qb.val, qb.x, qb.q, s); else mv2_t(n, st.maxnb, qb, qb.col, qb.val, qb.x, qb.q, s);
+  }
+  hipLaunchKernelGGL(k_pk_cg_init, dim3(pk_blocks(n, 256)), dim3(256), 0, s, n, qb, pq.rowv);
+  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pk_blocks((long long)pq.S * 64, 256)), dim3(256), 0, s, 0, pq.S, pq.sys_start, pq.rowv,
+                     (const double*)qb.x, pq.pk, pq.pki, 0, imax, tol, pq.any);
+}
+
+void launch_pk_qeq_iteration(const State& st, QeqBuffers& qb, const PackedQeq& pq, int it, int imax, double tol, hipStream_t s) {
+  const int n = st.n;
+  if (qb.fp32) {
+    if (qb.narrow) pk_mvdir_t(n, st.maxnb, qb, qb.col16, qb.val32, pq, it, s); else pk_mvdir_t(n, st.maxnb, qb, qb.col, qb.val32, pq, it, s);
+  } else {
+    if (qb.narrow) pk_mvdir_t(n, st.maxnb, qb, qb.col16, qb.val, pq, it, s); else pk_mvdir_t(n, st.maxnb, qb, qb.col, qb.val, pq, it, s);
+  }
+  const dim3 sg(pk_blocks((long long)pq.S * 64, 256));
+  hipLaunchKernelGGL(k_pk_cg_seg, sg, dim3(256), 0, s, 1, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
+  hipLaunchKernelGGL(k_pk_cg_update, dim3(pk_blocks(n, 256)), dim3(256), 0, s, n, qb, pq.sysid, (const double*)pq.pk, (const int*)pq.pki, pq.rowv);
+  hipLaunchKernelGGL(k_pk_cg_seg, sg, dim3(256), 0, s, 2, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
+}
+
+void launch_pk_qeq_finish(const State& st, QeqBuffers& qb, const PackedQeq& pq, hipStream_t s) {
+  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pk_blocks((long long)pq.S * 64, 256)), dim3(256), 0, s, 3, pq.S, pq.sys_start, pq.rowv,
+                     (const double*)qb.x, pq.pk, pq.pki, 0, 0, 0.0, pq.any);
+  hipLaunchKernelGGL(k_pk_cg_charges, dim3(pk_blocks(st.n, 256)), dim3(256), 0, s, st.n, qb, pq.sysid, (const double*)pq.pk, st.q);
 }
 
 }  // namespace rxa

@@ -80,6 +80,23 @@ struct QeqBuffers {
   double *s_hist, *t_hist;   // 4 * n each, [k * n + i], k = 0 most recent
 };
 constexpr int kCgBlocks = 256;
+// several independent systems packed in one engine: per-system CG scalars and row scratch (k_qeq.hip)
+struct PackedQeq {
+  int S = 0;
+  const int* sys_start = nullptr;   // S + 1 row offsets
+  const int* sysid = nullptr;       // n: system of each row
+  double* pk = nullptr;             // 10 S: per solve |b|, r.p, alpha, beta; sums of s and t
+  int* pki = nullptr;               // 4 S: per solve active, stock loop index
+  double* rowv = nullptr;           // 4 n: per-row d.q and r.p of both solves
+  int* any = nullptr;               // imax + 2: some system active for iteration it + 1 (set by iteration it; [0] by the start)
+};
+void launch_pk_qeq_init(const State& st, QeqBuffers& qb, const PackedQeq& pq, int imax, double tol, int use_hist, hipStream_t s);
+void launch_pk_qeq_iteration(const State& st, QeqBuffers& qb, const PackedQeq& pq, int it, int imax, double tol, hipStream_t s);
+void launch_pk_qeq_finish(const State& st, QeqBuffers& qb, const PackedQeq& pq, hipStream_t s);
+// packed systems: neighbour search, integrator and energy sums per system (k_packed.hip)
+void launch_pk_shift_table(const Cell* cells, int S, double4* shv, const int smx[3], hipStream_t s);
+void launch_pk_wrap(const State& st, const Cell* cells, const int* reb, hipStream_t s);
+void launch_pk_nbr(const State& st, const Cell* cells, const int* sys_start, const int4* ext, const int* reb, double rs, hipStream_t s);
 void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s);
 // Non-bonded terms over the charge-matrix rows instead of the skin list (mixed / FP32 modes): valid while the rows belong to the
 // current positions, carry keys, and the matrix cutoff covers nonb_cut.
@@ -105,6 +122,11 @@ void launch_md_drift(const State& st, MdBuffers& md, double dt, hipStream_t s);
 void launch_md_thermostat(const State& st, MdBuffers& md, hipStream_t s);
 void launch_md_ref_positions(const State& st, MdBuffers& md, hipStream_t s);
 void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s);
+void launch_pk_ref(const State& st, MdBuffers& md, const int* reb, hipStream_t s);
+void launch_pk_disp(const State& st, MdBuffers& md, const int* sys_start, int S, double lim, int* reb, int* any, double* disp,
+                    hipStream_t s);
+void launch_pk_thermostat(const State& st, MdBuffers& md, const int* sys_start, int S, double* nh_all, hipStream_t s);
+void launch_pk_energy(const State& st, const int* sys_start, int S, double* out, hipStream_t s);
 
 void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);   // bytes a multiple of 16
 

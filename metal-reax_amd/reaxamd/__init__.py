@@ -43,6 +43,9 @@ def lib():
             "rxa_pair_style": (I, [P, I, PP, S]),
             "rxa_pair_coeff": (I, [P, S, I, PP, P]),
             "rxa_extract": (I, [P, S, P]),
+            "rxa_set_systems": (I, [P, I, P, P, P, P, P, P]),
+            "rxa_system_results": (I, [P, P, P, P]),
+            "rxa_md_system_targets": (I, [P, P, P]),
             "rxa_cutoffs": (I, [P, P]),
             "rxa_table_dump": (LONG, [P, I, S, LONG]),
             "rxa_set_box": (I, [P, P, P, P, P]),
@@ -193,6 +196,37 @@ class Engine:
         xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
         self._chk(lib().rxa_set_atoms(self.h, t.size, _p(t), _p(g), _p(xx)))
         self.n = t.size
+
+    def set_systems(self, systems):
+        """Packs independent systems (objects with types, tags, x, lo, hi and optionally periodic) into this context."""
+        counts = np.array([len(w.types) for w in systems], dtype=np.int64)
+        boxes = np.zeros((len(systems), 9))
+        per = np.ones((len(systems), 3), dtype=np.int32)
+        for k, w in enumerate(systems):
+            boxes[k, 0:3] = w.lo
+            boxes[k, 3:6] = w.hi
+            if getattr(w, "periodic", None) is not None:
+                per[k] = [int(bool(v)) for v in w.periodic]
+        t = np.ascontiguousarray(np.concatenate([w.types for w in systems]), dtype=np.int32)
+        g = np.ascontiguousarray(np.concatenate([w.tags for w in systems]), dtype=np.int64)
+        xx = np.ascontiguousarray(np.concatenate([np.asarray(w.x).reshape(-1, 3) for w in systems]), dtype=np.float64).reshape(-1)
+        self._chk(lib().rxa_set_systems(self.h, len(systems), _p(counts), _p(boxes), _p(per), _p(t), _p(g), _p(xx)))
+        self.n = t.size
+        self.nsys = len(systems)
+        self.counts = counts
+
+    def system_results(self):
+        S = self.nsys
+        e = np.zeros((S, len(TERMS)))
+        v = np.zeros((S, 6))
+        th = np.zeros((S, 4))
+        self._chk(lib().rxa_system_results(self.h, _p(e), _p(v), _p(th)))
+        return {"energy": e, "virial": v, "ke": th[:, 0], "temp": th[:, 1], "iters_s": th[:, 2].astype(int), "iters_t": th[:, 3].astype(int)}
+
+    def md_system_targets(self, t_target=None, t_damp=None):
+        tt = None if t_target is None else np.ascontiguousarray(t_target, dtype=np.float64)
+        td = None if t_damp is None else np.ascontiguousarray(t_damp, dtype=np.float64)
+        self._chk(lib().rxa_md_system_targets(self.h, _p(tt), _p(td)))
 
     def set_positions(self, x):
         xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)

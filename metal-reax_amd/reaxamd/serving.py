@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Engine, systems
+from . import TERMS, Engine, systems
 
 
 @dataclass
@@ -65,6 +65,34 @@ def run_one(engine, ffield_text, req, precision):
     r = e.compute()
     x, vel, q = e.state()
     return {"energy": r["energy"], "grad": r["grad"], "charges": q, "positions": x, "thermo": thermo}
+
+
+def run_packed(engine, ffield_text, requests):
+    """Requests packed in one context (rxa_set_systems): one launch of each kernel advances them all. They must share the time
+    step and the step count; thermostat targets and damping may differ. Returns one result dict per request (as run_one)."""
+    if len({r.steps for r in requests}) != 1 or len({r.dt for r in requests}) != 1 or len({r.qeq_tol for r in requests}) != 1:
+        raise ValueError("packed requests must share steps, dt and the charge-solve tolerance")
+    e = engine
+    e.set_systems([r.system for r in requests])
+    e.qeq_setup(0.0, 10.0, e.extract("chi"), e.extract("eta"), e.extract("gamma"))
+    v = np.concatenate([systems.maxwell_boltzmann(r.system.masses(), r.t_target, seed=r.seed) for r in requests])
+    masses = np.array([0.0] + [systems.MASS[el] for el in systems.ELEMENTS])
+    r0 = requests[0]
+    e.md_setup(v, masses, dt=r0.dt, t_target=r0.t_target, t_damp=r0.t_damp, skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=r0.qeq_tol)
+    if any(r.t_target != r0.t_target or r.t_damp != r0.t_damp for r in requests):
+        e.md_system_targets([r.t_target for r in requests], [r.t_damp for r in requests])
+    thermo = e.md_run(r0.steps)
+    g = e.compute()["grad"]
+    x, vel, q = e.state()
+    per = e.system_results()
+    out, a = [], 0
+    for k, r in enumerate(requests):
+        b = a + r.system.n
+        out.append({"energy": dict(zip(TERMS, per["energy"][k].tolist())), "grad": g[a:b], "charges": q[a:b], "positions": x[a:b],
+                    "thermo": {"ke": float(per["ke"][k]), "temp": float(per["temp"][k]), "iters_s": int(per["iters_s"][k]),
+                               "iters_t": int(per["iters_t"][k]), "pack": thermo}})
+        a = b
+    return out
 
 
 def new_engine(ffield_text, precision):
