@@ -50,8 +50,8 @@ def batch_and_serving(ffield_text, precision, slots=16):
     one-at-a-time capacity (so one-at-a-time serving could not keep up), with p50/p99 latency."""
     from reaxamd import serving
 
-    def timed(reqs, k, realtime=False):
-        srv = serving.Server(ffield_text, slots=k, precision=precision)
+    def timed(reqs, k, realtime=False, packed=False):
+        srv = serving.Server(ffield_text, slots=k, precision=precision, packed=packed)
         t0 = time.perf_counter()
         done = srv.run(reqs, realtime=realtime)
         span = time.perf_counter() - t0
@@ -85,11 +85,21 @@ def batch_and_serving(ffield_text, precision, slots=16):
     for r in reqs:
         t += rng.exponential(1.0 / rate)
         r.arrival = t
+    arrivals = [r.arrival for r in reqs]
     done, span = timed(reqs, slots, realtime=True)
     lat = serving.latency_stats(done)
-    out["serving"] = {"requests": len(reqs), "atom_steps": work, "slots": slots, "sequential_atom_steps_per_s": seq,
-                      "all_queued_atom_steps_per_s": queued, "poisson_rate_per_s": rate, "poisson_atom_steps_per_s": work / span,
-                      "p50_latency_s": lat["p50_s"], "p99_latency_s": lat["p99_s"]}
+    # dynamic batching: a free slot runs a large request alone or packs every queued small one into one context
+    pq = work / timed(serving.serving_requests(32, seed=2), slots, packed=True)[1]
+    preqs = serving.serving_requests(32, seed=2)
+    for r, a in zip(preqs, arrivals):
+        r.arrival = a
+    pdone, pspan = timed(preqs, slots, realtime=True, packed=True)
+    plat = serving.latency_stats(pdone)
+    out["serving"] = {"requests": len(reqs), "atom_steps": work, "poisson_rate_per_s": rate, "sequential_atom_steps_per_s": seq,
+                      "packed": {"slots": slots, "pack_below_atoms": 3000, "all_queued_atom_steps_per_s": pq, "poisson_atom_steps_per_s": work / pspan,
+                                 "p50_latency_s": plat["p50_s"], "p99_latency_s": plat["p99_s"]},
+                      "slot_executor": {"slots": slots, "all_queued_atom_steps_per_s": queued, "poisson_atom_steps_per_s": work / span,
+                                        "p50_latency_s": lat["p50_s"], "p99_latency_s": lat["p99_s"]}}
     print("bench: serving %s" % out["serving"], file=sys.stderr, flush=True)
     return out
 

@@ -95,6 +95,47 @@ def run_packed(engine, ffield_text, requests):
     return out
 
 
+def run_pack_chunks(engine, ffield_text, requests, t0=None):
+    """Requests of any step counts packed in one context: the pack advances in chunks that end where some request's step
+    count is reached, and that request's result is taken there (its later steps are not used). Splitting a run changes no
+    state and a packed system's results do not depend on the others, so each result is the one the request gets alone.
+    Sets start/end (seconds after t0) on the requests."""
+    if len({r.dt for r in requests}) != 1 or len({r.qeq_tol for r in requests}) != 1:
+        raise ValueError("packed requests must share dt and the charge-solve tolerance")
+    now = (lambda: time.perf_counter() - t0) if t0 is not None else (lambda: 0.0)
+    e = engine
+    start = now()
+    for r in requests:
+        r.start = start
+    e.set_systems([r.system for r in requests])
+    e.qeq_setup(0.0, 10.0, e.extract("chi"), e.extract("eta"), e.extract("gamma"))
+    v = np.concatenate([systems.maxwell_boltzmann(r.system.masses(), r.t_target, seed=r.seed) for r in requests])
+    masses = np.array([0.0] + [systems.MASS[el] for el in systems.ELEMENTS])
+    r0 = requests[0]
+    e.md_setup(v, masses, dt=r0.dt, t_target=r0.t_target, t_damp=r0.t_damp, skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=r0.qeq_tol)
+    if any(r.t_target != r0.t_target or r.t_damp != r0.t_damp for r in requests):
+        e.md_system_targets([r.t_target for r in requests], [r.t_damp for r in requests])
+    offs = np.concatenate([[0], np.cumsum([r.system.n for r in requests])])
+    done = 0
+    for target in sorted({r.steps for r in requests}):
+        thermo = e.md_run(target - done)
+        done = target
+        g = e.compute()["grad"]
+        x, vel, q = e.state()
+        per = e.system_results()
+        t = now()
+        for k, r in enumerate(requests):
+            if r.steps != target:
+                continue
+            a, b = offs[k], offs[k + 1]
+            r.result = {"energy": dict(zip(TERMS, per["energy"][k].tolist())), "grad": g[a:b].copy(), "charges": q[a:b].copy(),
+                        "positions": x[a:b].copy(),
+                        "thermo": {"ke": float(per["ke"][k]), "temp": float(per["temp"][k]), "iters_s": int(per["iters_s"][k]),
+                                   "iters_t": int(per["iters_t"][k]), "pack": thermo}}
+            r.end = t
+    return requests
+
+
 def new_engine(ffield_text, precision):
     e = Engine(0)
     e.pair_style(["NULL", "precision", precision])
@@ -114,12 +155,19 @@ def engine_bytes_per_atom(e):
 
 
 class Server:
-    """Slot-based concurrent executor of MD requests (see the module docstring)."""
+    """Concurrent executor of MD requests (see the module docstring). packed=False: each slot runs one request at a time in its
+    own context. packed=True: a free slot takes the oldest queued request alone if it has at least pack_below atoms (the
+    single-system path, with cell lists), else every queued request below that size that fits the memory budget (at most
+    max_pack_atoms atoms), packed in one context (run_pack_chunks)."""
 
-    def __init__(self, ffield_text, slots=4, precision="mixed", mem_budget_bytes=200 << 30):
+    def __init__(self, ffield_text, slots=4, precision="mixed", mem_budget_bytes=200 << 30, packed=False, max_pack_atoms=4_000_000,
+                 pack_below=3000):
         self.ffield_text = ffield_text
         self.precision = precision
         self.slots = slots
+        self.packed = packed
+        self.max_pack_atoms = max_pack_atoms
+        self.pack_below = pack_below   # packed mode: requests of at least this many atoms run alone on the single-system path
         self.budget = mem_budget_bytes
         self.bytes_per_atom = DEFAULT_BYTES_PER_ATOM
         self._free = queue.Queue()
@@ -139,6 +187,8 @@ class Server:
     def run(self, requests, realtime=False):
         """Runs `requests` (in arrival order). realtime=True honours Request.arrival (seconds after start); otherwise all
         requests are queued at once. Returns the requests with start/end times (seconds after start) and results."""
+        if self.packed:
+            return self._run_packed(requests, realtime)
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
         threads = []
@@ -176,6 +226,79 @@ class Server:
             th = threading.Thread(target=work, args=(slot, req), daemon=True)
             th.start()
             threads.append(th)
+        for th in threads:
+            th.join()
+        if errors:
+            rid, ex = errors[0]
+            raise RuntimeError("request %d failed: %s" % (rid, ex))
+        return reqs
+
+
+    def _run_packed(self, requests, realtime):
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        t0 = time.perf_counter()
+        waiting = []
+        state = {"submitted": 0}
+        cv = threading.Condition()
+        errors = []
+
+        def take():   # the oldest request alone if it is large, else every queued small request that fits (arrival order)
+            first = waiting[0]
+            if first.system.n * self.bytes_per_atom > self.budget:
+                raise ValueError("request %d needs more device memory than the budget" % first.rid)
+            if first.system.n >= self.pack_below:
+                return [waiting.pop(0)]
+            pack, atoms, k = [], 0, 0
+            while k < len(waiting):
+                r = waiting[k]
+                if r.system.n >= self.pack_below:
+                    k += 1
+                    continue
+                if pack and (atoms + r.system.n > self.max_pack_atoms or (atoms + r.system.n) * self.bytes_per_atom > self.budget):
+                    break
+                pack.append(waiting.pop(k))
+                atoms += r.system.n
+            return pack
+
+        def worker(slot):
+            while True:
+                with cv:
+                    while not waiting and state["submitted"] < len(reqs):
+                        cv.wait()
+                    if not waiting:
+                        return
+                    try:
+                        pack = take()
+                    except Exception as ex:
+                        errors.append((waiting[0].rid, ex))
+                        waiting.clear()
+                        return
+                try:
+                    if len(pack) == 1 and pack[0].system.n >= self.pack_below:
+                        r = pack[0]
+                        r.start = time.perf_counter() - t0
+                        r.result = run_one(self._engines[slot], self.ffield_text, r, self.precision)
+                        r.end = time.perf_counter() - t0
+                    else:
+                        run_pack_chunks(self._engines[slot], self.ffield_text, pack, t0)
+                    self.bytes_per_atom = max(self.bytes_per_atom, engine_bytes_per_atom(self._engines[slot]))
+                except Exception as ex:   # surfaced after the run
+                    errors.append((pack[0].rid, ex))
+
+        threads = [threading.Thread(target=worker, args=(k,), daemon=True) for k in range(self.slots)]
+        for th in threads:
+            th.start()
+        for r in reqs:
+            if realtime:
+                wait = r.arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+            with cv:
+                waiting.append(r)
+                state["submitted"] += 1
+                cv.notify()
+        with cv:
+            cv.notify_all()
         for th in threads:
             th.join()
         if errors:

@@ -76,3 +76,46 @@ def test_engine_reuse_matches_fresh_engine(ffield_text):
     assert np.array_equal(reused["charges"], fresh["charges"])
     assert np.array_equal(reused["positions"], fresh["positions"])
     assert np.array_equal(reused["grad"], fresh["grad"])
+
+
+@pytest.mark.gpu
+def test_packed_server_same_result_alone_and_in_the_mix(ffield_text):
+    """Dynamic batching: requests of uneven sizes and step counts packed in one context give, bit for bit, what each gives
+    alone (a pack of one); the pack's chunks end at every request's own step count."""
+    from reaxamd.serving import Request, Server, new_engine, run_pack_chunks
+    syst = small_batch(8, seed=4)
+
+    def reqs():
+        return [Request(rid=k, system=s, steps=6 + 3 * k, t_target=2500.0, dt=0.1, seed=200 + k) for k, s in enumerate(syst)]
+    srv = Server(ffield_text, slots=1, packed=True)
+    mixed = srv.run(reqs())
+    srv.close()
+    assert all(r.end >= r.start for r in mixed)
+    e = new_engine(ffield_text, "mixed")
+    for k in (0, 3, 7):
+        alone = run_pack_chunks(e, ffield_text, [reqs()[k]])[0].result
+        m = mixed[k].result
+        for name in alone["energy"]:
+            assert alone["energy"][name] == m["energy"][name], (k, name)
+        for key in ("grad", "charges", "positions"):
+            assert np.array_equal(alone[key], m[key]), (k, key)
+    e.close()
+
+
+@pytest.mark.gpu
+def test_packed_server_poisson_trace(ffield_text):
+    from reaxamd.serving import Server, latency_stats
+    syst = small_batch(12, seed=6)
+    reqs = requests(syst, steps=12)
+    rng = np.random.default_rng(2)
+    t = 0.0
+    for r in reqs:
+        t += rng.exponential(0.01)
+        r.arrival = t
+    srv = Server(ffield_text, slots=2, packed=True)
+    done = srv.run(reqs, realtime=True)
+    srv.close()
+    st = latency_stats(done)
+    print(st, flush=True)
+    assert all(r.end > r.start >= r.arrival - 1e-6 and r.result for r in done)
+    assert st["p99_s"] >= st["p50_s"] > 0
