@@ -4,7 +4,7 @@ For each mode: the error of one step against the FP64 oracle (energy per atom, l
 error; charges from each side's own charge solve at tol 1e-10), then an NVE run after a short NVT equilibration, reporting the
 drift of the total energy (linear fit, kcal/mol per atom per ps) and its RMS fluctuation.
 
-    python -m reaxamd.precision [--nside 6|20] [--nve-steps N]
+    python -m reaxamd.precision [--nside 6|20] [--nvt-steps N] [--nve-steps N] [--chunk N] [--modes fp64,mixed,fp32]
 
 The oracle is test infrastructure (oracle/); this study uses it as the checker only.
 """
@@ -70,6 +70,8 @@ def nve_drift(ff, w, prec, nvt_steps, nve_steps, chunk, dt=0.25):
         th = e.md_run(chunk)
         t.append((k + 1) * chunk * dt / 1000.0)
         etot.append((th["pe"] + th["ke"]) / w.n)
+        if (k + 1) % max(1, 10000 // chunk) == 0:
+            print("  %s NVE %d steps, E/atom %.6f" % (prec, (k + 1) * chunk, etot[-1]), file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     e.close()
     t, etot = np.array(t), np.array(etot)
@@ -85,10 +87,11 @@ def main():
     ap.add_argument("--nvt-steps", type=int, default=200)
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--modes", default="fp64,mixed,fp32")
     args = ap.parse_args()
     ff = systems.load_ffield()
     w = systems.water_box(args.nside, seed=1)
-    precs = ("fp64", "mixed", "fp32")
+    precs = tuple(args.modes.split(","))
     res = {"atoms": w.n}
     if not args.no_oracle:
         res["step_error_vs_fp64_oracle"] = step_error(ff, w, precs)
