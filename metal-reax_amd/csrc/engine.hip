@@ -186,6 +186,7 @@ struct Engine {
   MdBuffers md{};
   DBuf<double> md_buf;
   bool md_ready = false, md_nve = false;
+  bool v_scale_pending = false;   // the last thermostat velocity factor (md.nh[19]) is still to be applied
   double md_dt = 0.25;
   int md_qeq_every = 1, md_qeq_imax = 200;
   double md_qeq_tol = 1e-6;
@@ -868,12 +869,19 @@ struct Engine {
     timer.end(PH_BONDS, t0);
     HIPCHK(hipMemcpyAsync(flags_h, st.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, b));
     HIPCHK(hipEventRecord(ev_flags, b));
+    bonded_pending = true;
     t0 = timer.begin(PH_BOCORR);
     launch_bo_correct(prec, st, g, tab, b);
     timer.end(PH_BOCORR, t0);
     t0 = timer.begin(PH_TERMS);
     launch_atom_terms(prec, st, g, tab, b);
     timer.end(PH_TERMS, t0);
+  }
+  // queues the second half of the bonded chain as soon as the capacity flags are back, without waiting for them: called between
+  // the launches of a CG batch, so the angle kernels need not wait for the host to finish queueing the batch
+  bool bonded_pending = false;
+  void bonded_poll() {
+    if (bonded_pending && hipEventQuery(ev_flags) == hipSuccess) bonded_end();
   }
   void bonded_end() {
     for (int attempt = 0;; ++attempt) {
@@ -897,6 +905,7 @@ struct Engine {
     launch_bond_derivatives(st, b);
     timer.end(PH_DBOND, t0);
     if (b != stream) HIPCHK(hipEventRecord(ev_bonded, b));
+    bonded_pending = false;
   }
   void nonbonded_launch() {
     hipEvent_t t0 = timer.begin(PH_NONB);
@@ -1082,6 +1091,7 @@ struct Engine {
         launch_pk_qeq_iteration(st, qb, pq, cg_it, imax, cg_tol, stream);
         timer.end(PH_QEQ_SPMV, t0);
         ++cg_it;
+        bonded_poll();
       }
       HIPCHK(hipMemcpyAsync(cgf_h, d_cg_any.p, (size_t)cg_it * sizeof(int), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipEventRecord(ev_cg, stream));
@@ -1095,6 +1105,7 @@ struct Engine {
       t0 = timer.begin(PH_QEQ_SPMV);
       launch_qeq_mvdir(st, qb, cg_it, imax, cg_tol, stream);
       timer.end(PH_QEQ_SPMV, t0);
+      bonded_poll();
     }
     HIPCHK(hipMemcpyAsync(cgf_h, qb.iscal, 2 * (size_t)cg_it * sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(cgs_h, qb.scal, (8 + 2 * (size_t)cg_it) * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -1265,6 +1276,7 @@ struct Engine {
     md_qeq_every = qeq_every; md_qeq_imax = qimax; md_qeq_tol = qtol;
     md_ready = true;
     md_steps = 0;
+    v_scale_pending = false;
     // initial forces (and charges)
     build_neighbors();
     launch_md_ref_positions(st, md, stream);
@@ -1282,16 +1294,19 @@ struct Engine {
     double disp = 0;
     for (int s = 0; s < steps; ++s) {
       hipEvent_t t0 = timer.begin(PH_MD);
-      if (packed) launch_pk_thermostat(st, md, d_sys_start.p, S, d_nh_pk.p, stream);
-      else if (!md_nve) launch_md_thermostat(st, md, stream);
-      launch_md_half_kick(st, md, dtf, stream);
-      launch_md_drift(st, md, md_dt, stream);
+      if (packed) {
+        launch_pk_thermostat(st, md, d_sys_start.p, S, d_nh_pk.p, stream);
+        launch_md_half_kick(st, md, dtf, stream);
+        launch_md_drift(st, md, md_dt, stream);
+      } else {   // thermostat, half kick, drift and the displacement partials in fused kernels
+        launch_md_step_start(st, md, dtf, md_dt, !md_nve, v_scale_pending, stream);
+        v_scale_pending = false;
+      }
       if (packed) {   // each system's own displacement decides its own rebuild
         launch_fill_int(d_rebany.p, 1, 0, stream);
         launch_pk_disp(st, md, d_sys_start.p, S, 0.5 * skin, d_reb.p, d_rebany.p, d_disp_pk.p, stream);
         HIPCHK(hipMemcpyAsync(rebany_h, d_rebany.p, sizeof(int), hipMemcpyDeviceToHost, stream));
       } else {
-        launch_md_displacement(st, md, stream);
         HIPCHK(hipMemcpyAsync(disp_h, md.disp, sizeof(double), hipMemcpyDeviceToHost, stream));
       }
       timer.end(PH_MD, t0);
@@ -1316,16 +1331,24 @@ struct Engine {
       // bonded chain on the bonded stream while the main stream solves for the charges, then the non-bonded pass and the join;
       // energy totals are read once, after the last step
       bonded_begin();
-      if (do_qeq) qeq_begin(md_qeq_imax, md_qeq_tol);
-      bonded_end();
+      if (do_qeq) qeq_begin(md_qeq_imax, md_qeq_tol);   // polls for the bonded chain's flags between its launches
+      if (bonded_pending) bonded_end();
       if (do_qeq) qeq_end(md_qeq_imax, 0);
       nonbonded_launch();
       join_gather(s == steps - 1);
       t0 = timer.begin(PH_MD);
-      launch_md_half_kick(st, md, dtf, stream);
-      if (packed) launch_pk_thermostat(st, md, d_sys_start.p, S, d_nh_pk.p, stream);
-      else if (!md_nve || s == steps - 1) launch_md_thermostat(st, md, stream);   // NVE: kinetic energy of the last step only
+      if (packed) {
+        launch_md_half_kick(st, md, dtf, stream);
+        launch_pk_thermostat(st, md, d_sys_start.p, S, d_nh_pk.p, stream);
+      } else {   // NVE: kinetic energy of the last step only (factor 1); NVT: the velocity factor is applied by the next kernel
+        launch_md_step_end(st, md, dtf, !md_nve || s == steps - 1, stream);
+        v_scale_pending = !md_nve;
+      }
       timer.end(PH_MD, t0);
+    }
+    if (v_scale_pending) {   // the last thermostat factor, before anything reads the velocities
+      launch_md_scale(st, md, stream);
+      v_scale_pending = false;
     }
     sync();
     if (packed) pk_collect();

@@ -179,9 +179,96 @@ __global__ void __launch_bounds__(kRT) k_disp_final(MdBuffers md) {
   if (threadIdx.x == 0) md.disp[0] = sqrt(sh[0]);
 }
 
+// Fused forms of the integrator's element-wise kernels for the step loop, with the same arithmetic and the same partial-sum layout
+// (kRBlocks blocks, grid-stride) as the separate kernels above:
+//   k_scale_ke:   v *= nh[19] (the end-of-step thermostat factor still to apply, if `apply`), then the partials of sum m v^2;
+//   k_kick_drift: v *= nh[19] (if `scale`), v -= dtf g / m, x += dt v, packed positions, partials of the largest displacement;
+//   k_kick_ke:    v -= dtf g / m, then the partials of sum m v^2.
+__global__ void __launch_bounds__(kRT) k_scale_ke(State st, MdBuffers md, int apply) {
+  __shared__ double sh[kRT];
+  const double f = md.nh[19];
+  double v = 0;
+  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT) {
+    double vx = md.vx[i], vy = md.vy[i], vz = md.vz[i];
+    if (apply) {
+      vx *= f; vy *= f; vz *= f;
+      md.vx[i] = vx; md.vy[i] = vy; md.vz[i] = vz;
+    }
+    v += md.mass[i] * (vx * vx + vy * vy + vz * vz);
+  }
+  block_sum(&v, 1, sh);
+  if (threadIdx.x == 0) md.partial[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(kRT) k_kick_drift(State st, MdBuffers md, double dtf, double dt, int scale) {
+  __shared__ double sh[kRT];
+  const double fs = md.nh[19];
+  double m = 0;
+  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT) {
+    double vx = md.vx[i], vy = md.vy[i], vz = md.vz[i];
+    if (scale) { vx *= fs; vy *= fs; vz *= fs; }
+    const double f = dtf / md.mass[i];
+    vx -= f * st.gx[i];
+    vy -= f * st.gy[i];
+    vz -= f * st.gz[i];
+    md.vx[i] = vx; md.vy[i] = vy; md.vz[i] = vz;
+    const double x = st.x[i] + dt * vx, y = st.y[i] + dt * vy, z = st.z[i] + dt * vz;
+    st.x[i] = x;
+    st.y[i] = y;
+    st.z[i] = z;
+    st.pos[i] = make_double4(x, y, z, pos_w(st, i));
+    const double dx = x - md.x0[i], dy = y - md.y0[i], dz = z - md.z0[i];
+    m = fmax(m, dx * dx + dy * dy + dz * dz);
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kRT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) md.partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kRT) k_kick_ke(State st, MdBuffers md, double dtf) {
+  __shared__ double sh[kRT];
+  double v = 0;
+  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT) {
+    const double f = dtf / md.mass[i];
+    const double vx = md.vx[i] - f * st.gx[i], vy = md.vy[i] - f * st.gy[i], vz = md.vz[i] - f * st.gz[i];
+    md.vx[i] = vx; md.vy[i] = vy; md.vz[i] = vz;
+    v += md.mass[i] * (vx * vx + vy * vy + vz * vz);
+  }
+  block_sum(&v, 1, sh);
+  if (threadIdx.x == 0) md.partial[blockIdx.x] = v;
+}
+
 inline int nblk(int n, int per) { return (n + per - 1) / per; }
 
 }  // namespace
+
+// step start: thermostat half step (the pending end-of-step factor applied first), velocity half step, drift, displacement
+void launch_md_step_start(const State& st, MdBuffers& md, double dtf, double dt, bool thermostat, bool pending, hipStream_t s) {
+  if (thermostat) {
+    hipLaunchKernelGGL(k_scale_ke, dim3(kRBlocks), dim3(kRT), 0, s, st, md, pending ? 1 : 0);
+    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(kRT), 0, s, md);
+  } else if (pending) {
+    hipLaunchKernelGGL(k_scale_v, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md);
+  }
+  hipLaunchKernelGGL(k_kick_drift, dim3(kRBlocks), dim3(kRT), 0, s, st, md, dtf, dt, thermostat ? 1 : 0);
+  hipLaunchKernelGGL(k_disp_final, dim3(1), dim3(kRT), 0, s, md);
+}
+// step end: velocity half step, then (if `thermostat`) the thermostat half step, whose velocity factor stays pending
+void launch_md_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s) {
+  if (thermostat) {
+    hipLaunchKernelGGL(k_kick_ke, dim3(kRBlocks), dim3(kRT), 0, s, st, md, dtf);
+    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(kRT), 0, s, md);
+  } else {
+    hipLaunchKernelGGL(k_half_kick, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md, dtf);
+  }
+}
+void launch_md_scale(const State& st, MdBuffers& md, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_v, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md);
+}
 
 void launch_energy_reduce(const State& st, double* out, hipStream_t s) {
   // out[kTerms + 6] receives the sums; out + kRed .. holds kRBlocks * kRed partials (scratch)

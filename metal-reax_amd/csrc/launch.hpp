@@ -122,6 +122,10 @@ void launch_md_drift(const State& st, MdBuffers& md, double dt, hipStream_t s);
 void launch_md_thermostat(const State& st, MdBuffers& md, hipStream_t s);
 void launch_md_ref_positions(const State& st, MdBuffers& md, hipStream_t s);
 void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s);
+// fused step kernels (the separate ones above, same arithmetic): see k_md.hip
+void launch_md_step_start(const State& st, MdBuffers& md, double dtf, double dt, bool thermostat, bool pending, hipStream_t s);
+void launch_md_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s);
+void launch_md_scale(const State& st, MdBuffers& md, hipStream_t s);
 void launch_pk_ref(const State& st, MdBuffers& md, const int* reb, hipStream_t s);
 void launch_pk_disp(const State& st, MdBuffers& md, const int* sys_start, int S, double lim, int* reb, int* any, double* disp,
                     hipStream_t s);
