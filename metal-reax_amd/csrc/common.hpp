@@ -222,7 +222,8 @@ __device__ inline int shift_index(const State& st, int code) {
 #pragma clang fp contract(off)
 // pair_dvec for a list entry e = code | flag of neighbour j (position pj) of the atom at pi, without tags or branches: the same
 // operands in the same order as pair_dvec (the table holds shift_vec of every code), so the result is bitwise identical
-__device__ inline double pair_dvec_pos(const State& st, const double4& pi, const double4& pj, int e, double& dx, double& dy, double& dz) {
+// (pi and pj by value: selecting between a reference into global memory and one to a local would put the local on the stack)
+__device__ inline double pair_dvec_pos(const State& st, const double4 pi, const double4 pj, int e, double& dx, double& dy, double& dz) {
   const bool canon = (e & kCanon) != 0;
   const int code = entry_code(e);
   const double4 S = st.shv[pos_sys(pi.w) * st.shv_stride + shift_index(st, canon ? code : shift_neg(code))];

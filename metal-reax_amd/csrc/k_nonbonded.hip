@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(64 * kNbWaves) k_nonbonded_f(State st, Globals
 
 // pair_dvec_list for a charge-matrix entry: the same operands in the same order, the shift vector found through the entry's key
 #pragma clang fp contract(off)
-__device__ inline double pair_dvec_key(const State& st, const double4& pi, const double4& pj, int key, double& dx, double& dy, double& dz) {
+__device__ inline double pair_dvec_key(const State& st, const double4 pi, const double4 pj, int key, double& dx, double& dy, double& dz) {
   const bool canon = (key & 0x8000) != 0;
   const double4 S = st.shv[pos_sys(pi.w) * st.shv_stride + (key & 0x7FFF)];
   const double ax = canon ? pj.x : pi.x, ay = canon ? pj.y : pi.y, az = canon ? pj.z : pi.z;

@@ -1,0 +1,48 @@
+"""Code generation of the engine's kernels: no hot kernel may place locals in scratch (private) memory.
+
+A scratch-resident local turns every access into a memory round trip; it happened once silently when a helper selected between a
+reference into global memory and a reference to a local (k_qeq_build went from 117 to 185 us per 24k-atom step). The device
+assembly of every source is generated here (hipcc, gfx950, the build's flags) and each kernel's private segment size read
+from its descriptor. Two kernels keep a reserved but unused frame (FP32 per-atom terms) or spill in FP64 mode only.
+"""
+import concurrent.futures
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(REPO, "metal-reax_amd", "csrc")
+HIPCC = "/opt/rocm/bin/hipcc"
+ALLOWED = {"k_atom_termsIfE", "k_hbond_donorIdE"}
+
+
+def _asm(src, out):
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "--offload-device-only", "-S", src, "-o", out]
+    subprocess.run(cmd, check=True, cwd=CSRC, capture_output=True, timeout=600)
+    return open(out).read()
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_no_scratch_in_engine_kernels(tmp_path):
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        asms = list(ex.map(lambda f: _asm(f, str(tmp_path / (f + ".s"))), srcs))
+    bad = []
+    seen = 0
+    for asm in asms:
+        for m in re.finditer(r"\.amdhsa_kernel (\S+)(.*?)\.end_amdhsa_kernel", asm, re.S):
+            name, body = m.group(1), m.group(2)
+            if not name.startswith("_ZN3rxa"):
+                continue   # library kernels (hipCUB radix sort)
+            seen += 1
+            size = int(re.search(r"amdhsa_private_segment_fixed_size (\d+)", body).group(1))
+            short = re.sub(r"(ENS_|EvNS_).*", "", name.replace("_ZN3rxa12_GLOBAL__N_1", ""))
+            short = re.sub(r"^\d+", "", short)
+            if size and short not in ALLOWED:
+                bad.append((short, size))
+    assert seen > 30
+    assert not bad, bad
