@@ -160,8 +160,9 @@ __device__ inline void bo_prime_batch(const State& st, const Globals& g, const T
   nb += __popcll(mb);
 }
 
+// 4 waves per SIMD (108 VGPRs without spills; 129 and 3 waves otherwise)
 template <class TB>
-__global__ void __launch_bounds__(64 * kBuildWaves) k_bond_build(State st, Cell c, Globals g, Tables t) {
+__global__ void __attribute__((amdgpu_waves_per_eu(4))) __launch_bounds__(64 * kBuildWaves) k_bond_build(State st, Cell c, Globals g, Tables t) {
   __shared__ int rj_[kBuildWaves][kRing], rs_[kBuildWaves][kRing];
   __shared__ double rd_[kBuildWaves][kRing], rx_[kBuildWaves][kRing], ry_[kBuildWaves][kRing], rz_[kBuildWaves][kRing];
   const int w = threadIdx.x >> 6;
