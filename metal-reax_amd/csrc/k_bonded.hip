@@ -648,7 +648,7 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
   const int nbw = wave_max(nbj);
   double e_tor = 0, e_con = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0;
   double own_cdbo = 0, own_fx = 0, own_fy = 0, own_fz = 0;
-  double w[6] = {0, 0, 0, 0, 0, 0};   // this end's half of the torsion virial, about the midpoint of the central bond
+  double fIx = 0, fIy = 0, fIz = 0;   // this lane's forces on the outer partners i, summed (for the virial)
   const TR thb_cut = (TR)g.thb_cut;
   const TR p_tor2 = (TR)g.p_tor2, p_tor3 = (TR)g.p_tor3, p_tor4 = (TR)g.p_tor4, p_cot2 = (TR)g.p_cot2;
   const TR MINS = TR(1e-10);
@@ -768,11 +768,6 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
                        gjz = (double)(a7 * dcj.z + a8 * ecj.z + a9 * doj.z);
           fix += gix; fiy += giy; fiz += giz;
           fjx += gjx; fjy += gjy; fjz += gjz;
-          {   // x_i - m = d_ji - d_jk / 2, x_j - m = -d_jk / 2 (m: midpoint of the central bond)
-            const double hx = 0.5 * (double)djk.x, hy = 0.5 * (double)djk.y, hz = 0.5 * (double)djk.z;
-            vir_add(w, -1.0, (double)dji.x - hx, (double)dji.y - hy, (double)dji.z - hz, gix, giy, giz);
-            vir_add(w, -1.0, -hx, -hy, -hz, gjx, gjy, gjz);
-          }
           if (counted) {   // energy and the central bond once per torsion
             e_tor += (double)(fn10 * sijk * sjkl * CV);
             e_con += (double)(p_cot1 * fn12 * gg);
@@ -782,9 +777,19 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
         }
       }
     }
+    fIx += fix; fIy += fiy; fIz += fiz;
     cdbo_ij = gsum<G>(cdbo_ij); fix = gsum<G>(fix); fiy = gsum<G>(fiy); fiz = gsum<G>(fiz);
     if (pk == pi) { own_cdbo += cdbo_ij; own_fx += fix; own_fy += fiy; own_fz += fiz; }
   }
+  // This end's half of the torsion virial about the midpoint m of each central bond: with x_i - m = d_ji - d_jk / 2 and
+  // x_j - m = -d_jk / 2, - sum (x_i - m) (x) g_i - sum (x_j - m) (x) g_j = - sum_p d_jp (x) F_p + sum_lanes (d_jk / 2) (x) (F_I + F_j),
+  // F_p the group's force on the partner in slot p, F_I and F_j this lane's forces on its outer partners and on j.
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  if (live && tj >= 0 && pk < nbj) {
+    const size_t q = sl(pk, j, n);
+    vir_add(w, -1.0, st.b_dx[q], st.b_dy[q], st.b_dz[q], own_fx, own_fy, own_fz);
+  }
+  if (cen) vir_add(w, 0.5, (double)djk.x, (double)djk.y, (double)djk.z, fIx + fjx, fIy + fjy, fIz + fjz);
   e_tor = gsum<G>(e_tor); e_con = gsum<G>(e_con); cdself = gsum<G>(cdself);
   fjx = gsum<G>(fjx); fjy = gsum<G>(fjy); fjz = gsum<G>(fjz);
   for (int c = 0; c < 6; ++c) w[c] = gsum<G>(w[c]);
