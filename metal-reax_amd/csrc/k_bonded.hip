@@ -639,7 +639,7 @@ __device__ inline TR tan_inv(TR s, TR c) {
 // the energy and the central-bond coefficients. The outer-bond contributions of an i are summed over the group and kept by the
 // lane owning slot i.
 template <class TR, int G>
-__global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) {
+__device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = gid / G, pk = gid % G, n = st.n;
   const bool live = j < n;
@@ -647,7 +647,7 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
   const int nbj = live ? min(st.b_cnt[j], st.maxb) : 0;
   const int nbw = wave_max(nbj);
   double e_tor = 0, e_con = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0;
-  double own_cdbo = 0, own_fx = 0, own_fy = 0, own_fz = 0;
+  double own_fx = 0, own_fy = 0, own_fz = 0;
   double fIx = 0, fIy = 0, fIz = 0;   // this lane's forces on the outer partners i, summed (for the virial)
   const TR thb_cut = (TR)g.thb_cut;
   const TR p_tor2 = (TR)g.p_tor2, p_tor3 = (TR)g.p_tor3, p_tor4 = (TR)g.p_tor4, p_cot2 = (TR)g.p_cot2;
@@ -683,7 +683,7 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
   }
   double cdbo_jk = 0, cdbopi_jk = 0;
   for (int pi = 0; pi < nbw; ++pi) {
-    double cdbo_ij = 0, fix = 0, fiy = 0, fiz = 0;
+    TR cdbo_ij = 0, fix = 0, fiy = 0, fiz = 0;   // this lane's terms of partner i: at most one per bond of k
     if (cen && pi != pk && pi < nbj) {
       const size_t qi = sl(pi, j, n);
       const TR BOij = (TR)st.bo[qi];
@@ -761,9 +761,9 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
           const TR CC6 = TR(2) * p_cot1 * fn12 * cw * sijk * sjkl;
           // this end's half: j itself, its outer partner i, its outer bond ij, its CdDelta
           cdself += (double)CT3;
-          cdbo_ij += (double)(CT4 + CC1);
+          cdbo_ij += CT4 + CC1;
           const TR a7 = CT7 + CC4, a8 = CT8 + CC5, a9 = CT9 + CC6;
-          const double gix = (double)(a7 * dci.x + a9 * doi.x), giy = (double)(a7 * dci.y + a9 * doi.y), giz = (double)(a7 * dci.z + a9 * doi.z);
+          const TR gix = a7 * dci.x + a9 * doi.x, giy = a7 * dci.y + a9 * doi.y, giz = a7 * dci.z + a9 * doi.z;
           const double gjx = (double)(a7 * dcj.x + a8 * ecj.x + a9 * doj.x), gjy = (double)(a7 * dcj.y + a8 * ecj.y + a9 * doj.y),
                        gjz = (double)(a7 * dcj.z + a8 * ecj.z + a9 * doj.z);
           fix += gix; fiy += giy; fiz += giz;
@@ -777,9 +777,12 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
         }
       }
     }
-    fIx += fix; fIy += fiy; fIz += fiz;
-    cdbo_ij = gsum<G>(cdbo_ij); fix = gsum<G>(fix); fiy = gsum<G>(fiy); fiz = gsum<G>(fiz);
-    if (pk == pi) { own_cdbo += cdbo_ij; own_fx += fix; own_fy += fiy; own_fz += fiz; }
+    fIx += (double)fix; fIy += (double)fiy; fIz += (double)fiz;
+    const double gc = gsum<G>((double)cdbo_ij), gx = gsum<G>((double)fix), gy = gsum<G>((double)fiy), gz = gsum<G>((double)fiz);
+    if (pk == pi) {   // the lane owning slot i: its BO coefficient now, its force for the virial and the end
+      if (live && tj >= 0) st.Cdbo[sl(pk, j, n)] += gc;
+      own_fx += gx; own_fy += gy; own_fz += gz;
+    }
   }
   // This end's half of the torsion virial about the midpoint m of each central bond: with x_i - m = d_ji - d_jk / 2 and
   // x_j - m = -d_jk / 2, - sum (x_i - m) (x) g_i - sum (x_j - m) (x) g_j = - sum_p d_jp (x) F_p + sum_lanes (d_jk / 2) (x) (F_I + F_j),
@@ -795,7 +798,7 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
   for (int c = 0; c < 6; ++c) w[c] = gsum<G>(w[c]);
   if (live && tj >= 0 && pk < nbj) {
     const size_t q = sl(pk, j, n);
-    st.Cdbo[q] += own_cdbo + cdbo_jk;
+    st.Cdbo[q] += cdbo_jk;
     st.Cdbopi[q] += cdbopi_jk;
     st.fpx[q] += own_fx; st.fpy[q] += own_fy; st.fpz[q] += own_fz;
   }
@@ -806,6 +809,16 @@ __global__ void __launch_bounds__(256) k_torsion(State st, Globals g, Tables t) 
     st.eatom[T_TOR * n + j] = e_tor;
     st.eatom[T_CON * n + j] = e_con;
   }
+}
+
+// FP32 terms at 3 waves per SIMD (168 VGPRs, no spills); FP64 at the compiler's choice
+template <int G>
+__global__ void __attribute__((amdgpu_waves_per_eu(3))) __launch_bounds__(256) k_torsion_f(State st, Globals g, Tables t) {
+  torsion_body<float, G>(st, g, t);
+}
+template <int G>
+__global__ void __launch_bounds__(256) k_torsion_d(State st, Globals g, Tables t) {
+  torsion_body<double, G>(st, g, t);
 }
 
 // ================================================================================================================
@@ -1075,7 +1088,8 @@ template <class TR, int G>
 void launch_terms_g(const State& st, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * G, 256)), blk(256), wgrid(nblk(st.n, 4));
   hipLaunchKernelGGL((k_valence<TR, G>), grid, blk, 0, s, st, g, t);
-  hipLaunchKernelGGL((k_torsion<TR, G>), grid, blk, 0, s, st, g, t);
+  if (sizeof(TR) == 4) hipLaunchKernelGGL((k_torsion_f<G>), grid, blk, 0, s, st, g, t);
+  else hipLaunchKernelGGL((k_torsion_d<G>), grid, blk, 0, s, st, g, t);
   hipLaunchKernelGGL(k_hbond_donor<TR>, dim3(nblk(st.n, kHbWaves)), dim3(64 * kHbWaves), 0, s, st, g, t);
   hipLaunchKernelGGL(k_hbond_acceptor<TR>, wgrid, blk, 0, s, st, g, t);
 }

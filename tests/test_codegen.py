@@ -3,7 +3,9 @@
 A scratch-resident local turns every access into a memory round trip; it happened once silently when a helper selected between a
 reference into global memory and a reference to a local (k_qeq_build went from 117 to 185 us per 24k-atom step). The device
 assembly of every source is generated here (hipcc, gfx950, the build's flags) and each kernel's private segment size read
-from its descriptor. Two kernels keep a reserved but unused frame (FP32 per-atom terms) or spill in FP64 mode only.
+from its descriptor. Allowed: a reserved but unused frame (FP32 per-atom terms), spills of the FP64-mode H-bond donor, and the
+FP32 torsion kept at 3 waves per SIMD, which spills one 8-byte value that is reloaded once per outer partner, outside the inner
+loop (24 us per step faster than the spill-free 2-wave build).
 """
 import concurrent.futures
 import os
@@ -16,7 +18,9 @@ import pytest
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(REPO, "metal-reax_amd", "csrc")
 HIPCC = "/opt/rocm/bin/hipcc"
-ALLOWED = {"k_atom_termsIfE", "k_hbond_donorIdE"}
+ALLOWED = {"k_atom_termsIfE", "k_hbond_donorIdE", "k_torsion_fILi8EE", "k_torsion_fILi16EE", "k_torsion_fILi32EE",
+           "k_torsion_fILi64EE"}
+MAX_ALLOWED_BYTES = 72
 
 
 def _asm(src, out):
@@ -42,7 +46,7 @@ def test_no_scratch_in_engine_kernels(tmp_path):
             size = int(re.search(r"amdhsa_private_segment_fixed_size (\d+)", body).group(1))
             short = re.sub(r"(ENS_|EvNS_).*", "", name.replace("_ZN3rxa12_GLOBAL__N_1", ""))
             short = re.sub(r"^\d+", "", short)
-            if size and short not in ALLOWED:
+            if size and (short not in ALLOWED or size > MAX_ALLOWED_BYTES):
                 bad.append((short, size))
     assert seen > 30
     assert not bad, bad
