@@ -884,6 +884,16 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
   double e_hb = 0, fjx = 0, fjy = 0, fjz = 0;
   double w[6] = {0, 0, 0, 0, 0, 0};   // virial of the triples about the donor
   int head = 0, fill = 0, inrange = 0;
+  // the bonds (j, i) that can carry an H-bond (acceptor-type partner, BO >= 0.01), tested by the lanes at once (nb <= 64)
+  unsigned long long hbonds;
+  {
+    bool ok = false;
+    if (lane < nb) {
+      const size_t q = sl(lane, j, n);
+      ok = t.sp[st.type[st.b_j[q]]].p_hbond == 2 && (TR)st.bo[q] >= TR(1e-2);
+    }
+    hbonds = __ballot(ok);
+  }
   // the triples of the m ring entries from head on
   auto batch = [&](int m) {
     const bool has = lane < m;
@@ -897,10 +907,10 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
     const V3<TR> djk{(TR)dx, (TR)dy, (TR)dz};
     const TR rjk = (TR)r;
     double fkx = 0, fky = 0, fkz = 0;
-    for (int p = 0; p < nb; ++p) {
+    for (unsigned long long left = hbonds; left; left &= left - 1) {   // wave-uniform, ascending slots
+      const int p = __ffsll((long long)left) - 1;
       const size_t q = sl(p, j, n);
       const int i = st.b_j[q], ti = st.type[i];
-      if (t.sp[ti].p_hbond != 2 || !((TR)st.bo[q] >= TR(1e-2))) continue;   // wave-uniform
       const V3<TR> dji{(TR)st.b_dx[q], (TR)st.b_dy[q], (TR)st.b_dz[q]};
       const TR rij = (TR)st.b_d[q];
       double c1 = 0, fx = 0, fy = 0, fz = 0;
