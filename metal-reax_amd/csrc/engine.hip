@@ -970,7 +970,7 @@ struct Engine {
     const int imax_cap = std::max(qeq_imax_cap, 2048);
     // sub-arrays start on 256-byte boundaries (the CG reads (s, t) pairs as 16-byte loads)
     auto rnd = [](size_t k) { return (k + 31) & ~(size_t)31; };
-    const size_t nmv = 2 * (size_t)qeq_mv_partials(n);
+    const size_t nmv = 4 * (size_t)qeq_mv_partials(n);
     const size_t sz = rnd((size_t)n * maxnb) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + rnd(nmv) +
                       rnd(8 + 2 * (size_t)(imax_cap + 1)) + 2 * rnd(4 * (size_t)n);
     std::vector<double> keep;

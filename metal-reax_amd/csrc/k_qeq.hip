@@ -4,7 +4,7 @@
 // stops exactly where the stock loop would (i < imax && sqrt(r.M^-1 r)/|b| > tol), so iteration counts are those of two separate
 // stock solves. Vectors are interleaved per atom ([2 i + system]).
 //
-// Determinism: all dot products are reduced in a fixed order (per-thread grid-stride partials, a fixed LDS tree per block, and the
+// Determinism: all dot products are reduced in a fixed order (per-thread grid-stride partials, a fixed per-block order, and the
 // per-block partials reduced again in the same fixed order by every block that needs the scalar), so every block computes the same
 // alpha/beta bitwise and results repeat run to run. Scalars of iteration `it` live in slot `it` of device arrays, so the blocks of one
 // launch never read a value another block of the same launch writes.
@@ -15,7 +15,7 @@ namespace rxa {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kUpdBlocks = 64, kUpdThreads = 1024;   // CG update kernel
+constexpr int kUpdBlocks = 240, kUpdThreads = 256;   // CG update kernel
 constexpr int kMvThreads = 256;             // product kernels: 4 waves per block,
 constexpr int kMvRowsPerBlock = kMvThreads / 16;   // 16 lanes per row (kRowLanes below)
 inline __host__ __device__ int mv_blocks(int n) { return (n + kMvRowsPerBlock - 1) / kMvRowsPerBlock; }
@@ -24,20 +24,23 @@ __device__ inline double& SIG(double* scal, int it, int sys) { return scal[8 + 2
 
 template <int K, int NT = kThreads>
 __device__ inline void block_reduce(double (&v)[K], double* sh) {
-  const int tid = threadIdx.x;
-  for (int k = 0; k < K; ++k) sh[k * NT + tid] = v[k];
+  // xor butterfly inside each wave, then every thread sums the per-wave values in wave order: one barrier instead of a
+  // log2(NT)-level LDS tree, and the same fixed order in every block
+  constexpr int kWaves = NT / 64;
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (lane_id() == 0)
+    for (int k = 0; k < K; ++k) sh[k * kWaves + threadIdx.x / 64] = v[k];
   __syncthreads();
-  for (int stride = NT / 2; stride > 0; stride >>= 1) {
-    if (tid < stride)
-      for (int k = 0; k < K; ++k) sh[k * NT + tid] += sh[k * NT + tid + stride];
-    __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    double s = sh[k * kWaves];
+    for (int w = 1; w < kWaves; ++w) s += sh[k * kWaves + w];
+    v[k] = s;
   }
-  for (int k = 0; k < K; ++k) v[k] = sh[k * NT];
   __syncthreads();
 }
 
 // every block re-reduces the `np` partials of a previous launch (K values per block) in the same fixed order: thread t sums
-// partials t, t + NT, ... in sequence, then the fixed LDS tree
+// partials t, t + NT, ... in sequence, then block_reduce's fixed order
 template <int K, int NT = kThreads>
 __device__ inline void reduce_partials(const double* part, double (&v)[K], double* sh, int np = kCgBlocks) {
   for (int k = 0; k < K; ++k) v[k] = 0.0;
@@ -107,19 +110,42 @@ __device__ inline double2 row_product(int row, int n, int maxnb, const int* __re
 
 __device__ inline int product_row() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes); }
 
-// w = (diag(eta) + H) v for both systems (the initial residual's product)
-template <class TV, class TC>
+// w = (diag(eta) + H) v for both systems (the initial residual's product). With kInit the CG start is fused in: the row's
+// owner writes r = b - w and p = r / eta instead of w, and each block leaves the partials of b.b and r.p of its rows.
+template <class TV, class TC, bool kInit>
 __global__ void __launch_bounds__(kMvThreads) k_spmv2(int n, int maxnb, const int* __restrict__ cnt, const TC* __restrict__ col,
                                                      const TV* __restrict__ val, const double* __restrict__ eta, const double* v,
-                                                     double* w) {
+                                                     double* w, QeqBuffers qb) {
   const double2* __restrict__ v2 = reinterpret_cast<const double2*>(v);
   double2* w2 = reinterpret_cast<double2*>(w);
   const int i = product_row();
   const double2 y = row_product<TV, TC>(i, n, maxnb, cnt, col, val, v2);
+  double acc[4] = {0, 0, 0, 0};
   if (i < n && lane_id() % kRowLanes == 0) {
     const double2 xi = v2[i];
     const double ei = eta[i];
-    w2[i] = make_double2(y.x + ei * xi.x, y.y + ei * xi.y);
+    const double2 wi = make_double2(y.x + ei * xi.x, y.y + ei * xi.y);
+    if (!kInit) {
+      w2[i] = wi;
+    } else {
+      const double hi = qb.hinv[i];
+      const double wv[2] = {wi.x, wi.y};
+      for (int s = 0; s < 2; ++s) {
+        const double b = qb.b[2 * i + s];
+        const double r = b - wv[s];
+        const double d = r * hi;
+        qb.r[2 * i + s] = r;
+        qb.p[2 * i + s] = d;
+        acc[s] = b * b;
+        acc[2 + s] = r * d;
+      }
+    }
+  }
+  if (kInit) {
+    __shared__ double sh[4 * kMvThreads / 64];
+    block_reduce<4, kMvThreads>(acc, sh);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 4; ++k) qb.partial_mv[blockIdx.x * 4 + k] = acc[k];
   }
 }
 
@@ -132,7 +158,7 @@ __global__ void __launch_bounds__(kMvThreads) k_spmv2(int n, int maxnb, const in
 template <class TV, class TC>
 __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const TC* __restrict__ col, const TV* __restrict__ val, QeqBuffers qb,
                                                         int it, int imax, double tol) {
-  __shared__ double sh[2 * kMvThreads];
+  __shared__ double sh[2 * kMvThreads / 64];
   int act[2];
   double beta[2] = {0.0, 0.0};
   if (it == 1) {
@@ -190,29 +216,10 @@ __global__ void __launch_bounds__(kThreads) k_spmv1(int n, int maxnb, const int*
 }
 
 
-__global__ void __launch_bounds__(kThreads) k_cg_init(int n, QeqBuffers qb) {
-  __shared__ double sh[4 * kThreads];
-  double acc[4] = {0, 0, 0, 0};
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    const double hi = qb.hinv[i];
-    for (int s = 0; s < 2; ++s) {
-      const double r = qb.b[2 * i + s] - qb.q[2 * i + s];
-      const double d = r * hi;
-      qb.r[2 * i + s] = r;
-      qb.p[2 * i + s] = d;
-      acc[s] += qb.b[2 * i + s] * qb.b[2 * i + s];
-      acc[2 + s] += r * d;
-    }
-  }
-  block_reduce<4>(acc, sh);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 4; ++k) qb.partial[blockIdx.x * 4 + k] = acc[k];
-}
-
-__global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, int imax, double tol) {
-  __shared__ double sh[4 * kThreads];
+__global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, int nparts, int imax, double tol) {
+  __shared__ double sh[4 * kThreads / 64];
   double v[4];
-  reduce_partials<4>(qb.partial, v, sh);
+  reduce_partials<4>(qb.partial_mv, v, sh, nparts);
   if (threadIdx.x == 0) {
     for (int s = 0; s < 2; ++s) {
       const double bn = sqrt(v[s]);
@@ -226,7 +233,7 @@ __global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, int ima
 // x += alpha d, r -= alpha q, p = r / eta, partial r.p   (iteration it >= 1); few large blocks, since every block first
 // re-reduces all partials of the product
 __global__ void __launch_bounds__(kUpdThreads) k_cg_update(int n, QeqBuffers qb, int it) {
-  __shared__ double sh[2 * kUpdThreads];
+  __shared__ double sh[2 * kUpdThreads / 64];
   const int* act = qb.iscal + 2 * (it - 1);
   const int a[2] = {act[0], act[1]};
   if (!a[0] && !a[1]) return;
@@ -252,7 +259,7 @@ __global__ void __launch_bounds__(kUpdThreads) k_cg_update(int n, QeqBuffers qb,
 }
 
 __global__ void __launch_bounds__(kThreads) k_cg_sums(int n, QeqBuffers qb) {
-  __shared__ double sh[2 * kThreads];
+  __shared__ double sh[2 * kThreads / 64];
   double acc[2] = {0, 0};
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     acc[0] += qb.x[2 * i];
@@ -264,7 +271,7 @@ __global__ void __launch_bounds__(kThreads) k_cg_sums(int n, QeqBuffers qb) {
 
 // q = s - (sum s / sum t) t, then the history shift of calculate_Q
 __global__ void __launch_bounds__(kThreads) k_cg_charges(int n, QeqBuffers qb, double* q) {
-  __shared__ double sh[2 * kThreads];
+  __shared__ double sh[2 * kThreads / 64];
   double v[2];
   reduce_partials<2>(qb.partial, v, sh);
   const double u = v[0] / v[1];
@@ -419,9 +426,9 @@ void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, 
   }
 }
 
-template <class TV, class TC>
+template <bool kInit, class TV, class TC>
 void mv2_t(int n, int maxnb, const QeqBuffers& qb, const TC* col, const TV* val, const double* x, double* y, hipStream_t s) {
-  hipLaunchKernelGGL((k_spmv2<TV, TC>), dim3(mv_blocks(n)), dim3(kMvThreads), 0, s, n, maxnb, qb.cnt, col, val, qb.eta, x, y);
+  hipLaunchKernelGGL((k_spmv2<TV, TC, kInit>), dim3(mv_blocks(n)), dim3(kMvThreads), 0, s, n, maxnb, qb.cnt, col, val, qb.eta, x, y, qb);
 }
 
 template <class TV, class TC>
@@ -433,12 +440,11 @@ void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int 
   const int n = st.n;
   hipLaunchKernelGGL(k_cg_guess, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, use_hist);
   if (qb.fp32) {
-    if (qb.narrow) mv2_t(n, st.maxnb, qb, qb.col16, qb.val32, qb.x, qb.q, s); else mv2_t(n, st.maxnb, qb, qb.col, qb.val32, qb.x, qb.q, s);
+    if (qb.narrow) mv2_t<true>(n, st.maxnb, qb, qb.col16, qb.val32, qb.x, qb.q, s); else mv2_t<true>(n, st.maxnb, qb, qb.col, qb.val32, qb.x, qb.q, s);
   } else {
-    if (qb.narrow) mv2_t(n, st.maxnb, qb, qb.col16, qb.val, qb.x, qb.q, s); else mv2_t(n, st.maxnb, qb, qb.col, qb.val, qb.x, qb.q, s);
+    if (qb.narrow) mv2_t<true>(n, st.maxnb, qb, qb.col16, qb.val, qb.x, qb.q, s); else mv2_t<true>(n, st.maxnb, qb, qb.col, qb.val, qb.x, qb.q, s);
   }
-  hipLaunchKernelGGL(k_cg_init, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb);
-  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kThreads), 0, s, qb, imax, tol);
+  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kThreads), 0, s, qb, mv_blocks(n), imax, tol);
 }
 
 void launch_qeq_mvdir(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s) {
@@ -474,9 +480,9 @@ void launch_pk_qeq_init(const State& st, QeqBuffers& qb, const PackedQeq& pq, in
   const int n = st.n;
   hipLaunchKernelGGL(k_cg_guess, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, use_hist);
   if (qb.fp32) {
-    if (qb.narrow) mv2_t(n, st.maxnb, qb, qb.col16, qb.val32, qb.x, qb.q, s); else mv2_t(n, st.maxnb, qb, qb.col, qb.val32, qb.x, qb.q, s);
+    if (qb.narrow) mv2_t<false>(n, st.maxnb, qb, qb.col16, qb.val32, qb.x, qb.q, s); else mv2_t<false>(n, st.maxnb, qb, qb.col, qb.val32, qb.x, qb.q, s);
   } else {
-    if (qb.narrow) mv2_t(n, st.maxnb, qb, qb.col16, qb.val, qb.x, qb.q, s); else mv2_t(n, st.maxnb, qb, qb.col, qb.val, qb.x, qb.q, s);
+    if (qb.narrow) mv2_t<false>(n, st.maxnb, qb, qb.col16, qb.val, qb.x, qb.q, s); else mv2_t<false>(n, st.maxnb, qb, qb.col, qb.val, qb.x, qb.q, s);
   }
   hipLaunchKernelGGL(k_pk_cg_init, dim3(pk_blocks(n, 256)), dim3(256), 0, s, n, qb, pq.rowv);
   hipLaunchKernelGGL(k_pk_cg_seg, dim3(pk_blocks((long long)pq.S * 64, 256)), dim3(256), 0, s, 0, pq.S, pq.sys_start, pq.rowv,
