@@ -219,14 +219,20 @@ __device__ inline int shift_index(const State& st, int code) {
   return ((s2 + st.smx2) * (2 * st.smx1 + 1) + (s1 + st.smx1)) * (2 * st.smx0 + 1) + (s0 + st.smx0);
 }
 
+// table index of the shift that list entry e applies to its canonical end: the entry's code when the listing atom is that end,
+// else the code's negation; the table enumerates the box [-smx, smx]^3 in shift_index order, so negating a code maps index k
+// to N - 1 - k (N the table size)
+__device__ inline int entry_shift_index(const State& st, int e) {
+  const int k = shift_index(st, entry_code(e));
+  return (e & kCanon) ? k : (2 * st.smx0 + 1) * (2 * st.smx1 + 1) * (2 * st.smx2 + 1) - 1 - k;
+}
+
 #pragma clang fp contract(off)
 // pair_dvec for a list entry e = code | flag of neighbour j (position pj) of the atom at pi, without tags or branches: the same
 // operands in the same order as pair_dvec (the table holds shift_vec of every code), so the result is bitwise identical
 // (pi and pj by value: selecting between a reference into global memory and one to a local would put the local on the stack)
-__device__ inline double pair_dvec_pos(const State& st, const double4 pi, const double4 pj, int e, double& dx, double& dy, double& dz) {
-  const bool canon = (e & kCanon) != 0;
-  const int code = entry_code(e);
-  const double4 S = st.shv[pos_sys(pi.w) * st.shv_stride + shift_index(st, canon ? code : shift_neg(code))];
+__device__ inline double pair_dvec_shift(const double4 pi, const double4 pj, bool canon, const double4 S, double& dx, double& dy,
+                                         double& dz) {
   const double ax = canon ? pj.x : pi.x, ay = canon ? pj.y : pi.y, az = canon ? pj.z : pi.z;
   const double bx = canon ? pi.x : pj.x, by = canon ? pi.y : pj.y, bz = canon ? pi.z : pj.z;
   const double ex = (ax + S.x) - bx, ey = (ay + S.y) - by, ez = (az + S.z) - bz;
@@ -234,6 +240,9 @@ __device__ inline double pair_dvec_pos(const State& st, const double4 pi, const 
   dy = canon ? ey : -ey;
   dz = canon ? ez : -ez;
   return dx * dx + dy * dy + dz * dz;
+}
+__device__ inline double pair_dvec_pos(const State& st, const double4 pi, const double4 pj, int e, double& dx, double& dy, double& dz) {
+  return pair_dvec_shift(pi, pj, (e & kCanon) != 0, st.shv[pos_sys(pi.w) * st.shv_stride + entry_shift_index(st, e)], dx, dy, dz);
 }
 __device__ inline double pair_dvec_list(const State& st, const double4& pi, int j, int e, double& dx, double& dy, double& dz) {
   return pair_dvec_pos(st, pi, st.pos[j], e, dx, dy, dz);

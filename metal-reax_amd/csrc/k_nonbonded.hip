@@ -289,8 +289,9 @@ __device__ inline double pair_dvec_key(const State& st, const double4 pi, const 
 // cutoff, so every lane carries a pair (no distance pre-pass, no compaction) and a row costs its column and key (4 bytes per
 // pair with 16-bit columns) instead of the 8-byte skin entries of all pairs out to the list radius. Lane l takes entries l,
 // l + 64, ..., the assignment of the skin-list kernel's compacted batches, so the sums are the same.
+// (5 waves per SIMD: 96 VGPRs without spills; the compiler's choice, 106 VGPRs and 4 waves, is slower)
 template <class TC>
-__global__ void __launch_bounds__(256) k_nonbonded_rows(State st, Globals g, Tables t, const TC* __restrict__ col,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_nonbonded_rows(State st, Globals g, Tables t, const TC* __restrict__ col,
                                                         const unsigned short* __restrict__ key, const int* __restrict__ rcnt) {
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
@@ -304,14 +305,27 @@ __global__ void __launch_bounds__(256) k_nonbonded_rows(State st, Globals g, Tab
     const double4 pi = st.pos[i];
     const double cut = g.nonb_cut, cut2 = cut * cut * (1.0 + 1e-12);
     const NbPairF* __restrict__ row = t.nbf + ti * g.ntypes;
+    // two-deep software pipeline as in the matrix build: columns/keys two strides ahead, positions and charges one ahead
+    // (lanes past the row's end carry the row atom itself and key 0, valid indices never used)
+    int jB = i, kyB = 0, jC = i, kyC = 0;
+    if (lane < cnt) { jC = (int)col[base + lane]; kyC = key[base + lane]; }
+    double4 pC = st.pos[jC];
+    float qC = (float)st.q[jC];
+    if (lane + 64 < cnt) { jB = (int)col[base + lane + 64]; kyB = key[base + lane + 64]; }
     for (int k = lane; k < cnt; k += 64) {
-      const int j = (int)col[base + k];
-      const double4 pj = st.pos[j];
+      const double4 pj = pC;
+      const float qj = qC;
+      const int ky = kyC;
+      jC = jB; kyC = kyB;
+      pC = st.pos[jC];
+      qC = (float)st.q[jC];
+      jB = i; kyB = 0;
+      if (k + 128 < cnt) { jB = (int)col[base + k + 128]; kyB = key[base + k + 128]; }
       double dx, dy, dz;
-      const double r2 = pair_dvec_key(st, pi, pj, (int)key[base + k], dx, dy, dz);
+      const double r2 = pair_dvec_key(st, pi, pj, ky, dx, dy, dz);
       if (r2 <= cut2) {
         const double rd = sqrt(r2);
-        if (rd <= cut) nb_pair_f(g, row, qi, (float)st.q[j], pos_type(pj.w), rd, dx, dy, dz, a);
+        if (rd <= cut) nb_pair_f(g, row, qi, qj, pos_type(pj.w), rd, dx, dy, dz, a);
       }
     }
   }
@@ -335,50 +349,89 @@ __global__ void __launch_bounds__(256) k_nonbonded_rows(State st, Globals g, Tab
   }
 }
 
+constexpr int kQbWaves = 4, kQbRing = 128;
+
+// One wave per row atom over its skin list: a filter pass (distance only, list entries two chunks ahead and positions one chunk
+// ahead in flight) pushes the pairs within the matrix cutoff, in list order, into a per-wave LDS ring; every 64 of them are
+// finished together (taper, shielded 1/r, stores), so the costly part runs on full waves. Row entries are kept in skin-list
+// order: slot m of the row is the m-th pair of the list within the cutoff. The 16-bit key is the canonical flag and the
+// shift-table index the distance used.
 __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t, QeqBuffers qb) {
+  __shared__ int ring_j[kQbWaves][kQbRing];
+  __shared__ int ring_k[kQbWaves][kQbRing];   // key | neighbour type << 16
+  __shared__ double ring_r2[kQbWaves][kQbRing];
   const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= st.n) return;
   const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  int* rj = ring_j[wv];
+  int* rk = ring_k[wv];
+  double* rr = ring_r2[wv];
   const int ti = st.type[i];
   const size_t base = (size_t)i * st.maxnb;
-  const int cnt = min(st.nb_cnt[i], st.maxnb);
+  const int cnt = ti >= 0 ? min(st.nb_cnt[i], st.maxnb) : 0;
   const double swb2 = g.q_swb * g.q_swb;
   const double* T = g.qtap;
   const double4 pi = st.pos[i];
-  int m = 0;
+  const double4* __restrict__ shv = st.shv + pos_sys(pi.w) * st.shv_stride;
+  const double* __restrict__ shld = t.qeq_shld + max(ti, 0) * g.ntypes;
+  int m = 0, head = 0, fill = 0;
+  auto batch = [&](int count) {
+    if (lane < count) {
+      const int e = (head + lane) & (kQbRing - 1);
+      const int j = rj[e], kt = rk[e];
+      const double r = sqrt(rr[e]);
+      double Tap = T[7] * r + T[6];
+      Tap = Tap * r + T[5]; Tap = Tap * r + T[4]; Tap = Tap * r + T[3]; Tap = Tap * r + T[2]; Tap = Tap * r + T[1]; Tap = Tap * r + T[0];
+      const double h = (Tap * 14.4) * rcbrt_f64(r * r * r + shld[kt >> 16]);
+      const size_t o = base + m + lane;
+      if (qb.narrow) qb.col16[o] = (unsigned short)j; else qb.col[o] = j;
+      if (qb.fp32) qb.val32[o] = (float)h; else qb.val[o] = h;
+      if (qb.keyed) qb.key[o] = (unsigned short)(kt & 0xFFFF);
+    }
+    m += count;
+  };
+  const int* __restrict__ lj = st.nb_j + base;
+  const int* __restrict__ ls = st.nb_s + base;
+  int jB = i, eB = 0, jC = i, eC = 0;
+  if (lane < cnt) { jC = lj[lane]; eC = ls[lane]; }
+  double4 pC = st.pos[jC];
+  if (64 + lane < cnt) { jB = lj[64 + lane]; eB = ls[64 + lane]; }
   for (int k0 = 0; k0 < cnt; k0 += 64) {
     const int k = k0 + lane;
+    const int j = jC, e = eC;
+    const double4 pj = pC;
+    jC = jB; eC = eB;
+    if (k0 + 64 < cnt) pC = st.pos[jC];
+    jB = i; eB = 0;
+    if (k + 128 < cnt) { jB = lj[k + 128]; eB = ls[k + 128]; }
     bool keep = false;
-    int j = 0;
-    double h = 0;
-    int e = 0;
-    if (k < cnt && ti >= 0) {
-      j = st.nb_j[base + k];
-      e = st.nb_s[base + k];
+    double r2 = 0;
+    int kt = 0;
+    if (k < cnt) {
+      const bool canon = (e & kCanon) != 0;
+      const int sidx = entry_shift_index(st, e);
       double dx, dy, dz;
-      int tj;
-      const double r2 = pair_dvec_list(st, pi, j, e, dx, dy, dz, tj);
-      if (tj >= 0 && r2 <= swb2) {
-        const double r = sqrt(r2);
-        double Tap = T[7] * r + T[6];
-        Tap = Tap * r + T[5]; Tap = Tap * r + T[4]; Tap = Tap * r + T[3]; Tap = Tap * r + T[2]; Tap = Tap * r + T[1]; Tap = Tap * r + T[0];
-        h = (Tap * 14.4) * rcbrt_f64(r * r * r + t.qeq_shld[ti * g.ntypes + tj]);
-        keep = true;
-      }
+      r2 = pair_dvec_shift(pi, pj, canon, shv[sidx], dx, dy, dz);
+      const int tj = pos_type(pj.w);
+      keep = tj >= 0 && r2 <= swb2;
+      kt = (qb.keyed ? (canon ? 0x8000 : 0) | sidx : 0) | (tj << 16);   // a keyed table has at most 32768 entries
     }
     const unsigned long long msk = __ballot(keep);
     if (keep) {
-      const int slot = m + lanes_below(msk);
-      if (qb.narrow) qb.col16[base + slot] = (unsigned short)j; else qb.col[base + slot] = j;
-      if (qb.fp32) qb.val32[base + slot] = (float)h; else qb.val[base + slot] = h;
-      if (qb.keyed) {
-        const bool canon = (e & kCanon) != 0;
-        const int code = entry_code(e);
-        qb.key[base + slot] = (unsigned short)((canon ? 0x8000 : 0) | shift_index(st, canon ? code : shift_neg(code)));
-      }
+      const int e2 = (head + fill + lanes_below(msk)) & (kQbRing - 1);
+      rj[e2] = j; rk[e2] = kt; rr[e2] = r2;
     }
-    m += __popcll(msk);
+    fill += __popcll(msk);
+    __builtin_amdgcn_wave_barrier();
+    if (fill >= 64) {
+      batch(64);
+      head = (head + 64) & (kQbRing - 1);
+      fill -= 64;
+      __builtin_amdgcn_wave_barrier();
+    }
   }
+  if (fill > 0) batch(fill);
   if (lane == 0) {
     qb.cnt[i] = m;
     const double eta = ti >= 0 ? t.qeq_eta[ti] : 1.0;
