@@ -34,7 +34,9 @@ struct DevPair {
 };
 
 // van der Waals / Coulomb parameters of a type pair for the single-precision non-bonded kernel (pgw = (1 / gamma_w)^p_vdW1)
-struct NbPairF { float D, alpha, rvdw, pgw, ecore, acore, rcore, gamma, lgcij, lgre, pad0, pad1; };
+// single-precision non-bonded pair constants, reciprocals and quotients precomputed (in FP64, then rounded) so the pair
+// arithmetic needs no division by a constant
+struct NbPairF { float D, alpha, irvdw, a_r, pgw, ecore, acore, ircore, ac_r, gamma, lgcij, re6; };   // a_r = alpha / r_vdW, ac_r = acore / rcore, re6 = lgre^6
 
 struct DevAngle { double theta_00, p_val1, p_val2, p_val4, p_val7, p_pen1, p_coa1, pad; };
 struct DevTorsion { double V1, V2, V3, p_tor1, p_cot1, pad[3]; };

@@ -313,10 +313,10 @@ struct Engine {
     for (size_t k = 0; k < nbf.size(); ++k) {
       const DevPair& d = pr[k];
       NbPairF f{};
-      f.D = (float)d.D; f.alpha = (float)d.alpha; f.rvdw = (float)d.r_vdW;
+      f.D = (float)d.D; f.alpha = (float)d.alpha; f.irvdw = (float)(1.0 / d.r_vdW); f.a_r = (float)(d.alpha / d.r_vdW);
       f.pgw = (float)std::pow(1.0 / d.gamma_w, ff.gp[28]);
-      f.ecore = (float)d.ecore; f.acore = (float)d.acore; f.rcore = (float)d.rcore; f.gamma = (float)d.gamma;
-      f.lgcij = (float)d.lgcij; f.lgre = (float)d.lgre;
+      f.ecore = (float)d.ecore; f.acore = (float)d.acore; f.ircore = (float)(1.0 / d.rcore); f.ac_r = (float)(d.acore / d.rcore);
+      f.gamma = (float)d.gamma; f.lgcij = (float)d.lgcij; f.re6 = (float)std::pow(d.lgre, 6.0);
       nbf[k] = f;
     }
     up(d_sp, sp); up(d_pair, pr); up(d_angle_idx, aidx); up(d_angle, ang); up(d_tors_idx, tidx); up(d_tors, tors); up(d_hb, hb);

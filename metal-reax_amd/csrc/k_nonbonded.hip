@@ -143,45 +143,47 @@ struct NbAcc { double gx = 0, gy = 0, gz = 0, ev = 0, ee = 0, v[6] = {0, 0, 0, 0
 // one pair (i, j) within nonb_cut at distance rd, dvec = x_j - x_i: this end's gradient, half of the pair energy and virial
 __device__ inline void nb_pair_f(const Globals& g, const NbPairF* __restrict__ row, float qi, float qj, int tj, double rd, double dx,
                              double dy, double dz, NbAcc& a) {
-  const float p_vdW1 = (float)g.p_vdW1, p1i = 1.0f / p_vdW1;
+  // divisions by constants are products with reciprocals (table or loop-invariant), divisions by pair quantities use the
+  // hardware reciprocal (1 ulp): single-precision terms, within their own rounding
+  const float p_vdW1 = (float)g.p_vdW1, p1i = (float)(1.0 / g.p_vdW1);
   const int vt = g.vdw_type, lg = g.lg;
-  const float swa = (float)g.nonb_low, tw = (float)g.nonb_cut - (float)g.nonb_low;
+  const float swa = (float)g.nonb_low, itw = (float)(1.0 / (g.nonb_cut - g.nonb_low));
   const float L2E = 1.4426950408889634f;
   const NbPairF P = row[tj];
-  const float r = (float)rd;
-  const float x = (r - swa) / tw, x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+  const float r = (float)rd, ir = __builtin_amdgcn_rcpf(r);
+  const float x = (r - swa) * itw, x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
   const float Tap = 1.0f - x4 * (35.0f - 84.0f * x + 70.0f * x2 - 20.0f * x3);
-  const float dTap = (x3 * (-140.0f + 420.0f * x - 420.0f * x2 + 140.0f * x3)) / (tw * r);
+  const float dTap = (x3 * (-140.0f + 420.0f * x - 420.0f * x2 + 140.0f * x3)) * (itw * ir);
   float evdw, CEvd;
   if (vt == 1 || vt == 3) {
     const float pr = fexp2(p_vdW1 * flog2(r)), S = pr + P.pgw;
     const float fn13 = fexp2(p1i * flog2(S));
-    const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - fn13 / P.rvdw)), e1 = e2 * e2;
+    const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - fn13 * P.irvdw)), e1 = e2 * e2;
     const float e0 = P.D * (e1 - 2.0f * e2);
     evdw = Tap * e0;
-    const float dfn13 = fn13 / S * pr / (r * r);
-    CEvd = dTap * e0 - Tap * P.D * (P.alpha / P.rvdw) * (e1 - e2) * dfn13;
+    const float dfn13 = fn13 * __builtin_amdgcn_rcpf(S) * pr * (ir * ir);
+    CEvd = dTap * e0 - Tap * P.D * P.a_r * (e1 - e2) * dfn13;
   } else {
-    const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - r / P.rvdw)), e1 = e2 * e2;
+    const float e2 = fexp2((0.5f * L2E) * P.alpha * (1.0f - r * P.irvdw)), e1 = e2 * e2;
     const float e0 = P.D * (e1 - 2.0f * e2);
     evdw = Tap * e0;
-    CEvd = dTap * e0 - Tap * P.D * (P.alpha / P.rvdw) * (e1 - e2) / r;
+    CEvd = dTap * e0 - Tap * P.D * P.a_r * (e1 - e2) * ir;
   }
   if (vt == 2 || vt == 3) {
-    const float ec = P.ecore * fexp2(L2E * P.acore * (1.0f - r / P.rcore));
+    const float ec = P.ecore * fexp2(L2E * P.acore * (1.0f - r * P.ircore));
     evdw += Tap * ec;
-    CEvd += dTap * ec + Tap * (-(P.acore / P.rcore) * ec) / r;
+    CEvd += dTap * ec + Tap * (-P.ac_r * ec) * ir;
     if (lg) {
-      const float r6 = (r * r * r) * (r * r * r), r5 = r6 / r, re6 = (P.lgre * P.lgre * P.lgre) * (P.lgre * P.lgre * P.lgre);
-      const float elg = -(P.lgcij / (r6 + re6));
+      const float r3 = r * r * r, r6 = r3 * r3, idn = __builtin_amdgcn_rcpf(r6 + P.re6);
+      const float elg = -P.lgcij * idn;
       evdw += Tap * elg;
-      CEvd += dTap * elg + Tap * (-6.0f * elg * r5 / (r6 + re6)) / r;
+      CEvd += dTap * elg + Tap * (-6.0f * elg * (r6 * ir) * idn) * ir;
     }
   }
   const float d1 = r * r * r + P.gamma, rc = fexp2(flog2(d1) * (-1.0f / 3.0f));   // d1^(-1/3)
   const float qq = 332.06371f * qi * qj;
   const float eel = qq * Tap * rc;
-  const float CE = CEvd + qq * (dTap - Tap * r / d1) * rc;
+  const float CE = CEvd + qq * (dTap - Tap * r * __builtin_amdgcn_rcpf(d1)) * rc;
   const double ce = (double)CE;
   a.gx += -ce * dx; a.gy += -ce * dy; a.gz += -ce * dz;
   a.ev += 0.5 * (double)evdw;
