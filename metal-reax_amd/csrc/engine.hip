@@ -856,12 +856,16 @@ struct Engine {
   // the per-atom terms (valid whenever the lists fit); the main stream meanwhile runs the charge solve and the non-bonded pass.
   // Only the group-size dependent angle kernels wait for the flags. An overflow drains both streams, grows the lists and redoes
   // the chain. The gather on the main stream joins the two.
+  // an event on the main stream after which the bonded chain may start (set by the MD loop for a step that keeps its lists:
+  // the positions are final there, before the charge matrix of the step is queued); otherwise the fork is recorded here
+  hipEvent_t fork_after = nullptr;
   void bonded_begin() {
     const hipStream_t b = bstream();
     if (b != stream) {
-      HIPCHK(hipEventRecord(ev_fork, stream));
-      HIPCHK(hipStreamWaitEvent(b, ev_fork, 0));
+      if (!fork_after) HIPCHK(hipEventRecord(ev_fork, stream));
+      HIPCHK(hipStreamWaitEvent(b, fork_after ? fork_after : ev_fork, 0));
     }
+    fork_after = nullptr;
     hipEvent_t t0 = timer.begin(PH_BONDS);
     launch_fill_int(st.flags, 8, 0, b);
     launch_bond_build(prec, st, cell, g, tab, b);
@@ -1316,16 +1320,19 @@ struct Engine {
       const bool do_qeq = md_qeq_every > 0 && md_steps % md_qeq_every == 0;
       if (do_qeq && nbr_valid) qeq_matrix();   // speculative: the common step keeps its lists; redone after a rebuild
       HIPCHK(hipEventSynchronize(ev_disp));
+      fork_after = ev_disp;   // the bonded chain need not wait for the charge matrix queued above
       if (packed) {
         if (*rebany_h) {
           build_neighbors_packed(false);   // the flagged systems only; invalidates the matrix
           launch_pk_ref(st, md, d_reb.p, stream);
+          fork_after = nullptr;
         }
       } else {
         disp = *disp_h;
         if (disp > 0.5 * skin) {
           build_neighbors();   // invalidates the matrix
           launch_md_ref_positions(st, md, stream);
+          fork_after = nullptr;
         }
       }
       // bonded chain on the bonded stream while the main stream solves for the charges, then the non-bonded pass and the join;

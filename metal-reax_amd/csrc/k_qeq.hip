@@ -166,6 +166,11 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
   } else {
     const int* pa = qb.iscal + 2 * (it - 2);
     const int pact[2] = {pa[0], pa[1]};
+    if (!pact[0] && !pact[1]) {   // both stopped before: nothing to reduce, the slots of it - 1 carry the last values on
+      if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int s = 0; s < 2; ++s) { SIG(qb.scal, it - 1, s) = SIG(qb.scal, it - 2, s); qb.iscal[2 * (it - 1) + s] = 0; }
+      return;
+    }
     double sn[2];
     reduce_partials<2, kMvThreads>(qb.partial + 2 * kCgBlocks, sn, sh, kUpdBlocks);
     for (int s = 0; s < 2; ++s) {
