@@ -269,4 +269,39 @@ __device__ inline int lane_id() { return threadIdx.x & 63; }
 // number of set bits of `mask` below this lane
 __device__ inline int lanes_below(unsigned long long mask) { return __popcll(mask & ((1ull << lane_id()) - 1ull)); }
 
+// slot p of atom i in the slot-major per-bond arrays
+__device__ inline size_t sl(int p, int i, int n) { return (size_t)p * (size_t)n + (size_t)i; }
+
+// The last step of a force call for atom i (the bonded chain and the non-bonded pass complete): the gradient from the atom's own
+// sums, the forces its bond partners left in the mirror slots and the BO-derivative terms of its bonds, plus the non-bonded
+// gradient; the per-atom virial gets the bonded part. Writes st.gx/gy/gz[i] and returns them.
+__device__ inline void gather_atom(const State& st, int i, double& gx_out, double& gy_out, double& gz_out) {
+  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
+  const double Si = st.S[i];
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  double gx = st.fs_x[i] + st.Dx[i] + Si * st.dDps_x[i];
+  double gy = st.fs_y[i] + st.Dy[i] + Si * st.dDps_y[i];
+  double gz = st.fs_z[i] + st.Dz[i] + Si * st.dDps_z[i];
+  for (int p = 0; p < nb; ++p) {
+    const size_t q = sl(p, i, n);
+    const int j = st.b_j[q];
+    const size_t m = sl(st.b_sym[q], j, n);
+    gx += st.fpx[m]; gy += st.fpy[m]; gz += st.fpz[m];
+    const double c = st.S[j] * st.dbop_c[q];
+    gx += c * st.b_dx[q]; gy += c * st.b_dy[q]; gz += c * st.b_dz[q];
+    vir_add(w, 0.5 * (c + Si * st.dbop_c[q]), st.b_dx[q], st.b_dy[q], st.b_dz[q], st.b_dx[q], st.b_dy[q], st.b_dz[q]);
+  }
+  gx_out = gx + st.gnb_x[i];
+  gy_out = gy + st.gnb_y[i];
+  gz_out = gz + st.gnb_z[i];
+  st.gx[i] = gx_out;
+  st.gy[i] = gy_out;
+  st.gz[i] = gz_out;
+  for (int c = 0; c < 6; ++c) st.vatom[(size_t)c * n + i] += st.vb[(size_t)c * n + i] + w[c];
+}
+__device__ inline void gather_atom(const State& st, int i) {
+  double gx, gy, gz;
+  gather_atom(st, i, gx, gy, gz);
+}
+
 }  // namespace rxa

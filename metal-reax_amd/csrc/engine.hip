@@ -1311,6 +1311,7 @@ struct Engine {
         launch_pk_disp(st, md, d_sys_start.p, S, 0.5 * skin, d_reb.p, d_rebany.p, d_disp_pk.p, stream);
         HIPCHK(hipMemcpyAsync(rebany_h, d_rebany.p, sizeof(int), hipMemcpyDeviceToHost, stream));
       } else {
+        launch_md_disp_final(md, stream);
         HIPCHK(hipMemcpyAsync(disp_h, md.disp, sizeof(double), hipMemcpyDeviceToHost, stream));
       }
       timer.end(PH_MD, t0);
@@ -1342,16 +1343,21 @@ struct Engine {
       if (bonded_pending) bonded_end();
       if (do_qeq) qeq_end(md_qeq_imax, 0);
       nonbonded_launch();
-      join_gather(s == steps - 1);
-      t0 = timer.begin(PH_MD);
       if (packed) {
+        join_gather(s == steps - 1);
+        t0 = timer.begin(PH_MD);
         launch_md_half_kick(st, md, dtf, stream);
         launch_pk_thermostat(st, md, d_sys_start.p, S, d_nh_pk.p, stream);
-      } else {   // NVE: kinetic energy of the last step only (factor 1); NVT: the velocity factor is applied by the next kernel
-        launch_md_step_end(st, md, dtf, !md_nve || s == steps - 1, stream);
+        timer.end(PH_MD, t0);
+      } else {   // the gather fused with the half kick; NVE: thermostat pass (kinetic energy, factor 1) after the last step only;
+                 // NVT: the velocity factor is applied by the next kernel
+        if (bstream() != stream) HIPCHK(hipStreamWaitEvent(stream, ev_bonded, 0));
+        t0 = timer.begin(PH_DBOND);
+        launch_md_gather_step_end(st, md, dtf, !md_nve || s == steps - 1, stream);
+        if (s == steps - 1) launch_energy_reduce(st, red.p, stream);
+        timer.end(PH_DBOND, t0);
         v_scale_pending = !md_nve;
       }
-      timer.end(PH_MD, t0);
     }
     if (v_scale_pending) {   // the last thermostat factor, before anything reads the velocities
       launch_md_scale(st, md, stream);

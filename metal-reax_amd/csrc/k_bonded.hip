@@ -68,8 +68,6 @@ template <class T> __device__ inline T ratio2_deriv(T a, T b, T ka, T kb, T inv)
   return ((-ka * a - T(2) * kb * b + (ka - kb) * (a * b)) * inv) * inv;
 }
 
-__device__ inline size_t sl(int p, int i, int n) { return (size_t)p * (size_t)n + (size_t)i; }
-
 template <class T> struct V3 { T x, y, z; };
 template <class T> __device__ inline T dot3(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <class T> __device__ inline V3<T> cross3(const V3<T>& a, const V3<T>& b) {
@@ -1049,26 +1047,7 @@ __global__ void __launch_bounds__(128) k_dbond(State st) {
 // final gradient: bonded explicit-geometry terms (own + partner slots), BO-derivative pair forces, non-bonded
 __global__ void __launch_bounds__(128) k_gather(State st) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= st.n) return;
-  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
-  const double Si = st.S[i];
-  double w[6] = {0, 0, 0, 0, 0, 0};
-  double gx = st.fs_x[i] + st.Dx[i] + Si * st.dDps_x[i];
-  double gy = st.fs_y[i] + st.Dy[i] + Si * st.dDps_y[i];
-  double gz = st.fs_z[i] + st.Dz[i] + Si * st.dDps_z[i];
-  for (int p = 0; p < nb; ++p) {
-    const size_t q = sl(p, i, n);
-    const int j = st.b_j[q];
-    const size_t m = sl(st.b_sym[q], j, n);
-    gx += st.fpx[m]; gy += st.fpy[m]; gz += st.fpz[m];
-    const double c = st.S[j] * st.dbop_c[q];
-    gx += c * st.b_dx[q]; gy += c * st.b_dy[q]; gz += c * st.b_dz[q];
-    vir_add(w, 0.5 * (c + Si * st.dbop_c[q]), st.b_dx[q], st.b_dy[q], st.b_dz[q], st.b_dx[q], st.b_dy[q], st.b_dz[q]);
-  }
-  st.gx[i] = gx + st.gnb_x[i];
-  st.gy[i] = gy + st.gnb_y[i];
-  st.gz[i] = gz + st.gnb_z[i];
-  for (int c = 0; c < 6; ++c) st.vatom[(size_t)c * n + i] += st.vb[(size_t)c * n + i] + w[c];
+  if (i < st.n) gather_atom(st, i);
 }
 
 inline int nblk(long long items, int per) { return (int)((items + per - 1) / per); }

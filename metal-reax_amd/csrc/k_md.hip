@@ -242,6 +242,22 @@ __global__ void __launch_bounds__(kRT) k_kick_ke(State st, MdBuffers md, double 
   if (threadIdx.x == 0) md.partial[blockIdx.x] = v;
 }
 
+// the gather (gather_atom) and k_kick_ke in one pass: the same arithmetic and partial-sum layout as the two kernels in sequence
+__global__ void __launch_bounds__(kRT) k_gather_kick_ke(State st, MdBuffers md, double dtf) {
+  __shared__ double sh[kRT];
+  double v = 0;
+  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT) {
+    double gx, gy, gz;
+    gather_atom(st, i, gx, gy, gz);
+    const double f = dtf / md.mass[i];
+    const double vx = md.vx[i] - f * gx, vy = md.vy[i] - f * gy, vz = md.vz[i] - f * gz;
+    md.vx[i] = vx; md.vy[i] = vy; md.vz[i] = vz;
+    v += md.mass[i] * (vx * vx + vy * vy + vz * vz);
+  }
+  block_sum(&v, 1, sh);
+  if (threadIdx.x == 0) md.partial[blockIdx.x] = v;
+}
+
 inline int nblk(int n, int per) { return (n + per - 1) / per; }
 
 }  // namespace
@@ -255,6 +271,15 @@ void launch_md_step_start(const State& st, MdBuffers& md, double dtf, double dt,
     hipLaunchKernelGGL(k_scale_v, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md);
   }
   hipLaunchKernelGGL(k_kick_drift, dim3(kRBlocks), dim3(kRT), 0, s, st, md, dtf, dt, thermostat ? 1 : 0);
+}
+// step end fused with the force gather: gather + velocity half step + kinetic-energy partials, then (if `thermostat`) the
+// thermostat half step, whose velocity factor stays pending
+void launch_md_gather_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_kick_ke, dim3(kRBlocks), dim3(kRT), 0, s, st, md, dtf);
+  if (thermostat) hipLaunchKernelGGL(k_nhc, dim3(1), dim3(kRT), 0, s, md);
+}
+// the largest displacement from the kick-drift partials (launch_md_step_start), into md.disp
+void launch_md_disp_final(MdBuffers& md, hipStream_t s) {
   hipLaunchKernelGGL(k_disp_final, dim3(1), dim3(kRT), 0, s, md);
 }
 // step end: velocity half step, then (if `thermostat`) the thermostat half step, whose velocity factor stays pending
