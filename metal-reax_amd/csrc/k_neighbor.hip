@@ -97,22 +97,27 @@ __global__ void __launch_bounds__(256) k_nbr_cells(State st, Cell c, int nc0, in
         if (q2 < 0) { q2 += nc2; s2 = -1; } else if (q2 >= nc2) { q2 -= nc2; s2 = 1; }
         const int cell = (q2 * nc1 + q1) * nc0 + q0;
         const int sc = shift_code(s0, s1, s2);
+        // the image's shift vector is the same for the whole cell; the list radius lies a skin beyond every consumer's
+        // cut-off, so the test takes the image from this side (no canonical orientation) and only kept entries need the
+        // tags that decide their canonical end
+        const double4 S = st.shv[pos_sys(pi.w) * st.shv_stride + shift_index(st, sc)];
         const int pe = cend[cell];
         for (int p0 = cstart[cell]; p0 < pe; p0 += 64) {
           const int p = p0 + lane;
           bool keep = false;
-          int j = 0, e = 0;
+          int j = 0;
           if (p < pe) {
             j = sidx[p];
             if (!(j == i && sc == 0)) {
-              e = canonical_first(i, j, sc, st.tag) ? (sc | kCanon) : sc;
-              double ddx, ddy, ddz;
-              keep = pair_dvec_list(st, pi, j, e, ddx, ddy, ddz) <= rs2;
+              const double4 pj = st.pos[j];
+              const double ex = (pj.x + S.x) - pi.x, ey = (pj.y + S.y) - pi.y, ez = (pj.z + S.z) - pi.z;
+              keep = ex * ex + ey * ey + ez * ez <= rs2;
             }
           }
           const unsigned long long m = __ballot(keep);
           if (keep) {
             const int slot = count + lanes_below(m);
+            const int e = canonical_first(i, j, sc, st.tag) ? (sc | kCanon) : sc;
             if (slot < st.maxnb) { st.nb_j[base + slot] = j; st.nb_s[base + slot] = e; }
           }
           count += __popcll(m);
@@ -181,8 +186,11 @@ __global__ void __launch_bounds__(256) k_sublists(State st, Globals g, Tables t,
         e = st.nb_s[base + k];
         const int tj = st.type[j];
         if (tj >= 0) {
-          double dx, dy, dz;
-          const double d2 = pair_dvec_list(st, pi, j, e, dx, dy, dz);
+          // list radii with a skin beyond the consumers' cut-offs: the image from this side, no canonical orientation
+          const double4 pj = st.pos[j];
+          const double4 S = st.shv[pos_sys(pi.w) * st.shv_stride + shift_index(st, entry_code(e))];
+          const double ex = (pj.x + S.x) - pi.x, ey = (pj.y + S.y) - pi.y, ez = (pj.z + S.z) - pi.z;
+          const double d2 = ex * ex + ey * ey + ez * ez;
           bk = d2 <= rb2;
           if (role == 1 && d2 <= rh2) hk = t.sp[tj].p_hbond == 2;   // donors list acceptors; acceptor rows come from the transpose
         }
@@ -226,17 +234,28 @@ __global__ void __launch_bounds__(256) k_hc_transpose(State st, Globals g, Table
   }
 }
 
+// (kLds: the row's keys staged in the wave's share of dynamic LDS, maxhc ints per wave, so the rank loop reads LDS broadcasts;
+// rows too long for LDS read the keys from global memory)
+template <bool kLds>
 __global__ void __launch_bounds__(256) k_hc_sort(State st, Globals g, Tables t) {
+  extern __shared__ int hc_keys[];
   const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (k >= st.n) return;
   const int tk = st.type[k];
   if (tk < 0 || t.sp[tk].p_hbond != 2) return;
   const int m = min(st.hc_cnt[k], st.maxhc);
   const size_t kb = (size_t)k * st.maxhc;
+  const int* keys = st.hc_m + kb;
+  if (kLds) {
+    int* lk = hc_keys + (threadIdx.x >> 6) * st.maxhc;
+    for (int h = lane_id(); h < m; h += 64) lk[h] = st.hc_m[kb + h];
+    __builtin_amdgcn_wave_barrier();
+    keys = lk;
+  }
   for (int h = lane_id(); h < m; h += 64) {
-    const int key = st.hc_m[kb + h];
+    const int key = keys[h];
     int rank = 0;
-    for (int q = 0; q < m; ++q) rank += st.hc_m[kb + q] < key ? 1 : 0;
+    for (int q = 0; q < m; ++q) rank += keys[q] < key ? 1 : 0;
     st.hc_j[kb + rank] = key;
   }
 }
@@ -317,7 +336,9 @@ void launch_sublists(const State& st, const Globals& g, const Tables& t, double 
   hipLaunchKernelGGL(k_sublists, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t, rb * rb * m, rh * rh * m);
   if (rh > 0) {
     hipLaunchKernelGGL(k_hc_transpose, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t);
-    hipLaunchKernelGGL(k_hc_sort, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t);
+    const size_t lds = 4 * (size_t)st.maxhc * sizeof(int);   // four waves per block
+    if (lds <= 64 * 1024) hipLaunchKernelGGL(k_hc_sort<true>, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), lds, s, st, g, t);
+    else hipLaunchKernelGGL(k_hc_sort<false>, dim3(blocks_for((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t);
   }
 }
 
