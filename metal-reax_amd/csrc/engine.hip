@@ -207,8 +207,8 @@ struct Engine {
   bool overlap = true;
   hipStream_t sb = nullptr;
   hipEvent_t ev_fork = nullptr, ev_bonded = nullptr, ev_cg = nullptr;
-  int* cgf_h = nullptr;      // pinned: CG activity flags read back per batch
-  double* cgs_h = nullptr;   // pinned: CG scalars read back per batch
+  int* cgf_h = nullptr;      // pinned: packed systems' any-active flags read back per batch
+  double* cgs_h = nullptr;   // pinned: CG scalars and activity flags (the device layout of QeqBuffers::scal) read back per batch
   size_t cg_cap_h = 0;
 
   Engine(int dev) : device(dev) {
@@ -976,7 +976,7 @@ struct Engine {
     auto rnd = [](size_t k) { return (k + 31) & ~(size_t)31; };
     const size_t nmv = 4 * (size_t)qeq_mv_partials(n);
     const size_t sz = rnd((size_t)n * maxnb) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + rnd(nmv) +
-                      rnd(8 + 2 * (size_t)(imax_cap + 1)) + 2 * rnd(4 * (size_t)n);
+                      rnd(8 + 3 * (size_t)(imax_cap + 2)) + 2 * rnd(4 * (size_t)n);
     std::vector<double> keep;
     const bool had_hist = q_dbl.p && hist_count > 0 && q_dbl_n == n;   // history of the same atoms only
     if (had_hist) {
@@ -1001,13 +1001,11 @@ struct Engine {
     qb.q = take(2 * (size_t)n);
     qb.partial = take(4 * kCgBlocks);
     qb.partial_mv = take(nmv);
-    qb.scal = take(8 + 2 * (size_t)(imax_cap + 1));
+    qb.scal = take(8 + 3 * (size_t)(imax_cap + 2));
     qb.s_hist = take(4 * (size_t)n);
     qb.t_hist = take(4 * (size_t)n);
     qeq_imax_cap = imax_cap;
-    static_assert(sizeof(int) <= sizeof(double), "");
-    iscal_buf.alloc(2 * (size_t)(imax_cap + 2));
-    qb.iscal = iscal_buf.p;
+    static_assert(2 * sizeof(int) <= sizeof(double), "");
     if (had_hist) {
       HIPCHK(hipMemcpy(qb.s_hist, keep.data(), 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(qb.t_hist, keep.data() + 4 * (size_t)n, 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
@@ -1015,7 +1013,6 @@ struct Engine {
       hist_count = 0;
     }
   }
-  DBuf<int> iscal_buf;
   void alloc_qeq() { alloc_qeq_matrix(); }
 
   // A new atom set starts the stock extrapolation from a zero history: the first solve starts from zero and the later ones
@@ -1081,12 +1078,14 @@ struct Engine {
       if (cgf_h) HIPCHK(hipHostFree(cgf_h));
       if (cgs_h) HIPCHK(hipHostFree(cgs_h));
       HIPCHK(hipHostMalloc((void**)&cgf_h, need * sizeof(int), hipHostMallocDefault));
-      HIPCHK(hipHostMalloc((void**)&cgs_h, (8 + need) * sizeof(double), hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void**)&cgs_h, (8 + 3 * need) * sizeof(double), hipHostMallocDefault));
       cg_cap_h = need;
     }
   }
   int cg_guess = 8;
   int cg_it = 0, cg_batch = 0;
+  // activity flags of iteration index k in the read-back scalars (the layout of QeqBuffers::scal)
+  const int* cg_act(int k) const { return reinterpret_cast<const int*>(cgs_h + 8 + 3 * (size_t)k + 2); }
   // launches the next batch of iterations, then the asynchronous readback of the flags and scalars (pinned) and its event
   void cg_launch_batch(int imax) {
     if (packed) {   // iteration it sets any[it]: some system active for the next one
@@ -1111,8 +1110,7 @@ struct Engine {
       timer.end(PH_QEQ_SPMV, t0);
       bonded_poll();
     }
-    HIPCHK(hipMemcpyAsync(cgf_h, qb.iscal, 2 * (size_t)cg_it * sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(cgs_h, qb.scal, (8 + 2 * (size_t)cg_it) * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(cgs_h, qb.scal, (8 + 3 * (size_t)cg_it) * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(ev_cg, stream));
   }
   double cg_tol = 1e-6;
@@ -1150,16 +1148,16 @@ struct Engine {
   void cg_wait_end(int imax) {
     while (true) {
       HIPCHK(hipEventSynchronize(ev_cg));
-      const int* a = cgf_h + 2 * (cg_it - 1);
+      const int* a = cg_act(cg_it - 1);
       if ((!a[0] && !a[1]) || cg_it >= imax) break;
       cg_batch = 2;
       cg_launch_batch(imax);
     }
     for (int s = 0; s < 2; ++s) {
       int k = 0;
-      while (k < cg_it && cgf_h[2 * k + s]) ++k;
+      while (k < cg_it && cg_act(k)[s]) ++k;
       last_iters[s] = k + 1;   // the stock loop index at exit
-      last_rel[s] = std::sqrt(cgs_h[8 + 2 * (last_iters[s] - 1) + s]) / cgs_h[s];
+      last_rel[s] = std::sqrt(cgs_h[8 + 3 * (last_iters[s] - 1) + s]) / cgs_h[s];
     }
     cg_guess = std::max(last_iters[0], last_iters[1]) - 1;
   }

@@ -19,8 +19,10 @@ constexpr int kUpdBlocks = 240, kUpdThreads = 256;   // CG update kernel
 constexpr int kMvThreads = 256;             // product kernels: 4 waves per block,
 constexpr int kMvRowsPerBlock = kMvThreads / 16;   // 16 lanes per row (kRowLanes below)
 inline __host__ __device__ int mv_blocks(int n) { return (n + kMvRowsPerBlock - 1) / kMvRowsPerBlock; }
-// scal layout: [0,1] |b| of s and t, [2,3] u terms, then sig[it][sys] at 8 + 2 it + sys
-__device__ inline double& SIG(double* scal, int it, int sys) { return scal[8 + 2 * it + sys]; }
+// scal layout: [0,1] |b| of s and t, [2,3] u terms, then three doubles per iteration index k: sig of the two systems at 8 + 3 k
+// + sys, and the two activity flags (ints) in 8 + 3 k + 2, so one copy of the first 8 + 3 k doubles brings both to the host
+__device__ inline double& SIG(double* scal, int k, int sys) { return scal[8 + 3 * k + sys]; }
+__device__ inline int* ACT(double* scal, int k) { return reinterpret_cast<int*>(scal + 8 + 3 * k + 2); }
 
 template <int K, int NT = kThreads>
 __device__ inline void block_reduce(double (&v)[K], double* sh) {
@@ -162,13 +164,13 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
   int act[2];
   double beta[2] = {0.0, 0.0};
   if (it == 1) {
-    act[0] = qb.iscal[0]; act[1] = qb.iscal[1];
+    act[0] = ACT(qb.scal, 0)[0]; act[1] = ACT(qb.scal, 0)[1];
   } else {
-    const int* pa = qb.iscal + 2 * (it - 2);
+    const int* pa = ACT(qb.scal, it - 2);
     const int pact[2] = {pa[0], pa[1]};
     if (!pact[0] && !pact[1]) {   // both stopped before: nothing to reduce, the slots of it - 1 carry the last values on
       if (blockIdx.x == 0 && threadIdx.x == 0)
-        for (int s = 0; s < 2; ++s) { SIG(qb.scal, it - 1, s) = SIG(qb.scal, it - 2, s); qb.iscal[2 * (it - 1) + s] = 0; }
+        for (int s = 0; s < 2; ++s) { SIG(qb.scal, it - 1, s) = SIG(qb.scal, it - 2, s); ACT(qb.scal, it - 1)[s] = 0; }
       return;
     }
     double sn[2];
@@ -180,7 +182,7 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
     if (blockIdx.x == 0 && threadIdx.x == 0)
       for (int s = 0; s < 2; ++s) {
         SIG(qb.scal, it - 1, s) = pact[s] ? sn[s] : SIG(qb.scal, it - 2, s);
-        qb.iscal[2 * (it - 1) + s] = act[s];
+        ACT(qb.scal, it - 1)[s] = act[s];
       }
   }
   if (!act[0] && !act[1]) return;
@@ -230,7 +232,7 @@ __global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, int npa
       const double bn = sqrt(v[s]);
       qb.scal[s] = bn;
       SIG(qb.scal, 0, s) = v[2 + s];
-      qb.iscal[s] = (1 < imax && sqrt(v[2 + s]) / bn > tol) ? 1 : 0;
+      ACT(qb.scal, 0)[s] = (1 < imax && sqrt(v[2 + s]) / bn > tol) ? 1 : 0;
     }
   }
 }
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, int npa
 // re-reduces all partials of the product
 __global__ void __launch_bounds__(kUpdThreads) k_cg_update(int n, QeqBuffers qb, int it) {
   __shared__ double sh[2 * kUpdThreads / 64];
-  const int* act = qb.iscal + 2 * (it - 1);
+  const int* act = ACT(qb.scal, it - 1);
   const int a[2] = {act[0], act[1]};
   if (!a[0] && !a[1]) return;
   double dq[2];

@@ -75,8 +75,7 @@ struct QeqBuffers {
   double* partial;           // 4 * kCgBlocks: per-block partials of the element-wise CG kernels
   double* partial_mv;        // 2 * qeq_mv_partials(n): per-block partials of the matrix-vector product
   int fp32 = 0;              // the matrix values are val32 (mixed and FP32 modes) instead of val
-  double* scal;              // device scalars (see k_qeq.hip)
-  int* iscal;                // device integer scalars
+  double* scal;              // device scalars and activity flags per iteration (see k_qeq.hip)
   double *s_hist, *t_hist;   // 4 * n each, [k * n + i], k = 0 most recent
 };
 constexpr int kCgBlocks = 256;
