@@ -75,7 +75,19 @@ int rxa_get_charges(rxa_context* ctx, double* q);         /* n */
  * energies[RXA_NTERMS] (may be NULL), grad[3n] = dE/dx of the owned atoms (the adapter SUBTRACTS it from atom->f, as the reference
  * does at pair_reaxff_metal.cpp:501-504; may be NULL), virial[6] = xx yy zz xy xz yz of all terms (may be NULL). */
 #define RXA_COMPUTE_REBUILD 1   /* force a neighbour-list rebuild */
+#define RXA_COMPUTE_PERATOM 2   /* also tally per-atom energies and virials (read them with rxa_peratom) */
 int rxa_compute(rxa_context* ctx, int flags, double* energies, double* grad, double* virial);
+/* per-atom energy eatom[n] and virial vatom[6n] (xx yy zz xy xz yz per atom) of the last rxa_compute made with
+ * RXA_COMPUTE_PERATOM, owned atoms in the caller's order, every image folded onto its owner (either pointer may be NULL).
+ * Replaces the per-atom tallies of PairReaxFFMetal::compute with eflag_atom / vflag_atom (pair_reaxff_metal.cpp:412-524), which
+ * the reference takes from its CPU engine (BondedOptions::per_atom / NonbondedOptions::per_atom, include/reaxmetal/bonded.hpp:28,
+ * nonbonded.hpp:22). Energies follow its attribution (bonded.cpp:212-704, nonbonded.cpp:55-76): bond, triple-bond and C2 terms
+ * half to each atom of the bond, lone pair and over/under to the atom, valence to the centre, torsion half to each atom of the
+ * central bond, H-bond a third to each atom of the triple, van der Waals and Coulomb half to each atom of the pair,
+ * polarisation to the atom. The virial is split as the engine forms it: non-bonded pair terms half to each atom (as the
+ * reference), bonded terms to the atoms whose kernels form them, not the reference's even split over a term's atoms; the
+ * per-atom virials add up to the virial of rxa_compute. */
+int rxa_peratom(rxa_context* ctx, double* eatom, double* vatom);
 /* the 13 terms -> LAMMPS' 14-slot pvector (eb ea elp emol ev epen ecoa ehb et eco ew ep efi eqeq), energy_terms.cpp:48-63 */
 void rxa_pvector(const double* energies, double* pvector14);
 /* per owned atom decision census, 7 * n ints: bonds, angle_sets(-1: not collected), torsions(-1), hbonds(-1), nonbonded, sbo_region(-1),

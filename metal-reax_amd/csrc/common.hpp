@@ -126,6 +126,13 @@ struct State {
   double* eatom;                  // kTerms * n per-atom energies
   double* vatom;                  // 6 * n per-atom virial (xx yy zz xy xz yz): non-bonded half pairs + vb, complete after k_gather
   double* vb;                     // 6 * n bonded part of the per-atom virial, accumulated by the bonded kernels
+  // per-atom energies in the reference's attribution (filled only when peratom != 0; bonded.cpp:212-704 ev_tally convention):
+  int peratom;
+  double* eslot;                  // maxb * n: energy share of the slot's partner (bond, C2, torsion halves; H-bond third of i)
+  double* eown;                   // n: the atom's own share of the bond, lone-pair/C2, over/under, torsion and H-bond terms
+  double* ehc;                    // maxhc * n: H-bond energy third of the acceptor in each donor candidate slot
+  double* eacc;                   // n: an acceptor's gathered H-bond thirds
+  double* eref;                   // n: the per-atom energy (k_eatom_ref)
   double *gnb_x, *gnb_y, *gnb_z;  // non-bonded gradient
 };
 

@@ -156,6 +156,7 @@ struct Engine {
   DBuf<double> a_bdbl;   // bond slot doubles
   DBuf<int> a_bc, a_hc;  // bond / H-bond candidate sub-lists (j, entry; H-bond: + mirror position)
   DBuf<double> a_hf;     // per H-bond candidate slot: force on the acceptor
+  DBuf<double> a_ehc;    // per H-bond candidate slot: the acceptor's energy third (per-atom calls)
   BinGrid grid;
   BinBuffers bins;
   DBuf<unsigned> b_keys, b_skeys, b_akey;
@@ -393,7 +394,7 @@ struct Engine {
     a_x.alloc(n); a_y.alloc(n); a_z.alloc(n); a_q.alloc(n); a_type.alloc(n); a_tag.alloc(n); a_pos.alloc(n);
     d_perm.alloc(n); d_src.alloc(n); d_psd.alloc(24 * (size_t)n); d_psi.alloc(4 * (size_t)n); d_psl.alloc(2 * (size_t)n);
     HIPCHK(hipMemset(a_q.p, 0, n * sizeof(double)));
-    const int NA = 38 + kTerms + 6;   // number of per-atom double arrays below
+    const int NA = 38 + kTerms + 9;   // number of per-atom double arrays below
     a_atom.alloc((size_t)NA * n);
     HIPCHK(hipMemset(a_atom.p, 0, (size_t)NA * n * sizeof(double)));
     a_iatom.alloc(6 * (size_t)n + 8);
@@ -412,6 +413,7 @@ struct Engine {
     k = 38;
     s.eatom = atom_arr(k);
     s.vatom = atom_arr(k + kTerms);
+    s.eown = atom_arr(k + kTerms + 6); s.eacc = atom_arr(k + kTerms + 7); s.eref = atom_arr(k + kTerms + 8);
     s.nb_cnt = a_iatom.p; s.b_cnt = a_iatom.p + n; s.h_cnt = a_iatom.p + 2 * (size_t)n; s.bc_cnt = a_iatom.p + 3 * (size_t)n;
     s.hc_cnt = a_iatom.p + 4 * (size_t)n; s.sysid = a_iatom.p + 5 * (size_t)n; s.flags = a_iatom.p + 6 * (size_t)n;
     red.alloc(19 + 240 * 19 + 8);
@@ -438,17 +440,19 @@ struct Engine {
     a_bc.alloc(2 * (size_t)n * maxbc);
     a_hc.alloc(3 * (size_t)n * maxhc);
     a_hf.alloc(3 * (size_t)n * maxhc);
+    a_ehc.alloc((size_t)n * maxhc);
     st.maxbc = maxbc; st.maxhc = maxhc;
     st.bc_j = a_bc.p; st.bc_s = a_bc.p + (size_t)n * maxbc;
     st.hc_j = a_hc.p; st.hc_s = a_hc.p + (size_t)n * maxhc; st.hc_m = a_hc.p + 2 * (size_t)n * maxhc;
     st.hf = a_hf.p;
+    st.ehc = a_ehc.p;
   }
 
   void alloc_bonds(int cap_b) {
     maxb = cap_b;
     const size_t nb = (size_t)n * maxb;
     a_bint.alloc(3 * nb);
-    a_bdbl.alloc(29 * nb);
+    a_bdbl.alloc(30 * nb);
     st.maxb = maxb;
     st.b_j = a_bint.p; st.b_s = a_bint.p + nb; st.b_sym = a_bint.p + 2 * nb;
     double* b = a_bdbl.p;
@@ -461,7 +465,8 @@ struct Engine {
     for (int c = 8; c < 11; ++c) st.C[c] = nullptr;
     st.Cdbo = nxt(); st.Cdbopi = nxt(); st.Cdbopi2 = nxt(); st.fpx = nxt(); st.fpy = nxt(); st.fpz = nxt();
     st.cdp = nxt();
-    if (k != 29) throw ApiError(RXA_EINVAL, "internal: bond slot layout");
+    st.eslot = nxt();
+    if (k != 30) throw ApiError(RXA_EINVAL, "internal: bond slot layout");
   }
 
   // expected entries within r of an atom at the mean density (at least that of a condensed phase for open cells), with margin
@@ -923,6 +928,7 @@ struct Engine {
     if (bstream() != stream) HIPCHK(hipStreamWaitEvent(stream, ev_bonded, 0));
     hipEvent_t t0 = timer.begin(PH_DBOND);
     launch_gather(st, stream);
+    if (st.peratom) launch_eatom_ref(st, stream);
     if (energies) launch_energy_reduce(st, red.p, stream);   // the per-atom arrays hold the energies either way
     timer.end(PH_DBOND, t0);
   }
@@ -934,9 +940,19 @@ struct Engine {
     join_gather(energies);
   }
 
+  bool peratom_valid = false;   // the last force call of rxa_compute left per-atom tallies
   void compute(int flags, double* energies, double* grad, double* virial) {
     if (!n) throw ApiError(RXA_ESTATE, "no atoms");
-    forces((flags & RXA_COMPUTE_REBUILD) != 0);
+    st.peratom = (flags & RXA_COMPUTE_PERATOM) ? 1 : 0;
+    peratom_valid = false;
+    try {
+      forces((flags & RXA_COMPUTE_REBUILD) != 0);
+    } catch (...) {
+      st.peratom = 0;
+      throw;
+    }
+    peratom_valid = st.peratom != 0;
+    st.peratom = 0;
     double out[kTerms + 6];
     HIPCHK(hipMemcpyAsync(out, red.p, sizeof(out), hipMemcpyDeviceToHost, stream));
     std::vector<double> gx, gy, gz;
@@ -954,6 +970,21 @@ struct Engine {
         const int u = perm_h[i];
         grad[3 * u] = gx[i]; grad[3 * u + 1] = gy[i]; grad[3 * u + 2] = gz[i];
       }
+  }
+
+  // per-atom energy and virial of the last rxa_compute call made with RXA_COMPUTE_PERATOM, in the caller's order
+  void peratom_out(double* eatom, double* vatom) {
+    if (!peratom_valid) throw ApiError(RXA_ESTATE, "no per-atom tallies: call rxa_compute with RXA_COMPUTE_PERATOM first");
+    std::vector<double> e(n), v(6 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(e.data(), st.eref, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(v.data(), st.vatom, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int i = 0; i < n; ++i) {
+      const int u = perm_h[i];
+      if (eatom) eatom[u] = e[i];
+      if (vatom)
+        for (int c = 0; c < 6; ++c) vatom[6 * (size_t)u + c] = v[(size_t)c * n + i];
+    }
   }
 
   // ---------------------------------------------------------------------------------------------------------------
@@ -1611,6 +1642,10 @@ int rxa_get_charges(rxa_context* c, double* q) {
 
 int rxa_compute(rxa_context* c, int flags, double* energies, double* grad, double* virial) {
   return guard(c, [&](Engine& e) { e.compute(flags, energies, grad, virial); });
+}
+
+int rxa_peratom(rxa_context* c, double* eatom, double* vatom) {
+  return guard(c, [&](Engine& e) { e.peratom_out(eatom, vatom); });
 }
 
 void rxa_pvector(const double* e, double* pv) {

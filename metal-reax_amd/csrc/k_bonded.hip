@@ -345,10 +345,12 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= st.n) return;
   const int n = st.n, ti = st.type[i];
-  double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, cdself = 0;
+  double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, cdself = 0, e_c2 = 0;
   if (ti >= 0) {
     const DevSpecies& si = t.sp[ti];
     const int nb = min(st.b_cnt[i], st.maxb);
+    if (st.peratom)
+      for (int p = 0; p < nb; ++p) st.eslot[sl(p, i, n)] = 0.0;
     // bond energy of the bonds whose canonical end is this atom (the reference's tag order, bonded.cpp:222-252)
     for (int p = 0; p < nb; ++p) {
       const size_t q = sl(p, i, n);
@@ -360,7 +362,9 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
       const TR pw = BOs == TR(0) ? TR(0) : rpow<TR>(BOs, (TR)P.p_be2);
       const TR ex = rexp<TR>((TR)P.p_be1 * (TR(1) - pw));
       const TR CEbo = -(TR)P.De_s * ex * (TR(1) - (TR)P.p_be1 * (TR)P.p_be2 * pw);
-      e_bond += (double)(-(TR)P.De_s * BOs * ex - (TR)P.De_p * BOpi - (TR)P.De_pp * BOpi2);
+      const double e_b = (double)(-(TR)P.De_s * BOs * ex - (TR)P.De_p * BOpi - (TR)P.De_pp * BOpi2);
+      e_bond += e_b;
+      if (st.peratom) st.eslot[q] += 0.5 * e_b;
       double cdbo = (double)CEbo;
       st.Cdbopi[q] += -(double)(CEbo + (TR)P.De_p);
       st.Cdbopi2[q] += -(double)(CEbo + (TR)P.De_pp);
@@ -371,7 +375,9 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
         const TR ea = rexp<TR>(-gp3 * ((TR)st.tbo[i] - BO)), eb = rexp<TR>(-gp3 * ((TR)st.tbo[j] - BO));
         const TR eov = safe_exp<TR>(gp4 * ((TR)st.Delta[i] + (TR)st.Delta[j]));
         const TR hov = TR(1) / (TR(1) + TR(25.0) * eov);
-        e_bond += (double)(gp10 * exphu * hov * (ea + eb));
+        const double e_t = (double)(gp10 * exphu * hov * (ea + eb));
+        e_bond += e_t;
+        if (st.peratom) st.eslot[q] += 0.5 * e_t;
         cdbo += (double)(gp10 * exphu * hov * (ea + eb) * (gp3 - TR(2.0) * gp7 * (BO - TR(2.50))));
         cdself += (double)(-gp10 * exphu * hov * (gp3 * ea + TR(25.0) * gp4 * eov * hov * (ea + eb)));
         st.cdp[q] += (double)(-gp10 * exphu * hov * (gp3 * eb + TR(25.0) * gp4 * eov * hov * (ea + eb)));
@@ -393,7 +399,10 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
         if (!t.sp[st.type[st.b_j[q]]].c2) continue;
         const TR vov3 = (TR)st.bo[q] - Di - TR(0.040) * rpow<TR>(Di, TR(4.));
         if (vov3 > TR(3.)) {
-          e_lp += (double)(p_lp3 * ((vov3 - TR(3.0)) * (vov3 - TR(3.0))));
+          const double e_c = (double)(p_lp3 * ((vov3 - TR(3.0)) * (vov3 - TR(3.0))));
+          e_lp += e_c;
+          e_c2 += e_c;
+          if (st.peratom) st.eslot[q] += 0.5 * e_c;
           st.Cdbo[q] += (double)(TR(2.) * p_lp3 * (vov3 - TR(3.)));
           cdself += (double)(TR(2.) * p_lp3 * (vov3 - TR(3.)) * (-TR(1.) - TR(0.16) * rpow<TR>(Di, TR(3.))));
         }
@@ -452,6 +461,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
   st.eatom[T_LP * n + i] = e_lp;
   st.eatom[T_OV * n + i] = e_ov;
   st.eatom[T_UN * n + i] = e_un;
+  if (st.peratom) st.eown[i] = 0.5 * e_bond + (e_lp - 0.5 * e_c2) + e_ov + e_un;   // halves of the pair-split terms stay here
 }
 
 // ================================================================================================================
@@ -791,6 +801,7 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
     vir_add(w, -1.0, st.b_dx[q], st.b_dy[q], st.b_dz[q], own_fx, own_fy, own_fz);
   }
   if (cen) vir_add(w, 0.5, (double)djk.x, (double)djk.y, (double)djk.z, fIx + fjx, fIy + fjy, fIz + fjz);
+  if (st.peratom && live && tj >= 0 && pk < nbj) st.eslot[sl(pk, j, n)] += 0.5 * (e_tor + e_con);   // this lane's central bond
   e_tor = gsum<G>(e_tor); e_con = gsum<G>(e_con); cdself = gsum<G>(cdself);
   fjx = gsum<G>(fjx); fjy = gsum<G>(fjy); fjz = gsum<G>(fjz);
   for (int c = 0; c < 6; ++c) w[c] = gsum<G>(w[c]);
@@ -806,6 +817,7 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
     for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + j] += w[c];
     st.eatom[T_TOR * n + j] = e_tor;
     st.eatom[T_CON * n + j] = e_con;
+    if (st.peratom) st.eown[j] += 0.5 * (e_tor + e_con);
   }
 }
 
@@ -904,14 +916,14 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
     const long long tagk = has ? st.tag[k] : 0;
     const V3<TR> djk{(TR)dx, (TR)dy, (TR)dz};
     const TR rjk = (TR)r;
-    double fkx = 0, fky = 0, fkz = 0;
+    double fkx = 0, fky = 0, fkz = 0, ek = 0;
     for (unsigned long long left = hbonds; left; left &= left - 1) {   // wave-uniform, ascending slots
       const int p = __ffsll((long long)left) - 1;
       const size_t q = sl(p, j, n);
       const int i = st.b_j[q], ti = st.type[i];
       const V3<TR> dji{(TR)st.b_dx[q], (TR)st.b_dy[q], (TR)st.b_dz[q]};
       const TR rij = (TR)st.b_d[q];
-      double c1 = 0, fx = 0, fy = 0, fz = 0;
+      double c1 = 0, fx = 0, fy = 0, fz = 0, et = 0;
       // the same atom as i, possibly another image, is not an acceptor of this triple (reference quirk Q-32)
       if (has && tagk != st.tag[i]) {
         const DevHBond P = t.hb[(ti * g.ntypes + tj) * g.ntypes + tk];
@@ -921,6 +933,7 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
           V3<TR> dti, dtj, dtk;
           hbond_term<TR>(P, e2, dji, rij, djk, rjk, e, CH1, CH2, CH3, dti, dtj, dtk);
           e_hb += (double)e;
+          et = (double)e;
           c1 = (double)CH1;
           fx = (double)(CH2 * dti.x); fy = (double)(CH2 * dti.y); fz = (double)(CH2 * dti.z);
           const TR c3 = CH3 / rjk;
@@ -936,10 +949,16 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
       }
       c1 = wave_sum(c1); fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
       if (lane == 0) { st.Cdbo[q] += c1; st.fpx[q] += fx; st.fpy[q] += fy; st.fpz[q] += fz; }
+      if (st.peratom) {   // a third of each triple's energy to i (through the bond slot) and to k (through the candidate slot)
+        ek += et;
+        et = wave_sum(et);
+        if (lane == 0) st.eslot[q] += et / 3.0;
+      }
     }
     if (has) {
       const size_t o = 3 * (hbase + h);
       hf[o] = (TR)fkx; hf[o + 1] = (TR)fky; hf[o + 2] = (TR)fkz;
+      if (st.peratom) st.ehc[hbase + h] = ek / 3.0;
     }
   };
   for (int h0 = 0; h0 < nc; h0 += 64) {
@@ -954,6 +973,7 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
       if (!ok) {   // beyond hbond_cut this step: no force on the acceptor from this slot
         const size_t o = 3 * (hbase + h);
         hf[o] = TR(0); hf[o + 1] = TR(0); hf[o + 2] = TR(0);
+        if (st.peratom) st.ehc[hbase + h] = 0.0;
       }
     }
     const unsigned long long mk = __ballot(ok);
@@ -979,6 +999,7 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
     for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + j] += w[c];
     st.eatom[T_HB * n + j] = e_hb;
     st.h_cnt[j] = inrange;
+    if (st.peratom) st.eown[j] += e_hb / 3.0;
   }
 }
 
@@ -990,17 +1011,40 @@ __global__ void __launch_bounds__(256) k_hbond_acceptor(State st, Globals g, Tab
   if (k >= st.n) return;
   const int lane = lane_id(), tk = st.type[k];
   const int role = (tk >= 0 && g.hbond_cut > 0) ? t.sp[tk].p_hbond : 0;
-  if (role != 2) return;
+  if (role != 2) {
+    if (st.peratom && lane == 0) st.eacc[k] = 0.0;
+    return;
+  }
   const int nc = min(st.hc_cnt[k], st.maxhc);
   const size_t hbase = (size_t)k * st.maxhc;
   const TR* hf = reinterpret_cast<const TR*>(st.hf);
-  double fx = 0, fy = 0, fz = 0;
+  double fx = 0, fy = 0, fz = 0, e = 0;
   for (int h = lane; h < nc; h += 64) {
-    const size_t o = 3 * (size_t)st.hc_j[hbase + h];   // flat donor slot
-    fx += (double)hf[o]; fy += (double)hf[o + 1]; fz += (double)hf[o + 2];
+    const size_t o = (size_t)st.hc_j[hbase + h];   // flat donor slot
+    fx += (double)hf[3 * o]; fy += (double)hf[3 * o + 1]; fz += (double)hf[3 * o + 2];
+    if (st.peratom) e += st.ehc[o];
   }
   fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
   if (lane == 0) { st.fs_x[k] += fx; st.fs_y[k] += fy; st.fs_z[k] += fz; }
+  if (st.peratom) {
+    e = wave_sum(e);
+    if (lane == 0) st.eacc[k] = e;
+  }
+}
+
+// per-atom energy in the reference's attribution: the atom's own shares, the valence terms of its centre, its non-bonded halves and
+// polarisation, the shares its bond partners left in their mirror slots, and its H-bond thirds as an acceptor
+__global__ void __launch_bounds__(128) k_eatom_ref(State st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
+  double e = st.eown[i] + st.eatom[T_ANG * n + i] + st.eatom[T_PEN * n + i] + st.eatom[T_COA * n + i] + st.eatom[T_VDW * n + i] +
+             st.eatom[T_ELE * n + i] + st.eatom[T_POL * n + i] + st.eacc[i];
+  for (int p = 0; p < nb; ++p) {
+    const size_t q = sl(p, i, n);
+    e += st.eslot[sl(st.b_sym[q], st.b_j[q], n)];
+  }
+  st.eref[i] = e;
 }
 
 // ================================================================================================================
@@ -1106,6 +1150,10 @@ void launch_bond_derivatives(const State& st, hipStream_t s) {
   const dim3 grid(nblk(st.n, 128)), blk(128);
   hipLaunchKernelGGL(k_cdelta, grid, blk, 0, s, st);
   hipLaunchKernelGGL(k_dbond, grid, blk, 0, s, st);
+}
+
+void launch_eatom_ref(const State& st, hipStream_t s) {
+  hipLaunchKernelGGL(k_eatom_ref, dim3(nblk(st.n, 128)), dim3(128), 0, s, st);
 }
 
 void launch_gather(const State& st, hipStream_t s) {

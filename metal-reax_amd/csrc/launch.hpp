@@ -50,6 +50,7 @@ void launch_angle_terms(Precision p, const State& st, const Globals& g, const Ta
 // BO-derivative pass (CdDelta, then the per-atom bond sums); the final gather adds the non-bonded gradient and the virial
 void launch_bond_derivatives(const State& st, hipStream_t s);
 void launch_gather(const State& st, hipStream_t s);
+void launch_eatom_ref(const State& st, hipStream_t s);   // per-atom energies of the reference's attribution (peratom calls)
 
 // ---- non-bonded terms and the charge-equilibration matrix (k_nonbonded.hip)
 void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);

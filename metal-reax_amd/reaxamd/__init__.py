@@ -54,6 +54,7 @@ def lib():
             "rxa_set_charges": (I, [P, P]),
             "rxa_get_charges": (I, [P, P]),
             "rxa_compute": (I, [P, I, P, P, P]),
+            "rxa_peratom": (I, [P, P, P]),
             "rxa_pvector": (None, [P, P]),
             "rxa_census": (I, [P, P]),
             "rxa_bond_table": (LONG, [P, P, P, P, P, P]),
@@ -242,12 +243,20 @@ class Engine:
         return q
 
     # ---- evaluation
-    def compute(self, rebuild=False, want_grad=True):
+    def compute(self, rebuild=False, want_grad=True, peratom=False):
+        """Energies, gradient and virial; with peratom=True also "eatom" (n) and "vatom" (n, 6) of the same call (rxa_peratom)."""
         e = np.zeros(13)
         g = np.zeros(3 * self.n) if want_grad else None
         v = np.zeros(6)
-        self._chk(lib().rxa_compute(self.h, 1 if rebuild else 0, _p(e), _p(g), _p(v)))
-        return {"energy": dict(zip(TERMS, e.tolist())), "grad": None if g is None else g.reshape(self.n, 3), "virial": v}
+        self._chk(lib().rxa_compute(self.h, (1 if rebuild else 0) | (2 if peratom else 0), _p(e), _p(g), _p(v)))
+        r = {"energy": dict(zip(TERMS, e.tolist())), "grad": None if g is None else g.reshape(self.n, 3), "virial": v}
+        if peratom:
+            ea = np.zeros(self.n)
+            va = np.zeros(6 * self.n)
+            self._chk(lib().rxa_peratom(self.h, _p(ea), _p(va)))
+            r["eatom"] = ea
+            r["vatom"] = va.reshape(self.n, 6)
+        return r
 
     def census(self):
         out = np.zeros(7 * self.n, dtype=np.int32)

@@ -12,7 +12,7 @@ SRCS := src/core/capabilities.cpp src/core/compat_flags.cpp src/core/energy_term
         src/io/control_parser.cpp src/io/pair_settings.cpp src/io/table_dump.cpp src/metal/metal_backend_stub.cpp
 OBJS := $(patsubst %.cpp,$(OUT)/obj/%.o,$(SRCS))
 
-all: $(OUT)/reaxmetal_energy_tool $(OUT)/reaxmetal_ffield_dump
+all: $(OUT)/reaxmetal_energy_tool $(OUT)/reaxmetal_ffield_dump $(OUT)/ref_peratom
 
 $(OUT)/obj/%.o: $(REF)/%.cpp
 	@mkdir -p $(dir $@)
@@ -20,6 +20,10 @@ $(OUT)/obj/%.o: $(REF)/%.cpp
 
 $(OUT)/reaxmetal_energy_tool: $(OBJS) $(REF)/tools/energy_tool.cpp
 	$(CXX) $(CXXFLAGS) $(REF)/tools/energy_tool.cpp $(OBJS) -o $@ -lpthread
+
+# per-atom tallies through the reference's public interface (a driver of this directory, not a reference source)
+$(OUT)/ref_peratom: $(OBJS) oracle/ref_peratom.cpp
+	$(CXX) $(CXXFLAGS) oracle/ref_peratom.cpp $(OBJS) -o $@ -lpthread
 
 $(OUT)/reaxmetal_ffield_dump: $(OBJS) $(REF)/tools/ffield_dump.cpp
 	$(CXX) $(CXXFLAGS) $(REF)/tools/ffield_dump.cpp $(OBJS) -o $@ -lpthread
