@@ -110,7 +110,13 @@ __device__ inline double2 row_product(int row, int n, int maxnb, const int* __re
   return make_double2(y0, y1);
 }
 
-__device__ inline int product_row() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes); }
+// Grid of the product kernels: a multiple of the 8 XCDs. Workgroup b is dispatched to XCD b % 8, so it takes the virtual block
+// (b % 8) C + b / 8 (C = grid / 8): each XCD walks one contiguous eighth of the rows, and with atoms in cell order that eighth
+// is a compact region whose column gathers stay in that XCD's L2. Virtual blocks past the last row compute nothing.
+constexpr int kXcds = 8;
+inline __host__ __device__ int mv_grid(int n) { return (mv_blocks(n) + kXcds - 1) / kXcds * kXcds; }
+__device__ inline int mv_vblock() { return (int)((blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds); }
+__device__ inline int product_row() { return (int)((mv_vblock() * blockDim.x + threadIdx.x) / kRowLanes); }
 
 // w = (diag(eta) + H) v for both systems (the initial residual's product). With kInit the CG start is fused in: the row's
 // owner writes r = b - w and p = r / eta instead of w, and each block leaves the partials of b.b and r.p of its rows.
@@ -147,7 +153,8 @@ __global__ void __launch_bounds__(kMvThreads) k_spmv2(int n, int maxnb, const in
     __shared__ double sh[4 * kMvThreads / 64];
     block_reduce<4, kMvThreads>(acc, sh);
     if (threadIdx.x == 0)
-      for (int k = 0; k < 4; ++k) qb.partial_mv[blockIdx.x * 4 + k] = acc[k];
+      if (mv_vblock() < mv_blocks(n))
+        for (int k = 0; k < 4; ++k) qb.partial_mv[mv_vblock() * 4 + k] = acc[k];
   }
 }
 
@@ -205,7 +212,7 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
     }
   }
   block_reduce<2, kMvThreads>(dq, sh);
-  if (threadIdx.x == 0) { qb.partial_mv[blockIdx.x * 2] = dq[0]; qb.partial_mv[blockIdx.x * 2 + 1] = dq[1]; }
+  if (threadIdx.x == 0 && mv_vblock() < mv_blocks(n)) { qb.partial_mv[mv_vblock() * 2] = dq[0]; qb.partial_mv[mv_vblock() * 2 + 1] = dq[1]; }
 }
 
 // single-vector matvec (the fix's sparse_matvec), owned rows
@@ -435,12 +442,12 @@ void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, 
 
 template <bool kInit, class TV, class TC>
 void mv2_t(int n, int maxnb, const QeqBuffers& qb, const TC* col, const TV* val, const double* x, double* y, hipStream_t s) {
-  hipLaunchKernelGGL((k_spmv2<TV, TC, kInit>), dim3(mv_blocks(n)), dim3(kMvThreads), 0, s, n, maxnb, qb.cnt, col, val, qb.eta, x, y, qb);
+  hipLaunchKernelGGL((k_spmv2<TV, TC, kInit>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, maxnb, qb.cnt, col, val, qb.eta, x, y, qb);
 }
 
 template <class TV, class TC>
 void mvdir_t(int n, int maxnb, const QeqBuffers& qb, const TC* col, const TV* val, int it, int imax, double tol, hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_mvdir<TV, TC>), dim3(mv_blocks(n)), dim3(kMvThreads), 0, s, n, maxnb, col, val, qb, it, imax, tol);
+  hipLaunchKernelGGL((k_cg_mvdir<TV, TC>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, maxnb, col, val, qb, it, imax, tol);
 }
 
 void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s) {
@@ -479,7 +486,7 @@ namespace {
 inline int pk_blocks(long long items, int per) { return (int)((items + per - 1) / per); }
 template <class TV, class TC>
 void pk_mvdir_t(int n, int maxnb, const QeqBuffers& qb, const TC* col, const TV* val, const PackedQeq& pq, int it, hipStream_t s) {
-  hipLaunchKernelGGL((k_pk_cg_mvdir<TV, TC>), dim3(mv_blocks(n)), dim3(kMvThreads), 0, s, n, maxnb, col, val, qb, pq.sysid, pq.pk, pq.pki, pq.rowv, it);
+  hipLaunchKernelGGL((k_pk_cg_mvdir<TV, TC>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, maxnb, col, val, qb, pq.sysid, pq.pk, pq.pki, pq.rowv, it);
 }
 }  // namespace
 
