@@ -88,15 +88,17 @@ def batch_and_serving(ffield_text, precision, slots=16):
     arrivals = [r.arrival for r in reqs]
     done, span = timed(reqs, slots, realtime=True)
     lat = serving.latency_stats(done)
-    # dynamic batching: a free slot runs a large request alone or packs every queued small one into one context
-    pq = work / timed(serving.serving_requests(32, seed=2), slots, packed=True)[1]
+    # dynamic batching: a free slot runs a large request alone or packs every queued small one into one context; 8 slots (the
+    # large requests run as single-system contexts through the 4 hardware queues: 8 concurrent ones served 15% more than 16)
+    pslots = 8
+    pq = work / timed(serving.serving_requests(32, seed=2), pslots, packed=True)[1]
     preqs = serving.serving_requests(32, seed=2)
     for r, a in zip(preqs, arrivals):
         r.arrival = a
-    pdone, pspan = timed(preqs, slots, realtime=True, packed=True)
+    pdone, pspan = timed(preqs, pslots, realtime=True, packed=True)
     plat = serving.latency_stats(pdone)
     out["serving"] = {"requests": len(reqs), "atom_steps": work, "poisson_rate_per_s": rate, "sequential_atom_steps_per_s": seq,
-                      "packed": {"slots": slots, "pack_below_atoms": 3000, "all_queued_atom_steps_per_s": pq, "poisson_atom_steps_per_s": work / pspan,
+                      "packed": {"slots": pslots, "pack_below_atoms": 3000, "all_queued_atom_steps_per_s": pq, "poisson_atom_steps_per_s": work / pspan,
                                  "p50_latency_s": plat["p50_s"], "p99_latency_s": plat["p99_s"]},
                       "slot_executor": {"slots": slots, "all_queued_atom_steps_per_s": queued, "poisson_atom_steps_per_s": work / span,
                                         "p50_latency_s": lat["p50_s"], "p99_latency_s": lat["p99_s"]}}
