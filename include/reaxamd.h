@@ -161,6 +161,10 @@ int rxa_check_pair_style(int nargs, const char* const* args, const char* control
 #define RXA_NPHASES 10
 int rxa_profile(rxa_context* ctx, int mask);
 int rxa_profile_read(rxa_context* ctx, double* ms, int64_t* counts);
+/* time only every `every`-th launch sequence of each timed phase (1 = all; clears the totals): fewer events in a timed loop.
+ * rxa_profile_read then reports every launch sequence in counts and the timed mean times that count in ms. No reference
+ * counterpart (measurement of this engine). */
+int rxa_profile_every(rxa_context* ctx, int every);
 
 /* ---- device-resident inputs (bench / serving): pointers to device memory owned by the caller (HIP, same device) */
 int rxa_set_positions_device(rxa_context* ctx, const double* dx);

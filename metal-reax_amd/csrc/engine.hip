@@ -60,6 +60,8 @@ enum Phase { PH_NBR, PH_BONDS, PH_BOCORR, PH_TERMS, PH_NONB, PH_DBOND, PH_QEQ_H,
 struct PhaseTimer {
   bool on = false;
   unsigned mask = 0;   // phases being timed (bit k = phase k)
+  int every = 1;       // time every `every`-th launch sequence of a phase (the others carry no events)
+  long long seen[PH_COUNT] = {0};
   hipStream_t s = nullptr;
   std::vector<hipEvent_t> pool;
   struct Pending { int ph; hipEvent_t a, b; };
@@ -78,6 +80,7 @@ struct PhaseTimer {
   }
   hipEvent_t begin(int ph) {
     if (!on || !(mask >> ph & 1u)) return nullptr;
+    if (seen[ph]++ % every != 0) return nullptr;
     hipEvent_t a = take();
     HIPCHK(hipEventRecord(a, s));
     return a;
@@ -99,7 +102,7 @@ struct PhaseTimer {
     }
     pending.clear();
   }
-  void reset() { for (int k = 0; k < PH_COUNT; ++k) { ms[k] = 0; cnt[k] = 0; } }
+  void reset() { for (int k = 0; k < PH_COUNT; ++k) { ms[k] = 0; cnt[k] = 0; seen[k] = 0; } }
   ~PhaseTimer() {
     for (auto e : pool) (void)hipEventDestroy(e);
     for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -1854,10 +1857,20 @@ int rxa_profile(rxa_context* c, int enable) {
 int rxa_profile_read(rxa_context* c, double* ms, int64_t* counts) {
   return guard(c, [&](Engine& e) {
     e.sync();
-    for (int k = 0; k < rxa::PH_COUNT; ++k) {
-      ms[k] = e.timer.ms[k];
-      counts[k] = e.timer.cnt[k];
+    for (int k = 0; k < rxa::PH_COUNT; ++k) {   // sampled phases: the timed mean times every launch sequence
+      const long long all = e.timer.seen[k], timed = e.timer.cnt[k];
+      ms[k] = timed ? e.timer.ms[k] * (double)all / (double)timed : 0.0;
+      counts[k] = timed ? all : 0;
     }
+  });
+}
+
+int rxa_profile_every(rxa_context* c, int every) {
+  return guard(c, [&](Engine& e) {
+    if (every < 1) throw ApiError(RXA_EINVAL, "profile sampling period must be >= 1");
+    e.sync();
+    e.timer.every = every;
+    e.timer.reset();
   });
 }
 

@@ -55,6 +55,7 @@ def lib():
             "rxa_get_charges": (I, [P, P]),
             "rxa_compute": (I, [P, I, P, P, P]),
             "rxa_peratom": (I, [P, P, P]),
+            "rxa_profile_every": (I, [P, I]),
             "rxa_pvector": (None, [P, P]),
             "rxa_census": (I, [P, P]),
             "rxa_bond_table": (LONG, [P, P, P, P, P, P]),
@@ -336,13 +337,15 @@ class Engine:
     PHASES = ("neighbor", "bond_lists", "bo_correct", "bonded_terms", "nonbonded", "dbond_gather", "qeq_matrix", "qeq_spmv", "qeq_cg",
               "integrate")
 
-    def profile(self, enable=True, phases=None):
-        """Device timing of the engine's phases (HIP events on its stream): off, every phase, or only the named phases."""
+    def profile(self, enable=True, phases=None, every=1):
+        """Device timing of the engine's phases (HIP events on its stream): off, every phase, or only the named phases; with
+        every > 1 only every every-th launch sequence of a phase is timed (profile_read scales the timed mean to all of them)."""
         mask = 0
         if enable:
             mask = 1 if phases is None else sum(1 << self.PHASES.index(p) for p in phases)
             if mask == 1 and phases is not None:   # only phase 0: bit 0 alone would read as "every phase"
                 mask = 1 | (1 << 31)
+        self._chk(lib().rxa_profile_every(self.h, int(every)))
         self._chk(lib().rxa_profile(self.h, int(np.int32(np.uint32(mask)))))
 
     def profile_read(self):
