@@ -231,7 +231,7 @@ struct Engine {
     HIPCHK(hipEventCreateWithFlags(&ev_cg, hipEventDisableTiming));
     timer.s = stream;
     HIPCHK(hipHostMalloc((void**)&flags_h, 8 * sizeof(int), hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void**)&disp_h, sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&disp_h, md_disp_partials() * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipEventCreateWithFlags(&ev_flags, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev_disp, hipEventDisableTiming));
   }
@@ -1342,9 +1342,8 @@ struct Engine {
         launch_fill_int(d_rebany.p, 1, 0, stream);
         launch_pk_disp(st, md, d_sys_start.p, S, 0.5 * skin, d_reb.p, d_rebany.p, d_disp_pk.p, stream);
         HIPCHK(hipMemcpyAsync(rebany_h, d_rebany.p, sizeof(int), hipMemcpyDeviceToHost, stream));
-      } else {
-        launch_md_disp_final(md, stream);
-        HIPCHK(hipMemcpyAsync(disp_h, md.disp, sizeof(double), hipMemcpyDeviceToHost, stream));
+      } else {   // the kick-drift kernel's per-block largest squared displacements, reduced on the host (exact: a maximum)
+        HIPCHK(hipMemcpyAsync(disp_h, md.partial, md_disp_partials() * sizeof(double), hipMemcpyDeviceToHost, stream));
       }
       timer.end(PH_MD, t0);
       HIPCHK(hipEventRecord(ev_disp, stream));
@@ -1361,7 +1360,9 @@ struct Engine {
           fork_after = nullptr;
         }
       } else {
-        disp = *disp_h;
+        double m = 0;
+        for (int k = 0; k < md_disp_partials(); ++k) m = std::max(m, disp_h[k]);
+        disp = std::sqrt(m);
         if (disp > 0.5 * skin) {
           build_neighbors();   // invalidates the matrix
           launch_md_ref_positions(st, md, stream);

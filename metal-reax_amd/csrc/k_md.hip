@@ -272,15 +272,13 @@ void launch_md_step_start(const State& st, MdBuffers& md, double dtf, double dt,
   }
   hipLaunchKernelGGL(k_kick_drift, dim3(kRBlocks), dim3(kRT), 0, s, st, md, dtf, dt, thermostat ? 1 : 0);
 }
+int md_disp_partials() { return kRBlocks; }
+
 // step end fused with the force gather: gather + velocity half step + kinetic-energy partials, then (if `thermostat`) the
 // thermostat half step, whose velocity factor stays pending
 void launch_md_gather_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_kick_ke, dim3(kRBlocks), dim3(kRT), 0, s, st, md, dtf);
   if (thermostat) hipLaunchKernelGGL(k_nhc, dim3(1), dim3(kRT), 0, s, md);
-}
-// the largest displacement from the kick-drift partials (launch_md_step_start), into md.disp
-void launch_md_disp_final(MdBuffers& md, hipStream_t s) {
-  hipLaunchKernelGGL(k_disp_final, dim3(1), dim3(kRT), 0, s, md);
 }
 // step end: velocity half step, then (if `thermostat`) the thermostat half step, whose velocity factor stays pending
 void launch_md_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s) {

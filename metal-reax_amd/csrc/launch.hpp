@@ -124,7 +124,7 @@ void launch_md_ref_positions(const State& st, MdBuffers& md, hipStream_t s);
 void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s);
 // fused step kernels (the separate ones above, same arithmetic): see k_md.hip
 void launch_md_step_start(const State& st, MdBuffers& md, double dtf, double dt, bool thermostat, bool pending, hipStream_t s);
-void launch_md_disp_final(MdBuffers& md, hipStream_t s);
+int md_disp_partials();   // per-block partials of the kick-drift kernel's largest squared displacement (in md.partial)
 void launch_md_gather_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s);
 void launch_md_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s);
 void launch_md_scale(const State& st, MdBuffers& md, hipStream_t s);
