@@ -140,8 +140,8 @@ def main():
                     help="also run a streaming kernel (MIB MiB in, MIB MiB out, 10 times) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; --gpus %d is not supported (systems shard across GPUs without "
-              "communication, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported (independent systems would be assigned to "
+              "GPUs whole, with no communication between them)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import reaxamd
