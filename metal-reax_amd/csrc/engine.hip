@@ -877,14 +877,13 @@ struct Engine {
     hipEvent_t t0 = timer.begin(PH_BONDS);
     launch_fill_int(st.flags, 8, 0, b);
     launch_bond_build(prec, st, cell, g, tab, b);
-    launch_bond_sym(st, b);
     timer.end(PH_BONDS, t0);
-    HIPCHK(hipMemcpyAsync(flags_h, st.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, b));
-    HIPCHK(hipEventRecord(ev_flags, b));
-    bonded_pending = true;
     t0 = timer.begin(PH_BOCORR);
     launch_bo_correct(prec, st, g, tab, b);
     timer.end(PH_BOCORR, t0);
+    HIPCHK(hipMemcpyAsync(flags_h, st.flags, 8 * sizeof(int), hipMemcpyDeviceToHost, b));
+    HIPCHK(hipEventRecord(ev_flags, b));
+    bonded_pending = true;
     t0 = timer.begin(PH_TERMS);
     launch_atom_terms(prec, st, g, tab, b);
     timer.end(PH_TERMS, t0);

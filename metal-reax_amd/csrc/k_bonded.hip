@@ -3,9 +3,9 @@
 // Pipeline per step (one launch each, in stream order):
 //   bond_build  (wave per atom)   bond list from the skin list: d <= bond_cut and BO' >= bo_cut, uncorrected BO', dBO'/dx, Delta'
 //                                 [reference src/cpu/bonded.cpp:109-162, terms.hpp:50-56]; H-bond partner lists [bonded.cpp:120-124]
-//   bond_sym    (thread per atom) index of the mirror slot of every bond
 //   bo_correct  (thread per atom) corrected BO and its 11 derivative coefficients, always from the pair's canonical end so both ends
-//                                 hold bitwise-identical values [bonded.cpp:164-204, terms.hpp:66-164]
+//                                 hold bitwise-identical values [bonded.cpp:164-204, terms.hpp:66-164]; index of the mirror slot of
+//                                 every bond and the step's largest bond count
 //   atom_terms  (thread per atom) bond energy + triple-bond stabilisation, lone pair + C2, over/under coordination [bonded.cpp:219-318]
 //   valence     (thread per atom) valence angle, penalty, 3-body conjugation at the centre [bonded.cpp:321-474]
 //   torsion     (thread per atom) torsion + 4-body conjugation, evaluated at BOTH atoms of the central bond; each end keeps the force
@@ -222,7 +222,13 @@ __global__ void __attribute__((amdgpu_waves_per_eu(4))) __launch_bounds__(64 * k
   }
 }
 
-__global__ void __launch_bounds__(128) k_bond_sym(State st) {
+// ================================================================================================================
+// corrected bond orders (always from the canonical end) and atom quantities
+// ================================================================================================================
+// Also finds each slot's mirror on the other atom (b_sym) and the step's largest bond count (flags[7]); a slot without a mirror
+// sets flags[3].
+template <class TB>
+__global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   {   // the largest bond count of this step (flags[7]), one atomic per wave
     int c = i < st.n ? st.b_cnt[i] : 0;
@@ -230,31 +236,20 @@ __global__ void __launch_bounds__(128) k_bond_sym(State st) {
     if (lane_id() == 0 && c > 0) atomicMax(&st.flags[7], c);
   }
   if (i >= st.n) return;
-  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
-  for (int p = 0; p < nb; ++p) {
-    const int j = st.b_j[sl(p, i, n)], want = shift_neg(st.b_s[sl(p, i, n)]);
-    const int nbj = min(st.b_cnt[j], st.maxb);
-    int found = -1;
-    for (int q = 0; q < nbj; ++q)
-      if (st.b_j[sl(q, j, n)] == i && st.b_s[sl(q, j, n)] == want) { found = q; break; }
-    st.b_sym[sl(p, i, n)] = found < 0 ? 0 : found;
-    if (found < 0) st.flags[3] = 1;
-  }
-}
-
-// ================================================================================================================
-// corrected bond orders (always from the canonical end) and atom quantities
-// ================================================================================================================
-template <class TB>
-__global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables t) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= st.n) return;
   const int n = st.n, ti = st.type[i];
   const int nb = min(st.b_cnt[i], st.maxb);
   TB tbo = 0;
   for (int p = 0; p < nb; ++p) {
     const size_t q = sl(p, i, n);
     const int j = st.b_j[q];
+    {   // mirror slot of this bond on atom j
+      const int want = shift_neg(st.b_s[q]), nbj = min(st.b_cnt[j], st.maxb);
+      int found = -1;
+      for (int r = 0; r < nbj; ++r)
+        if (st.b_j[sl(r, j, n)] == i && st.b_s[sl(r, j, n)] == want) { found = r; break; }
+      st.b_sym[q] = found < 0 ? 0 : found;
+      if (found < 0) st.flags[3] = 1;
+    }
     const bool first = canonical_first(i, j, st.b_s[q], st.tag);
     const int a = first ? i : j, b = first ? j : i;
     const int ta = st.type[a], tb = st.type[b];
@@ -1108,9 +1103,6 @@ void launch_bond_build(Precision p, const State& st, const Cell& c, const Global
   else bond_build_t<double>(st, c, g, t, s);
 }
 
-void launch_bond_sym(const State& st, hipStream_t s) {
-  hipLaunchKernelGGL(k_bond_sym, dim3(nblk(st.n, 128)), dim3(128), 0, s, st);
-}
 
 void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
   if (p == Precision::FP32) hipLaunchKernelGGL(k_bo_correct<float>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);

@@ -41,7 +41,6 @@ void launch_sublists(const State& st, const Globals& g, const Tables& t, double 
 
 // ---- bond-order chain and bonded terms (k_bonded.hip)
 void launch_bond_build(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
-void launch_bond_sym(const State& st, hipStream_t s);
 void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s);
 // bond energy / lone pair / over-under (no group size needed), then valence, torsion, H-bonds with the group size for the
 // largest bond count bmax
