@@ -196,9 +196,9 @@ def main():
     bytes_step = model_bytes_per_atom_step(n_f, R, k, B, n_h, (w.types == 2).mean())
 
     # roofline of the charge-matrix product k_cg_mvdir (the kernel with the largest device time; it also forms the CG direction):
-    # algorithmic bytes of one fused two-vector pass over the half matrix (4-byte value + 4-byte column per entry) / mean launch
-    # duration from the HIP events around each launch
-    alg = 8 * n_f * n
+    # algorithmic bytes of one fused two-vector pass over the half matrix (4-byte value + 4-byte column per entry; the value is
+    # 8 bytes in FP64 mode) / mean launch duration from the HIP events around each launch
+    alg = (12 if args.precision == "fp64" else 8) * n_f * n
     avg_s = spmv_ms / max(1, spmv_cnt) * 1e-3
     achieved = alg / avg_s / 1e9
     traffic = read_traffic()
@@ -218,8 +218,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": value / PUBLISHED_24K,
         "dtype": {"fp64": "f64", "mixed": "f64+f32", "fp32": "f32"}[args.precision],
-        "data": "synthetic: seeded 8000-H2O lattice box at 1.0 g/cm^3, Maxwell-Boltzmann 300 K; stand-in C/H/O ffield",
-        "config": {"workload": "water_24000_nvt_qeq", "atoms": n, "box_A": float(w.hi[0]), "precision": args.precision,
+        "data": "synthetic: seeded %d-H2O lattice box at 1.0 g/cm^3, Maxwell-Boltzmann 300 K; stand-in C/H/O ffield" % (n // 3),
+        "config": {"workload": "water_%d_nvt_qeq" % n, "atoms": n, "box_A": float(w.hi[0]), "precision": args.precision,
                    "dt_fs": 0.25, "qeq_tol": 1e-6, "skin_A": 2.0},
         "roofline": {"bound": "hbm", "kernel": "k_cg_mvdir", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
