@@ -76,6 +76,7 @@ int rxa_get_charges(rxa_context* ctx, double* q);         /* n */
  * does at pair_reaxff_metal.cpp:501-504; may be NULL), virial[6] = xx yy zz xy xz yz of all terms (may be NULL). */
 #define RXA_COMPUTE_REBUILD 1   /* force a neighbour-list rebuild */
 #define RXA_COMPUTE_PERATOM 2   /* also tally per-atom energies and virials (read them with rxa_peratom) */
+#define RXA_COMPUTE_CENSUS 4    /* also record the decision census (read it with rxa_census) */
 int rxa_compute(rxa_context* ctx, int flags, double* energies, double* grad, double* virial);
 /* per-atom energy eatom[n] and virial vatom[6n] (xx yy zz xy xz yz per atom) of the last rxa_compute made with
  * RXA_COMPUTE_PERATOM, owned atoms in the caller's order, every image folded onto its owner (either pointer may be NULL).
@@ -90,8 +91,14 @@ int rxa_compute(rxa_context* ctx, int flags, double* energies, double* grad, dou
 int rxa_peratom(rxa_context* ctx, double* eatom, double* vatom);
 /* the 13 terms -> LAMMPS' 14-slot pvector (eb ea elp emol ev epen ecoa ehb et eco ew ep efi eqeq), energy_terms.cpp:48-63 */
 void rxa_pvector(const double* energies, double* pvector14);
-/* per owned atom decision census, 7 * n ints: bonds, angle_sets(-1: not collected), torsions(-1), hbonds(-1), nonbonded, sbo_region(-1),
- * lp_trunc (replaces BondedOptions::census / DecisionCensus, include/reaxmetal/census.hpp:12-20) */
+/* per owned atom decision census, 7 * n ints [k * n + atom] in the caller's order (replaces BondedOptions::census / DecisionCensus,
+ * include/reaxmetal/census.hpp:12-20; the reference fills it in src/cpu/bonded.cpp and nonbonded.cpp):
+ *   0 bonds (the bond-order cut-off), 1 angle_sets: bond pairs of the valence centre through the three-body gates (BO > thb_cut
+ *   each, product > thb_cutsq), 2 torsions: chains through the four-body gates with parameters, at the central bond's counted
+ *   end, 3 hbonds: triples evaluated at the donor hydrogen, 4 nonbonded: pairs within nonb_cut at the counted end, 5 sbo_region:
+ *   0 SBO <= 0, 1 (0, 1], 2 (1, 2), 3 >= 2, 6 lp_trunc: (int)(Delta_e / 2).
+ * The kernels that take the decisions record them (in each mode's own precision). Reads the census of the last rxa_compute made
+ * with RXA_COMPUTE_CENSUS; otherwise first runs such a force call at the current positions and charges. */
 int rxa_census(rxa_context* ctx, int* out);
 /* bond table of the owned atoms: start[n+1], nbr[], bo[] (nbr = owned index of the partner), total_bo[n], nlp[n]; returns the
  * number of bonds; call with nbr == NULL to size (replaces BondResult::table for fix reaxff/bonds|species, bonded.hpp:29-35,
@@ -112,6 +119,14 @@ int rxa_qeq_solve(rxa_context* ctx, const double* b, const double* x0, int imax,
  * into RXA_ENOCONV (the fix's 'strict' keyword). */
 int rxa_qeq_charges(rxa_context* ctx, int imax, double tol, int strict, double* q_out, int* iters, double* rel);
 int rxa_qeq_reset_history(rxa_context* ctx);
+/* FP64 iterative refinement of the charge solve (replaces the fix's 'resident' keyword, plugin/adapter/fix_qeq_reaxff_metal.cpp:38,
+ * 67-80, i.e. mtl::Context::qeq_solve's double-single refinement, src/metal/metal_backend.mm:435-530): the CG multiplies with the
+ * FP32 matrix values, while its initial residual and a check of the true residual after the iterations stop use the matrix to
+ * about 48 bits (FP32 value + FP32 remainder); a solve whose true residual fails the stock condition restarts from it, within
+ * one imax budget. The charges then solve the FP64 matrix to the requested tolerance. mode -1: the precision's default (on in
+ * mixed mode, off in FP32 mode), 0 off, 1 on; FP64 mode keeps FP64 values and is not affected. rxa_qeq_solve / rxa_qeq_charges
+ * then report the total iterations (+1) and the true relative residual of the last check. */
+int rxa_qeq_refine(rxa_context* ctx, int mode);
 
 /* ---- standalone MD loop (bench.py and the serving harness; LAMMPS integrates in plugin mode)
  * masses by LAMMPS type (index 0 unused); NVT Nose-Hoover chain (3) at t_target with damping t_damp (fs), or NVE when t_damp <= 0;

@@ -134,7 +134,16 @@ struct State {
   double* eacc;                   // n: an acceptor's gathered H-bond thirds
   double* eref;                   // n: the per-atom energy (k_eatom_ref)
   double *gnb_x, *gnb_y, *gnb_z;  // non-bonded gradient
+  // decision census (filled only when census != 0): kCenCount * n ints, [k * n + i] for the categories below
+  int census;
+  int* cen;
 };
+
+// census categories recorded by the kernels that take the decisions (the bond count and the lone-pair truncation are read from
+// the bond list and Delta_e): valence centre bond pairs through the three-body gates, torsions through the four-body gates at
+// the central bond's counted end, H-bond triples evaluated at the donor, non-bonded pairs within nonb_cut at the counted end,
+// the SBO branch of the valence centre (0: SBO <= 0, 1: (0, 1], 2: (1, 2), 3: >= 2)
+enum CensusIndex { kCenAngles, kCenTorsions, kCenHbonds, kCenNonbonded, kCenSbo, kCenCount };
 
 // ---- shift codes ------------------------------------------------------------------------------------------------
 __host__ __device__ inline int shift_code(int s0, int s1, int s2) { return (s0 & 0xFF) | ((s1 & 0xFF) << 8) | ((s2 & 0xFF) << 16); }
@@ -269,6 +278,10 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 __device__ inline float wave_sum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline int wave_sum_int(int v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }

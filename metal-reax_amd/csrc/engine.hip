@@ -177,11 +177,17 @@ struct Engine {
   DBuf<int> q_int;
   DBuf<double> q_dbl;
   int q_dbl_n = -1;   // atom count the charge buffers were sized for
-  DBuf<float> q_f32;
+  DBuf<float> q_f32;        // FP32 values (and their FP32 remainders under refinement): mixed and FP32 modes
   DBuf<unsigned short> q_u16;
   DBuf<unsigned short> q_key;
+  DBuf<double> q_scal;       // CG scalars and flags per iteration, sized by the iteration cap alone (never reallocated with the
+                             // matrix or the per-atom vectors, so growing it cannot invalidate a built matrix)
   bool qeq_ready = false, qeq_h_valid = false;
-  int qeq_imax_cap = 0;
+  int qeq_imax_cap = 0;      // iteration cap the scalars, the packed flags and the pinned readback buffers are sized for
+  int q_fp32_alloc = -1;     // value format the matrix arrays were allocated for (qb.fp32 at the time)
+  // FP64 iterative refinement of the charge solve: -1 the precision's default (on in mixed mode), 0 off, 1 on (the fix's
+  // 'resident' keyword; no effect in FP64 mode, whose matrix values are FP64)
+  int qeq_refine_mode = -1;
   int hist_count = 0;
   double q_swa = 0, q_swb = 0;
   std::vector<double> q_chi_t, q_eta_t, q_gam_t;   // per ffield type
@@ -213,7 +219,6 @@ struct Engine {
   hipEvent_t ev_fork = nullptr, ev_bonded = nullptr, ev_cg = nullptr;
   int* cgf_h = nullptr;      // pinned: packed systems' any-active flags read back per batch
   double* cgs_h = nullptr;   // pinned: CG scalars and activity flags (the device layout of QeqBuffers::scal) read back per batch
-  size_t cg_cap_h = 0;
 
   Engine(int dev) : device(dev) {
     const char* sort_env = std::getenv("RXA_ATOM_SORT");
@@ -361,7 +366,13 @@ struct Engine {
     const std::string p = ps.effective_precision();
     prec = p == "fp64" ? Precision::FP64 : (p == "fp32" ? Precision::FP32 : Precision::MIXED);
     qb.fp32 = prec == Precision::FP64 ? 0 : 1;   // the charge solve multiplies with FP32 matrix values outside FP64 mode
+    apply_refine();
     qeq_h_valid = false;
+  }
+  void apply_refine() {
+    const int r = (prec == Precision::MIXED && qeq_refine_mode != 0) || (prec == Precision::FP32 && qeq_refine_mode == 1) ? 1 : 0;
+    if (r != qb.refine) qeq_h_valid = false;   // the matrix must be rebuilt with (or may drop) its FP32 remainders
+    qb.refine = r;
   }
 
   double list_cut() const { return std::max({g.nonb_cut, g.hbond_cut, g.bond_cut, qeq_ready ? q_swb : 0.0}); }
@@ -400,8 +411,9 @@ struct Engine {
     const int NA = 38 + kTerms + 9;   // number of per-atom double arrays below
     a_atom.alloc((size_t)NA * n);
     HIPCHK(hipMemset(a_atom.p, 0, (size_t)NA * n * sizeof(double)));
-    a_iatom.alloc(6 * (size_t)n + 8);
-    HIPCHK(hipMemset(a_iatom.p, 0, (6 * (size_t)n + 8) * sizeof(int)));
+    const size_t ni = (6 + kCenCount) * (size_t)n + 8;   // 6 count arrays, 8 flags, the census
+    a_iatom.alloc(ni);
+    HIPCHK(hipMemset(a_iatom.p, 0, ni * sizeof(int)));
     State& s = st;
     s.n = n;
     s.x = a_x.p; s.y = a_y.p; s.z = a_z.p; s.q = a_q.p; s.type = a_type.p; s.tag = a_tag.p; s.pos = a_pos.p;
@@ -419,6 +431,9 @@ struct Engine {
     s.eown = atom_arr(k + kTerms + 6); s.eacc = atom_arr(k + kTerms + 7); s.eref = atom_arr(k + kTerms + 8);
     s.nb_cnt = a_iatom.p; s.b_cnt = a_iatom.p + n; s.h_cnt = a_iatom.p + 2 * (size_t)n; s.bc_cnt = a_iatom.p + 3 * (size_t)n;
     s.hc_cnt = a_iatom.p + 4 * (size_t)n; s.sysid = a_iatom.p + 5 * (size_t)n; s.flags = a_iatom.p + 6 * (size_t)n;
+    s.cen = a_iatom.p + 6 * (size_t)n + 8;
+    s.census = 0;
+    census_valid = false;
     red.alloc(19 + 240 * 19 + 8);
     // binning scratch
     b_keys.alloc(n); b_skeys.alloc(n); b_akey.alloc(n); b_idx.alloc(n); b_sidx.alloc(n);
@@ -943,18 +958,25 @@ struct Engine {
   }
 
   bool peratom_valid = false;   // the last force call of rxa_compute left per-atom tallies
+  bool census_valid = false;    // the last force call of rxa_compute recorded the decision census
   void compute(int flags, double* energies, double* grad, double* virial) {
     if (!n) throw ApiError(RXA_ESTATE, "no atoms");
     st.peratom = (flags & RXA_COMPUTE_PERATOM) ? 1 : 0;
+    st.census = (flags & RXA_COMPUTE_CENSUS) ? 1 : 0;
     peratom_valid = false;
+    census_valid = false;
     try {
       forces((flags & RXA_COMPUTE_REBUILD) != 0);
+      if (st.census) launch_census_nb(st, g, stream);
     } catch (...) {
       st.peratom = 0;
+      st.census = 0;
       throw;
     }
     peratom_valid = st.peratom != 0;
+    census_valid = st.census != 0;
     st.peratom = 0;
+    st.census = 0;
     double out[kTerms + 6];
     HIPCHK(hipMemcpyAsync(out, red.p, sizeof(out), hipMemcpyDeviceToHost, stream));
     std::vector<double> gx, gy, gz;
@@ -996,20 +1018,29 @@ struct Engine {
     qb.cnt = q_int.p;
     qb.col = qb.narrow ? nullptr : q_int.p + n;
     if (qb.narrow) { q_u16.alloc((size_t)n * maxnb); qb.col16 = q_u16.p; } else { q_u16.release(); qb.col16 = nullptr; }
-    q_f32.alloc((size_t)n * maxnb);
-    qb.val32 = q_f32.p;
+    // FP32 values and their FP32 remainders (mixed / FP32), or FP64 values in the double pool (FP64)
+    if (qb.fp32) {
+      q_f32.alloc(2 * (size_t)n * maxnb);
+      qb.val32 = q_f32.p;
+      qb.lo32 = q_f32.p + (size_t)n * maxnb;
+    } else {
+      q_f32.release();
+      qb.val32 = nullptr;
+      qb.lo32 = nullptr;
+    }
+    q_fp32_alloc = qb.fp32;
     q_key.alloc((size_t)n * maxnb);
     qb.key = q_key.p;
     // val shares the double pool
     alloc_qeq_doubles();
   }
   void alloc_qeq_doubles() {
-    const int imax_cap = std::max(qeq_imax_cap, 2048);
     // sub-arrays start on 256-byte boundaries (the CG reads (s, t) pairs as 16-byte loads)
     auto rnd = [](size_t k) { return (k + 31) & ~(size_t)31; };
     const size_t nmv = 4 * (size_t)qeq_mv_partials(n);
-    const size_t sz = rnd((size_t)n * maxnb) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + rnd(nmv) +
-                      rnd(8 + 3 * (size_t)(imax_cap + 2)) + 2 * rnd(4 * (size_t)n);
+    const size_t nval = qb.fp32 ? 0 : (size_t)n * maxnb;   // FP64 values (FP64 mode only)
+    const size_t sz = (nval ? rnd(nval) : 0) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + 2 * rnd(nmv) +
+                      2 * rnd(4 * (size_t)n);
     std::vector<double> keep;
     const bool had_hist = q_dbl.p && hist_count > 0 && q_dbl_n == n;   // history of the same atoms only
     if (had_hist) {
@@ -1022,7 +1053,7 @@ struct Engine {
     HIPCHK(hipMemset(q_dbl.p, 0, sz * sizeof(double)));
     double* p = q_dbl.p;
     auto take = [&](size_t k) { double* r = p; p += rnd(k); return r; };
-    qb.val = take((size_t)n * maxnb);
+    qb.val = nval ? take(nval) : nullptr;
     qb.eta = take(n);
     qb.hinv = take(n);
     qb.chi = take(n);
@@ -1034,10 +1065,9 @@ struct Engine {
     qb.q = take(2 * (size_t)n);
     qb.partial = take(4 * kCgBlocks);
     qb.partial_mv = take(nmv);
-    qb.scal = take(8 + 3 * (size_t)(imax_cap + 2));
+    qb.partial_rs = take(nmv);
     qb.s_hist = take(4 * (size_t)n);
     qb.t_hist = take(4 * (size_t)n);
-    qeq_imax_cap = imax_cap;
     static_assert(2 * sizeof(int) <= sizeof(double), "");
     if (had_hist) {
       HIPCHK(hipMemcpy(qb.s_hist, keep.data(), 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
@@ -1092,6 +1122,11 @@ struct Engine {
   void qeq_matrix() {
     if (!qeq_ready) throw ApiError(RXA_ESTATE, "charge equilibration is not set up");
     if (!nbr_valid) build_neighbors();
+    if (q_fp32_alloc != qb.fp32) {   // the precision changed since the arrays were sized
+      sync();
+      alloc_qeq_matrix();
+      qeq_h_valid = false;
+    }
     if (!qeq_h_valid) {
       hipEvent_t t0 = timer.begin(PH_QEQ_H);
       qb.keyed = (2 * smx[0] + 1) * (2 * smx[1] + 1) * (2 * smx[2] + 1) <= 32768 ? 1 : 0;
@@ -1102,23 +1137,37 @@ struct Engine {
   }
 
   // runs CG iterations until both systems stopped; batch launches, one readback per batch
+  // The CG's per-iteration scalars (device), the packed systems' per-iteration flags (device) and their pinned host copies, all
+  // sized for iterations 0 .. cap + 1 (cap >= imax). Only these grow here: the matrix, the diagonal and the per-atom vectors
+  // are untouched, so a built matrix stays valid. Queued work may still read the old buffers or copy them into the old pinned
+  // memory, so the streams are drained before anything is freed.
   void ensure_cg_cap(int imax) {
-    if (imax + 2 > qeq_imax_cap) { qeq_imax_cap = imax + 2; alloc_qeq_doubles(); }
-    if (d_cg_any.n < (size_t)qeq_imax_cap + 2) { d_cg_any.alloc((size_t)qeq_imax_cap + 2); }
-    pq.any = d_cg_any.p;
-    const size_t need = 2 * (size_t)(std::max(qeq_imax_cap, imax + 2) + 2);
-    if (need > cg_cap_h) {
+    if (imax + 2 > qeq_imax_cap || !q_scal.p) {
+      sync();
+      const int cap = std::max(imax + 2, 2048);
+      q_scal.alloc(qeq_scal_doubles(cap));
+      HIPCHK(hipMemset(q_scal.p, 0, q_scal.n * sizeof(double)));
+      d_cg_any.alloc((size_t)cap + 2);
       if (cgf_h) HIPCHK(hipHostFree(cgf_h));
       if (cgs_h) HIPCHK(hipHostFree(cgs_h));
-      HIPCHK(hipHostMalloc((void**)&cgf_h, need * sizeof(int), hipHostMallocDefault));
-      HIPCHK(hipHostMalloc((void**)&cgs_h, (8 + 3 * need) * sizeof(double), hipHostMallocDefault));
-      cg_cap_h = need;
+      cgf_h = nullptr;
+      cgs_h = nullptr;
+      HIPCHK(hipHostMalloc((void**)&cgf_h, ((size_t)cap + 2) * sizeof(int), hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void**)&cgs_h, qeq_scal_doubles(cap) * sizeof(double), hipHostMallocDefault));
+      qeq_imax_cap = cap;
     }
+    qb.scal = q_scal.p;
+    pq.any = d_cg_any.p;
   }
   int cg_guess = 8;
   int cg_it = 0, cg_batch = 0;
-  // activity flags of iteration index k in the read-back scalars (the layout of QeqBuffers::scal)
-  const int* cg_act(int k) const { return reinterpret_cast<const int*>(cgs_h + 8 + 3 * (size_t)k + 2); }
+  // refinement: the last batch ended with update(cg_it) and a check of slot cg_it, so the next launch is the product of
+  // iteration cg_it + 1 in its checking form
+  bool cg_checked = false;
+  // activity flags, restart records and sig of iteration slot k in the read-back scalars (the layout of QeqBuffers::scal)
+  const int* cg_act(int k) const { return reinterpret_cast<const int*>(cgs_h + 8 + kScalSlot * (size_t)k + 2); }
+  const int* cg_rst(int k) const { return reinterpret_cast<const int*>(cgs_h + 8 + kScalSlot * (size_t)k + 3); }
+  double cg_sig(int k, int sys) const { return cgs_h[8 + kScalSlot * (size_t)k + sys]; }
   // launches the next batch of iterations, then the asynchronous readback of the flags and scalars (pinned) and its event
   void cg_launch_batch(int imax) {
     if (packed) {   // iteration it sets any[it]: some system active for the next one
@@ -1129,21 +1178,37 @@ struct Engine {
         ++cg_it;
         bonded_poll();
       }
+      if (qb.refine) {   // stopped solves: true residual, restart or verify (sets any[cg_it - 1] on a restart)
+        hipEvent_t t0 = timer.begin(PH_QEQ_CG);
+        launch_pk_qeq_restart(st, qb, pq, cg_it - 1, imax, cg_tol, stream);
+        timer.end(PH_QEQ_CG, t0);
+      }
       HIPCHK(hipMemcpyAsync(cgf_h, d_cg_any.p, (size_t)cg_it * sizeof(int), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipEventRecord(ev_cg, stream));
       return;
     }
     for (int b = 0; b < cg_batch && cg_it < imax; ++b) {
       hipEvent_t t0 = timer.begin(PH_QEQ_CG);
-      launch_qeq_update(st, qb, cg_it, stream);
+      if (!cg_checked) launch_qeq_update(st, qb, cg_it, stream);
       timer.end(PH_QEQ_CG, t0);
       ++cg_it;
       t0 = timer.begin(PH_QEQ_SPMV);
-      launch_qeq_mvdir(st, qb, cg_it, imax, cg_tol, stream);
+      if (cg_checked) launch_qeq_mvdir_chk(st, qb, cg_it, imax, cg_tol, stream);
+      else launch_qeq_mvdir(st, qb, cg_it, imax, cg_tol, stream);
       timer.end(PH_QEQ_SPMV, t0);
+      cg_checked = false;
       bonded_poll();
     }
-    HIPCHK(hipMemcpyAsync(cgs_h, qb.scal, (8 + 3 * (size_t)cg_it) * sizeof(double), hipMemcpyDeviceToHost, stream));
+    int slots = cg_it;
+    if (qb.refine) {   // close the batch with the update of iteration cg_it and a refinement check in slot cg_it
+      hipEvent_t t0 = timer.begin(PH_QEQ_CG);
+      if (!cg_checked) launch_qeq_update(st, qb, cg_it, stream);
+      launch_qeq_restart(st, qb, cg_it, imax, cg_tol, stream);
+      timer.end(PH_QEQ_CG, t0);
+      cg_checked = true;
+      slots = cg_it + 1;
+    }
+    HIPCHK(hipMemcpyAsync(cgs_h, qb.scal, (8 + kScalSlot * (size_t)slots) * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(ev_cg, stream));
   }
   double cg_tol = 1e-6;
@@ -1172,6 +1237,7 @@ struct Engine {
   void cg_start(int imax, double tol) {
     cg_tol = tol;
     cg_it = 1;
+    cg_checked = false;
     cg_batch = std::max(1, cg_guess);
     hipEvent_t t0 = timer.begin(PH_QEQ_SPMV);
     launch_qeq_mvdir(st, qb, 1, imax, tol, stream);
@@ -1181,16 +1247,22 @@ struct Engine {
   void cg_wait_end(int imax) {
     while (true) {
       HIPCHK(hipEventSynchronize(ev_cg));
-      const int* a = cg_act(cg_it - 1);
-      if ((!a[0] && !a[1]) || cg_it >= imax) break;
+      if (qb.refine) {   // done when the batch's check found both true residuals within tol (or the budget is spent)
+        const int* r = cg_rst(cg_it);
+        if ((r[0] == 0 && r[1] == 0) || cg_it >= imax) break;
+      } else {
+        const int* a = cg_act(cg_it - 1);
+        if ((!a[0] && !a[1]) || cg_it >= imax) break;
+      }
       cg_batch = 2;
       cg_launch_batch(imax);
     }
     for (int s = 0; s < 2; ++s) {
-      int k = 0;
-      while (k < cg_it && cg_act(k)[s]) ++k;
-      last_iters[s] = k + 1;   // the stock loop index at exit
-      last_rel[s] = std::sqrt(cgs_h[8 + 3 * (last_iters[s] - 1) + s]) / cgs_h[s];
+      int done = 0;   // iterations the system ran (over every restart)
+      for (int k = 0; k < cg_it; ++k) done += cg_act(k)[s] == 1;
+      last_iters[s] = done + 1;   // the stock loop index at exit
+      // the relative residual the solve stopped on: the true one of the last check under refinement, else the recursion's
+      last_rel[s] = std::sqrt(qb.refine ? cg_sig(cg_it, s) : cg_sig(last_iters[s] - 1, s)) / cgs_h[s];
     }
     cg_guess = std::max(last_iters[0], last_iters[1]) - 1;
   }
@@ -1659,17 +1731,24 @@ void rxa_pvector(const double* e, double* pv) {
 
 int rxa_census(rxa_context* c, int* out) {
   return guard(c, [&](Engine& e) {
+    if (!e.n) throw ApiError(RXA_ESTATE, "no atoms");
+    // the census of the last rxa_compute made with RXA_COMPUTE_CENSUS, else a force call at the current positions and charges
+    // that records it (the same energies and forces)
+    if (!e.census_valid) e.compute(RXA_COMPUTE_CENSUS, nullptr, nullptr, nullptr);
     const int n = e.n;
-    std::vector<int> b(n);
+    std::vector<int> b(n), cen((size_t)rxa::kCenCount * n);
     std::vector<double> de(n);
     e.sync();
     HIPCHK(hipMemcpy(b.data(), e.st.b_cnt, n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(de.data(), e.st.Delta_e, n * sizeof(double), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 7 * n; ++i) out[i] = -1;
+    HIPCHK(hipMemcpy(cen.data(), e.st.cen, cen.size() * sizeof(int), hipMemcpyDeviceToHost));
+    // reference order (census.hpp): bonds, angle_sets, torsions, hbonds, nonbonded, sbo_region, lp_trunc
+    const int from[7] = {-1, rxa::kCenAngles, rxa::kCenTorsions, rxa::kCenHbonds, rxa::kCenNonbonded, rxa::kCenSbo, -1};
     for (int i = 0; i < n; ++i) {
       const int u = e.perm_h[i];
-      out[u] = b[i];
-      out[6 * n + u] = (int)(de[i] / 2.0);
+      out[u] = std::min(b[i], e.maxb);
+      for (int k = 1; k < 6; ++k) out[(size_t)k * n + u] = cen[(size_t)from[k] * n + i];
+      out[6 * (size_t)n + u] = (int)(de[i] / 2.0);
     }
   });
 }
@@ -1760,6 +1839,15 @@ int rxa_qeq_charges(rxa_context* c, int imax, double tol, int strict, double* q_
     if (q_out) e.charges_out(q_out);
     if (iters) { iters[0] = e.last_iters[0]; iters[1] = e.last_iters[1]; }
     if (rel) { rel[0] = e.last_rel[0]; rel[1] = e.last_rel[1]; }
+  });
+}
+
+int rxa_qeq_refine(rxa_context* c, int mode) {
+  return guard(c, [&](Engine& e) {
+    if (mode < -1 || mode > 1) throw ApiError(RXA_EINVAL, "qeq refine mode must be -1, 0 or 1");
+    e.sync();
+    e.qeq_refine_mode = mode;
+    e.apply_refine();
   });
 }
 

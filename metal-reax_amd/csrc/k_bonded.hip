@@ -465,6 +465,11 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
 // ---- atom groups: G lanes per atom (G a power of two >= the largest bond count), lane p owns bond slot p of its atom ----------
 // Sums over a group use xor shuffles with offsets below G, so they stay inside the group and run in a fixed order.
 template <int G>
+__device__ inline int gsum_int(int v) {
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+template <int G>
 __device__ inline double gsum(double v) {
   for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -486,6 +491,9 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
   const int nb = live ? min(st.b_cnt[j], st.maxb) : 0;
   const int nbw = wave_max(nb);
   const bool act = tj >= 0 && nb >= 2;
+  // the census also wants the SBO branch of atoms with fewer than two bonds (no angle of theirs is evaluated)
+  const bool sbo_on = act || (st.census && tj >= 0);
+  int n_ang = 0, sbo_region = 0;   // census: bond pairs through the three-body gates, SBO branch
   double e_ang = 0, e_pen = 0, e_coa = 0, cdself = 0, fjx = 0, fjy = 0, fjz = 0, sCE5 = 0, sCE6 = 0;
   double own_cdbo = 0, own_cdp = 0, own_fx = 0, own_fy = 0, own_fz = 0;
   // centre quantities (every lane of the group computes them)
@@ -494,7 +502,7 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
   const TR p_val6 = (TR)g.p_val6, p_val8 = (TR)g.p_val8, p_val9 = (TR)g.p_val9, p_val10 = (TR)g.p_val10;
   const TR p_pen2 = (TR)g.p_pen2, p_pen3 = (TR)g.p_pen3, p_pen4 = (TR)g.p_pen4;
   const TR p_coa2 = (TR)g.p_coa2, p_coa3 = (TR)g.p_coa3, p_coa4 = (TR)g.p_coa4;
-  if (act) {
+  if (sbo_on) {
     const DevSpecies& sj = t.sp[tj];
     p_val3 = (TR)sj.p_val3; p_val5 = (TR)sj.p_val5;
     TR SBOp = 0, prod = 1;
@@ -514,10 +522,10 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
     Dboc = (TR)st.Delta_boc[j];
     const TR SBO = SBOp + (TR(1) - prod) * (-Dboc - p_val8 * vlpadj);
     dSBO1 = -TR(8) * prod * (Dboc + p_val8 * vlpadj);
-    if (SBO <= TR(0)) { SBO2 = 0; CSBO2 = 0; }
-    else if (SBO <= TR(1)) { SBO2 = rpow<TR>(SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(SBO, p_val9 - TR(1)); }
-    else if (SBO < TR(2)) { SBO2 = TR(2) - rpow<TR>(TR(2) - SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(TR(2) - SBO, p_val9 - TR(1)); }
-    else { SBO2 = 2; CSBO2 = 0; }
+    if (SBO <= TR(0)) { SBO2 = 0; CSBO2 = 0; sbo_region = 0; }
+    else if (SBO <= TR(1)) { SBO2 = rpow<TR>(SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(SBO, p_val9 - TR(1)); sbo_region = 1; }
+    else if (SBO < TR(2)) { SBO2 = TR(2) - rpow<TR>(TR(2) - SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(TR(2) - SBO, p_val9 - TR(1)); sbo_region = 2; }
+    else { SBO2 = 2; CSBO2 = 0; sbo_region = 3; }
     ev6 = safe_exp<TR>(p_val6 * Dboc);
     const TR Delta = (TR)st.Delta[j], Dval = (TR)st.Delta_val[j];
     const TR ep3 = safe_exp<TR>(-p_pen3 * Delta), ep4 = safe_exp<TR>(p_pen4 * Delta), itrm = TR(1) / (TR(1) + ep3 + ep4);
@@ -548,7 +556,9 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
       const TR BOA_jk = BOjk - thb_cut;
       const int k = st.b_j[qk], tk = st.type[k];
       const int2 ai = t.angle_idx[(ti * g.ntypes + tj) * g.ntypes + tk];
-      if (BOA_jk > TR(0) && BOij > thb_cut && BOjk > thb_cut && BOij * BOjk > thb_cutsq && ai.y != 0) {
+      const bool gate = BOA_jk > TR(0) && BOij > thb_cut && BOjk > thb_cut && BOij * BOjk > thb_cutsq;
+      n_ang += gate ? 1 : 0;
+      if (gate && ai.y != 0) {
         const V3<TR> vjk{(TR)st.b_dx[qk], (TR)st.b_dy[qk], (TR)st.b_dz[qk]};
         const TR djk = (TR)st.b_d[qk];
         TR th, ct, sth;
@@ -626,6 +636,10 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
     st.eatom[T_PEN * n + j] = e_pen;
     st.eatom[T_COA * n + j] = e_coa;
   }
+  if (st.census) {
+    n_ang = gsum_int<G>(n_ang);
+    if (live && p == 0) { st.cen[kCenAngles * n + j] = n_ang; st.cen[kCenSbo * n + j] = sbo_region; }
+  }
 }
 
 template <class TR>
@@ -641,7 +655,7 @@ __device__ inline TR tan_inv(TR s, TR c) {
 // its own half (its atom, its outer partner i and outer bond (j, i)); the end the reference counts (canonical_first) also adds
 // the energy and the central-bond coefficients. The outer-bond contributions of an i are summed over the group and kept by the
 // lane owning slot i.
-template <class TR, int G>
+template <class TR, int G, bool kCen>
 __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = gid / G, pk = gid % G, n = st.n;
@@ -685,6 +699,7 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
     dfn11 = ratio2_deriv<TR>(et3, et4, -p_tor3, p_tor4, it34);
   }
   double cdbo_jk = 0, cdbopi_jk = 0;
+  int n_tor = 0;   // census: torsions through the four-body gates that this lane's central bond counts
   for (int pi = 0; pi < nbw; ++pi) {
     TR cdbo_ij = 0, fix = 0, fiy = 0, fiz = 0;   // this lane's terms of partner i: at most one per bond of k
     if (cen && pi != pk && pi < nbj) {
@@ -709,6 +724,7 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
           const int ti4 = t.tors_idx[((ti * g.ntypes + tj) * g.ntypes + tk) * g.ntypes + tl];
           if (ti4 < 0) continue;
           if (!(BOij * BOjk * BOkl > thb_cut)) continue;
+          if (kCen) n_tor += counted ? 1 : 0;
           const DevTorsion P = t.tors[ti4];
           const TR rkl = (TR)st.b_d[ql], BOA_kl = BOkl - thb_cut;
           const V3<TR> dkl{(TR)st.b_dx[ql], (TR)st.b_dy[ql], (TR)st.b_dz[ql]};
@@ -814,16 +830,21 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
     st.eatom[T_CON * n + j] = e_con;
     if (st.peratom) st.eown[j] += 0.5 * (e_tor + e_con);
   }
+  if (kCen) {
+    n_tor = gsum_int<G>(n_tor);
+    if (live && pk == 0) st.cen[kCenTorsions * n + j] = n_tor;
+  }
 }
 
-// FP32 terms at 3 waves per SIMD (168 VGPRs, no spills); FP64 at the compiler's choice
-template <int G>
+// FP32 terms at 3 waves per SIMD (168 VGPRs); FP64 at the compiler's choice. kCen: the variant that also records the torsion
+// census (launched only for a force call that asks for it, so the counter costs the ordinary step nothing)
+template <int G, bool kCen>
 __global__ void __attribute__((amdgpu_waves_per_eu(3))) __launch_bounds__(256) k_torsion_f(State st, Globals g, Tables t) {
-  torsion_body<float, G>(st, g, t);
+  torsion_body<float, G, kCen>(st, g, t);
 }
-template <int G>
+template <int G, bool kCen>
 __global__ void __launch_bounds__(256) k_torsion_d(State st, Globals g, Tables t) {
-  torsion_body<double, G>(st, g, t);
+  torsion_body<double, G, kCen>(st, g, t);
 }
 
 // ================================================================================================================
@@ -875,7 +896,11 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
   const int lane = lane_id(), n = st.n, tj = st.type[j];
   const int role = (tj >= 0 && g.hbond_cut > 0) ? t.sp[tj].p_hbond : 0;
   if (role != 1) {
-    if (lane == 0) { st.eatom[T_HB * n + j] = 0.0; st.h_cnt[j] = 0; }
+    if (lane == 0) {
+      st.eatom[T_HB * n + j] = 0.0;
+      st.h_cnt[j] = 0;
+      if (st.census) st.cen[kCenHbonds * n + j] = 0;
+    }
     return;
   }
   int* rk = rk_[wv]; int* rh = rh_[wv]; int* rt = rt_[wv];
@@ -889,6 +914,7 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
   double e_hb = 0, fjx = 0, fjy = 0, fjz = 0;
   double w[6] = {0, 0, 0, 0, 0, 0};   // virial of the triples about the donor
   int head = 0, fill = 0, inrange = 0;
+  int n_hb = 0;   // census: triples evaluated (bond BO >= 0.01 to an acceptor type, parameters set, i and k not one atom)
   // the bonds (j, i) that can carry an H-bond (acceptor-type partner, BO >= 0.01), tested by the lanes at once (nb <= 64)
   unsigned long long hbonds;
   {
@@ -923,6 +949,7 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
       if (has && tagk != st.tag[i]) {
         const DevHBond P = t.hb[(ti * g.ntypes + tj) * g.ntypes + tk];
         if (P.r0 > 0.0) {
+          ++n_hb;
           const TR e2 = rexp<TR>(-(TR)P.p2 * (TR)st.bo[q]);
           TR e, CH1, CH2, CH3;
           V3<TR> dti, dtj, dtk;
@@ -994,6 +1021,12 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
     for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + j] += w[c];
     st.eatom[T_HB * n + j] = e_hb;
     st.h_cnt[j] = inrange;
+  }
+  if (st.census) {
+    n_hb = wave_sum_int(n_hb);
+    if (lane == 0) st.cen[kCenHbonds * n + j] = n_hb;
+  }
+  if (lane == 0) {
     if (st.peratom) st.eown[j] += e_hb / 3.0;
   }
 }
@@ -1113,8 +1146,13 @@ template <class TR, int G>
 void launch_terms_g(const State& st, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * G, 256)), blk(256), wgrid(nblk(st.n, 4));
   hipLaunchKernelGGL((k_valence<TR, G>), grid, blk, 0, s, st, g, t);
-  if (sizeof(TR) == 4) hipLaunchKernelGGL((k_torsion_f<G>), grid, blk, 0, s, st, g, t);
-  else hipLaunchKernelGGL((k_torsion_d<G>), grid, blk, 0, s, st, g, t);
+  if (sizeof(TR) == 4) {
+    if (st.census) hipLaunchKernelGGL((k_torsion_f<G, true>), grid, blk, 0, s, st, g, t);
+    else hipLaunchKernelGGL((k_torsion_f<G, false>), grid, blk, 0, s, st, g, t);
+  } else {
+    if (st.census) hipLaunchKernelGGL((k_torsion_d<G, true>), grid, blk, 0, s, st, g, t);
+    else hipLaunchKernelGGL((k_torsion_d<G, false>), grid, blk, 0, s, st, g, t);
+  }
   hipLaunchKernelGGL(k_hbond_donor<TR>, dim3(nblk(st.n, kHbWaves)), dim3(64 * kHbWaves), 0, s, st, g, t);
   hipLaunchKernelGGL(k_hbond_acceptor<TR>, wgrid, blk, 0, s, st, g, t);
 }

@@ -388,7 +388,14 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
       const double h = (Tap * 14.4) * rcbrt_f64(r * r * r + shld[kt >> 16]);
       const size_t o = base + m + lane;
       if (qb.narrow) qb.col16[o] = (unsigned short)j; else qb.col[o] = j;
-      if (qb.fp32) qb.val32[o] = (float)h; else qb.val[o] = h;
+      if (qb.fp32) {
+        const float hf = (float)h;
+        qb.val32[o] = hf;
+        // h - hf is exact in FP64; rounded to FP32 it leaves hf + lo within about 2^-48 (relative) of h
+        if (qb.refine) qb.lo32[o] = (float)(h - (double)hf);
+      } else {
+        qb.val[o] = h;
+      }
       if (qb.keyed) qb.key[o] = (unsigned short)(kt & 0xFFFF);
     }
     m += count;
@@ -443,9 +450,39 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
   }
 }
 
+// Census of the non-bonded cut-off decision: per atom, the pairs its skin list holds at the counted (canonical) end within
+// nonb_cut. The test is the one every non-bonded kernel takes (the FP64 square root of the canonical squared distance <=
+// nonb_cut, in all precisions), on the same list entries, so the count is the number of pairs those kernels evaluate there.
+__global__ void __launch_bounds__(256) k_census_nb(State st, Globals g) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  const int lane = lane_id();
+  const int ti = st.type[i];
+  int c = 0;
+  if (ti >= 0) {
+    const size_t base = (size_t)i * st.maxnb;
+    const int cnt = min(st.nb_cnt[i], st.maxnb);
+    const double4 pi = st.pos[i];
+    for (int k = lane; k < cnt; k += 64) {
+      const int e = st.nb_s[base + k];
+      if (!(e & kCanon)) continue;
+      double dx, dy, dz;
+      int tj;
+      const double r2 = pair_dvec_list(st, pi, st.nb_j[base + k], e, dx, dy, dz, tj);
+      if (tj >= 0 && sqrt(r2) <= g.nonb_cut) ++c;
+    }
+  }
+  c = wave_sum_int(c);
+  if (lane == 0) st.cen[kCenNonbonded * st.n + i] = c;
+}
+
 inline int nblk(long long items, int per) { return (int)((items + per - 1) / per); }
 
 }  // namespace
+
+void launch_census_nb(const State& st, const Globals& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_census_nb, dim3(nblk((long long)st.n * 64, 256)), dim3(256), 0, s, st, g);
+}
 
 void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * 64, 256)), blk(256);

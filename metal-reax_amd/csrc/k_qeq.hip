@@ -4,10 +4,18 @@
 // stops exactly where the stock loop would (i < imax && sqrt(r.M^-1 r)/|b| > tol), so iteration counts are those of two separate
 // stock solves. Vectors are interleaved per atom ([2 i + system]).
 //
+// FP64 iterative refinement (qb.refine: mixed mode, and FP32 mode with the fix's 'resident' keyword; the reference's resident
+// solve, src/metal/metal_backend.mm:435-530): the CG itself multiplies with the FP32 matrix values, but its initial residual and
+// a verification residual after the iterations stop are formed with the two-float matrix hi + lo (about 48 bits). A system
+// whose true residual still fails the stock condition restarts from it (beta 0), and its iterations count on, bounded by imax
+// as one budget, so the charges solve the FP64 matrix to the requested tolerance.
+//
 // Determinism: all dot products are reduced in a fixed order (per-thread grid-stride partials, a fixed per-block order, and the
 // per-block partials reduced again in the same fixed order by every block that needs the scalar), so every block computes the same
 // alpha/beta bitwise and results repeat run to run. Scalars of iteration `it` live in slot `it` of device arrays, so the blocks of one
 // launch never read a value another block of the same launch writes.
+#include <type_traits>
+
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -19,10 +27,12 @@ constexpr int kUpdBlocks = 240, kUpdThreads = 256;   // CG update kernel
 constexpr int kMvThreads = 256;             // product kernels: 4 waves per block,
 constexpr int kMvRowsPerBlock = kMvThreads / 16;   // 16 lanes per row (kRowLanes below)
 inline __host__ __device__ int mv_blocks(int n) { return (n + kMvRowsPerBlock - 1) / kMvRowsPerBlock; }
-// scal layout: [0,1] |b| of s and t, [2,3] u terms, then three doubles per iteration index k: sig of the two systems at 8 + 3 k
-// + sys, and the two activity flags (ints) in 8 + 3 k + 2, so one copy of the first 8 + 3 k doubles brings both to the host
-__device__ inline double& SIG(double* scal, int k, int sys) { return scal[8 + 3 * k + sys]; }
-__device__ inline int* ACT(double* scal, int k) { return reinterpret_cast<int*>(scal + 8 + 3 * k + 2); }
+// scal layout: [0,1] |b| of s and t, [2,3] u terms, then kScalSlot doubles per iteration index k: sig of the two systems at
+// 8 + 4 k + sys, the two activity flags (ints) in 8 + 4 k + 2 and the two restart records of a refinement check (ints: -1 no
+// check ran, else the restart flags) in 8 + 4 k + 3, so one copy of the first 8 + 4 k doubles brings them all to the host
+__device__ inline double& SIG(double* scal, int k, int sys) { return scal[8 + kScalSlot * k + sys]; }
+__device__ inline int* ACT(double* scal, int k) { return reinterpret_cast<int*>(scal + 8 + kScalSlot * k + 2); }
+__device__ inline int* RST(double* scal, int k) { return reinterpret_cast<int*>(scal + 8 + kScalSlot * k + 3); }
 
 template <int K, int NT = kThreads>
 __device__ inline void block_reduce(double (&v)[K], double* sh) {
@@ -75,11 +85,27 @@ __global__ void __launch_bounds__(kThreads) k_cg_guess(int n, QeqBuffers qb, int
 constexpr int kRowLanes = 16;
 constexpr int kInFlight = 8;
 
+// Matrix value accessors: FP64 values, FP32 values, or the two-float pair hi + lo (FP64 to about 48 bits) of the refinement's
+// residuals. Products and sums are FP64 in every case.
+struct ValD {
+  const double* v;
+  __device__ double operator()(size_t k) const { return v[k]; }
+};
+struct ValF {
+  const float* v;
+  __device__ double operator()(size_t k) const { return (double)v[k]; }
+};
+struct ValDS {
+  const float* hi;
+  const float* lo;
+  __device__ double operator()(size_t k) const { return (double)hi[k] + (double)lo[k]; }
+};
+
 // (H v)_row for both systems (v interleaved), off-diagonal part, summed over the row's lane group (all its lanes get the sum);
-// rows >= n give zero. Values FP64 or FP32 (TV), columns 32 or 16 bits (TC); products and sums FP64.
-template <class TV, class TC>
+// rows >= n give zero. Columns 32 or 16 bits (TC).
+template <class VA, class TC>
 __device__ inline double2 row_product(int row, int n, int maxnb, const int* __restrict__ cnt, const TC* __restrict__ col,
-                                      const TV* __restrict__ val, const double2* __restrict__ v2) {
+                                      const VA val, const double2* __restrict__ v2) {
   const int li = lane_id() % kRowLanes;
   double y0 = 0, y1 = 0;
   if (row < n) {
@@ -91,14 +117,14 @@ __device__ inline double2 row_product(int row, int n, int maxnb, const int* __re
       double h[kInFlight];
       double2 x[kInFlight];
 #pragma unroll
-      for (int u = 0; u < kInFlight; ++u) { j[u] = col[base + k + u * kRowLanes]; h[u] = (double)val[base + k + u * kRowLanes]; }
+      for (int u = 0; u < kInFlight; ++u) { j[u] = col[base + k + u * kRowLanes]; h[u] = val(base + k + u * kRowLanes); }
 #pragma unroll
       for (int u = 0; u < kInFlight; ++u) x[u] = v2[j[u]];
 #pragma unroll
       for (int u = 0; u < kInFlight; ++u) { y0 += h[u] * x[u].x; y1 += h[u] * x[u].y; }
     }
     for (; k < c; k += kRowLanes) {
-      const double h = (double)val[base + k];
+      const double h = val(base + k);
       const double2 x = v2[col[base + k]];
       y0 += h * x.x; y1 += h * x.y;
     }
@@ -118,22 +144,28 @@ inline __host__ __device__ int mv_grid(int n) { return (mv_blocks(n) + kXcds - 1
 __device__ inline int mv_vblock() { return (int)((blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds); }
 __device__ inline int product_row() { return (int)((mv_vblock() * blockDim.x + threadIdx.x) / kRowLanes); }
 
-// w = (diag(eta) + H) v for both systems (the initial residual's product). With kInit the CG start is fused in: the row's
-// owner writes r = b - w and p = r / eta instead of w, and each block leaves the partials of b.b and r.p of its rows.
-template <class TV, class TC, bool kInit>
+// w = (diag(eta) + H) v for both systems. kMode 0 writes w. kMode 1 (CG start) and 2 (refinement check) fuse the start in: the
+// row's owner writes r = b - w and p = r / eta instead of w, and each block leaves the partials of b.b and r.p of its rows
+// (1: partial_mv, 2: partial_rs). kMode 2 runs only when neither system is active in iteration `it` (activity slot it - 1): the
+// check waits until both systems have stopped and otherwise leaves the iteration's vectors alone.
+template <class VA, class TC, int kMode>
 __global__ void __launch_bounds__(kMvThreads) k_spmv2(int n, int maxnb, const int* __restrict__ cnt, const TC* __restrict__ col,
-                                                     const TV* __restrict__ val, const double* __restrict__ eta, const double* v,
-                                                     double* w, QeqBuffers qb) {
+                                                     const VA val, const double* __restrict__ eta, const double* v, double* w,
+                                                     QeqBuffers qb, int it) {
+  if (kMode == 2) {
+    const int* a = ACT(qb.scal, it - 1);
+    if (a[0] != 0 || a[1] != 0) return;   // uniform over the grid
+  }
   const double2* __restrict__ v2 = reinterpret_cast<const double2*>(v);
   double2* w2 = reinterpret_cast<double2*>(w);
   const int i = product_row();
-  const double2 y = row_product<TV, TC>(i, n, maxnb, cnt, col, val, v2);
+  const double2 y = row_product<VA, TC>(i, n, maxnb, cnt, col, val, v2);
   double acc[4] = {0, 0, 0, 0};
   if (i < n && lane_id() % kRowLanes == 0) {
     const double2 xi = v2[i];
     const double ei = eta[i];
     const double2 wi = make_double2(y.x + ei * xi.x, y.y + ei * xi.y);
-    if (!kInit) {
+    if (kMode == 0) {
       w2[i] = wi;
     } else {
       const double hi = qb.hinv[i];
@@ -149,12 +181,13 @@ __global__ void __launch_bounds__(kMvThreads) k_spmv2(int n, int maxnb, const in
       }
     }
   }
-  if (kInit) {
+  if (kMode != 0) {
     __shared__ double sh[4 * kMvThreads / 64];
     block_reduce<4, kMvThreads>(acc, sh);
-    if (threadIdx.x == 0)
-      if (mv_vblock() < mv_blocks(n))
-        for (int k = 0; k < 4; ++k) qb.partial_mv[mv_vblock() * 4 + k] = acc[k];
+    if (threadIdx.x == 0 && mv_vblock() < mv_blocks(n)) {
+      double* part = kMode == 1 ? qb.partial_mv : qb.partial_rs;
+      for (int k = 0; k < 4; ++k) part[mv_vblock() * 4 + k] = acc[k];
+    }
   }
 }
 
@@ -164,14 +197,22 @@ __global__ void __launch_bounds__(kMvThreads) k_spmv2(int n, int maxnb, const in
 //   then for every row i: w = (diag(eta) + H) p, d_i = p_i + beta d_i, q_i = w_i + beta q_i (= (diag(eta) + H) d_i by
 //   linearity, so the direction never has to be complete before the product), and per-block partials of d.q.
 // Block 0 records sig (slot it-1) and the activity flags of iteration it for the later kernels and the host.
-template <class TV, class TC>
-__global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const TC* __restrict__ col, const TV* __restrict__ val, QeqBuffers qb,
-                                                        int it, int imax, double tol) {
+// chk = 1: a refinement check ran before this launch (after update(it - 1)); if it restarted the solve (its record in slot
+// it - 1 is not the marker -1), iteration it starts afresh from the true residual (beta 0, activity from the record), else the
+// iteration continues as usual.
+template <class VA, class TC>
+__global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const TC* __restrict__ col, const VA val, QeqBuffers qb,
+                                                        int it, int imax, double tol, int chk) {
   __shared__ double sh[2 * kMvThreads / 64];
   int act[2];
   double beta[2] = {0.0, 0.0};
-  if (it == 1) {
+  bool fresh = it == 1;
+  if (fresh) {
     act[0] = ACT(qb.scal, 0)[0]; act[1] = ACT(qb.scal, 0)[1];
+  } else if (chk && RST(qb.scal, it - 1)[0] >= 0) {
+    fresh = true;
+    act[0] = RST(qb.scal, it - 1)[0]; act[1] = RST(qb.scal, it - 1)[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) { ACT(qb.scal, it - 1)[0] = act[0]; ACT(qb.scal, it - 1)[1] = act[1]; }
   } else {
     const int* pa = ACT(qb.scal, it - 2);
     const int pact[2] = {pa[0], pa[1]};
@@ -195,7 +236,7 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
   if (!act[0] && !act[1]) return;
   const double2* __restrict__ p2 = reinterpret_cast<const double2*>(qb.p);
   const int i = product_row();
-  const double2 y = row_product<TV, TC>(i, n, maxnb, qb.cnt, col, val, p2);
+  const double2 y = row_product<VA, TC>(i, n, maxnb, qb.cnt, col, val, p2);
   double dq[2] = {0.0, 0.0};
   if (i < n && lane_id() % kRowLanes == 0) {
     const double2 pi = p2[i];
@@ -204,8 +245,8 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
     for (int s = 0; s < 2; ++s) {
       if (!act[s]) continue;
       const int k = 2 * i + s;
-      const double d = it == 1 ? pv[s] : pv[s] + beta[s] * qb.d[k];
-      const double q = it == 1 ? w[s] : w[s] + beta[s] * qb.q[k];
+      const double d = fresh ? pv[s] : pv[s] + beta[s] * qb.d[k];
+      const double q = fresh ? w[s] : w[s] + beta[s] * qb.q[k];
       qb.d[k] = d;
       qb.q[k] = q;
       dq[s] = d * q;
@@ -216,19 +257,18 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
 }
 
 // single-vector matvec (the fix's sparse_matvec), owned rows
-template <class TV, class TC>
-__global__ void __launch_bounds__(kThreads) k_spmv1(int n, int maxnb, const int* cnt, const TC* col, const TV* val, const double* eta,
+template <class VA, class TC>
+__global__ void __launch_bounds__(kThreads) k_spmv1(int n, int maxnb, const int* cnt, const TC* col, const VA val, const double* eta,
                                                    const double* v, double* w) {
   const int lane = lane_id();
   const int i = (blockIdx.x * kThreads + threadIdx.x) >> 6;
   if (i >= n) return;
   const size_t base = (size_t)i * maxnb;
   double y = 0;
-  for (int k = lane; k < cnt[i]; k += 64) y += (double)val[base + k] * v[col[base + k]];
+  for (int k = lane; k < cnt[i]; k += 64) y += val(base + k) * v[col[base + k]];
   y = wave_sum(y);
   if (lane == 0) w[i] = y + eta[i] * v[i];
 }
-
 
 __global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, int nparts, int imax, double tol) {
   __shared__ double sh[4 * kThreads / 64];
@@ -242,6 +282,27 @@ __global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, int npa
       ACT(qb.scal, 0)[s] = (1 < imax && sqrt(v[2 + s]) / bn > tol) ? 1 : 0;
     }
   }
+}
+
+// End of a refinement check (after update(it)): if either system was active in iteration it, the marker -1 (the iteration
+// continues); else the true residual's r.p gives sig of slot it and each system's restart flag for iteration it + 1 (the stock
+// condition with the true residual: it + 1 < imax and sqrt(r.M^-1 r)/|b| > tol).
+__global__ void __launch_bounds__(kThreads) k_cg_check_fin(QeqBuffers qb, int nparts, int it, int imax, double tol) {
+  __shared__ double sh[4 * kThreads / 64];
+  const int* pa = ACT(qb.scal, it - 1);
+  int* rs = RST(qb.scal, it);
+  if (pa[0] != 0 || pa[1] != 0) {
+    if (threadIdx.x == 0) { rs[0] = -1; rs[1] = -1; }
+    return;
+  }
+  double v[4];
+  reduce_partials<4>(qb.partial_rs, v, sh, nparts);
+  if (threadIdx.x == 0)
+    for (int s = 0; s < 2; ++s) {
+      const double bn = sqrt(v[s]);
+      SIG(qb.scal, it, s) = v[2 + s];
+      rs[s] = (it + 1 < imax && sqrt(v[2 + s]) / bn > tol) ? 1 : 0;
+    }
 }
 
 // x += alpha d, r -= alpha q, p = r / eta, partial r.p   (iteration it >= 1); few large blocks, since every block first
@@ -306,7 +367,8 @@ __global__ void __launch_bounds__(kThreads) k_cg_charges(int n, QeqBuffers qb, d
 // The same two CG systems (s, t) per packed system, with that system's own scalars: one wave per system sums its rows' values
 // (lane = row offset within the system, then a fixed butterfly), so a system's solve does not depend on the systems beside it.
 // pk[10 s + 4 k + {0, 1, 2, 3}] = |b|, r.p, alpha, beta of solve k; pk[10 s + 8, 9] = the sums of s and t for the charges;
-// pki[4 s + 2 k + {0, 1}] = active, the stock loop index. rowv[4 i + {0, 1}] = d.q (b.b at the start), rowv[4 i + {2, 3}] = r.p.
+// pki[4 s + 2 k + {0, 1}] = state, the stock loop index. State 1: active; 0: stopped; 2: stopped and its true residual verified
+// by the refinement check (only with refinement). rowv[4 i + {0, 1}] = d.q (b.b at the start), rowv[4 i + {2, 3}] = r.p.
 __global__ void k_pk_cg_init(int n, QeqBuffers qb, double* rowv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -321,8 +383,9 @@ __global__ void k_pk_cg_init(int n, QeqBuffers qb, double* rowv) {
   }
 }
 
-// kind 0: start (|b|, r.p, activity for iteration 1); 1: alpha of iteration it; 2: r.p, beta and activity after iteration it;
-// 3: sums of s and t
+// kind 0: start (|b|, r.p, state for iteration 1); 1: alpha of iteration it; 2: r.p, beta and state after iteration it;
+// 3: sums of s and t; 4: refinement check after iteration it (stopped solves: the true residual's r.p restarts the solve with
+// beta 0 while it exceeds tol, else the solve is verified)
 __global__ void __launch_bounds__(256) k_pk_cg_seg(int kind, int S, const int* sys_start, const double* rowv, const double* x, double* pk,
                                                   int* pki, int it, int imax, double tol, int* any) {
   const int sy = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -349,31 +412,42 @@ __global__ void __launch_bounds__(256) k_pk_cg_seg(int kind, int S, const int* s
       a[1] = 1;
       if (a[0]) atomicOr(any, 1);
     } else if (kind == 1) {
-      if (a[0]) q[2] = q[1] / v[k];
-    } else if (a[0]) {
-      q[3] = v[2 + k] / q[1];
+      if (a[0] == 1) q[2] = q[1] / v[k];
+    } else if (kind == 2) {
+      if (a[0] == 1) {
+        q[3] = v[2 + k] / q[1];
+        q[1] = v[2 + k];
+        a[1] += 1;
+        a[0] = (a[1] < imax && sqrt(v[2 + k]) / q[0] > tol) ? 1 : 0;
+        if (a[0]) atomicOr(any + it, 1);
+      }
+    } else if (a[0] == 0) {   // kind 4
       q[1] = v[2 + k];
-      a[1] = it + 1;
-      a[0] = (it + 1 < imax && sqrt(v[2 + k]) / q[0] > tol) ? 1 : 0;
-      if (a[0]) atomicOr(any + it, 1);
+      if (a[1] < imax && sqrt(v[2 + k]) / q[0] > tol) {
+        a[0] = 1;
+        q[3] = 0.0;
+        atomicOr(any + it, 1);
+      } else {
+        a[0] = 2;
+      }
     }
   }
 }
 
 // w = (diag(eta) + H) p, d = p + beta d, q = w + beta q and d.q per row of the active solves (iteration it)
-template <class TV, class TC>
-__global__ void __launch_bounds__(kMvThreads) k_pk_cg_mvdir(int n, int maxnb, const TC* __restrict__ col, const TV* __restrict__ val, QeqBuffers qb,
+template <class VA, class TC>
+__global__ void __launch_bounds__(kMvThreads) k_pk_cg_mvdir(int n, int maxnb, const TC* __restrict__ col, const VA val, QeqBuffers qb,
                                                            const int* sysid, const double* pk, const int* pki, double* rowv, int it) {
   const int i = product_row();
   int act[2] = {0, 0}, sy = 0;
   if (i < n) {
     sy = sysid[i];
-    act[0] = pki[4 * sy];
-    act[1] = pki[4 * sy + 2];
+    act[0] = pki[4 * sy] == 1;
+    act[1] = pki[4 * sy + 2] == 1;
   }
   if (!act[0] && !act[1]) return;   // uniform over the row's lane group
   const double2* __restrict__ p2 = reinterpret_cast<const double2*>(qb.p);
-  const double2 y = row_product<TV, TC>(i, n, maxnb, qb.cnt, col, val, p2);
+  const double2 y = row_product<VA, TC>(i, n, maxnb, qb.cnt, col, val, p2);
   if (lane_id() % kRowLanes != 0) return;
   const double2 pi = p2[i];
   const double ei = qb.eta[i];
@@ -390,6 +464,36 @@ __global__ void __launch_bounds__(kMvThreads) k_pk_cg_mvdir(int n, int maxnb, co
   }
 }
 
+// refinement check, rows of the systems with a stopped solve: the true residual r = b - (diag(eta) + H) x with the two-float
+// matrix, p = r / eta and r.p for those solves (the rows of the other solves are left alone)
+template <class TC>
+__global__ void __launch_bounds__(kMvThreads) k_pk_cg_check(int n, int maxnb, const TC* __restrict__ col, const ValDS val, QeqBuffers qb,
+                                                           const int* sysid, const int* pki, double* rowv) {
+  const int i = product_row();
+  int chk[2] = {0, 0};
+  if (i < n) {
+    const int sy = sysid[i];
+    chk[0] = pki[4 * sy] == 0;
+    chk[1] = pki[4 * sy + 2] == 0;
+  }
+  if (!chk[0] && !chk[1]) return;   // uniform over the row's lane group
+  const double2* __restrict__ x2 = reinterpret_cast<const double2*>(qb.x);
+  const double2 y = row_product<ValDS, TC>(i, n, maxnb, qb.cnt, col, val, x2);
+  if (lane_id() % kRowLanes != 0) return;
+  const double2 xi = x2[i];
+  const double ei = qb.eta[i], hi = qb.hinv[i];
+  const double w[2] = {y.x + ei * xi.x, y.y + ei * xi.y};
+  for (int s = 0; s < 2; ++s) {
+    if (!chk[s]) continue;
+    const int k = 2 * i + s;
+    const double r = qb.b[k] - w[s];
+    const double p = r * hi;
+    qb.r[k] = r;
+    qb.p[k] = p;
+    rowv[4 * i + 2 + s] = r * p;
+  }
+}
+
 // x += alpha d, r -= alpha q, p = r / eta and r.p per row of the active solves
 __global__ void k_pk_cg_update(int n, QeqBuffers qb, const int* sysid, const double* pk, const int* pki, double* rowv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -397,7 +501,7 @@ __global__ void k_pk_cg_update(int n, QeqBuffers qb, const int* sysid, const dou
   const int sy = sysid[i];
   const double hi = qb.hinv[i];
   for (int s = 0; s < 2; ++s) {
-    if (!pki[4 * sy + 2 * s]) continue;
+    if (pki[4 * sy + 2 * s] != 1) continue;
     const int k = 2 * i + s;
     const double alpha = pk[10 * sy + 4 * s + 2];
     qb.x[k] += alpha * qb.d[k];
@@ -424,50 +528,68 @@ __global__ void k_pk_cg_charges(int n, QeqBuffers qb, const int* sysid, const do
   qb.t_hist[i] = t;
 }
 
-}  // namespace
+inline int pk_blocks(long long items, int per) { return (int)((items + per - 1) / per); }
 
-template <class TV, class TC>
-void mv1_t(const State& st, const QeqBuffers& qb, const TC* col, const TV* val, const double* x, double* y, hipStream_t s) {
-  hipLaunchKernelGGL((k_spmv1<TV, TC>), dim3((st.n * 64 + kThreads - 1) / kThreads), dim3(kThreads), 0, s, st.n, st.maxnb, qb.cnt, col, val,
-                     qb.eta, x, y);
-}
-
-void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, double* y, hipStream_t s) {
-  if (qb.fp32) {
-    if (qb.narrow) mv1_t(st, qb, qb.col16, qb.val32, x, y, s); else mv1_t(st, qb, qb.col, qb.val32, x, y, s);
+// Calls f(values, columns) with the matrix in its stored format: FP64 or FP32 values, or (ds) the two-float values of the
+// refinement's residuals; 16- or 32-bit columns.
+template <class F>
+void with_matrix(const QeqBuffers& qb, bool ds, F&& f) {
+  if (ds) {
+    const ValDS v{qb.val32, qb.lo32};
+    if (qb.narrow) f(v, (const unsigned short*)qb.col16); else f(v, (const int*)qb.col);
+  } else if (qb.fp32) {
+    const ValF v{qb.val32};
+    if (qb.narrow) f(v, (const unsigned short*)qb.col16); else f(v, (const int*)qb.col);
   } else {
-    if (qb.narrow) mv1_t(st, qb, qb.col16, qb.val, x, y, s); else mv1_t(st, qb, qb.col, qb.val, x, y, s);
+    const ValD v{qb.val};
+    if (qb.narrow) f(v, (const unsigned short*)qb.col16); else f(v, (const int*)qb.col);
   }
 }
+template <class P>
+using col_t = typename std::remove_const<typename std::remove_pointer<P>::type>::type;
 
-template <bool kInit, class TV, class TC>
-void mv2_t(int n, int maxnb, const QeqBuffers& qb, const TC* col, const TV* val, const double* x, double* y, hipStream_t s) {
-  hipLaunchKernelGGL((k_spmv2<TV, TC, kInit>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, maxnb, qb.cnt, col, val, qb.eta, x, y, qb);
+template <int kMode>
+void spmv2(int n, int maxnb, QeqBuffers& qb, bool ds, const double* x, double* y, int it, hipStream_t s) {
+  with_matrix(qb, ds, [&](auto v, auto col) {
+    hipLaunchKernelGGL((k_spmv2<decltype(v), col_t<decltype(col)>, kMode>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, maxnb, qb.cnt,
+                       col, v, qb.eta, x, y, qb, it);
+  });
 }
 
-template <class TV, class TC>
-void mvdir_t(int n, int maxnb, const QeqBuffers& qb, const TC* col, const TV* val, int it, int imax, double tol, hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_mvdir<TV, TC>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, maxnb, col, val, qb, it, imax, tol);
+}  // namespace
+
+void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, double* y, hipStream_t s) {
+  // the fix's sparse_matvec: FP64 values where the engine keeps them (FP64 mode, or the two-float pair under refinement)
+  with_matrix(qb, qb.fp32 && qb.refine, [&](auto v, auto col) {
+    hipLaunchKernelGGL((k_spmv1<decltype(v), col_t<decltype(col)>>), dim3((st.n * 64 + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
+                       st.n, st.maxnb, qb.cnt, col, v, qb.eta, x, y);
+  });
 }
 
 void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s) {
   const int n = st.n;
   hipLaunchKernelGGL(k_cg_guess, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, use_hist);
-  if (qb.fp32) {
-    if (qb.narrow) mv2_t<true>(n, st.maxnb, qb, qb.col16, qb.val32, qb.x, qb.q, s); else mv2_t<true>(n, st.maxnb, qb, qb.col, qb.val32, qb.x, qb.q, s);
-  } else {
-    if (qb.narrow) mv2_t<true>(n, st.maxnb, qb, qb.col16, qb.val, qb.x, qb.q, s); else mv2_t<true>(n, st.maxnb, qb, qb.col, qb.val, qb.x, qb.q, s);
-  }
+  spmv2<1>(n, st.maxnb, qb, qb.fp32 && qb.refine, qb.x, qb.q, 0, s);
   hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kThreads), 0, s, qb, mv_blocks(n), imax, tol);
 }
 
-void launch_qeq_mvdir(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s) {
+namespace {
+void mvdir_launch(const State& st, QeqBuffers& qb, int it, int imax, double tol, int chk, hipStream_t s) {
   const int n = st.n;
-  if (qb.fp32) {
-    if (qb.narrow) mvdir_t(n, st.maxnb, qb, qb.col16, qb.val32, it, imax, tol, s); else mvdir_t(n, st.maxnb, qb, qb.col, qb.val32, it, imax, tol, s);
-  } else {
-    if (qb.narrow) mvdir_t(n, st.maxnb, qb, qb.col16, qb.val, it, imax, tol, s); else mvdir_t(n, st.maxnb, qb, qb.col, qb.val, it, imax, tol, s);
-  }
+  with_matrix(qb, false, [&](auto v, auto col) {
+    hipLaunchKernelGGL((k_cg_mvdir<decltype(v), col_t<decltype(col)>>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, st.maxnb, col, v, qb,
+                       it, imax, tol, chk);
+  });
+}
+}  // namespace
+
+void launch_qeq_mvdir(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s) { mvdir_launch(st, qb, it, imax, tol, 0, s); }
+void launch_qeq_mvdir_chk(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s) { mvdir_launch(st, qb, it, imax, tol, 1, s); }
+
+void launch_qeq_restart(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s) {
+  const int n = st.n;
+  spmv2<2>(n, st.maxnb, qb, true, qb.x, qb.q, it, s);
+  hipLaunchKernelGGL(k_cg_check_fin, dim3(1), dim3(kThreads), 0, s, qb, mv_blocks(n), it, imax, tol);
 }
 
 int qeq_mv_partials(int n) { return mv_blocks(n); }
@@ -482,22 +604,10 @@ void launch_qeq_finish(const State& st, QeqBuffers& qb, hipStream_t s) {
 }
 
 // ---- packed systems ----
-namespace {
-inline int pk_blocks(long long items, int per) { return (int)((items + per - 1) / per); }
-template <class TV, class TC>
-void pk_mvdir_t(int n, int maxnb, const QeqBuffers& qb, const TC* col, const TV* val, const PackedQeq& pq, int it, hipStream_t s) {
-  hipLaunchKernelGGL((k_pk_cg_mvdir<TV, TC>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, maxnb, col, val, qb, pq.sysid, pq.pk, pq.pki, pq.rowv, it);
-}
-}  // namespace
-
 void launch_pk_qeq_init(const State& st, QeqBuffers& qb, const PackedQeq& pq, int imax, double tol, int use_hist, hipStream_t s) {
   const int n = st.n;
   hipLaunchKernelGGL(k_cg_guess, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, use_hist);
-  if (qb.fp32) {
-    if (qb.narrow) mv2_t<false>(n, st.maxnb, qb, qb.col16, qb.val32, qb.x, qb.q, s); else mv2_t<false>(n, st.maxnb, qb, qb.col, qb.val32, qb.x, qb.q, s);
-  } else {
-    if (qb.narrow) mv2_t<false>(n, st.maxnb, qb, qb.col16, qb.val, qb.x, qb.q, s); else mv2_t<false>(n, st.maxnb, qb, qb.col, qb.val, qb.x, qb.q, s);
-  }
+  spmv2<0>(n, st.maxnb, qb, qb.fp32 && qb.refine, qb.x, qb.q, 0, s);
   hipLaunchKernelGGL(k_pk_cg_init, dim3(pk_blocks(n, 256)), dim3(256), 0, s, n, qb, pq.rowv);
   hipLaunchKernelGGL(k_pk_cg_seg, dim3(pk_blocks((long long)pq.S * 64, 256)), dim3(256), 0, s, 0, pq.S, pq.sys_start, pq.rowv,
                      (const double*)qb.x, pq.pk, pq.pki, 0, imax, tol, pq.any);
@@ -505,15 +615,27 @@ void launch_pk_qeq_init(const State& st, QeqBuffers& qb, const PackedQeq& pq, in
 
 void launch_pk_qeq_iteration(const State& st, QeqBuffers& qb, const PackedQeq& pq, int it, int imax, double tol, hipStream_t s) {
   const int n = st.n;
-  if (qb.fp32) {
-    if (qb.narrow) pk_mvdir_t(n, st.maxnb, qb, qb.col16, qb.val32, pq, it, s); else pk_mvdir_t(n, st.maxnb, qb, qb.col, qb.val32, pq, it, s);
-  } else {
-    if (qb.narrow) pk_mvdir_t(n, st.maxnb, qb, qb.col16, qb.val, pq, it, s); else pk_mvdir_t(n, st.maxnb, qb, qb.col, qb.val, pq, it, s);
-  }
+  with_matrix(qb, false, [&](auto v, auto col) {
+    hipLaunchKernelGGL((k_pk_cg_mvdir<decltype(v), col_t<decltype(col)>>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, st.maxnb, col, v, qb,
+                       pq.sysid, (const double*)pq.pk, (const int*)pq.pki, pq.rowv, it);
+  });
   const dim3 sg(pk_blocks((long long)pq.S * 64, 256));
   hipLaunchKernelGGL(k_pk_cg_seg, sg, dim3(256), 0, s, 1, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
   hipLaunchKernelGGL(k_pk_cg_update, dim3(pk_blocks(n, 256)), dim3(256), 0, s, n, qb, pq.sysid, (const double*)pq.pk, (const int*)pq.pki, pq.rowv);
   hipLaunchKernelGGL(k_pk_cg_seg, sg, dim3(256), 0, s, 2, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
+}
+
+void launch_pk_qeq_restart(const State& st, QeqBuffers& qb, const PackedQeq& pq, int it, int imax, double tol, hipStream_t s) {
+  const int n = st.n;
+  const ValDS v{qb.val32, qb.lo32};
+  if (qb.narrow)
+    hipLaunchKernelGGL((k_pk_cg_check<unsigned short>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, st.maxnb, (const unsigned short*)qb.col16, v,
+                       qb, pq.sysid, (const int*)pq.pki, pq.rowv);
+  else
+    hipLaunchKernelGGL((k_pk_cg_check<int>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, st.maxnb, (const int*)qb.col, v, qb, pq.sysid,
+                       (const int*)pq.pki, pq.rowv);
+  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pk_blocks((long long)pq.S * 64, 256)), dim3(256), 0, s, 4, pq.S, pq.sys_start, pq.rowv,
+                     (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
 }
 
 void launch_pk_qeq_finish(const State& st, QeqBuffers& qb, const PackedQeq& pq, hipStream_t s) {

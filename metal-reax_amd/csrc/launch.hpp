@@ -61,7 +61,13 @@ struct QeqBuffers {
   unsigned short* col16 = nullptr;  // n * maxnb: the same in 16 bits, when narrow == 1 (n <= 65535)
   double* val = nullptr;            // n * maxnb: H_ij in FP64 (FP64 mode)
   float* val32 = nullptr;           // n * maxnb: H_ij in FP32 (mixed and FP32 modes)
+  // n * maxnb: the FP32 remainder H_ij - (double)val32 (written when refine == 1), so that (double)val32 + (double)lo32
+  // carries H_ij to about 48 bits for the FP64 residuals of the iterative refinement
+  float* lo32 = nullptr;
   int narrow = 0;
+  // FP64 iterative refinement of the FP32-valued solve: initial and verification residuals with the two-float matrix, CG
+  // restarted from the true residual while it exceeds the tolerance (mixed mode; FP32 mode with the fix's 'resident' keyword)
+  int refine = 0;
   // n * maxnb: how the pair of each entry is reached from the row's atom, for the non-bonded pass over the same rows: bit 15 set
   // if the row's atom is the pair's canonical end, bits 0-14 the index in the shift-vector table of the lattice shift applied to
   // the canonical end's partner. Written when keyed == 1 (shift table of at most 32768 vectors).
@@ -73,30 +79,41 @@ struct QeqBuffers {
   // CG state, two systems (s: b = -chi, t: b = -1) interleaved per atom: [2*i + sys]
   double *b, *x, *r, *d, *p, *q;
   double* partial;           // 4 * kCgBlocks: per-block partials of the element-wise CG kernels
-  double* partial_mv;        // 2 * qeq_mv_partials(n): per-block partials of the matrix-vector product
+  double* partial_mv;        // 4 * qeq_mv_partials(n): per-block partials of the matrix-vector product
+  double* partial_rs;        // 4 * qeq_mv_partials(n): per-block partials of the refinement's verification residual
   int fp32 = 0;              // the matrix values are val32 (mixed and FP32 modes) instead of val
-  double* scal;              // device scalars and activity flags per iteration (see k_qeq.hip)
+  double* scal;              // device scalars and activity flags per iteration (see k_qeq.hip), own allocation sized by imax
   double *s_hist, *t_hist;   // 4 * n each, [k * n + i], k = 0 most recent
 };
 constexpr int kCgBlocks = 256;
+// doubles per iteration slot of QeqBuffers::scal after its 8-double header (sig of s and t, the activity pair, the restart-record
+// pair); the scalars of iterations 0 .. imax + 1 take 8 + kScalSlot (imax + 2) doubles
+constexpr int kScalSlot = 4;
+inline size_t qeq_scal_doubles(int imax) { return 8 + (size_t)kScalSlot * (size_t)(imax + 2); }
 // several independent systems packed in one engine: per-system CG scalars and row scratch (k_qeq.hip)
 struct PackedQeq {
   int S = 0;
   const int* sys_start = nullptr;   // S + 1 row offsets
   const int* sysid = nullptr;       // n: system of each row
   double* pk = nullptr;             // 10 S: per solve |b|, r.p, alpha, beta; sums of s and t
-  int* pki = nullptr;               // 4 S: per solve active, stock loop index
+  int* pki = nullptr;               // 4 S: per solve state (1 active, 0 stopped, 2 stopped and verified by the refinement
+                                    // check), stock loop index
   double* rowv = nullptr;           // 4 n: per-row d.q and r.p of both solves
   int* any = nullptr;               // imax + 2: some system active for iteration it + 1 (set by iteration it; [0] by the start)
 };
 void launch_pk_qeq_init(const State& st, QeqBuffers& qb, const PackedQeq& pq, int imax, double tol, int use_hist, hipStream_t s);
 void launch_pk_qeq_iteration(const State& st, QeqBuffers& qb, const PackedQeq& pq, int it, int imax, double tol, hipStream_t s);
+// refinement check after iteration it (qb.refine): every solve that has stopped and is not yet verified gets its true residual
+// with the two-float matrix; a solve above tol restarts (beta 0) for iteration it + 1, the others are verified
+void launch_pk_qeq_restart(const State& st, QeqBuffers& qb, const PackedQeq& pq, int it, int imax, double tol, hipStream_t s);
 void launch_pk_qeq_finish(const State& st, QeqBuffers& qb, const PackedQeq& pq, hipStream_t s);
 // packed systems: neighbour search, integrator and energy sums per system (k_packed.hip)
 void launch_pk_shift_table(const Cell* cells, int S, double4* shv, const int smx[3], hipStream_t s);
 void launch_pk_wrap(const State& st, const Cell* cells, const int* reb, hipStream_t s);
 void launch_pk_nbr(const State& st, const Cell* cells, const int* sys_start, const int4* ext, const int* reb, double rs, hipStream_t s);
 void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s);
+// census of the non-bonded cut-off decision (st.cen, category kCenNonbonded)
+void launch_census_nb(const State& st, const Globals& g, hipStream_t s);
 // Non-bonded terms over the charge-matrix rows instead of the skin list (mixed / FP32 modes): valid while the rows belong to the
 // current positions, carry keys, and the matrix cutoff covers nonb_cut.
 void launch_nonbonded_rows(const State& st, const Globals& g, const Tables& t, const QeqBuffers& qb, hipStream_t s);
@@ -104,6 +121,11 @@ void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, 
 void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s);
 void launch_qeq_mvdir(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);   // beta, d, q = A d, d.q partials
 void launch_qeq_update(const State& st, QeqBuffers& qb, int it, hipStream_t s);                        // alpha, x, r, p, r.p partials
+// refinement check after update(it): if both systems have stopped, the true residual b - (diag(eta) + H) x with the two-float
+// matrix restarts every system whose residual still exceeds tol (slot `it` of the scalars); otherwise it leaves a marker and
+// the next product continues the iteration (launch_qeq_mvdir with chk = 1)
+void launch_qeq_restart(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);
+void launch_qeq_mvdir_chk(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);
 int qeq_mv_partials(int n);                                                                             // partials per product
 void launch_qeq_finish(const State& st, QeqBuffers& qb, hipStream_t s);
 

@@ -64,6 +64,7 @@ def lib():
             "rxa_qeq_solve": (I, [P, P, P, I, D, P, P, P, P]),
             "rxa_qeq_charges": (I, [P, I, D, I, P, P, P]),
             "rxa_qeq_reset_history": (I, [P]),
+            "rxa_qeq_refine": (I, [P, I]),
             "rxa_md_setup": (I, [P, P, P, D, D, D, D, I, I, D]),
             "rxa_md_run": (I, [P, I, P]),
             "rxa_get_state": (I, [P, P, P, P]),
@@ -244,12 +245,14 @@ class Engine:
         return q
 
     # ---- evaluation
-    def compute(self, rebuild=False, want_grad=True, peratom=False):
-        """Energies, gradient and virial; with peratom=True also "eatom" (n) and "vatom" (n, 6) of the same call (rxa_peratom)."""
+    def compute(self, rebuild=False, want_grad=True, peratom=False, census=False):
+        """Energies, gradient and virial; with peratom=True also "eatom" (n) and "vatom" (n, 6) of the same call (rxa_peratom);
+        census=True records the decision census of the call (read it with census())."""
         e = np.zeros(13)
         g = np.zeros(3 * self.n) if want_grad else None
         v = np.zeros(6)
-        self._chk(lib().rxa_compute(self.h, (1 if rebuild else 0) | (2 if peratom else 0), _p(e), _p(g), _p(v)))
+        flags = (1 if rebuild else 0) | (2 if peratom else 0) | (4 if census else 0)
+        self._chk(lib().rxa_compute(self.h, flags, _p(e), _p(g), _p(v)))
         r = {"energy": dict(zip(TERMS, e.tolist())), "grad": None if g is None else g.reshape(self.n, 3), "virial": v}
         if peratom:
             ea = np.zeros(self.n)
@@ -259,7 +262,11 @@ class Engine:
             r["vatom"] = va.reshape(self.n, 6)
         return r
 
+    CENSUS = ("bonds", "angle_sets", "torsions", "hbonds", "nonbonded", "sbo_region", "lp_trunc")
+
     def census(self):
+        """Per-atom decision census (7, n) in the order of CENSUS: that of the last compute(census=True), else of a force call
+        made now at the current positions and charges."""
         out = np.zeros(7 * self.n, dtype=np.int32)
         self._chk(lib().rxa_census(self.h, _p(out)))
         return out.reshape(7, self.n)
@@ -307,6 +314,11 @@ class Engine:
 
     def qeq_reset_history(self):
         self._chk(lib().rxa_qeq_reset_history(self.h))
+
+    def qeq_refine(self, mode):
+        """FP64 iterative refinement of the charge solve: -1 the precision's default (on in mixed mode), 0 off, 1 on (the fix's
+        'resident' keyword)."""
+        self._chk(lib().rxa_qeq_refine(self.h, int(mode)))
 
     # ---- MD
     def md_setup(self, v, mass_by_type, dt=0.25, t_target=300.0, t_damp=25.0, skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=1e-6):
