@@ -21,7 +21,10 @@
 // partner's slot of the writer's own bond list (fpx/fpy/fpz/cdp) and are collected by the partner through the mirror index.
 //
 // Precision: TB is the type of the bond-order chain (BO', corrections, Delta quantities and their threshold tests); TR the type of
-// the term arithmetic. FP64: <double, double>; mixed: <double, float>; FP32: <float, float>. Energies, CdDelta and forces are
+// the term arithmetic. FP64: <double, double>; mixed: <double, float>; FP32: <float, float>. The terms' own discrete decisions
+// (three- and four-body gates, the H-bond BO gate, the triple-bond and C2 gates, the SBO branch) are taken in TS, the type of the
+// chain whose values they test: double in FP64 and mixed modes, so they match the FP64 engine outside its rounding, float in
+// FP32 mode. Energies, CdDelta and forces are
 // accumulated in double in every mode.
 #include "common.hpp"
 #include "launch.hpp"
@@ -32,6 +35,9 @@ namespace {
 template <class T> __device__ inline T rexp(T x);
 template <> __device__ inline double rexp(double x) { return exp(x); }
 template <> __device__ inline float rexp(float x) { return expf(x); }
+template <class T> __device__ inline T rexpm1(T x);
+template <> __device__ inline double rexpm1(double x) { return expm1(x); }
+template <> __device__ inline float rexpm1(float x) { return expm1f(x); }
 template <class T> __device__ inline T rpow(T x, T y);
 template <> __device__ inline double rpow(double x, double y) { return pow(x, y); }
 template <> __device__ inline float rpow(float x, float y) { return powf(x, y); }
@@ -335,7 +341,7 @@ __global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables 
 // ================================================================================================================
 // bond energy, triple-bond stabilisation, lone pair (+ C2), over/under coordination
 // ================================================================================================================
-template <class TR>
+template <class TR, class TS>
 __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= st.n) return;
@@ -364,7 +370,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
       st.Cdbopi[q] += -(double)(CEbo + (TR)P.De_p);
       st.Cdbopi2[q] += -(double)(CEbo + (TR)P.De_pp);
       const TR BO = (TR)st.bo[q];
-      if (BO >= TR(1.00) && P.tbs) {
+      if ((TS)st.bo[q] >= TS(1.00) && P.tbs) {
         const TR gp3 = (TR)g.gp3, gp4 = (TR)g.gp4, gp7 = (TR)g.gp7, gp10 = (TR)g.gp10;
         const TR exphu = rexp<TR>(-gp7 * ((BO - TR(2.50)) * (BO - TR(2.50))));
         const TR ea = rexp<TR>(-gp3 * ((TR)st.tbo[i] - BO)), eb = rexp<TR>(-gp3 * ((TR)st.tbo[j] - BO));
@@ -393,7 +399,8 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
         const size_t q = sl(p, i, n);
         if (!t.sp[st.type[st.b_j[q]]].c2) continue;
         const TR vov3 = (TR)st.bo[q] - Di - TR(0.040) * rpow<TR>(Di, TR(4.));
-        if (vov3 > TR(3.)) {
+        const TS Ds = (TS)st.Delta[i];
+        if ((TS)st.bo[q] - Ds - TS(0.040) * rpow<TS>(Ds, TS(4.)) > TS(3.)) {
           const double e_c = (double)(p_lp3 * ((vov3 - TR(3.0)) * (vov3 - TR(3.0))));
           e_lp += e_c;
           e_c2 += e_c;
@@ -482,7 +489,7 @@ __device__ inline int wave_max(int v) {
 // Valence angle, penalty and three-body conjugation at the centre j [reference bonded.cpp:321-474]. Lane p of j's group evaluates
 // the pairs (p, k) with p < k of j's bonds, k running over a wave-uniform loop: the contributions to slot p stay in the lane, the
 // contributions to slot k are summed over the group and kept by lane k, and the centre's are summed over the group at the end.
-template <class TR, int G>
+template <class TR, class TS, int G>
 __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = gid / G, p = gid % G, n = st.n;
@@ -498,33 +505,45 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
   double own_cdbo = 0, own_cdp = 0, own_fx = 0, own_fy = 0, own_fz = 0;
   // centre quantities (every lane of the group computes them)
   TR SBO2 = 0, CSBO2 = 0, dSBO1 = 0, dSBO2 = 0, ev6 = 0, f9 = 0, Cf9 = 0, ec2 = 0, Dboc = 0, p_val3 = 0, p_val5 = 0;
-  const TR thb_cut = (TR)g.thb_cut, thb_cutsq = (TR)g.thb_cutsq;
-  const TR p_val6 = (TR)g.p_val6, p_val8 = (TR)g.p_val8, p_val9 = (TR)g.p_val9, p_val10 = (TR)g.p_val10;
+  const TR thb_cut = (TR)g.thb_cut;
+  const TS thb_cut_s = (TS)g.thb_cut, thb_cutsq_s = (TS)g.thb_cutsq;
+  const TR p_val6 = (TR)g.p_val6, p_val10 = (TR)g.p_val10;
   const TR p_pen2 = (TR)g.p_pen2, p_pen3 = (TR)g.p_pen3, p_pen4 = (TR)g.p_pen4;
   const TR p_coa2 = (TR)g.p_coa2, p_coa3 = (TR)g.p_coa3, p_coa4 = (TR)g.p_coa4;
   if (sbo_on) {
     const DevSpecies& sj = t.sp[tj];
     p_val3 = (TR)sj.p_val3; p_val5 = (TR)sj.p_val5;
-    TR SBOp = 0, prod = 1;
-    for (int b = 0; b < nb; ++b) {
-      const size_t q = sl(b, j, n);
-      const TR bo = (TR)st.bo[q];
-      SBOp += (TR)st.bo_pi[q] + (TR)st.bo_pi2[q];
-      TR w = bo * bo;
-      w *= w;
-      w *= w;
-      prod *= rexp<TR>(-w);
+    // SBO and its branch in TS. omp = 1 - prod(exp(-BO^8)): in double the reference's product; in float from expm1 of the
+    // exponent sum, since a bond with BO^8 below FP32's epsilon leaves the float product at exactly 1 and would drop out of
+    // 1 - prod, which decides the sign of SBO at under-coordinated centres
+    TS SBOp = 0, prod = 1, omp;
+    {
+      TS sw = 0;
+      for (int b = 0; b < nb; ++b) {
+        const size_t q = sl(b, j, n);
+        const TS bo = (TS)st.bo[q];
+        SBOp += (TS)st.bo_pi[q] + (TS)st.bo_pi2[q];
+        TS w = bo * bo;
+        w *= w;
+        w *= w;
+        if (sizeof(TS) == 8) prod *= rexp<TS>(-w); else sw += w;
+      }
+      if (sizeof(TS) == 8) omp = TS(1) - prod;
+      else { omp = -rexpm1<TS>(-sw); prod = TS(1) - omp; }
     }
-    const TR dDlp = (TR)st.dDelta_lp[j];
-    TR vlpadj;
-    if ((TR)st.vlpex[j] >= TR(0)) { vlpadj = 0; dSBO2 = prod - TR(1); }
-    else { vlpadj = (TR)st.nlp[j]; dSBO2 = (prod - TR(1)) * (TR(1) - p_val8 * dDlp); }
-    Dboc = (TR)st.Delta_boc[j];
-    const TR SBO = SBOp + (TR(1) - prod) * (-Dboc - p_val8 * vlpadj);
-    dSBO1 = -TR(8) * prod * (Dboc + p_val8 * vlpadj);
-    if (SBO <= TR(0)) { SBO2 = 0; CSBO2 = 0; sbo_region = 0; }
-    else if (SBO <= TR(1)) { SBO2 = rpow<TR>(SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(SBO, p_val9 - TR(1)); sbo_region = 1; }
-    else if (SBO < TR(2)) { SBO2 = TR(2) - rpow<TR>(TR(2) - SBO, p_val9); CSBO2 = p_val9 * rpow<TR>(TR(2) - SBO, p_val9 - TR(1)); sbo_region = 2; }
+    const TS p_val8s = (TS)g.p_val8, dDlp = (TS)st.dDelta_lp[j];
+    TS vlpadj, dSBO2s;
+    if ((TS)st.vlpex[j] >= TS(0)) { vlpadj = 0; dSBO2s = -omp; }
+    else { vlpadj = (TS)st.nlp[j]; dSBO2s = -omp * (TS(1) - p_val8s * dDlp); }
+    const TS Dbocs = (TS)st.Delta_boc[j];
+    Dboc = (TR)Dbocs;
+    const TS SBO = SBOp + omp * (-Dbocs - p_val8s * vlpadj);
+    dSBO1 = (TR)(-TS(8) * prod * (Dbocs + p_val8s * vlpadj));
+    dSBO2 = (TR)dSBO2s;
+    const TS pv9 = (TS)g.p_val9;
+    if (SBO <= TS(0)) { SBO2 = 0; CSBO2 = 0; sbo_region = 0; }
+    else if (SBO <= TS(1)) { SBO2 = (TR)rpow<TS>(SBO, pv9); CSBO2 = (TR)(pv9 * rpow<TS>(SBO, pv9 - TS(1))); sbo_region = 1; }
+    else if (SBO < TS(2)) { SBO2 = (TR)(TS(2) - rpow<TS>(TS(2) - SBO, pv9)); CSBO2 = (TR)(pv9 * rpow<TS>(TS(2) - SBO, pv9 - TS(1))); sbo_region = 2; }
     else { SBO2 = 2; CSBO2 = 0; sbo_region = 3; }
     ev6 = safe_exp<TR>(p_val6 * Dboc);
     const TR Delta = (TR)st.Delta[j], Dval = (TR)st.Delta_val[j];
@@ -538,9 +557,11 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
   size_t qi = 0;
   int i = 0, ti = 0;
   TR BOij = 0, BOA_ij = 0, dij = 1, tboi = 0;
+  TS BOij_s = 0;
   V3<TR> vij{0, 0, 0};
   if (mine) {
     qi = sl(p, j, n);
+    BOij_s = (TS)st.bo[qi];
     BOij = (TR)st.bo[qi];
     BOA_ij = BOij - thb_cut;
     i = st.b_j[qi]; ti = st.type[i];
@@ -550,13 +571,14 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
   }
   for (int pk = 1; pk < nbw; ++pk) {
     double to_cdbo = 0, to_cdp = 0, to_fx = 0, to_fy = 0, to_fz = 0;
-    if (mine && p < pk && pk < nb && BOA_ij > TR(0)) {
+    if (mine && p < pk && pk < nb && BOij_s - thb_cut_s > TS(0)) {
       const size_t qk = sl(pk, j, n);
+      const TS BOjk_s = (TS)st.bo[qk];
       const TR BOjk = (TR)st.bo[qk];
       const TR BOA_jk = BOjk - thb_cut;
       const int k = st.b_j[qk], tk = st.type[k];
       const int2 ai = t.angle_idx[(ti * g.ntypes + tj) * g.ntypes + tk];
-      const bool gate = BOA_jk > TR(0) && BOij > thb_cut && BOjk > thb_cut && BOij * BOjk > thb_cutsq;
+      const bool gate = BOjk_s - thb_cut_s > TS(0) && BOij_s > thb_cut_s && BOjk_s > thb_cut_s && BOij_s * BOjk_s > thb_cutsq_s;
       n_ang += gate ? 1 : 0;
       if (gate && ai.y != 0) {
         const V3<TR> vjk{(TR)st.b_dx[qk], (TR)st.b_dy[qk], (TR)st.b_dz[qk]};
@@ -655,7 +677,7 @@ __device__ inline TR tan_inv(TR s, TR c) {
 // its own half (its atom, its outer partner i and outer bond (j, i)); the end the reference counts (canonical_first) also adds
 // the energy and the central-bond coefficients. The outer-bond contributions of an i are summed over the group and kept by the
 // lane owning slot i.
-template <class TR, int G, bool kCen>
+template <class TR, class TS, int G, bool kCen>
 __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = gid / G, pk = gid % G, n = st.n;
@@ -667,6 +689,7 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   double own_fx = 0, own_fy = 0, own_fz = 0;
   double fIx = 0, fIy = 0, fIz = 0;   // this lane's forces on the outer partners i, summed (for the virial)
   const TR thb_cut = (TR)g.thb_cut;
+  const TS thb_cut_s = (TS)g.thb_cut;
   const TR p_tor2 = (TR)g.p_tor2, p_tor3 = (TR)g.p_tor3, p_tor4 = (TR)g.p_tor4, p_cot2 = (TR)g.p_cot2;
   const TR MINS = TR(1e-10);
   // this lane's central bond (j, k)
@@ -681,7 +704,7 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
     BOjk = (TR)st.bo[qk];
     k = st.b_j[qk]; sk = st.b_s[qk]; tk = st.type[k];
     nbk = min(st.b_cnt[k], st.maxb);
-    cen = BOjk > thb_cut && nbk >= 2;
+    cen = (TS)st.bo[qk] > thb_cut_s && nbk >= 2;
   }
   if (cen) {
     counted = canonical_first(j, k, sk, st.tag);
@@ -704,8 +727,11 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
     TR cdbo_ij = 0, fix = 0, fiy = 0, fiz = 0;   // this lane's terms of partner i: at most one per bond of k
     if (cen && pi != pk && pi < nbj) {
       const size_t qi = sl(pi, j, n);
+      const TS BOij_s = (TS)st.bo[qi];
       const TR BOij = (TR)st.bo[qi];
-      if (BOij > thb_cut) {
+      if (BOij_s > thb_cut_s) {
+        // the four-body gate's product, left to right as the reference (FP64 gates: the first two factors formed here)
+        const TS BOijk_s = sizeof(TS) == 8 ? BOij_s * (TS)st.bo[qk] : TS(0);
         const int i = st.b_j[qi], si_ = st.b_s[qi], ti = st.type[i];
         const TR rij = (TR)st.b_d[qi], BOA_ij = BOij - thb_cut;
         const V3<TR> dji{(TR)st.b_dx[qi], (TR)st.b_dy[qi], (TR)st.b_dz[qi]};
@@ -717,13 +743,14 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
         for (int pl = 0; pl < nbk; ++pl) {
           if (pl == symk) continue;
           const size_t ql = sl(pl, k, n);
+          const TS BOkl_s = (TS)st.bo[ql];
+          if (!(BOkl_s > thb_cut_s)) continue;
           const TR BOkl = (TR)st.bo[ql];
-          if (!(BOkl > thb_cut)) continue;
           const int l = st.b_j[ql], sl_abs = shift_add(sk, st.b_s[ql]), tl = st.type[l];
           if (l == i && sl_abs == si_) continue;   // i and l are the same image
           const int ti4 = t.tors_idx[((ti * g.ntypes + tj) * g.ntypes + tk) * g.ntypes + tl];
           if (ti4 < 0) continue;
-          if (!(BOij * BOjk * BOkl > thb_cut)) continue;
+          if (sizeof(TS) == 8 ? !(BOijk_s * BOkl_s > thb_cut_s) : !(BOij * BOjk * BOkl > thb_cut)) continue;
           if (kCen) n_tor += counted ? 1 : 0;
           const DevTorsion P = t.tors[ti4];
           const TR rkl = (TR)st.b_d[ql], BOA_kl = BOkl - thb_cut;
@@ -838,13 +865,13 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
 
 // FP32 terms at 3 waves per SIMD (168 VGPRs); FP64 at the compiler's choice. kCen: the variant that also records the torsion
 // census (launched only for a force call that asks for it, so the counter costs the ordinary step nothing)
-template <int G, bool kCen>
+template <class TS, int G, bool kCen>
 __global__ void __attribute__((amdgpu_waves_per_eu(3))) __launch_bounds__(256) k_torsion_f(State st, Globals g, Tables t) {
-  torsion_body<float, G, kCen>(st, g, t);
+  torsion_body<float, TS, G, kCen>(st, g, t);
 }
 template <int G, bool kCen>
 __global__ void __launch_bounds__(256) k_torsion_d(State st, Globals g, Tables t) {
-  torsion_body<double, G, kCen>(st, g, t);
+  torsion_body<double, double, G, kCen>(st, g, t);
 }
 
 // ================================================================================================================
@@ -886,7 +913,7 @@ __device__ inline void hbond_term(const DevHBond& P, TR eh2, const V3<TR>& dji, 
 constexpr int kHbWaves = 4;
 constexpr int kHbRing = 128;
 
-template <class TR>
+template <class TR, class TS>
 __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals g, Tables t) {
   __shared__ int rk_[kHbWaves][kHbRing], rh_[kHbWaves][kHbRing], rt_[kHbWaves][kHbRing];
   __shared__ double rx_[kHbWaves][kHbRing], ry_[kHbWaves][kHbRing], rz_[kHbWaves][kHbRing], rr_[kHbWaves][kHbRing];
@@ -921,7 +948,7 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
     bool ok = false;
     if (lane < nb) {
       const size_t q = sl(lane, j, n);
-      ok = t.sp[st.type[st.b_j[q]]].p_hbond == 2 && (TR)st.bo[q] >= TR(1e-2);
+      ok = t.sp[st.type[st.b_j[q]]].p_hbond == 2 && (TS)st.bo[q] >= TS(1e-2);
     }
     hbonds = __ballot(ok);
   }
@@ -1142,38 +1169,40 @@ void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tab
   else hipLaunchKernelGGL(k_bo_correct<double>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
 }
 
-template <class TR, int G>
+template <class TR, class TS, int G>
 void launch_terms_g(const State& st, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * G, 256)), blk(256), wgrid(nblk(st.n, 4));
-  hipLaunchKernelGGL((k_valence<TR, G>), grid, blk, 0, s, st, g, t);
+  hipLaunchKernelGGL((k_valence<TR, TS, G>), grid, blk, 0, s, st, g, t);
   if (sizeof(TR) == 4) {
-    if (st.census) hipLaunchKernelGGL((k_torsion_f<G, true>), grid, blk, 0, s, st, g, t);
-    else hipLaunchKernelGGL((k_torsion_f<G, false>), grid, blk, 0, s, st, g, t);
+    if (st.census) hipLaunchKernelGGL((k_torsion_f<TS, G, true>), grid, blk, 0, s, st, g, t);
+    else hipLaunchKernelGGL((k_torsion_f<TS, G, false>), grid, blk, 0, s, st, g, t);
   } else {
     if (st.census) hipLaunchKernelGGL((k_torsion_d<G, true>), grid, blk, 0, s, st, g, t);
     else hipLaunchKernelGGL((k_torsion_d<G, false>), grid, blk, 0, s, st, g, t);
   }
-  hipLaunchKernelGGL(k_hbond_donor<TR>, dim3(nblk(st.n, kHbWaves)), dim3(64 * kHbWaves), 0, s, st, g, t);
+  hipLaunchKernelGGL((k_hbond_donor<TR, TS>), dim3(nblk(st.n, kHbWaves)), dim3(64 * kHbWaves), 0, s, st, g, t);
   hipLaunchKernelGGL(k_hbond_acceptor<TR>, wgrid, blk, 0, s, st, g, t);
 }
 
-template <class TR>
+template <class TR, class TS>
 void launch_terms_t(const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
-  if (bmax <= 4) launch_terms_g<TR, 4>(st, g, t, s);
-  else if (bmax <= 8) launch_terms_g<TR, 8>(st, g, t, s);
-  else if (bmax <= 16) launch_terms_g<TR, 16>(st, g, t, s);
-  else if (bmax <= 32) launch_terms_g<TR, 32>(st, g, t, s);
-  else launch_terms_g<TR, 64>(st, g, t, s);
+  if (bmax <= 4) launch_terms_g<TR, TS, 4>(st, g, t, s);
+  else if (bmax <= 8) launch_terms_g<TR, TS, 8>(st, g, t, s);
+  else if (bmax <= 16) launch_terms_g<TR, TS, 16>(st, g, t, s);
+  else if (bmax <= 32) launch_terms_g<TR, TS, 32>(st, g, t, s);
+  else launch_terms_g<TR, TS, 64>(st, g, t, s);
 }
 
 void launch_atom_terms(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
-  if (p == Precision::FP64) hipLaunchKernelGGL(k_atom_terms<double>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
-  else hipLaunchKernelGGL(k_atom_terms<float>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+  if (p == Precision::FP64) hipLaunchKernelGGL((k_atom_terms<double, double>), dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+  else if (p == Precision::MIXED) hipLaunchKernelGGL((k_atom_terms<float, double>), dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+  else hipLaunchKernelGGL((k_atom_terms<float, float>), dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
 }
 
 void launch_angle_terms(Precision p, const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
-  if (p == Precision::FP64) launch_terms_t<double>(st, g, t, bmax, s);
-  else launch_terms_t<float>(st, g, t, bmax, s);
+  if (p == Precision::FP64) launch_terms_t<double, double>(st, g, t, bmax, s);
+  else if (p == Precision::MIXED) launch_terms_t<float, double>(st, g, t, bmax, s);
+  else launch_terms_t<float, float>(st, g, t, bmax, s);
 }
 
 void launch_bond_derivatives(const State& st, hipStream_t s) {

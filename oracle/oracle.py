@@ -33,7 +33,7 @@ def lib():
         L.orc_match.argtypes = [P, ctypes.c_char_p]
         L.orc_qeq_params.argtypes = [P, P, P, P]
         L.orc_cutoffs.argtypes = [P, P]
-        L.orc_compute.argtypes = [P, L64, P, P, P, P, P, P, D, I, P, P, P, P, P]
+        L.orc_compute.argtypes = [P, L64, P, P, P, P, P, P, D, I, P, P, P, P, P, P]
         L.orc_qeq.argtypes = [P, L64, P, P, P, P, P, P, P, P, D, D, D, I, P, P, P, P, P, P]
         L.orc_qeq_matvec.argtypes = [P, L64, P, P, P, P, P, P, P, P, D, D, P, P]
         _LIB = L
@@ -92,7 +92,7 @@ class Oracle:
         return x, t, g, b, p
 
     def compute(self, x, types, tags, box, periodic, q=None, shell=0.0, enobonds=True):
-        """FP64 energies (dict), gradient dE/dx (n,3) folded onto owners, counts, census, total bond order."""
+        """FP64 energies (dict), gradient dE/dx (n,3) folded onto owners, counts, census, total bond order, the SBO value per atom."""
         x, t, g, b, p = self._sys(x, types, tags, box, periodic)
         n = t.size
         qq = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
@@ -101,8 +101,9 @@ class Oracle:
         counts = np.zeros(8, dtype=np.int64)
         census = np.zeros(7 * n, dtype=np.int32)
         tbo = np.zeros(n)
+        sbo = np.zeros(n)
         rc = lib().orc_compute(self.h, n, _p(x), _p(t), _p(g), _p(b), _p(p), _p(qq), float(shell), int(enobonds), _p(e), _p(grad),
-                               _p(counts), _p(census), _p(tbo))
+                               _p(counts), _p(census), _p(tbo), _p(sbo))
         if rc:
             raise RuntimeError(lib().orc_last_error().decode())
         return {
@@ -111,6 +112,7 @@ class Oracle:
             "counts": dict(zip(("pairs_oo", "pairs_og", "pairs_self", "bonds", "nall", "angle_sets", "torsions", "hbonds"), counts.tolist())),
             "census": {k: census[i * n:(i + 1) * n].copy() for i, k in enumerate(CENSUS)},
             "total_bo": tbo,
+            "sbo": sbo,
         }
 
     def qeq(self, x, types, tags, box, periodic, swa=0.0, swb=10.0, tol=1e-6, imax=200, s0=None, t0=None, params=None):

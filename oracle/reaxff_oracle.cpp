@@ -594,6 +594,7 @@ struct Result {
   std::vector<double> grad;          // 3 * nall, dE/dx (ghosts included)
   std::vector<double> total_bo;      // nall
   std::vector<int> c_bonds, c_angles, c_tors, c_hb, c_nb, c_sbo, c_lp;   // per owned atom
+  std::vector<double> sbo;   // per owned atom: the SBO value whose branch c_sbo records (its distance to the branch thresholds)
   long long pairs_oo = 0, pairs_og = 0, pairs_self = 0, nbonds = 0;
 };
 
@@ -784,6 +785,7 @@ void bonded(const FF& ff, const Ctl& ctl, const Atoms& A, const Far& far, bool e
 
   std::vector<double> CdD(N, 0.0);
   R.c_bonds.assign(n, 0); R.c_angles.assign(n, 0); R.c_tors.assign(n, 0); R.c_hb.assign(n, 0); R.c_sbo.assign(n, 0); R.c_lp.assign(n, 0);
+  R.sbo.assign(n, 0.0);
   for (size_t i = 0; i < n; ++i) { R.c_lp[i] = (int)(aq[i].Delta_e / 2.0); R.c_bonds[i] = (int)B[i].size(); }
   double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0;
 
@@ -926,7 +928,7 @@ void bonded(const FF& ff, const Ctl& ctl, const Atoms& A, const Far& far, bool e
     else if (SBO < 2) { SBO2 = 2 - std::pow(2 - SBO, p_val9); CSBO2 = p_val9 * std::pow(2 - SBO, p_val9 - 1); }
     else { SBO2 = 2; CSBO2 = 0; }
     const double ev6 = std::exp(p_val6 * Dboc);
-    if (j < n) R.c_sbo[j] = SBO <= 0 ? 0 : (SBO <= 1 ? 1 : (SBO < 2 ? 2 : 3));
+    if (j < n) { R.c_sbo[j] = SBO <= 0 ? 0 : (SBO <= 1 ? 1 : (SBO < 2 ? 2 : 3)); R.sbo[j] = SBO; }
     for (size_t pi = 0; pi < bj.size(); ++pi) {
       Bnd& bij = bj[pi];
       const double BOA_ij = bij.BO - ctl.thb_cut;
@@ -1439,9 +1441,10 @@ void orc_cutoffs(void* hp, double* out) {
 // box: origin[3], a[3], b[3], c[3]; periodic[3]; q: n charges (NULL -> the nonbonded terms are skipped).
 // Outputs: energies[13] (bond, ov, un, lp, ang, pen, coa, hb, tor, con, vdW, ele, pol), grad[3n] (dE/dx folded onto owners),
 // counts[8] = pairs oo, og, self, bonds, nall, sum angle sets, sum torsions, sum hbonds; census (may be NULL): 7*n ints in the order
-// bonds, angle_sets, torsions, hbonds, nonbonded, sbo_region, lp_trunc; total_bo (may be NULL): n.
+// bonds, angle_sets, torsions, hbonds, nonbonded, sbo_region, lp_trunc; total_bo (may be NULL): n; sbo (may be NULL): n, the SBO
+// value of each owned atom (the quantity the sbo_region branch tests).
 int orc_compute(void* hp, long n, const double* x, const int* type, const long long* tag, const double* box, const int* periodic, const double* q,
-                double shell, int enobonds, double* energies, double* grad, long long* counts, int* census, double* total_bo) {
+                double shell, int enobonds, double* energies, double* grad, long long* counts, int* census, double* total_bo, double* sbo) {
   try {
     const Handle& h = *static_cast<Handle*>(hp);
     Box b;
@@ -1470,6 +1473,7 @@ int orc_compute(void* hp, long n, const double* x, const int* type, const long l
       for (int k = 0; k < 7; ++k) for (long i = 0; i < n; ++i) census[k * n + i] = (cs[k]->size() == (size_t)n) ? (*cs[k])[i] : 0;
     }
     if (total_bo) for (long i = 0; i < n; ++i) total_bo[i] = R.total_bo[i];
+    if (sbo) for (long i = 0; i < n; ++i) sbo[i] = R.sbo.size() == (size_t)n ? R.sbo[i] : 0.0;
     return 0;
   } catch (const std::exception& e) { return fail(e); }
 }

@@ -3,9 +3,9 @@
 A scratch-resident local turns every access into a memory round trip; it happened once silently when a helper selected between a
 reference into global memory and a reference to a local (k_qeq_build went from 117 to 185 us per 24k-atom step). The device
 assembly of every source is generated here (hipcc, gfx950, the build's flags) and each kernel's private segment size read
-from its descriptor. Allowed: a reserved but unused frame (FP32 per-atom terms), spills of the FP64-mode H-bond donor, and the
-FP32 torsion kept at 3 waves per SIMD, which spills one 8-byte value that is reloaded once per outer partner, outside the inner
-loop (24 us per step faster than the spill-free 2-wave build), and its census-recording variant.
+from its descriptor. Allowed: a reserved but unused frame (FP32-term per-atom terms), spills of the FP64-mode H-bond donor, and
+the FP32-term torsion kept at 3 waves per SIMD, which spills one 8-byte value that is reloaded once per outer partner, outside
+the inner loop (24 us per step faster than the spill-free 2-wave build), and its census-recording variant.
 """
 import concurrent.futures
 import os
@@ -18,10 +18,12 @@ import pytest
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(REPO, "metal-reax_amd", "csrc")
 HIPCC = "/opt/rocm/bin/hipcc"
-# FP32 torsion: the ordinary variants (Lb0) of groups of 8..64 lanes, and every census-recording variant (Lb1, launched only by a
-# force call that asks for the census)
-ALLOWED = {"k_atom_termsIfE", "k_hbond_donorIdE"} | {"k_torsion_fILi%dELb0EE" % g for g in (8, 16, 32, 64)} | \
-    {"k_torsion_fILi%dELb1EE" % g for g in (4, 8, 16, 32, 64)}
+# FP32-term torsion (template arguments: the gate type, d in mixed mode and f in FP32 mode, the group size, the census flag): the
+# ordinary FP32-mode variants (Lb0) of groups of 8..64 lanes, every ordinary mixed-mode variant (its FP64 gate product takes the
+# registers), and every census-recording variant (Lb1, launched only by a force call that asks for the census)
+ALLOWED = {"k_atom_termsIfdE", "k_atom_termsIffE", "k_hbond_donorIddE"} | \
+    {"k_torsion_fIfLi%dELb0EE" % g for g in (8, 16, 32, 64)} | \
+    {"k_torsion_fI%sLi%dELb%dEE" % (ts, g, c) for ts in "df" for g in (4, 8, 16, 32, 64) for c in (0, 1) if ts == "d" or c == 1}
 MAX_ALLOWED_BYTES = 72
 
 
