@@ -146,7 +146,7 @@ def main():
 
     import reaxamd
     from reaxamd import systems
-    ffield_text = systems.load_ffield()
+    ffield_text = systems.load_ffield(os.path.join(REPO, "tests", "golden", "ffield.cho_standin"))
     w = systems.water_box(args.nside, seed=1)
     v = systems.maxwell_boltzmann(w.masses(), 300.0, seed=2)
 

@@ -227,6 +227,9 @@ struct Engine {
     if (rows_env && std::atoi(rows_env) == 0) nb_rows = false;
     const char* ovl_env = std::getenv("RXA_OVERLAP");
     if (ovl_env && std::atoi(ovl_env) == 0) overlap = false;
+    // initial bond-slot capacity per atom (default 12); the list regrows on overflow whatever the start
+    const char* bcap_env = std::getenv("RXA_BOND_CAP");
+    if (bcap_env && std::atoi(bcap_env) > 0) maxb = std::atoi(bcap_env);
     HIPCHK(hipSetDevice(dev));
     st.shv_stride = 0;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -1541,9 +1544,9 @@ int rxa_create(rxa_context** out, int device) {
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return RXA_EDEVICE;
   if (device < 0 || device >= count) return RXA_EINVAL;
   try {
-    auto* c = new rxa_context;
+    auto c = std::make_unique<rxa_context>();
     c->e = std::make_unique<Engine>(device);
-    *out = c;
+    *out = c.release();
     return RXA_OK;
   } catch (...) {
     return RXA_EDEVICE;

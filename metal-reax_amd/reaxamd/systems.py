@@ -139,9 +139,7 @@ def maxwell_boltzmann(masses, T, seed=1):
     return v
 
 
-def load_ffield(name="ffield.cho_standin"):
-    import os
-    here = os.path.dirname(os.path.abspath(__file__))
-    path = os.path.join(os.path.dirname(os.path.dirname(here)), "tests", "golden", name)
+def load_ffield(path):
+    """The text of a ReaxFF force-field file (the engine parses it: rxa_pair_coeff)."""
     with open(path) as fh:
         return fh.read()

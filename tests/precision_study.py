@@ -4,9 +4,9 @@ For each mode: the error of one step against the FP64 oracle (energy per atom, l
 error; charges from each side's own charge solve at tol 1e-10), then an NVE run after a short NVT equilibration, reporting the
 drift of the total energy (linear fit, kcal/mol per atom per ps) and its RMS fluctuation.
 
-    python -m reaxamd.precision [--nside 6|20] [--nvt-steps N] [--nve-steps N] [--chunk N] [--modes fp64,mixed,fp32]
+    python tests/precision_study.py [--nside 6|20] [--nvt-steps N] [--nve-steps N] [--chunk N] [--modes fp64,mixed,fp32]
 
-The oracle is test infrastructure (oracle/); this study uses it as the checker only.
+A measurement script beside the tests: the oracle (oracle/, test infrastructure) is its checker only.
 """
 import argparse
 import json
@@ -16,9 +16,10 @@ import time
 
 import numpy as np
 
-from . import Engine, systems
-
-REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "metal-reax_amd"))
+from reaxamd import Engine, systems   # noqa: E402
 
 
 def engine(ff, w, prec):
@@ -89,7 +90,7 @@ def main():
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed,fp32")
     args = ap.parse_args()
-    ff = systems.load_ffield()
+    ff = systems.load_ffield(os.path.join(REPO, "tests", "golden", "ffield.cho_standin"))
     w = systems.water_box(args.nside, seed=1)
     precs = tuple(args.modes.split(","))
     res = {"atoms": w.n}

@@ -169,3 +169,26 @@ def test_control_file():
     assert any("qeq_freq" in n for n in s["notices"])
     with pytest.raises(reaxamd.ReaxError):
         reaxamd.check_pair_style(["control.reax"], "bogus_key 1\n")
+
+
+def test_pvector_maps_terms_to_lammps_slots():
+    """rxa_pvector against the 14 slots LAMMPS reported beside the 13 terms, on all 58 reference goldens (tests/golden/pvector.json,
+    data of the reference's tests/fixtures/reference/*.json): eb, ea = e_ov + e_un, elp, emol = 0, ev, epen, ecoa, ehb, et, eco, ew,
+    ep = e_ele, efi = 0, eqeq = e_pol (Q-31: eng_coul is e_ele + e_pol). A wrong or transposed slot fails on the cases whose terms
+    differ; the sum of the slots equals the total potential energy."""
+    import ctypes
+    import json
+    with open(os.path.join(REPO, "tests", "golden", "pvector.json")) as fh:
+        g = json.load(fh)
+    L = reaxamd.lib()
+    nonzero = np.zeros(14, dtype=bool)
+    for c in g["cases"]:
+        e = np.ascontiguousarray(c["terms"], dtype=np.float64)
+        pv = np.full(14, np.nan)
+        L.rxa_pvector(e.ctypes.data_as(ctypes.c_void_p), pv.ctypes.data_as(ctypes.c_void_p))
+        want = np.array(c["slots"])
+        assert np.allclose(pv, want, rtol=1e-12, atol=1e-12), (c["id"], pv.tolist(), want.tolist())
+        assert abs(pv.sum() - e.sum()) <= 1e-9 * max(1.0, abs(e.sum()))
+        nonzero |= want != 0
+    # the goldens exercise every slot the 13 terms feed (emol and efi stay zero)
+    assert nonzero.sum() == 12 and not nonzero[3] and not nonzero[12]
