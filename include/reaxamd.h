@@ -66,6 +66,17 @@ long rxa_table_dump(rxa_context* ctx, int portable, char* buf, long cap);
  * LAMMPS box: lo, hi, tilt (xy, xz, yz), periodic flags. types are LAMMPS types (1..ntypes), tags the atom ids. */
 int rxa_set_box(rxa_context* ctx, const double lo[3], const double hi[3], const double tilt[3], const int periodic[3]);
 int rxa_set_atoms(rxa_context* ctx, int64_t n, const int* types, const int64_t* tags, const double* x);
+/* LAMMPS' ghost-native arrays as they are (nlocal owned atoms, then nghost ghosts; types/tags/x of length nlocal + nghost),
+ * replacing host_atom_set (pair_reaxff_metal.cpp:244-291): every ghost must be a lattice image of the owned atom with its id
+ * and type, shifted along periodic directions only (checked to 1e-8 of the box scale, as AtomSet::validate); then the owned
+ * atoms are loaded as by rxa_set_atoms (the engine builds every image itself, and grad/peratom cover the owned atoms only:
+ * ghost forces stay zero). nranks = comm->nprocs: more than one rank returns RXA_EINVAL, since a sub-domain's ghosts are owned
+ * by other ranks and this engine evaluates whole periodic cells (the reference's multi-rank 'distributed' atom set). */
+int rxa_set_atoms_ghosted(rxa_context* ctx, int64_t nlocal, int64_t nghost, const int* types, const int64_t* tags, const double* x,
+                          int nranks);
+/* the same checks without a context (no device needed): RXA_OK, or the error code with its message copied to err */
+int rxa_check_ghosts(const double lo[3], const double hi[3], const double tilt[3], const int periodic[3], int64_t nlocal, int64_t nghost,
+                     const int* types, const int64_t* tags, const double* x, int nranks, char* err, long errcap);
 int rxa_set_positions(rxa_context* ctx, const double* x); /* 3n, owned atoms, same order */
 int rxa_set_charges(rxa_context* ctx, const double* q);   /* n */
 int rxa_get_charges(rxa_context* ctx, double* q);         /* n */

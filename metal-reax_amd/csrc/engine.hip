@@ -109,6 +109,44 @@ struct PhaseTimer {
   }
 };
 
+// LAMMPS' ghost-native atom set (nlocal owned atoms followed by nghost ghosts) checked against the explicit-image model: on one
+// rank every ghost must be an image of an owned atom with the same id and type, displaced by a whole lattice vector along
+// periodic directions only (the reference's host_atom_set, plugin/adapter/pair_reaxff_metal.cpp:244-291, and AtomSet::validate
+// at 1e-8). More than one rank is refused: the owners of a sub-domain's ghosts live on other ranks, and this engine evaluates
+// whole periodic cells. Throws ApiError with the first violation.
+inline void check_ghosts(const Cell& c, long long nlocal, long long nghost, const int* types, const int64_t* tags, const double* x,
+                         int nranks) {
+  if (nranks != 1)
+    throw ApiError(RXA_EINVAL, "multi-rank domain decomposition is not supported: this engine evaluates whole periodic cells on "
+                               "one rank (got " + std::to_string(nranks) + " ranks; run one MPI rank per system)");
+  if (nlocal <= 0 || nghost < 0) throw ApiError(RXA_EINVAL, "atom count out of range");
+  std::unordered_map<long long, long long> owner;
+  owner.reserve(2 * (size_t)nlocal);
+  for (long long i = 0; i < nlocal; ++i)
+    if (!owner.emplace(tags[i], i).second) throw ApiError(RXA_EINVAL, "duplicate atom id " + std::to_string(tags[i]) + " among owned atoms");
+  double scale = 1.0;
+  for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) scale = std::max(scale, std::fabs(c.v[a][b]));
+  for (long long g = nlocal; g < nlocal + nghost; ++g) {
+    const auto it = owner.find(tags[g]);
+    if (it == owner.end())
+      throw ApiError(RXA_EINVAL, "ghost " + std::to_string(g) + " (id " + std::to_string(tags[g]) + ") has no owned atom on this rank");
+    const long long o = it->second;
+    if (types[g] != types[o]) throw ApiError(RXA_EINVAL, "ghost " + std::to_string(g) + " has another type than its owned atom");
+    double shift[3];
+    for (int d = 0; d < 3; ++d) {
+      const double fg = frac_coord(c, d, x[3 * g], x[3 * g + 1], x[3 * g + 2]), fo = frac_coord(c, d, x[3 * o], x[3 * o + 1], x[3 * o + 2]);
+      shift[d] = std::nearbyint(fg - fo);
+      if (!c.per[d] && shift[d] != 0.0)
+        throw ApiError(RXA_EINVAL, "ghost " + std::to_string(g) + " is shifted along a non-periodic direction");
+    }
+    for (int k = 0; k < 3; ++k) {
+      const double img = x[3 * o + k] + shift[0] * c.v[0][k] + shift[1] * c.v[1][k] + shift[2] * c.v[2][k];
+      if (!(std::fabs(x[3 * g + k] - img) <= 1e-8 * scale))
+        throw ApiError(RXA_EINVAL, "ghost " + std::to_string(g) + " is not a lattice image of its owned atom (id " + std::to_string(tags[g]) + ")");
+    }
+  }
+}
+
 struct Engine {
   int device = 0;
   PhaseTimer timer;
@@ -1640,6 +1678,29 @@ int rxa_set_atoms(rxa_context* c, int64_t n, const int* types, const int64_t* ta
     e.set_atoms(n, types, tags, x);
     e.lmp_types.assign(types, types + n);
   });
+}
+
+int rxa_set_atoms_ghosted(rxa_context* c, int64_t nlocal, int64_t nghost, const int* types, const int64_t* tags, const double* x,
+                          int nranks) {
+  return guard(c, [&](Engine& e) {
+    if (!e.have_box) throw ApiError(RXA_ESTATE, "the box must be set before the atoms");
+    if (!types || !tags || !x) throw ApiError(RXA_EINVAL, "null atom arrays");
+    rxa::check_ghosts(e.cell, nlocal, nghost, types, tags, x, nranks);
+    e.set_atoms(nlocal, types, tags, x);   // the owned atoms; the engine generates every image itself
+  });
+}
+
+int rxa_check_ghosts(const double lo[3], const double hi[3], const double tilt[3], const int periodic[3], int64_t nlocal, int64_t nghost,
+                     const int* types, const int64_t* tags, const double* x, int nranks, char* err, long errcap) {
+  try {
+    const rxa::Cell cell = Engine::make_cell(lo, hi, tilt, periodic);
+    rxa::check_ghosts(cell, nlocal, nghost, types, tags, x, nranks);
+    if (err && errcap > 0) err[0] = 0;
+    return RXA_OK;
+  } catch (const ApiError& ex) {
+    if (err && errcap > 0) { std::strncpy(err, ex.what(), (size_t)errcap - 1); err[errcap - 1] = 0; }
+    return ex.code;
+  }
 }
 
 int rxa_set_systems(rxa_context* c, int nsys, const int64_t* counts, const double* boxes, const int* periodic, const int* types,

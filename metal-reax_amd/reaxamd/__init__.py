@@ -50,6 +50,8 @@ def lib():
             "rxa_table_dump": (LONG, [P, I, S, LONG]),
             "rxa_set_box": (I, [P, P, P, P, P]),
             "rxa_set_atoms": (I, [P, I64, P, P, P]),
+            "rxa_set_atoms_ghosted": (I, [P, I64, I64, P, P, P, I]),
+            "rxa_check_ghosts": (I, [P, P, P, P, I64, I64, P, P, P, I, S, LONG]),
             "rxa_set_positions": (I, [P, P]),
             "rxa_set_charges": (I, [P, P]),
             "rxa_get_charges": (I, [P, P]),
@@ -122,6 +124,19 @@ def check_pair_style(args, control_text=""):
         else:
             out[k] = v
     return out
+
+
+def check_ghosts(lo, hi, tilt, periodic, nlocal, types, tags, x, nranks=1):
+    """The ghost checks of rxa_set_atoms_ghosted without a context: (status, message)."""
+    lo, hi, tilt = (np.ascontiguousarray(v, dtype=np.float64) for v in (lo, hi, tilt))
+    per = np.ascontiguousarray([int(bool(v)) for v in periodic], dtype=np.int32)
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    g = np.ascontiguousarray(tags, dtype=np.int64)
+    xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    err = ctypes.create_string_buffer(512)
+    rc = lib().rxa_check_ghosts(_p(lo), _p(hi), _p(tilt), _p(per), int(nlocal), t.size - int(nlocal), _p(t), _p(g), _p(xx), int(nranks),
+                                err, 512)
+    return rc, err.value.decode()
 
 
 def box_from_lammps(lo, hi, tilt=(0.0, 0.0, 0.0)):
@@ -199,6 +214,14 @@ class Engine:
         xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
         self._chk(lib().rxa_set_atoms(self.h, t.size, _p(t), _p(g), _p(xx)))
         self.n = t.size
+
+    def set_atoms_ghosted(self, nlocal, types, tags, x, nranks=1):
+        """LAMMPS' owned + ghost arrays (length nlocal + nghost); the ghosts are checked and the owned atoms loaded."""
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        g = np.ascontiguousarray(tags, dtype=np.int64)
+        xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        self._chk(lib().rxa_set_atoms_ghosted(self.h, int(nlocal), t.size - int(nlocal), _p(t), _p(g), _p(xx), int(nranks)))
+        self.n = int(nlocal)
 
     def set_systems(self, systems):
         """Packs independent systems (objects with types, tags, x, lo, hi and optionally periodic) into this context."""

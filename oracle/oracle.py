@@ -36,6 +36,8 @@ def lib():
         L.orc_compute.argtypes = [P, L64, P, P, P, P, P, P, D, I, P, P, P, P, P, P]
         L.orc_qeq.argtypes = [P, L64, P, P, P, P, P, P, P, P, D, D, D, I, P, P, P, P, P, P]
         L.orc_qeq_matvec.argtypes = [P, L64, P, P, P, P, P, P, P, P, D, D, P, P]
+        L.orc_bond_table.restype = L64
+        L.orc_bond_table.argtypes = [P, L64, P, P, P, P, P, I, P, P, P, L64, P, P]
         _LIB = L
     return _LIB
 
@@ -114,6 +116,19 @@ class Oracle:
             "total_bo": tbo,
             "sbo": sbo,
         }
+
+    def bond_table(self, x, types, tags, box, periodic, enobonds=True):
+        """The owned atoms' bond table (the reference's BondTable, partners as owned atom indices): start, owner, bo, total_bo, nlp."""
+        x, t, g, b, p = self._sys(x, types, tags, box, periodic)
+        n = t.size
+        start = np.zeros(n + 1, dtype=np.int32)
+        tbo, nlp = np.zeros(n), np.zeros(n)
+        cap = 64 * n
+        owner, bo = np.zeros(cap, dtype=np.int32), np.zeros(cap)
+        nb = lib().orc_bond_table(self.h, n, _p(x), _p(t), _p(g), _p(b), _p(p), int(enobonds), _p(start), _p(owner), _p(bo), cap, _p(tbo), _p(nlp))
+        if nb < 0 or nb > cap:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return {"start": start, "owner": owner[:nb], "bo": bo[:nb], "total_bo": tbo, "nlp": nlp}
 
     def qeq(self, x, types, tags, box, periodic, swa=0.0, swb=10.0, tol=1e-6, imax=200, s0=None, t0=None, params=None):
         x, t, g, b, p = self._sys(x, types, tags, box, periodic)

@@ -595,6 +595,10 @@ struct Result {
   std::vector<double> total_bo;      // nall
   std::vector<int> c_bonds, c_angles, c_tors, c_hb, c_nb, c_sbo, c_lp;   // per owned atom
   std::vector<double> sbo;   // per owned atom: the SBO value whose branch c_sbo records (its distance to the branch thresholds)
+  // bond table of the owned atoms as the reference's BondTable (include/reaxmetal/bonded.hpp:32-37, filled at src/cpu/bonded.cpp:
+  // 782-792), partners given as owned atoms (the owner of the image): start[n+1], owner[], bo[], nlp[n]
+  std::vector<int> bt_start, bt_owner;
+  std::vector<double> bt_bo, bt_nlp;
   long long pairs_oo = 0, pairs_og = 0, pairs_self = 0, nbonds = 0;
 };
 
@@ -787,6 +791,13 @@ void bonded(const FF& ff, const Ctl& ctl, const Atoms& A, const Far& far, bool e
   R.c_bonds.assign(n, 0); R.c_angles.assign(n, 0); R.c_tors.assign(n, 0); R.c_hb.assign(n, 0); R.c_sbo.assign(n, 0); R.c_lp.assign(n, 0);
   R.sbo.assign(n, 0.0);
   for (size_t i = 0; i < n; ++i) { R.c_lp[i] = (int)(aq[i].Delta_e / 2.0); R.c_bonds[i] = (int)B[i].size(); }
+  R.bt_start.assign(n + 1, 0);
+  R.bt_owner.clear(); R.bt_bo.clear(); R.bt_nlp.assign(n, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    R.bt_start[i + 1] = R.bt_start[i] + (int)B[i].size();
+    for (const Bnd& b : B[i]) { R.bt_owner.push_back(A.owner[(size_t)b.nbr]); R.bt_bo.push_back(b.BO); }
+    R.bt_nlp[i] = aq[i].nlp;
+  }
   double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0;
 
   // bond energy: owned end with the smaller tag (coordinate tie-break for self images)
@@ -1508,6 +1519,32 @@ int orc_qeq(void* hp, long n, const double* x, const int* type, const long long*
     info[0] = is; info[1] = it; info[2] = rs; info[3] = rt;
     return 0;
   } catch (const std::exception& e) { return fail(e); }
+}
+
+// Bond table of the owned atoms after one bonded evaluation (the reference's BondTable of bonded.cpp:782-792, partners as owned
+// atom indices): start[n+1]; owner[], bo[] (capacity cap; the count is returned, -1 on error, and nothing is written past cap);
+// total_bo[n], nlp[n] (may be NULL).
+long orc_bond_table(void* hp, long n, const double* x, const int* type, const long long* tag, const double* box, const int* periodic,
+                    int enobonds, int* start, int* owner, double* bo, long cap, double* total_bo, double* nlp) {
+  try {
+    const Handle& h = *static_cast<Handle*>(hp);
+    Box b;
+    for (int c = 0; c < 3; ++c) b.o[c] = box[c];
+    for (int k = 0; k < 3; ++k) for (int c = 0; c < 3; ++c) b.v[k][c] = box[3 + 3 * k + c];
+    for (int d = 0; d < 3; ++d) b.per[d] = periodic[d] != 0;
+    const double nonb = h.ff.nonb_cut, bond = h.ctl.bond_cut, hb = h.ctl.hbond_cut;
+    std::vector<int64_t> tg(tag, tag + n);
+    Atoms A = expand(b, x, type, tg.data(), (size_t)n, std::max({nonb, hb, 2.0 * bond}));
+    Far F = far_list(A, nonb, bond);
+    Result R;
+    bonded(h.ff, h.ctl, A, F, enobonds != 0, R);
+    const long nb = R.bt_start[(size_t)n];
+    for (long i = 0; i <= n; ++i) start[i] = R.bt_start[(size_t)i];
+    for (long k = 0; k < std::min(nb, cap); ++k) { owner[k] = R.bt_owner[(size_t)k]; bo[k] = R.bt_bo[(size_t)k]; }
+    if (total_bo) for (long i = 0; i < n; ++i) total_bo[i] = R.total_bo[(size_t)i];
+    if (nlp) for (long i = 0; i < n; ++i) nlp[i] = R.bt_nlp[(size_t)i];
+    return nb;
+  } catch (const std::exception& e) { fail(e); return -1; }
 }
 
 // y = (diag(eta) + H) x with the stock half matrix (checker for the device matvec)

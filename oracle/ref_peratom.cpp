@@ -3,7 +3,9 @@
 //
 // Input: the case text of tests/golden/make_golden.py (origin / a / b / c / periodic / atoms N + "tag type x y z" lines /
 // charges ...). Output: "eatom tag e" and "vatom tag xx yy zz xy xz yz" per owned atom, the bonded and non-bonded tallies summed
-// and every image folded onto its owner (what the LAMMPS adapter's reverse communication does).
+// and every image folded onto its owner (what the LAMMPS adapter's reverse communication does); then the bond table the LAMMPS
+// analysis commands read (BondedOptions::bond_table): "bondatom tag total_bo nlp" per owned atom and "bond tag partner_tag bo"
+// per bond of an owned atom, partners named by their atom id (an image by its owner's).
 // The engine calls are the reference's public interface (include/reaxmetal/bonded.hpp:28, nonbonded.hpp:22 per_atom options;
 // system.hpp image expansion; neighbor.hpp far list), used the way tools/energy_tool.cpp:50-121 drives it.
 #include <array>
@@ -77,6 +79,7 @@ int main(int argc, char** argv) {
     const FarList far = build_far_list(a, cut);
     BondedOptions bo;
     bo.per_atom = true;
+    bo.bond_table = true;
     const BondedResult rb = compute_bonded_core(ff, ctl, a, far, bo);
     NonbondedOptions no;
     no.per_atom = true;
@@ -93,6 +96,13 @@ int main(int argc, char** argv) {
       std::printf("eatom %lld %.17g\n", static_cast<long long>(a.tag[i]), e[i]);
       std::printf("vatom %lld %.17g %.17g %.17g %.17g %.17g %.17g\n", static_cast<long long>(a.tag[i]), v[i][0], v[i][1], v[i][2],
                   v[i][3], v[i][4], v[i][5]);
+    }
+    const BondTable& t = rb.table;
+    for (std::size_t i = 0; i < nl; ++i) {
+      std::printf("bondatom %lld %.17g %.17g\n", static_cast<long long>(a.tag[i]), t.total_bo[i], t.nlp[i]);
+      for (std::int32_t k = t.start[i]; k < t.start[i + 1]; ++k)
+        std::printf("bond %lld %lld %.17g\n", static_cast<long long>(a.tag[i]),
+                    static_cast<long long>(a.tag[static_cast<std::size_t>(t.nbr[static_cast<std::size_t>(k)])]), t.bo[static_cast<std::size_t>(k)]);
     }
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "error: %s\n", ex.what());

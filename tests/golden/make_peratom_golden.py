@@ -7,7 +7,9 @@ For each C/H/O case of tests/golden/fixtures.json (same inputs, stand-in force f
 case order, the reference's per-atom energy (its ev_tally attribution: bond, triple-bond and C2 energies half to each end,
 torsion half to each atom of the central bond, H-bond a third to each atom of the triple, valence to the centre, lone pair and
 over/under to the atom, non-bonded half to each atom of the pair) and per-atom virial (xx yy zz xy xz yz), images folded onto
-their owners. It checks that the per-atom energies add up to the case's 13 reference terms.
+their owners, and the bond table the LAMMPS analysis commands read (the reference's BondTable: per owned atom its bonds as
+[partner atom index in case order, corrected BO] in the reference's order, total bond order and lone pairs). It checks that the
+per-atom energies add up to the case's 13 reference terms.
 """
 import json
 import os
@@ -37,16 +39,24 @@ def main():
         n = len(case["types"])
         e = [None] * n
         v = [None] * n
+        tbo = [None] * n
+        nlp = [None] * n
+        bonds = [[] for _ in range(n)]
         for line in res.stdout.splitlines():
             p = line.split()
             if p[0] == "eatom":
                 e[int(p[1]) - 1] = float(p[2])
             elif p[0] == "vatom":
                 v[int(p[1]) - 1] = [float(t) for t in p[2:8]]
-        assert all(a is not None for a in e) and all(a is not None for a in v), case["id"]
+            elif p[0] == "bondatom":
+                tbo[int(p[1]) - 1] = float(p[2])
+                nlp[int(p[1]) - 1] = float(p[3])
+            elif p[0] == "bond":
+                bonds[int(p[1]) - 1].append([int(p[2]) - 1, float(p[3])])
+        assert all(a is not None for a in e + v + tbo + nlp), case["id"]
         total = sum(case["ref_energy"].values())
         assert abs(sum(e) - total) <= 1e-9 * max(1.0, abs(total)), (case["id"], sum(e), total)
-        out["cases"][case["id"]] = {"eatom": e, "vatom": v}
+        out["cases"][case["id"]] = {"eatom": e, "vatom": v, "bonds": bonds, "total_bo": tbo, "nlp": nlp}
     with open(os.path.join(HERE, "peratom.json"), "w") as fh:
         json.dump(out, fh, separators=(",", ":"))
     print("wrote", len(out["cases"]), "cases")

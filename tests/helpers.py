@@ -53,3 +53,28 @@ def strain_virial(O, x, types, tags, box12, per, q, eps=1e-5):
             e.append(sum(O.compute(xs, types, tags, bs, per, q=q)["energy"].values()))
         w[k] = -(e[0] - e[1]) / (2 * eps)
     return w
+
+
+def lammps_ghosts(x, types, tags, lo, hi, tilt, per, shell):
+    """Owned atoms followed by LAMMPS-style ghosts: every periodic image (shifts in {-1, 0, 1}^3 along periodic directions) whose
+    fractional coordinates lie within `shell` (A, measured along each face normal) outside the cell, in a shuffled order.
+    Returns (types, tags, x) of length nlocal + nghost."""
+    x = np.asarray(x, dtype=np.float64)
+    V = np.array([[hi[0] - lo[0], 0, 0], [tilt[0], hi[1] - lo[1], 0], [tilt[1], tilt[2], hi[2] - lo[2]]], dtype=np.float64)
+    f = (x - np.asarray(lo)) @ np.linalg.inv(V)
+    hgt = abs(np.linalg.det(V)) / np.linalg.norm(np.cross(V[[1, 2, 0]], V[[2, 0, 1]]), axis=1)
+    w = shell / hgt
+    gx, gt, gg = [], [], []
+    rng = np.random.default_rng(11)
+    for s in np.array(np.meshgrid([-1, 0, 1], [-1, 0, 1], [-1, 0, 1], indexing="ij")).reshape(3, -1).T:
+        if not s.any() or any(s[d] != 0 and not per[d] for d in range(3)):
+            continue
+        fs = f + s
+        keep = np.all((fs >= -w) & (fs < 1 + w), axis=1)
+        gx.append(x[keep] + s @ V)
+        gt.append(np.asarray(types)[keep])
+        gg.append(np.asarray(tags)[keep])
+    gx, gt, gg = np.concatenate(gx), np.concatenate(gt), np.concatenate(gg)
+    order = rng.permutation(len(gg))
+    return (np.concatenate([types, gt[order]]).astype(np.int32), np.concatenate([tags, gg[order]]).astype(np.int64),
+            np.concatenate([x, gx[order]]))

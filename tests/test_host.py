@@ -192,3 +192,59 @@ def test_pvector_maps_terms_to_lammps_slots():
         nonzero |= want != 0
     # the goldens exercise every slot the 13 terms feed (emol and efi stay zero)
     assert nonzero.sum() == 12 and not nonzero[3] and not nonzero[12]
+
+
+def test_ghost_native_input_checks(golden):
+    """rxa_check_ghosts (the checks of rxa_set_atoms_ghosted, without a device) on LAMMPS-style owned + ghost arrays of a periodic
+    fixture: accepted as generated; refused, with the reason, for more than one rank, a ghost whose owner is not on this rank
+    (what a sub-domain's ghosts are under MPI), a ghost off the lattice by 1e-4 A, a type mismatch, a shift along a non-periodic
+    direction and a duplicate owned id."""
+    from helpers import case_inputs, lammps_ghosts
+    case = next(c for c in golden["cases"] if c["id"] == "cho_water_box_8")
+    x, types, tags, lo, hi, tilt, per, q, _ = case_inputs(case)
+    n = len(types)
+    T, G, X = lammps_ghosts(x, types, tags, lo, hi, tilt, per, shell=10.0)
+    assert len(T) > 4 * n
+    rc, msg = reaxamd.check_ghosts(lo, hi, tilt, per, n, T, G, X)
+    assert rc == 0, msg
+    rc, msg = reaxamd.check_ghosts(lo, hi, tilt, per, n, T, G, X, nranks=2)
+    assert rc == -1 and "multi-rank" in msg
+    G2 = G.copy(); G2[n + 3] = 10 ** 6
+    rc, msg = reaxamd.check_ghosts(lo, hi, tilt, per, n, T, G2, X)
+    assert rc == -1 and "no owned atom" in msg
+    X2 = X.copy(); X2[n + 5, 1] += 1e-4
+    rc, msg = reaxamd.check_ghosts(lo, hi, tilt, per, n, T, G, X2)
+    assert rc == -1 and "lattice image" in msg
+    T2 = T.copy(); T2[n + 7] = 1 + (T2[n + 7] % 3)
+    rc, msg = reaxamd.check_ghosts(lo, hi, tilt, per, n, T2, G, X)
+    assert rc == -1 and "type" in msg
+    rc, msg = reaxamd.check_ghosts(lo, hi, tilt, [1, 1, 0], n, T, G, X)
+    assert rc == -1 and "non-periodic" in msg
+    G3 = G.copy(); G3[1] = G3[0]
+    rc, msg = reaxamd.check_ghosts(lo, hi, tilt, per, n, T, G3, X)
+    assert rc == -1 and "duplicate" in msg
+
+
+def test_compiled_c_caller_builds_against_the_header(golden, tmp_path):
+    """tests/abi_caller.c includes include/reaxamd.h from C99 (compiled with -Wall -Wextra -Werror by __graft_entry__.build and
+    again here); without a device it reads its inputs and stops at rxa_create with RXA_EDEVICE (exit status 3)."""
+    import subprocess
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
+    from make_golden import case_text
+    exe = str(tmp_path / "abi_caller")
+    subprocess.run(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(REPO, "include"),
+                    os.path.join(REPO, "tests", "abi_caller.c"), "-L", os.path.join(REPO, "metal-reax_amd"), "-lreaxamd",
+                    "-Wl,-rpath," + os.path.join(REPO, "metal-reax_amd"), "-o", exe], check=True)
+    case = next(c for c in golden["cases"] if c["id"] == "cho_water_box_8")
+    path = tmp_path / "case.txt"
+    path.write_text(case_text(case["origin"], case["vectors"], case["periodic"], case["positions"], case["types"], case["charges"]))
+    res = subprocess.run([exe, os.path.join(REPO, "tests", "golden", "ffield.cho_standin"), "C,H,O", str(path)], capture_output=True,
+                         text=True, timeout=60)
+    import ctypes
+    h = ctypes.c_void_p()
+    if reaxamd.lib().rxa_create(ctypes.byref(h), 0) == 0:   # a GPU is present: the caller runs through
+        reaxamd.lib().rxa_destroy(h)
+        assert res.returncode == 0 and res.stdout.count("grad ") == len(case["types"]), res.stderr
+    else:
+        assert res.returncode == 3 and "no device" in res.stderr

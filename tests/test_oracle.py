@@ -182,3 +182,29 @@ def test_strain_virial_of_an_open_molecule(golden, ffield_text):
         direct = np.array([-(x[:, a] * g[:, b]).sum() for a, b in VOIGT])
         fd = strain_virial(O, x, types - 1, tags, box12, per, q)
         assert np.abs(fd - direct).max() <= 1e-5 * np.abs(direct).max() + 1e-5, (cid, fd, direct)
+
+
+def test_oracle_bond_table_matches_reference(golden, ffield_text):
+    """The oracle's bond table (what fix reaxff/bonds and fix reaxff/species read) against the reference engine's BondTable on the
+    41 C/H/O fixtures (tests/golden/peratom.json, written by the reference's own engine): per owned atom the same partners in the
+    same order, corrected bond orders, total bond order and lone pairs within 1e-12."""
+    import json
+    from helpers import case_inputs
+    with open(os.path.join(REPO, "tests", "golden", "peratom.json")) as fh:
+        ref = json.load(fh)["cases"]
+    O = Oracle(ffield_text)
+    nb = 0
+    for case in golden["cases"]:
+        if case["id"] not in ref:
+            continue
+        x, types, tags, lo, hi, tilt, per, q, box12 = case_inputs(case)
+        bt = O.bond_table(x, types - 1, tags, box12, per)
+        r = ref[case["id"]]
+        for i, bonds in enumerate(r["bonds"]):
+            a, b = bt["start"][i], bt["start"][i + 1]
+            assert bt["owner"][a:b].tolist() == [p for p, _ in bonds], (case["id"], i)
+            assert np.allclose(bt["bo"][a:b], [v for _, v in bonds], rtol=1e-12, atol=1e-13), (case["id"], i)
+            nb += b - a
+        assert np.allclose(bt["total_bo"], r["total_bo"], rtol=1e-12, atol=1e-13), case["id"]
+        assert np.allclose(bt["nlp"], r["nlp"], rtol=1e-12, atol=1e-13), case["id"]
+    assert nb > 5000
