@@ -2,11 +2,14 @@
 
 One step = one NVT velocity-Verlet step (Nose-Hoover chain, 300 K, dt 0.25 fs) with the stock fix's charge update every step
 (history extrapolation + both CG solves at tol 1e-6) and all 13 ReaxFF energy terms and forces, in the engine's mixed precision
-(FP64 bond-order chain, thresholds, accumulations and charge solve; FP32 term arithmetic). Neighbour lists use a 2 A skin and are
-rebuilt when an atom moved more than 1 A. Inputs are resident on the device during the timed region.
+(FP64 bond-order chain, thresholds, accumulations and charge solve refined in FP64; FP32 term arithmetic). Neighbour lists use a
+2 A skin and are rebuilt when an atom moved more than 1 A. Inputs are resident on the device during the timed region.
+
+The box starts from a lattice, so before the warm-up it is equilibrated at 300 K (untimed: strong thermostat coupling, then the
+run's own). After the timed steps, an untimed longer window measures the neighbour-rebuild interval R of the equilibrated liquid.
 
 Prints one JSON line: the throughput (atom-steps/s), the roofline record of the dominant kernel measured live with HIP events on
-the engine's stream, and a CPU baseline (the FP64 oracle, one core, on a bounded sample of the same workload).
+the engine's stream, and a CPU baseline (the FP64 oracle on a bounded sample of the same trajectory, one core and all cores).
 """
 import argparse
 import json
@@ -106,25 +109,51 @@ def batch_and_serving(ffield_text, precision, slots=16):
     return out
 
 
-def cpu_baseline(ffield_text, sysm, budget_s=25.0):
-    """The FP64 oracle (serial, one core): charge solve + all terms + forces of the 24,000-atom system, repeated while the
-    time budget lasts (at least once)."""
-    from oracle.oracle import Oracle
+def cpu_baseline(ffield_text, sysm, traj, seed_steps=4):
+    """The FP64 oracle on consecutive positions of the engine's own equilibrated trajectory: for each, the stock fix's charge
+    update (history extrapolation s = 4(s0 + s2) - (6 s1 + s3), t = t2 + 3(t0 - t1) from the oracle's own previous solutions, both
+    solves at tol 1e-6) and all 13 terms and forces. The first `seed_steps` positions only fill the history (untimed); the rest
+    are timed, on one core and, with the OpenMP build, on every host core. Also returns the oracle's iteration count per step
+    (max of the two solves), the k of the step's byte model."""
+    from oracle.oracle import Oracle, omp_threads
     O = Oracle(ffield_text)
-    steps = 0
-    t0 = time.perf_counter()
-    s_prev = t_prev = None
-    while True:
-        ro = O.qeq(sysm.x, sysm.types - 1, sysm.tags, sysm.box12(), sysm.periodic, tol=1e-6, imax=200, s0=s_prev, t0=t_prev)
-        O.compute(sysm.x, sysm.types - 1, sysm.tags, sysm.box12(), sysm.periodic, q=ro["q"])
-        s_prev, t_prev = ro["s"], ro["t"]
-        steps += 1
-        if time.perf_counter() - t0 > budget_s * 0.5 or steps >= 20:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": sysm.n * steps / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d step(s) of charge solve (tol 1e-6, warm start) + all 13 terms and forces of the same 24,000-atom water system, "
-                      "FP64 oracle (oracle/reaxff_oracle.cpp), serial, %.1f s" % (steps, dt)}
+    hs, ht = [], []
+    kk = []
+
+    def step(x, timed):
+        guess_s = guess_t = None
+        if len(hs) == 4:
+            guess_s = 4 * (hs[0] + hs[2]) - (6 * hs[1] + hs[3])
+            guess_t = ht[2] + 3 * (ht[0] - ht[1])
+        ro = O.qeq(x, sysm.types - 1, sysm.tags, sysm.box12(), sysm.periodic, tol=1e-6, imax=200, s0=guess_s, t0=guess_t)
+        O.compute(x, sysm.types - 1, sysm.tags, sysm.box12(), sysm.periodic, q=ro["q"])
+        hs.insert(0, ro["s"]); ht.insert(0, ro["t"])
+        del hs[4:], ht[4:]
+        if timed:
+            kk.append(max(ro["iters_s"], ro["iters_t"]))
+
+    for x in traj[:seed_steps]:
+        step(x, False)
+    seeded = (list(hs), list(ht))
+    out = {}
+    for label, threads in (("serial", 1), ("omp", omp_threads())):
+        hs[:], ht[:] = list(seeded[0]), list(seeded[1])   # both passes continue from the same history
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        for x in traj[seed_steps:]:
+            step(x, label == "serial")
+        out[label] = (time.perf_counter() - t0, threads)
+    O.set_threads(1)
+    n_t = len(traj) - seed_steps
+    dt1, _ = out["serial"]
+    dtn, cores = out["omp"]
+    return {"value": sysm.n * n_t / dtn, "unit": "atom-steps/s", "cores": cores, "kind": "port",
+            "sample": "%d consecutive steps of the engine's equilibrated %d-atom water trajectory (stock history extrapolation "
+                      "from the oracle's own solutions after %d seeding steps, both solves at tol 1e-6, all 13 terms and forces), "
+                      "FP64 oracle (oracle/reaxff_oracle.cpp, cpu64 restatement, not stock LAMMPS): %.1f s on %d cores (OpenMP "
+                      "far list, non-bonded pass and charge solve, fixed-order per-thread sums; bonded terms serial); %.1f s on "
+                      "one core" % (n_t, sysm.n, seed_steps, dtn, cores, dt1),
+            "serial_value": sysm.n * n_t / dt1, "oracle_iters_per_step": float(np.mean(kk))}
 
 
 def main():
@@ -134,6 +163,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed", "fp32"))
     ap.add_argument("--nside", type=int, default=20, help="water molecules per box edge (20 -> 24,000 atoms)")
+    ap.add_argument("--equil", type=int, default=2000, help="untimed equilibration steps from the lattice before the warm-up")
+    ap.add_argument("--long-window", type=int, default=400, help="untimed steps after the timed region that measure R")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (512 systems) and served (Poisson trace) runs")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -157,8 +188,13 @@ def main():
     e.set_atoms(w.types, w.tags, w.x)
     e.qeq_setup(0.0, 10.0, e.extract("chi"), e.extract("eta"), e.extract("gamma"))
     masses = np.array([0.0] + [systems.MASS[el] for el in systems.ELEMENTS])
-    e.md_setup(v, masses, dt=0.25, t_target=300.0, t_damp=25.0, skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=1e-6)
-    print("bench: %d atoms, warmup %d steps" % (w.n, args.warmup), file=sys.stderr, flush=True)
+    # equilibration from the lattice (untimed): strong coupling first, then the run's own thermostat from the state reached
+    e.md_setup(v, masses, dt=0.25, t_target=300.0, t_damp=5.0, skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=1e-6)
+    th_eq = e.md_run(args.equil) if args.equil else None
+    _, v_eq, _ = e.state()
+    e.md_setup(v_eq, masses, dt=0.25, t_target=300.0, t_damp=25.0, skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=1e-6)
+    print("bench: %d atoms, equilibrated %d steps (%s), warmup %d steps" % (w.n, args.equil, th_eq, args.warmup), file=sys.stderr,
+          flush=True)
     if args.warmup:
         e.md_run(args.warmup)
 
@@ -184,14 +220,31 @@ def main():
     e.md_run(nb_steps)
     prof = e.profile_read()
     e.profile(False)
+    # untimed: a longer window of the equilibrated liquid for the rebuild interval R (and its own step time, rebuilds included)
+    t1 = time.perf_counter()
+    th_long = e.md_run(args.long_window) if args.long_window else {"rebuilds": 0.0}
+    sync()
+    long_s = time.perf_counter() - t1
+    # consecutive positions of the trajectory for the CPU baseline and the oracle's iteration count
+    traj = []
+    if not args.no_cpu_baseline:
+        for _ in range(8):
+            e.md_run(1)
+            traj.append(e.state()[0].copy())
 
     n = w.n
     value = n * args.steps / elapsed
-    # workload statistics for the byte model
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline (FP64 oracle, one core and all cores)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(ffield_text, w, traj)
+    # workload statistics for the byte model: k is the FP64 oracle's iteration count on the same trajectory (the device's own
+    # launch count is reported beside it), R the rebuild interval of the long window
     n_f = st["h_entries"] / n / 2.0                      # half of the full charge-matrix rows (pairs within 10 A)
     B = st["bond_slots"] / n
-    k = spmv_cnt / args.steps
-    R = args.steps / max(1.0, thermo["rebuilds"])
+    k_dev = spmv_cnt / args.steps
+    k = cpu["oracle_iters_per_step"] if cpu else k_dev
+    R = args.long_window / th_long["rebuilds"] if th_long["rebuilds"] else float(max(args.long_window, args.steps))
     n_h = st["hbond_entries"] / max(1, (w.types == 2).sum())        # acceptors within hbond_cut per donor hydrogen
     bytes_step = model_bytes_per_atom_step(n_f, R, k, B, n_h, (w.types == 2).mean())
 
@@ -227,8 +280,13 @@ def main():
                      "note": "timed region: the bonded chain runs concurrently on a second stream; every 4th launch timed",
                      "isolated_avg_launch_us": iso_s * 1e6, "isolated_frac": alg / iso_s / 1e9 / HBM_PEAK_GBS},
         "step_model": {"bytes_per_atom_step": bytes_step, "n_f": n_f, "R": R, "k": k, "B": B, "n_h": n_h,
-                       "roofline_atom_steps_per_s": HBM_PEAK_GBS * 1e9 / bytes_step,
+                       "k_source": "FP64 oracle iterations per step on the same trajectory" if cpu else "device launches per step",
+                       "k_device_launches": k_dev, "roofline_atom_steps_per_s": HBM_PEAK_GBS * 1e9 / bytes_step,
                        "frac_of_step_roofline": value / (HBM_PEAK_GBS * 1e9 / bytes_step)},
+        "timed_window": {"rebuilds": thermo["rebuilds"], "temp_K": thermo["temp"]},
+        "long_window": {"steps": args.long_window, "rebuilds": th_long["rebuilds"], "R": R,
+                        "ms_per_step": 1e3 * long_s / max(1, args.long_window),
+                        "atom_steps_per_s": n * args.long_window / long_s if args.long_window else None},
         "phase_ms_per_step": phase_ms,
         "thermo": thermo,
     }
@@ -239,9 +297,8 @@ def main():
             out.update(batch_and_serving(ffield_text, args.precision))
         except Exception as ex:   # the 24k line must still print
             print("bench: batch/serving failed: %s" % ex, file=sys.stderr, flush=True)
-    if not args.no_cpu_baseline:
-        print("bench: cpu baseline (FP64 oracle, one core)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(ffield_text, w)
+    if cpu:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

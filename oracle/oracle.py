@@ -36,10 +36,17 @@ def lib():
         L.orc_compute.argtypes = [P, L64, P, P, P, P, P, P, D, I, P, P, P, P, P, P]
         L.orc_qeq.argtypes = [P, L64, P, P, P, P, P, P, P, P, D, D, D, I, P, P, P, P, P, P]
         L.orc_qeq_matvec.argtypes = [P, L64, P, P, P, P, P, P, P, P, D, D, P, P]
+        L.orc_set_threads.argtypes = [I]
+        L.orc_max_threads.restype = I
         L.orc_bond_table.restype = L64
         L.orc_bond_table.argtypes = [P, L64, P, P, P, P, P, I, P, P, P, L64, P, P]
         _LIB = L
     return _LIB
+
+
+def omp_threads():
+    """the threads the OpenMP build would use for all host cores (OMP_NUM_THREADS, else the cores)"""
+    return int(lib().orc_max_threads())
 
 
 def _p(a):
@@ -59,6 +66,10 @@ class Oracle:
         if getattr(self, "h", None):
             lib().orc_destroy(self.h)
             self.h = None
+
+    def set_threads(self, threads):
+        """OpenMP threads of the oracle's parallel loops, for every handle of this process (1: the serial oracle)."""
+        lib().orc_set_threads(int(threads))
 
     def table_dump(self, portable=False):
         L = lib()

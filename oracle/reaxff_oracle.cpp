@@ -15,6 +15,11 @@
 // The traversal orders are those of the reference (they decide which bond/angle/torsion is seen first and which end of a bond
 // counts its energy); the code itself is written for this project. Built with -ffp-contract=off -fno-fast-math, as the
 // reference builds its oracle (cmake/ReaxMetalFlags.cmake:16-22).
+//
+// Threads (orc_set_threads, default 1): the far list, the non-bonded pass, the charge matrix and its product run over OpenMP
+// threads for the multi-core CPU baseline. Rows are independent, and every scatter goes into a per-thread buffer that is summed
+// in thread order after a static schedule, so results repeat for a given thread count; at one thread every sum runs in the
+// serial order, bit for bit the serial oracle (the tests use one thread).
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -27,8 +32,21 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace orc {
+
+int g_threads = 1;   // OpenMP threads of the parallel loops (orc_set_threads)
+inline int nthreads() { return g_threads; }
+inline int thread_id() {
+#ifdef _OPENMP
+  return omp_get_thread_num();
+#else
+  return 0;
+#endif
+}
 
 struct Err : std::runtime_error { using std::runtime_error::runtime_error; };
 
@@ -545,8 +563,7 @@ Far far_list(const Atoms& a, double nonb, double bond) {
     members[cell_of[i]].push_back((int)i);
   }
   struct E { int j; double d, v[3]; };
-  std::vector<E> row;
-  for (size_t i = 0; i < N; ++i) {
+  auto make_row = [&](size_t i, std::vector<E>& row) {
     const double rc = i < a.nlocal ? nonb : bond, rc2 = rc * rc;
     int c0 = cell_of[i] % nc[0], c1 = (cell_of[i] / nc[0]) % nc[1], c2 = cell_of[i] / (nc[0] * nc[1]);
     row.clear();
@@ -563,8 +580,18 @@ Far far_list(const Atoms& a, double nonb, double bond) {
             row.push_back(e);
           }
     std::sort(row.begin(), row.end(), [](const E& p, const E& q) { return p.j < q.j; });
-    for (auto& e : row) { F.nbr.push_back(e.j); F.d.push_back(e.d); F.dv.insert(F.dv.end(), e.v, e.v + 3); }
-    F.start.push_back(F.nbr.size());
+  };
+  // rows of a chunk are built independently (in parallel), then appended in row order
+  const size_t chunk = 4096;
+  std::vector<std::vector<E>> rows(std::min(chunk, N));
+  for (size_t i0 = 0; i0 < N; i0 += chunk) {
+    const long m = (long)std::min(chunk, N - i0);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads())
+    for (long k = 0; k < m; ++k) make_row(i0 + (size_t)k, rows[(size_t)k]);
+    for (long k = 0; k < m; ++k) {
+      for (auto& e : rows[(size_t)k]) { F.nbr.push_back(e.j); F.d.push_back(e.d); F.dv.insert(F.dv.end(), e.v, e.v + 3); }
+      F.start.push_back(F.nbr.size());
+    }
   }
   return F;
 }
@@ -1244,10 +1271,20 @@ void nonbonded(const FF& ff, double nonb, const Atoms& A, const Far& far, const 
   const double p_vdW1 = ff.gp[28];
   const int vt = ff.vdw_type;
   const Taper T = taper(ff.nonb_low, ff.nonb_cut);
-  std::vector<double> g(3 * N, 0.0);
   R.c_nb.assign(n, 0);
-  double evdw = 0, eele = 0, epol = 0;
-  for (size_t i = 0; i < n; ++i) {
+  double epol = 0;
+  // per-thread gradient, energies and pair counts (a static schedule, summed in thread order below)
+  const int nth = nthreads();
+  std::vector<std::vector<double>> gth((size_t)nth, std::vector<double>(3 * N, 0.0));
+  std::vector<double> evdw_t((size_t)nth, 0.0), eele_t((size_t)nth, 0.0);
+  std::vector<long long> poo_t((size_t)nth, 0), pog_t((size_t)nth, 0), pself_t((size_t)nth, 0);
+#pragma omp parallel for schedule(static) num_threads(nth)
+  for (long ii = 0; ii < (long)n; ++ii) {
+    const size_t i = (size_t)ii;
+    const int tid = thread_id();
+    double* g = gth[(size_t)tid].data();
+    double& evdw = evdw_t[(size_t)tid];
+    double& eele = eele_t[(size_t)tid];
     const int ti = A.type[i];
     if (ti < 0) continue;
     for (size_t p = far.start[i]; p < far.start[i + 1]; ++p) {
@@ -1259,7 +1296,7 @@ void nonbonded(const FF& ff, double nonb, const Atoms& A, const Far& far, const 
       const double* dv = &far.dv[3 * p];
       const int cls = pair_class(j, n, A.tag[i], A.tag[j], dv);
       if (!cls) continue;
-      if (cls == 1) ++R.pairs_oo; else if (cls == 2) ++R.pairs_og; else ++R.pairs_self;
+      if (cls == 1) ++poo_t[(size_t)tid]; else if (cls == 2) ++pog_t[(size_t)tid]; else ++pself_t[(size_t)tid];
       ++R.c_nb[i];
       const Pair2& t = ff.P(ti, tj);
       double Tap, dTap;
@@ -1297,6 +1334,13 @@ void nonbonded(const FF& ff, double nonb, const Atoms& A, const Far& far, const 
       for (int c = 0; c < 3; ++c) { g[3 * i + c] += -CE * dv[c]; g[3 * j + c] += CE * dv[c]; }
     }
   }
+  std::vector<double> g(3 * N, 0.0);
+  double evdw = 0, eele = 0;
+  for (int t = 0; t < nth; ++t) {
+    for (size_t k = 0; k < 3 * N; ++k) g[k] += gth[(size_t)t][k];
+    evdw += evdw_t[(size_t)t]; eele += eele_t[(size_t)t];
+    R.pairs_oo += poo_t[(size_t)t]; R.pairs_og += pog_t[(size_t)t]; R.pairs_self += pself_t[(size_t)t];
+  }
   for (size_t i = 0; i < n; ++i) {
     if (A.type[i] < 0) continue;
     const Atom1& s = ff.at[A.type[i]];
@@ -1328,8 +1372,9 @@ Qeq qeq_matrix(const FF& ff, const Atoms& A, const Far& far, double swa, double 
   const int nt = ff.n();
   std::vector<double> shld((size_t)nt * nt);
   for (int a = 0; a < nt; ++a) for (int b = 0; b < nt; ++b) shld[(size_t)a * nt + b] = std::pow(gam_t[a] * gam_t[b], -1.5);
-  for (size_t i = 0; i < n; ++i) {
-    Q.eta[i] = eta_t[A.type[i]];
+  struct Ent { int col; double val; };
+  auto make_row = [&](size_t i, std::vector<Ent>& row) {
+    row.clear();
     for (size_t p = far.start[i]; p < far.start[i + 1]; ++p) {
       const size_t j = (size_t)far.nbr[p];
       const double* dv = &far.dv[3 * p];
@@ -1340,22 +1385,55 @@ Qeq qeq_matrix(const FF& ff, const Atoms& A, const Far& far, double swa, double 
       double Tap = T.c[7] * r + T.c[6];
       Tap = Tap * r + T.c[5]; Tap = Tap * r + T.c[4]; Tap = Tap * r + T.c[3]; Tap = Tap * r + T.c[2]; Tap = Tap * r + T.c[1]; Tap = Tap * r + T.c[0];
       const double den = std::pow(r * r * r + shld[(size_t)A.type[i] * nt + A.type[j]], 1.0 / 3.0);
-      Q.col.push_back(A.owner[j]);
-      Q.val.push_back(Tap * 14.4 / den);
+      row.push_back(Ent{A.owner[j], Tap * 14.4 / den});
     }
-    Q.start.push_back(Q.col.size());
+  };
+  // rows of a chunk in parallel, appended in row order
+  const size_t chunk = 4096;
+  std::vector<std::vector<Ent>> rows(std::min(chunk, std::max<size_t>(n, 1)));
+  for (size_t i0 = 0; i0 < n; i0 += chunk) {
+    const long m = (long)std::min(chunk, n - i0);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads())
+    for (long k = 0; k < m; ++k) make_row(i0 + (size_t)k, rows[(size_t)k]);
+    for (long k = 0; k < m; ++k) {
+      const size_t i = i0 + (size_t)k;
+      Q.eta[i] = eta_t[A.type[i]];
+      for (const Ent& e : rows[(size_t)k]) { Q.col.push_back(e.col); Q.val.push_back(e.val); }
+      Q.start.push_back(Q.col.size());
+    }
   }
   return Q;
 }
 
 static void qeq_mv(const Qeq& Q, const double* x, double* y, size_t n) {
-  for (size_t i = 0; i < n; ++i) y[i] = Q.eta[i] * x[i];
-  for (size_t i = 0; i < n; ++i)
+  const int nth = nthreads();
+  if (nth == 1) {   // the serial order
+    for (size_t i = 0; i < n; ++i) y[i] = Q.eta[i] * x[i];
+    for (size_t i = 0; i < n; ++i)
+      for (size_t p = Q.start[i]; p < Q.start[i + 1]; ++p) {
+        const size_t j = (size_t)Q.col[p];
+        y[i] += Q.val[p] * x[j];
+        y[j] += Q.val[p] * x[i];
+      }
+    return;
+  }
+  // per-thread partial products over a static split of the rows, summed in thread order onto the diagonal term
+  std::vector<double> buf((size_t)nth * n, 0.0);
+#pragma omp parallel for schedule(static) num_threads(nth)
+  for (long ii = 0; ii < (long)n; ++ii) {
+    const size_t i = (size_t)ii;
+    double* yt = buf.data() + (size_t)thread_id() * n;
     for (size_t p = Q.start[i]; p < Q.start[i + 1]; ++p) {
       const size_t j = (size_t)Q.col[p];
-      y[i] += Q.val[p] * x[j];
-      y[j] += Q.val[p] * x[i];
+      yt[i] += Q.val[p] * x[j];
+      yt[j] += Q.val[p] * x[i];
     }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    double v = Q.eta[i] * x[i];
+    for (int t = 0; t < nth; ++t) v += buf[(size_t)t * n + i];
+    y[i] = v;
+  }
 }
 
 // the stock CG loop (fix_qeq_reaxff.cpp CG; kept line for line by the reference at fix_qeq_reaxff_metal.cpp:90-123).
@@ -1415,6 +1493,16 @@ void* orc_create(const char* ffield, const char* control, int lgvdw) {
   } catch (const std::exception& e) { fail(e); return nullptr; }
 }
 void orc_destroy(void* h) { delete static_cast<Handle*>(h); }
+
+// threads of the parallel loops (1 = the serial oracle); the most this build can use
+void orc_set_threads(int t) { g_threads = t < 1 ? 1 : t; }
+int orc_max_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 // canonical table dump into buf (returns the full length; truncated if cap is too small)
 long orc_table_dump(void* hp, int portable, char* buf, long cap) {
