@@ -198,11 +198,13 @@ def main():
     if args.warmup:
         e.md_run(args.warmup)
 
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None
+    # device-wide synchronisation on the HIP runtime the engine runs on (md_run also returns only after its streams drained)
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+
+    def sync():
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
     # timed region: the MD loop; only the SpMV launches carry HIP events (the roofline kernel), every 4th of them (a launch
     # bracketed by events cannot overlap its neighbours as freely; sampling keeps that perturbation of the timed loop small)
     e.profile(True, phases=["qeq_spmv"], every=4)
