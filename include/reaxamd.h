@@ -122,6 +122,14 @@ long rxa_bond_table(rxa_context* ctx, int32_t* start, int32_t* nbr, double* bo, 
 int rxa_qeq_setup(rxa_context* ctx, double swa, double swb, const double* chi, const double* eta, const double* gamma);
 /* y = (diag(eta) + H) x over the owned atoms (the fix's sparse_matvec; images folded) */
 int rxa_qeq_matvec(rxa_context* ctx, const double* x, double* y);
+/* The charge-matrix product y = (diag(eta) + H) x of rxa_qeq_matvec on DEVICE vectors in the caller's order (n doubles each),
+ * allocating nothing: no counterpart in the reference; for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Its staging vectors live in a file-scope device array of the library
+ * (2 x 262144 doubles, 4 MiB per device, held until the process ends): more than 262144 atoms returns RXA_EINVAL. The matrix
+ * of the current positions must already be built (rxa_qeq_matvec, rxa_qeq_solve, rxa_qeq_charges or an MD step), else
+ * RXA_ESTATE. Work is queued on the context's main stream (rxa_stream) and not waited for. Every context on one device shares
+ * that device's array, so calls that overlap on one GPU race. */
+int rxa_qeq_matvec_noalloc(rxa_context* ctx, const double* x_dev, double* y_dev);
 /* both systems at once (b, x0, x: 2n = system s then system t); iters[2] = the stock loop index at exit, rel[2] = sqrt(r.M^-1 r)/|b|,
  * status[2]: 1 converged, 3 iteration limit (the reference's CgOutcome, metal_backend.hpp:84-85) */
 int rxa_qeq_solve(rxa_context* ctx, const double* b, const double* x0, int imax, double tol, double* x, int* iters, double* rel, int* status);

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1550,6 +1551,41 @@ struct rxa_context {
   std::string err;
 };
 
+// ---- allocation-free charge-matrix product (rxa_qeq_matvec_noalloc) ----------------------------------------------------------
+// Its staging vectors (the caller's vector and the product, both in engine order) live in a file-scope device array instead of
+// engine allocations: kNoAllocMaxAtoms atoms at most, 2 * kNoAllocMaxAtoms doubles (4 MiB) on each device the library runs on,
+// held until the process ends. Host code never uses the array's name as an address: rxa_create looks its address up once per
+// device (hipGetSymbolAddress, with that device current) and the entry point passes that address to the kernels.
+namespace rxa {
+constexpr int kNoAllocMaxAtoms = 262144;
+static __device__ __align__(256) double scratch[2 * (size_t)kNoAllocMaxAtoms];
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+namespace {
+__global__ void __launch_bounds__(256) k_perm_gather(int n, const int* __restrict__ perm, const double* __restrict__ src, double* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void __launch_bounds__(256) k_perm_scatter(int n, const int* __restrict__ perm, const double* __restrict__ src, double* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
+}
+}  // namespace
+
+// the device address of `scratch` on the current device (device index dev), looked up once
+int scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return RXA_EDEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[dev]) return RXA_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return RXA_EDEVICE;
+  g_scratch_dev[dev] = static_cast<double*>(p);
+  return RXA_OK;
+}
+}  // namespace rxa
+
 namespace {
 template <class F>
 int guard(rxa_context* c, F&& f) {
@@ -1583,7 +1619,8 @@ int rxa_create(rxa_context** out, int device) {
   if (device < 0 || device >= count) return RXA_EINVAL;
   try {
     auto c = std::make_unique<rxa_context>();
-    c->e = std::make_unique<Engine>(device);
+    c->e = std::make_unique<Engine>(device);   // makes `device` current
+    if (rxa::scratch_init(device) != RXA_OK) return RXA_EDEVICE;
     *out = c.release();
     return RXA_OK;
   } catch (...) {
@@ -1866,6 +1903,26 @@ int rxa_qeq_matvec(rxa_context* c, const double* x, double* y) {
     HIPCHK(hipMemcpyAsync(yo.data(), e.qb.r, n * sizeof(double), hipMemcpyDeviceToHost, e.stream));
     e.sync();
     for (int i = 0; i < n; ++i) y[e.perm_h[i]] = yo[i];
+  });
+}
+
+int rxa_qeq_matvec_noalloc(rxa_context* c, const double* x_dev, double* y_dev) {
+  return guard(c, [&](Engine& e) {
+    const int n = e.n;
+    if (!n) throw ApiError(RXA_ESTATE, "no atoms");
+    if (n > rxa::kNoAllocMaxAtoms) throw ApiError(RXA_EINVAL, "rxa_qeq_matvec_noalloc: more than 262144 atoms (the size of its scratch)");
+    if (!x_dev || !y_dev) throw ApiError(RXA_EINVAL, "rxa_qeq_matvec_noalloc: null vector");
+    if (e.packed || !e.qeq_ready || !e.qeq_h_valid)
+      throw ApiError(RXA_ESTATE, "rxa_qeq_matvec_noalloc: the charge matrix of the current positions is not built (call rxa_qeq_matvec, "
+                                 "rxa_qeq_solve, rxa_qeq_charges or an MD step first)");
+    double* const sx = (e.device >= 0 && e.device < rxa::kMaxDevices) ? rxa::g_scratch_dev[e.device] : nullptr;
+    if (!sx) throw ApiError(RXA_EDEVICE, "rxa_qeq_matvec_noalloc: scratch address of this device unknown");
+    double* const sy = sx + rxa::kNoAllocMaxAtoms;
+    const dim3 blk(256), grid((n + 255) / 256);
+    hipLaunchKernelGGL(rxa::k_perm_gather, grid, blk, 0, e.stream, n, (const int*)e.d_perm.p, x_dev, sx);
+    rxa::launch_qeq_matvec1(e.st, e.qb, sx, sy, e.stream);
+    hipLaunchKernelGGL(rxa::k_perm_scatter, grid, blk, 0, e.stream, n, (const int*)e.d_perm.p, (const double*)sy, y_dev);
+    HIPCHK(hipGetLastError());
   });
 }
 

@@ -63,6 +63,7 @@ def lib():
             "rxa_bond_table": (LONG, [P, P, P, P, P, P]),
             "rxa_qeq_setup": (I, [P, D, D, P, P, P]),
             "rxa_qeq_matvec": (I, [P, P, P]),
+            "rxa_qeq_matvec_noalloc": (I, [P, P, P]),
             "rxa_qeq_solve": (I, [P, P, P, I, D, P, P, P, P]),
             "rxa_qeq_charges": (I, [P, I, D, I, P, P, P]),
             "rxa_qeq_reset_history": (I, [P]),
