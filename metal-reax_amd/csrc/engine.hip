@@ -1082,7 +1082,7 @@ struct Engine {
     const size_t nmv = 4 * (size_t)qeq_mv_partials(n);
     const size_t nval = qb.fp32 ? 0 : (size_t)n * maxnb;   // FP64 values (FP64 mode only)
     const size_t sz = (nval ? rnd(nval) : 0) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + 2 * rnd(nmv) +
-                      2 * rnd(4 * (size_t)n);
+                      2 * rnd(4 * (size_t)n) + rnd(n) + rnd(2 * (size_t)n) + rnd(3 * kCgBlocks);
     std::vector<double> keep;
     const bool had_hist = q_dbl.p && hist_count > 0 && q_dbl_n == n;   // history of the same atoms only
     if (had_hist) {
@@ -1110,6 +1110,9 @@ struct Engine {
     qb.partial_rs = take(nmv);
     qb.s_hist = take(4 * (size_t)n);
     qb.t_hist = take(4 * (size_t)n);
+    qb.lorow = take(n);
+    qb.x0 = take(2 * (size_t)n);
+    qb.partial_gate = take(3 * kCgBlocks);
     static_assert(2 * sizeof(int) <= sizeof(double), "");
     if (had_hist) {
       HIPCHK(hipMemcpy(qb.s_hist, keep.data(), 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));

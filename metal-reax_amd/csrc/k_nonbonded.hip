@@ -378,6 +378,7 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
   const double4* __restrict__ shv = st.shv + pos_sys(pi.w) * st.shv_stride;
   const double* __restrict__ shld = t.qeq_shld + max(ti, 0) * g.ntypes;
   int m = 0, head = 0, fill = 0;
+  double lo_sum = 0.0;   // this lane's part of sum_j |lo_ij| (refinement)
   auto batch = [&](int count) {
     if (lane < count) {
       const int e = (head + lane) & (kQbRing - 1);
@@ -392,7 +393,11 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
         const float hf = (float)h;
         qb.val32[o] = hf;
         // h - hf is exact in FP64; rounded to FP32 it leaves hf + lo within about 2^-48 (relative) of h
-        if (qb.refine) qb.lo32[o] = (float)(h - (double)hf);
+        if (qb.refine) {
+          const float lo = (float)(h - (double)hf);
+          qb.lo32[o] = lo;
+          lo_sum += fabs((double)lo);
+        }
       } else {
         qb.val[o] = h;
       }
@@ -441,8 +446,10 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
     }
   }
   if (fill > 0) batch(fill);
+  if (qb.refine) lo_sum = wave_sum(lo_sum);
   if (lane == 0) {
     qb.cnt[i] = m;
+    if (qb.refine) qb.lorow[i] = lo_sum;
     const double eta = ti >= 0 ? t.qeq_eta[ti] : 1.0;
     qb.eta[i] = eta;
     qb.hinv[i] = 1.0 / eta;

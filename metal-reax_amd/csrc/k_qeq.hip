@@ -62,8 +62,12 @@ __device__ inline void reduce_partials(const double* part, double (&v)[K], doubl
 }
 
 // guess mode: 0 zero start, 1 the stock history extrapolation, 2 keep x and b as the caller wrote them
+// The start is also kept in x0: the refinement's bound measures the correction x - x0 the FP32-valued iterations made.
 __global__ void __launch_bounds__(kThreads) k_cg_guess(int n, QeqBuffers qb, int use_hist) {
-  if (use_hist == 2) return;
+  if (use_hist == 2) {
+    for (int k = blockIdx.x * kThreads + threadIdx.x; k < 2 * n; k += gridDim.x * kThreads) qb.x0[k] = qb.x[k];
+    return;
+  }
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const double* sh = qb.s_hist;
     const double* th = qb.t_hist;
@@ -74,6 +78,8 @@ __global__ void __launch_bounds__(kThreads) k_cg_guess(int n, QeqBuffers qb, int
     }
     qb.x[2 * i] = s;
     qb.x[2 * i + 1] = t;
+    qb.x0[2 * i] = s;
+    qb.x0[2 * i + 1] = t;
     qb.b[2 * i] = -qb.chi[i];
     qb.b[2 * i + 1] = -1.0;
   }
@@ -144,6 +150,64 @@ inline __host__ __device__ int mv_grid(int n) { return (mv_blocks(n) + kXcds - 1
 __device__ inline int mv_vblock() { return (int)((blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds); }
 __device__ inline int product_row() { return (int)((mv_vblock() * blockDim.x + threadIdx.x) / kRowLanes); }
 
+// Refinement's bound. The CG starts from the true residual r0 = b - A x0 (two-float matrix A = hi + lo) and iterates with the FP32
+// values hi, so its recursive residual is r0 - hi (x - x0) while the true one is b - A x = r0 - A (x - x0): they differ by
+// lo (x - x0), whose M^-1-norm is at most max_j |x_j - x0_j| * S with S^2 = sum_i (sum_j |lo_ij|)^2 / eta_i. When the recursion's
+// sqrt(r.M^-1 r) plus that bound is within tol |b| (with a margin of 1e-3 for the recursion's own FP64 rounding), the true
+// residual passes the stock test and the check's product would change nothing: the check is skipped. k_check_gate leaves per
+// block max |x - x0| of both systems and the partial sum of S^2 (only once both systems stopped, like the check itself).
+__global__ void __launch_bounds__(kThreads) k_check_gate(int n, QeqBuffers qb, int it) {
+  __shared__ double sh[3 * kThreads / 64];
+  const int* a = ACT(qb.scal, it - 1);
+  if (a[0] != 0 || a[1] != 0) return;   // uniform over the grid
+  double m0 = 0.0, m1 = 0.0, s2 = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    m0 = fmax(m0, fabs(qb.x[2 * i] - qb.x0[2 * i]));
+    m1 = fmax(m1, fabs(qb.x[2 * i + 1] - qb.x0[2 * i + 1]));
+    const double l = qb.lorow[i];
+    s2 += l * l * qb.hinv[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m0 = fmax(m0, __shfl_xor(m0, off, 64));
+    m1 = fmax(m1, __shfl_xor(m1, off, 64));
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  constexpr int kW = kThreads / 64;
+  if (lane_id() == 0) { sh[threadIdx.x / 64] = m0; sh[kW + threadIdx.x / 64] = m1; sh[2 * kW + threadIdx.x / 64] = s2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v[3] = {sh[0], sh[kW], sh[2 * kW]};
+    for (int w = 1; w < kW; ++w) { v[0] = fmax(v[0], sh[w]); v[1] = fmax(v[1], sh[kW + w]); v[2] += sh[2 * kW + w]; }
+    for (int k = 0; k < 3; ++k) qb.partial_gate[blockIdx.x * 3 + k] = v[k];
+  }
+}
+// the check is skipped for both systems (every thread computes the same answer from the gate's partials, in a fixed order)
+template <int NT>
+__device__ inline bool check_skippable(QeqBuffers qb, int it, double tol, double* sh) {
+  double m0 = 0.0, m1 = 0.0, s2 = 0.0;
+  for (int b = threadIdx.x; b < kCgBlocks; b += NT) {
+    m0 = fmax(m0, qb.partial_gate[3 * b]);
+    m1 = fmax(m1, qb.partial_gate[3 * b + 1]);
+    s2 += qb.partial_gate[3 * b + 2];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m0 = fmax(m0, __shfl_xor(m0, off, 64));
+    m1 = fmax(m1, __shfl_xor(m1, off, 64));
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  constexpr int kW = NT / 64;
+  __syncthreads();
+  if (lane_id() == 0) { sh[threadIdx.x / 64] = m0; sh[kW + threadIdx.x / 64] = m1; sh[2 * kW + threadIdx.x / 64] = s2; }
+  __syncthreads();
+  double v[3] = {sh[0], sh[kW], sh[2 * kW]};
+  for (int w = 1; w < kW; ++w) { v[0] = fmax(v[0], sh[w]); v[1] = fmax(v[1], sh[kW + w]); v[2] += sh[2 * kW + w]; }
+  __syncthreads();
+  const double S = sqrt(v[2]);
+  bool ok = true;
+  for (int s = 0; s < 2; ++s) ok = ok && sqrt(SIG(qb.scal, it - 1, s)) + v[s] * S <= (1.0 - 1e-3) * tol * qb.scal[s];
+  return ok;
+}
+
 // w = (diag(eta) + H) v for both systems. kMode 0 writes w. kMode 1 (CG start) and 2 (refinement check) fuse the start in: the
 // row's owner writes r = b - w and p = r / eta instead of w, and each block leaves the partials of b.b and r.p of its rows
 // (1: partial_mv, 2: partial_rs). kMode 2 runs only when neither system is active in iteration `it` (activity slot it - 1): the
@@ -151,10 +215,12 @@ __device__ inline int product_row() { return (int)((mv_vblock() * blockDim.x + t
 template <class VA, class TC, int kMode>
 __global__ void __launch_bounds__(kMvThreads) k_spmv2(int n, int maxnb, const int* __restrict__ cnt, const TC* __restrict__ col,
                                                      const VA val, const double* __restrict__ eta, const double* v, double* w,
-                                                     QeqBuffers qb, int it) {
+                                                     QeqBuffers qb, int it, double tol) {
   if (kMode == 2) {
     const int* a = ACT(qb.scal, it - 1);
     if (a[0] != 0 || a[1] != 0) return;   // uniform over the grid
+    __shared__ double shg[3 * kMvThreads / 64];
+    if (check_skippable<kMvThreads>(qb, it, tol, shg)) return;   // uniform: the bound already passes both systems
   }
   const double2* __restrict__ v2 = reinterpret_cast<const double2*>(v);
   double2* w2 = reinterpret_cast<double2*>(w);
@@ -245,6 +311,7 @@ __global__ void __launch_bounds__(kMvThreads) k_cg_mvdir(int n, int maxnb, const
     for (int s = 0; s < 2; ++s) {
       if (!act[s]) continue;
       const int k = 2 * i + s;
+      if (fresh && it > 1) qb.x0[k] = qb.x[k];   // a restart: the correction is measured from here
       const double d = fresh ? pv[s] : pv[s] + beta[s] * qb.d[k];
       const double q = fresh ? w[s] : w[s] + beta[s] * qb.q[k];
       qb.d[k] = d;
@@ -293,6 +360,11 @@ __global__ void __launch_bounds__(kThreads) k_cg_check_fin(QeqBuffers qb, int np
   int* rs = RST(qb.scal, it);
   if (pa[0] != 0 || pa[1] != 0) {
     if (threadIdx.x == 0) { rs[0] = -1; rs[1] = -1; }
+    return;
+  }
+  if (check_skippable<kThreads>(qb, it, tol, sh)) {   // verified by the bound: the recursion's r.M^-1 r stands
+    if (threadIdx.x == 0)
+      for (int s = 0; s < 2; ++s) { SIG(qb.scal, it, s) = SIG(qb.scal, it - 1, s); rs[s] = 0; }
     return;
   }
   double v[4];
@@ -549,10 +621,10 @@ template <class P>
 using col_t = typename std::remove_const<typename std::remove_pointer<P>::type>::type;
 
 template <int kMode>
-void spmv2(int n, int maxnb, QeqBuffers& qb, bool ds, const double* x, double* y, int it, hipStream_t s) {
+void spmv2(int n, int maxnb, QeqBuffers& qb, bool ds, const double* x, double* y, int it, hipStream_t s, double tol = 0.0) {
   with_matrix(qb, ds, [&](auto v, auto col) {
     hipLaunchKernelGGL((k_spmv2<decltype(v), col_t<decltype(col)>, kMode>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, maxnb, qb.cnt,
-                       col, v, qb.eta, x, y, qb, it);
+                       col, v, qb.eta, x, y, qb, it, tol);
   });
 }
 
@@ -588,7 +660,8 @@ void launch_qeq_mvdir_chk(const State& st, QeqBuffers& qb, int it, int imax, dou
 
 void launch_qeq_restart(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s) {
   const int n = st.n;
-  spmv2<2>(n, st.maxnb, qb, true, qb.x, qb.q, it, s);
+  hipLaunchKernelGGL(k_check_gate, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, it);
+  spmv2<2>(n, st.maxnb, qb, true, qb.x, qb.q, it, s, tol);
   hipLaunchKernelGGL(k_cg_check_fin, dim3(1), dim3(kThreads), 0, s, qb, mv_blocks(n), it, imax, tol);
 }
 

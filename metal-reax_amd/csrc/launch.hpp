@@ -81,6 +81,11 @@ struct QeqBuffers {
   double* partial;           // 4 * kCgBlocks: per-block partials of the element-wise CG kernels
   double* partial_mv;        // 4 * qeq_mv_partials(n): per-block partials of the matrix-vector product
   double* partial_rs;        // 4 * qeq_mv_partials(n): per-block partials of the refinement's verification residual
+  // refinement's bound (k_qeq.hip k_check_gate): the FP32 remainders' row sums sum_j |lo_ij| (n), the CG start x0 (2 n, same
+  // layout as x) and the gate's per-block partials (3 * kCgBlocks)
+  double* lorow = nullptr;
+  double* x0 = nullptr;
+  double* partial_gate = nullptr;
   int fp32 = 0;              // the matrix values are val32 (mixed and FP32 modes) instead of val
   double* scal;              // device scalars and activity flags per iteration (see k_qeq.hip), own allocation sized by imax
   double *s_hist, *t_hist;   // 4 * n each, [k * n + i], k = 0 most recent

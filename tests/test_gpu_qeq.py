@@ -168,3 +168,34 @@ def test_qeq_history_extrapolation(golden, ffield_text):
             first = it.copy()
     assert it[0] <= first[0] // 2 and it[1] <= first[1] // 2
     e.close()
+
+
+@pytest.mark.gpu
+def test_refinement_bound_keeps_the_true_residual(ffield_text):
+    """Warm-started mixed solves at the stock tolerance, where the refinement's bound (max |x - x0| times the M^-1-norm of the
+    FP32 remainders' row sums) stands in for the check's two-float product: the charges still meet the stock stopping test
+    measured with the oracle's FP64 matrix, sqrt(r.M^-1 r)/|b| <= tol, for both systems. The starts lie 1e-4 to 1e-2 (relative)
+    from the tight solution, from well inside the bound to where the check has to run."""
+    from reaxamd import systems
+    w = systems.water_box(6, seed=3)
+    O = Oracle(ffield_text)
+    e = new_engine(ffield_text, ["C", "H", "O"], "mixed")
+    e.set_box(w.lo, w.hi)
+    e.set_atoms(w.types, w.tags, w.x)
+    chi, eta, gam = setup_qeq(e)
+    n = w.n
+    chi_a, eta_a = chi[w.types], eta[w.types]
+    b = np.concatenate([-chi_a, -np.ones(n)])
+    xt, it, rel, st = e.qeq_solve(b, np.zeros(2 * n), imax=1000, tol=1e-12)
+    rng = np.random.default_rng(11)
+    params = (chi[1:], eta[1:], gam[1:])
+    for scale in (1e-4, 1e-3, 1e-2):
+        x0 = xt * (1.0 + scale * rng.normal(size=2 * n))
+        x, it, rel, st = e.qeq_solve(b, x0, imax=200, tol=1e-6)
+        for s in range(2):
+            xs, bs = x[s * n:(s + 1) * n], b[s * n:(s + 1) * n]
+            r = bs - O.qeq_matvec(w.x, w.types - 1, w.tags, w.box12(), w.periodic, xs, params=params)
+            true_rel = np.sqrt((r * r / eta_a).sum()) / np.linalg.norm(bs)
+            print("start %.0e, system %d: iterations %d, reported %.2e, true %.2e" % (scale, s, it[s], rel[s], true_rel), flush=True)
+            assert true_rel <= 1e-6 and rel[s] <= 1e-6, (scale, s, true_rel, rel[s])
+    e.close()
