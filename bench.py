@@ -208,6 +208,7 @@ def main():
     # timed region: the MD loop; only the SpMV launches carry HIP events (the roofline kernel), every 4th of them (a launch
     # bracketed by events cannot overlap its neighbours as freely; sampling keeps that perturbation of the timed loop small)
     e.profile(True, phases=["qeq_spmv"], every=4)
+    qs0 = e.qeq_stats()
     sync()
     t0 = time.perf_counter()
     thermo = e.md_run(args.steps)
@@ -215,6 +216,8 @@ def main():
     elapsed = time.perf_counter() - t0
     spmv_ms, spmv_cnt = e.profile_read()["qeq_spmv"]
     st = e.stats()
+    qs1 = e.qeq_stats()
+    solve_counts = {k: (qs1[k] - qs0[k]) / args.steps for k in ("full_checks", "restarts", "batches")}
     print("bench: timed %d steps in %.3f s; thermo %s" % (args.steps, elapsed, thermo), file=sys.stderr, flush=True)
     # untimed: per-phase device time of a few more steps (every phase bracketed by events)
     nb_steps = min(10, max(1, args.steps))
@@ -285,7 +288,8 @@ def main():
                        "k_source": "FP64 oracle iterations per step on the same trajectory" if cpu else "device launches per step",
                        "k_device_launches": k_dev, "roofline_atom_steps_per_s": HBM_PEAK_GBS * 1e9 / bytes_step,
                        "frac_of_step_roofline": value / (HBM_PEAK_GBS * 1e9 / bytes_step)},
-        "timed_window": {"rebuilds": thermo["rebuilds"], "temp_K": thermo["temp"]},
+        "timed_window": {"rebuilds": thermo["rebuilds"], "temp_K": thermo["temp"],
+                         "charge_solve_per_step": dict(solve_counts, iterations=thermo["qeq_iters_mean"])},
         "long_window": {"steps": args.long_window, "rebuilds": th_long["rebuilds"], "R": R,
                         "ms_per_step": 1e3 * long_s / max(1, args.long_window),
                         "atom_steps_per_s": n * args.long_window / long_s if args.long_window else None},

@@ -177,6 +177,10 @@ int rxa_get_state(rxa_context* ctx, double* x, double* v, double* q);
  * [7] charge-matrix entries (full rows), [8] neighbour rebuilds, [9] precision (0 fp64, 1 mixed, 2 fp32), [10] bond candidates
  * (r <= bond_cut + skin, directed), [11] H-bond candidates (donor and acceptor lists, r <= hbond_cut + skin) */
 int rxa_stats(rxa_context* ctx, int64_t* out);
+/* charge-solve counters since the context's solve buffers were sized: out[0] solves (rxa_qeq_charges / MD steps), out[1]
+ * refinement checks whose two-float product ran (the others were settled by the bound), out[2] restarts from the true residual,
+ * out[3] batches of the single-system solve (each one host read-back of its flags) */
+int rxa_qeq_stats(rxa_context* ctx, int64_t* out);
 
 /* ---- context-free helpers (no device is touched; usable on a host without a GPU) ----------------------------------
  * rxa_ffield_table_dump: run-time reader + canonical dump of a force field text (0 on success, length in *len); err receives the

@@ -242,6 +242,7 @@ struct Engine {
   long long md_steps = 0;
   double qeq_iter_sum = 0;
   long long qeq_calls = 0;
+  long long cg_batches = 0;   // readbacks of the single-system solve (host round trips)
   int last_iters[2] = {0, 0};
   double last_rel[2] = {0, 0};
 
@@ -1255,6 +1256,7 @@ struct Engine {
     }
     HIPCHK(hipMemcpyAsync(cgs_h, qb.scal, (8 + kScalSlot * (size_t)slots) * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(ev_cg, stream));
+    ++cg_batches;
   }
   double cg_tol = 1e-6;
   // Charge solve in two halves so that other work can be queued in between: qeq_begin queues the matrix (if not current), the
@@ -2083,6 +2085,16 @@ int rxa_profile_every(rxa_context* c, int every) {
     e.sync();
     e.timer.every = every;
     e.timer.reset();
+  });
+}
+
+int rxa_qeq_stats(rxa_context* c, int64_t* out) {
+  return guard(c, [&](Engine& e) {
+    e.sync();
+    out[0] = e.qeq_calls;
+    out[1] = e.cgs_h ? (int64_t)e.cgs_h[4] : 0;
+    out[2] = e.cgs_h ? (int64_t)e.cgs_h[5] : 0;
+    out[3] = e.cg_batches;
   });
 }
 

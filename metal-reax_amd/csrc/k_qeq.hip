@@ -27,7 +27,8 @@ constexpr int kUpdBlocks = 240, kUpdThreads = 256;   // CG update kernel
 constexpr int kMvThreads = 256;             // product kernels: 4 waves per block,
 constexpr int kMvRowsPerBlock = kMvThreads / 16;   // 16 lanes per row (kRowLanes below)
 inline __host__ __device__ int mv_blocks(int n) { return (n + kMvRowsPerBlock - 1) / kMvRowsPerBlock; }
-// scal layout: [0,1] |b| of s and t, [2,3] u terms, then kScalSlot doubles per iteration index k: sig of the two systems at
+// scal layout: [0,1] |b| of s and t, [2,3] u terms, [4] refinement checks whose product ran, [5] restarts (running counts),
+// then kScalSlot doubles per iteration index k: sig of the two systems at
 // 8 + 4 k + sys, the two activity flags (ints) in 8 + 4 k + 2 and the two restart records of a refinement check (ints: -1 no
 // check ran, else the restart flags) in 8 + 4 k + 3, so one copy of the first 8 + 4 k doubles brings them all to the host
 __device__ inline double& SIG(double* scal, int k, int sys) { return scal[8 + kScalSlot * k + sys]; }
@@ -369,12 +370,16 @@ __global__ void __launch_bounds__(kThreads) k_cg_check_fin(QeqBuffers qb, int np
   }
   double v[4];
   reduce_partials<4>(qb.partial_rs, v, sh, nparts);
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     for (int s = 0; s < 2; ++s) {
       const double bn = sqrt(v[s]);
       SIG(qb.scal, it, s) = v[2 + s];
       rs[s] = (it + 1 < imax && sqrt(v[2 + s]) / bn > tol) ? 1 : 0;
     }
+    // running counts since the scalars were allocated: checks whose product ran, and restarts
+    qb.scal[4] += 1.0;
+    if (rs[0] || rs[1]) qb.scal[5] += 1.0;
+  }
 }
 
 // x += alpha d, r -= alpha q, p = r / eta, partial r.p   (iteration it >= 1); few large blocks, since every block first

@@ -72,6 +72,7 @@ def lib():
             "rxa_md_run": (I, [P, I, P]),
             "rxa_get_state": (I, [P, P, P, P]),
             "rxa_stats": (I, [P, P]),
+            "rxa_qeq_stats": (I, [P, P]),
             "rxa_set_positions_device": (I, [P, P]),
             "rxa_stream": (I, [P, ctypes.POINTER(P)]),
             "rxa_stream_copy": (I, [P, I64, I]),
@@ -369,6 +370,11 @@ class Engine:
         keys = ("n", "maxnb", "maxb", "maxhc", "skin_entries", "bond_slots", "hbond_entries", "h_entries", "rebuilds", "precision",
                 "bond_candidates", "hb_candidates")
         return dict(zip(keys, out.tolist()))
+
+    def qeq_stats(self):
+        out = np.zeros(4, dtype=np.int64)
+        self._chk(lib().rxa_qeq_stats(self.h, _p(out)))
+        return dict(zip(("solves", "full_checks", "restarts", "batches"), out.tolist()))
 
     PHASES = ("neighbor", "bond_lists", "bo_correct", "bonded_terms", "nonbonded", "dbond_gather", "qeq_matrix", "qeq_spmv", "qeq_cg",
               "integrate")
