@@ -96,9 +96,10 @@ int rxa_compute(rxa_context* ctx, int flags, double* energies, double* grad, dou
  * nonbonded.hpp:22). Energies follow its attribution (bonded.cpp:212-704, nonbonded.cpp:55-76): bond, triple-bond and C2 terms
  * half to each atom of the bond, lone pair and over/under to the atom, valence to the centre, torsion half to each atom of the
  * central bond, H-bond a third to each atom of the triple, van der Waals and Coulomb half to each atom of the pair,
- * polarisation to the atom. The virial is split as the engine forms it: non-bonded pair terms half to each atom (as the
- * reference), bonded terms to the atoms whose kernels form them, not the reference's even split over a term's atoms; the
- * per-atom virials add up to the virial of rxa_compute. */
+ * polarisation to the atom. The virial follows the reference's split (bonded.cpp:462-466, 636-647, 703-710, 737-766):
+ * non-bonded pairs half to each atom, each valence angle's virial a third to each of its atoms, each torsion's a quarter to each
+ * of four, each H-bond triple's a third to each of three, and the bond-order derivative pass per bond as the reference tallies
+ * it; the per-atom virials add up to the virial of rxa_compute. */
 int rxa_peratom(rxa_context* ctx, double* eatom, double* vatom);
 /* the 13 terms -> LAMMPS' 14-slot pvector (eb ea elp emol ev epen ecoa ehb et eco ew ep efi eqeq), energy_terms.cpp:48-63 */
 void rxa_pvector(const double* energies, double* pvector14);

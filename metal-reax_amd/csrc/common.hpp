@@ -133,6 +133,13 @@ struct State {
   double* ehc;                    // maxhc * n: H-bond energy third of the acceptor in each donor candidate slot
   double* eacc;                   // n: an acceptor's gathered H-bond thirds
   double* eref;                   // n: the per-atom energy (k_eatom_ref)
+  // per-atom virial in the reference's split (filled only when peratom != 0; bonded.cpp:462-466, 636-647, 703-710, 737-766):
+  // valence a third of each angle's virial to each of its three atoms, torsion a quarter to each of four, H-bond a third to each
+  // of three, the bond-order derivative pass per bond as the reference tallies it, non-bonded half pairs. Component-major.
+  double* vref;                   // 6 * n: the atom's own shares, complete after k_vatom_ref
+  double* vslot;                  // 6 * maxb * n: shares of the slot's partner (valence, torsion, H-bond i)
+  double* vhc;                    // 6 * maxhc * n: H-bond share of the acceptor in each donor candidate slot
+  double* vE;                     // 3 * n: sum over the atom's slots of the bond-order derivative coefficient times the bond vector
   double *gnb_x, *gnb_y, *gnb_z;  // non-bonded gradient
   // decision census (filled only when census != 0): kCenCount * n ints, [k * n + i] for the categories below
   int census;

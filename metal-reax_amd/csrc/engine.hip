@@ -199,6 +199,7 @@ struct Engine {
   DBuf<int> a_bc, a_hc;  // bond / H-bond candidate sub-lists (j, entry; H-bond: + mirror position)
   DBuf<double> a_hf;     // per H-bond candidate slot: force on the acceptor
   DBuf<double> a_ehc;    // per H-bond candidate slot: the acceptor's energy third (per-atom calls)
+  DBuf<double> a_vref;   // per-atom calls: the reference-split virial (vref, vE, vslot, vhc), sized on first use
   BinGrid grid;
   BinBuffers bins;
   DBuf<unsigned> b_keys, b_skeys, b_akey;
@@ -507,6 +508,18 @@ struct Engine {
     st.hc_j = a_hc.p; st.hc_s = a_hc.p + (size_t)n * maxhc; st.hc_m = a_hc.p + 2 * (size_t)n * maxhc;
     st.hf = a_hf.p;
     st.ehc = a_ehc.p;
+    if (st.peratom) alloc_peratom();
+  }
+
+  // the per-atom virial buffers of the reference's split for the current atom count and list capacities (per-atom calls only;
+  // called with no work queued that uses them)
+  void alloc_peratom() {
+    const size_t need = 9 * (size_t)n + 6 * (size_t)n * maxb + 6 * (size_t)n * maxhc;
+    if (a_vref.n != need) a_vref.alloc(need);
+    st.vref = a_vref.p;
+    st.vE = a_vref.p + 6 * (size_t)n;
+    st.vslot = a_vref.p + 9 * (size_t)n;
+    st.vhc = st.vslot + 6 * (size_t)n * maxb;
   }
 
   void alloc_bonds(int cap_b) {
@@ -528,6 +541,7 @@ struct Engine {
     st.cdp = nxt();
     st.eslot = nxt();
     if (k != 30) throw ApiError(RXA_EINVAL, "internal: bond slot layout");
+    if (st.peratom) alloc_peratom();
   }
 
   // expected entries within r of an atom at the mean density (at least that of a condensed phase for open cells), with margin
@@ -987,6 +1001,7 @@ struct Engine {
   void join_gather(bool energies) {
     if (bstream() != stream) HIPCHK(hipStreamWaitEvent(stream, ev_bonded, 0));
     hipEvent_t t0 = timer.begin(PH_DBOND);
+    if (st.peratom) launch_vatom_ref(st, stream);   // reads the non-bonded half pairs in vatom, before the gather adds to them
     launch_gather(st, stream);
     if (st.peratom) launch_eatom_ref(st, stream);
     if (energies) launch_energy_reduce(st, red.p, stream);   // the per-atom arrays hold the energies either way
@@ -1009,6 +1024,7 @@ struct Engine {
     peratom_valid = false;
     census_valid = false;
     try {
+      if (st.peratom) { sync(); alloc_peratom(); }
       forces((flags & RXA_COMPUTE_REBUILD) != 0);
       if (st.census) launch_census_nb(st, g, stream);
     } catch (...) {
@@ -1044,7 +1060,7 @@ struct Engine {
     if (!peratom_valid) throw ApiError(RXA_ESTATE, "no per-atom tallies: call rxa_compute with RXA_COMPUTE_PERATOM first");
     std::vector<double> e(n), v(6 * (size_t)n);
     HIPCHK(hipMemcpyAsync(e.data(), st.eref, n * sizeof(double), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(v.data(), st.vatom, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(v.data(), st.vref, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, stream));
     sync();
     for (int i = 0; i < n; ++i) {
       const int u = perm_h[i];

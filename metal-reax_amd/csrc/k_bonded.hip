@@ -347,11 +347,18 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
   if (i >= st.n) return;
   const int n = st.n, ti = st.type[i];
   double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, cdself = 0, e_c2 = 0;
+  if (st.peratom)   // the reference-split virial shares, accumulated by the angle kernels that follow
+    for (int c = 0; c < 6; ++c) st.vref[(size_t)c * n + i] = 0.0;
   if (ti >= 0) {
     const DevSpecies& si = t.sp[ti];
     const int nb = min(st.b_cnt[i], st.maxb);
-    if (st.peratom)
-      for (int p = 0; p < nb; ++p) st.eslot[sl(p, i, n)] = 0.0;
+    if (st.peratom) {
+      const size_t nsl = (size_t)st.maxb * n;
+      for (int p = 0; p < nb; ++p) {
+        st.eslot[sl(p, i, n)] = 0.0;
+        for (int c = 0; c < 6; ++c) st.vslot[c * nsl + sl(p, i, n)] = 0.0;
+      }
+    }
     // bond energy of the bonds whose canonical end is this atom (the reference's tag order, bonded.cpp:222-252)
     for (int p = 0; p < nb; ++p) {
       const size_t q = sl(p, i, n);
@@ -489,7 +496,9 @@ __device__ inline int wave_max(int v) {
 // Valence angle, penalty and three-body conjugation at the centre j [reference bonded.cpp:321-474]. Lane p of j's group evaluates
 // the pairs (p, k) with p < k of j's bonds, k running over a wave-uniform loop: the contributions to slot p stay in the lane, the
 // contributions to slot k are summed over the group and kept by lane k, and the centre's are summed over the group at the end.
-template <class TR, class TS, int G>
+// PA: also the reference's per-atom virial split (bonded.cpp:462-466): each angle's (x_i - x_j) (x) F_i + (x_k - x_j) (x) F_k,
+// a third to each of i, j and k (the ends through their slots).
+template <class TR, class TS, int G, bool PA>
 __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = gid / G, p = gid % G, n = st.n;
@@ -569,8 +578,10 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
     dij = (TR)st.b_d[qi];
     tboi = (TR)st.tbo[i];
   }
+  double vA[6] = {0, 0, 0, 0, 0, 0}, vC[6] = {0, 0, 0, 0, 0, 0};   // PA: slot p's partner, the centre (this lane's angles)
   for (int pk = 1; pk < nbw; ++pk) {
     double to_cdbo = 0, to_cdp = 0, to_fx = 0, to_fy = 0, to_fz = 0;
+    double vto[6] = {0, 0, 0, 0, 0, 0};
     if (mine && p < pk && pk < nb && BOij_s - thb_cut_s > TS(0)) {
       const size_t qk = sl(pk, j, n);
       const TS BOjk_s = (TS)st.bo[qk];
@@ -630,11 +641,25 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
           fjx += (double)(CE8 * dcj.x); fjy += (double)(CE8 * dcj.y); fjz += (double)(CE8 * dcj.z);
           own_fx += (double)(CE8 * dci.x); own_fy += (double)(CE8 * dci.y); own_fz += (double)(CE8 * dci.z);
           to_fx += (double)(CE8 * dck.x); to_fy += (double)(CE8 * dck.y); to_fz += (double)(CE8 * dck.z);
+          if (PA) {   // F_i = -CE8 dcos/dx_i, F_k = -CE8 dcos/dx_k
+            vir_add(vto, -1.0, (double)vij.x, (double)vij.y, (double)vij.z, (double)(CE8 * dci.x), (double)(CE8 * dci.y),
+                    (double)(CE8 * dci.z));
+            vir_add(vto, -1.0, (double)vjk.x, (double)vjk.y, (double)vjk.z, (double)(CE8 * dck.x), (double)(CE8 * dck.y),
+                    (double)(CE8 * dck.z));
+          }
         }
       }
     }
     to_cdbo = gsum<G>(to_cdbo); to_cdp = gsum<G>(to_cdp); to_fx = gsum<G>(to_fx); to_fy = gsum<G>(to_fy); to_fz = gsum<G>(to_fz);
     if (p == pk) { own_cdbo += to_cdbo; own_cdp += to_cdp; own_fx += to_fx; own_fy += to_fy; own_fz += to_fz; }
+    if (PA) {
+      for (int c = 0; c < 6; ++c) {
+        vA[c] += vto[c];
+        vC[c] += vto[c];
+        const double tk = gsum<G>(vto[c]);
+        if (p == pk) vA[c] += tk;
+      }
+    }
   }
   e_ang = gsum<G>(e_ang); e_pen = gsum<G>(e_pen); e_coa = gsum<G>(e_coa); cdself = gsum<G>(cdself);
   fjx = gsum<G>(fjx); fjy = gsum<G>(fjy); fjz = gsum<G>(fjz); sCE5 = gsum<G>(sCE5); sCE6 = gsum<G>(sCE6);
@@ -658,6 +683,14 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
     st.eatom[T_PEN * n + j] = e_pen;
     st.eatom[T_COA * n + j] = e_coa;
   }
+  if (PA) {
+    for (int c = 0; c < 6; ++c) vC[c] = gsum<G>(vC[c]);
+    const size_t nsl = (size_t)st.maxb * n;
+    if (mine)
+      for (int c = 0; c < 6; ++c) st.vslot[c * nsl + qi] += vA[c] * (1.0 / 3.0);
+    if (live && p == 0)
+      for (int c = 0; c < 6; ++c) st.vref[(size_t)c * n + j] += vC[c] * (1.0 / 3.0);
+  }
   if (st.census) {
     n_ang = gsum_int<G>(n_ang);
     if (live && p == 0) { st.cen[kCenAngles * n + j] = n_ang; st.cen[kCenSbo * n + j] = sbo_region; }
@@ -677,7 +710,10 @@ __device__ inline TR tan_inv(TR s, TR c) {
 // its own half (its atom, its outer partner i and outer bond (j, i)); the end the reference counts (canonical_first) also adds
 // the energy and the central-bond coefficients. The outer-bond contributions of an i are summed over the group and kept by the
 // lane owning slot i.
-template <class TR, class TS, int G, bool kCen>
+// PA: also the reference's per-atom virial split (bonded.cpp:636-647): each torsion's (x_l - x_i) (x) g_i + (x_l - x_j) (x) g_j +
+// (x_l - x_k) (x) g_k (g the gradients), a quarter to each of its four atoms. Both ends evaluate every torsion, so each end gives
+// the quarters of its own two atoms (j and the outer partner i, through i's slot).
+template <class TR, class TS, int G, bool kCen, bool PA>
 __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = gid / G, pk = gid % G, n = st.n;
@@ -723,8 +759,10 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   }
   double cdbo_jk = 0, cdbopi_jk = 0;
   int n_tor = 0;   // census: torsions through the four-body gates that this lane's central bond counts
+  double vJ[6] = {0, 0, 0, 0, 0, 0}, vIown[6] = {0, 0, 0, 0, 0, 0};   // PA: j's quarters, this lane's slot partner's quarters
   for (int pi = 0; pi < nbw; ++pi) {
     TR cdbo_ij = 0, fix = 0, fiy = 0, fiz = 0;   // this lane's terms of partner i: at most one per bond of k
+    double vI[6] = {0, 0, 0, 0, 0, 0};
     if (cen && pi != pk && pi < nbj) {
       const size_t qi = sl(pi, j, n);
       const TS BOij_s = (TS)st.bo[qi];
@@ -814,6 +852,19 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
                        gjz = (double)(a7 * dcj.z + a8 * ecj.z + a9 * doj.z);
           fix += gix; fiy += giy; fiz += giz;
           fjx += gjx; fjy += gjy; fjz += gjz;
+          if (PA) {   // g_k with the reference's dcos(omega)/dx_k; positions relative to l: x_l - x_i = -dli, x_l - x_j = djk + dkl
+            const TR htrc = rkl + c2 * (rij * c1 - rjk), hnrc = rij * s1 * s2, gcl = (htrc - arg * hnrc) / rkl;
+            const V3<TR> dok{sc * (-gcl * dkl.x + htrb / rjk * djk.x + gd * dck.x + ge * eck.x),
+                             sc * (-gcl * dkl.y + htrb / rjk * djk.y + gd * dck.y + ge * eck.y),
+                             sc * (-gcl * dkl.z + htrb / rjk * djk.z + gd * dck.z + ge * eck.z)};
+            const double gkx = (double)(a7 * dck.x + a8 * eck.x + a9 * dok.x), gky = (double)(a7 * dck.y + a8 * eck.y + a9 * dok.y),
+                         gkz = (double)(a7 * dck.z + a8 * eck.z + a9 * dok.z);
+            double v[6] = {0, 0, 0, 0, 0, 0};
+            vir_add(v, -1.0, (double)dli.x, (double)dli.y, (double)dli.z, (double)gix, (double)giy, (double)giz);
+            vir_add(v, 1.0, (double)(djk.x + dkl.x), (double)(djk.y + dkl.y), (double)(djk.z + dkl.z), gjx, gjy, gjz);
+            vir_add(v, 1.0, (double)dkl.x, (double)dkl.y, (double)dkl.z, gkx, gky, gkz);
+            for (int c = 0; c < 6; ++c) { vJ[c] += v[c]; vI[c] += v[c]; }
+          }
           if (counted) {   // energy and the central bond once per torsion
             e_tor += (double)(fn10 * sijk * sjkl * CV);
             e_con += (double)(p_cot1 * fn12 * gg);
@@ -829,6 +880,11 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
       if (live && tj >= 0) st.Cdbo[sl(pk, j, n)] += gc;
       own_fx += gx; own_fy += gy; own_fz += gz;
     }
+    if (PA)
+      for (int c = 0; c < 6; ++c) {
+        const double vs = gsum<G>(vI[c]);
+        if (pk == pi) vIown[c] += vs;
+      }
   }
   // This end's half of the torsion virial about the midpoint m of each central bond: with x_i - m = d_ji - d_jk / 2 and
   // x_j - m = -d_jk / 2, - sum (x_i - m) (x) g_i - sum (x_j - m) (x) g_j = - sum_p d_jp (x) F_p + sum_lanes (d_jk / 2) (x) (F_I + F_j),
@@ -861,17 +917,25 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
     n_tor = gsum_int<G>(n_tor);
     if (live && pk == 0) st.cen[kCenTorsions * n + j] = n_tor;
   }
+  if (PA) {
+    for (int c = 0; c < 6; ++c) vJ[c] = gsum<G>(vJ[c]);
+    const size_t nsl = (size_t)st.maxb * n;
+    if (live && tj >= 0 && pk < nbj)
+      for (int c = 0; c < 6; ++c) st.vslot[c * nsl + sl(pk, j, n)] += 0.25 * vIown[c];
+    if (live && pk == 0)
+      for (int c = 0; c < 6; ++c) st.vref[(size_t)c * n + j] += 0.25 * vJ[c];
+  }
 }
 
 // FP32 terms at 3 waves per SIMD (168 VGPRs); FP64 at the compiler's choice. kCen: the variant that also records the torsion
 // census (launched only for a force call that asks for it, so the counter costs the ordinary step nothing)
-template <class TS, int G, bool kCen>
+template <class TS, int G, bool kCen, bool PA>
 __global__ void __attribute__((amdgpu_waves_per_eu(3))) __launch_bounds__(256) k_torsion_f(State st, Globals g, Tables t) {
-  torsion_body<float, TS, G, kCen>(st, g, t);
+  torsion_body<float, TS, G, kCen, PA>(st, g, t);
 }
-template <int G, bool kCen>
+template <int G, bool kCen, bool PA>
 __global__ void __launch_bounds__(256) k_torsion_d(State st, Globals g, Tables t) {
-  torsion_body<double, double, G, kCen>(st, g, t);
+  torsion_body<double, double, G, kCen, PA>(st, g, t);
 }
 
 // ================================================================================================================
@@ -910,10 +974,12 @@ __device__ inline void hbond_term(const DevHBond& P, TR eh2, const V3<TR>& dji, 
 // of its candidate: energy and the force on j stay in the lane, the BO derivative coefficient and the force on i are reduced over
 // the wave into the bond slot, and the force on the acceptor k is written to the candidate's slot of hf for the acceptor to
 // collect. Non-donors only clear their H-bond energy.
+// PA (per-atom calls): also the reference's per-atom virial split (bonded.cpp:703-710): each triple's (x_j - x_i) (x) g_i +
+// (x_j - x_k) (x) g_k, a third to the donor, to i (through the bond slot) and to the acceptor (through the candidate slot).
 constexpr int kHbWaves = 4;
 constexpr int kHbRing = 128;
 
-template <class TR, class TS>
+template <class TR, class TS, bool PA>
 __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals g, Tables t) {
   __shared__ int rk_[kHbWaves][kHbRing], rh_[kHbWaves][kHbRing], rt_[kHbWaves][kHbRing];
   __shared__ double rx_[kHbWaves][kHbRing], ry_[kHbWaves][kHbRing], rz_[kHbWaves][kHbRing], rr_[kHbWaves][kHbRing];
@@ -965,7 +1031,9 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
     const V3<TR> djk{(TR)dx, (TR)dy, (TR)dz};
     const TR rjk = (TR)r;
     double fkx = 0, fky = 0, fkz = 0, ek = 0;
+    double vk[6] = {0, 0, 0, 0, 0, 0};   // PA: this candidate's triples
     for (unsigned long long left = hbonds; left; left &= left - 1) {   // wave-uniform, ascending slots
+      double vt[6] = {0, 0, 0, 0, 0, 0};
       const int p = __ffsll((long long)left) - 1;
       const size_t q = sl(p, j, n);
       const int i = st.b_j[q], ti = st.type[i];
@@ -994,10 +1062,22 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
           fkx += gkx; fky += gky; fkz += gkz;
           vir_add(w, -1.0, dx, dy, dz, gkx, gky, gkz);
           vir_add(w, -1.0, (double)dji.x, (double)dji.y, (double)dji.z, fx, fy, fz);
+          if (PA) {
+            vir_add(vt, -1.0, dx, dy, dz, gkx, gky, gkz);
+            vir_add(vt, -1.0, (double)dji.x, (double)dji.y, (double)dji.z, fx, fy, fz);
+          }
         }
       }
       c1 = wave_sum(c1); fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
       if (lane == 0) { st.Cdbo[q] += c1; st.fpx[q] += fx; st.fpy[q] += fy; st.fpz[q] += fz; }
+      if (PA) {
+        const size_t nsl = (size_t)st.maxb * n;
+        for (int c = 0; c < 6; ++c) {
+          vk[c] += vt[c];
+          const double vs = wave_sum(vt[c]);
+          if (lane == 0) st.vslot[c * nsl + q] += vs * (1.0 / 3.0);
+        }
+      }
       if (st.peratom) {   // a third of each triple's energy to i (through the bond slot) and to k (through the candidate slot)
         ek += et;
         et = wave_sum(et);
@@ -1008,6 +1088,8 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
       const size_t o = 3 * (hbase + h);
       hf[o] = (TR)fkx; hf[o + 1] = (TR)fky; hf[o + 2] = (TR)fkz;
       if (st.peratom) st.ehc[hbase + h] = ek / 3.0;
+      if (PA)
+        for (int c = 0; c < 6; ++c) st.vhc[c * ((size_t)st.maxhc * n) + hbase + h] = vk[c] * (1.0 / 3.0);
     }
   };
   for (int h0 = 0; h0 < nc; h0 += 64) {
@@ -1023,6 +1105,8 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
         const size_t o = 3 * (hbase + h);
         hf[o] = TR(0); hf[o + 1] = TR(0); hf[o + 2] = TR(0);
         if (st.peratom) st.ehc[hbase + h] = 0.0;
+        if (PA)
+          for (int c = 0; c < 6; ++c) st.vhc[c * ((size_t)st.maxhc * n) + hbase + h] = 0.0;
       }
     }
     const unsigned long long mk = __ballot(ok);
@@ -1046,6 +1130,8 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
   if (lane == 0) {
     st.fs_x[j] += fjx; st.fs_y[j] += fjy; st.fs_z[j] += fjz;
     for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + j] += w[c];
+    if (PA)   // the donor's third: w is the sum of the triples' virials
+      for (int c = 0; c < 6; ++c) st.vref[(size_t)c * n + j] += w[c] * (1.0 / 3.0);
     st.eatom[T_HB * n + j] = e_hb;
     st.h_cnt[j] = inrange;
   }
@@ -1074,16 +1160,25 @@ __global__ void __launch_bounds__(256) k_hbond_acceptor(State st, Globals g, Tab
   const size_t hbase = (size_t)k * st.maxhc;
   const TR* hf = reinterpret_cast<const TR*>(st.hf);
   double fx = 0, fy = 0, fz = 0, e = 0;
+  double va[6] = {0, 0, 0, 0, 0, 0};
+  const size_t nhc = (size_t)st.maxhc * st.n;
   for (int h = lane; h < nc; h += 64) {
     const size_t o = (size_t)st.hc_j[hbase + h];   // flat donor slot
     fx += (double)hf[3 * o]; fy += (double)hf[3 * o + 1]; fz += (double)hf[3 * o + 2];
-    if (st.peratom) e += st.ehc[o];
+    if (st.peratom) {
+      e += st.ehc[o];
+      for (int c = 0; c < 6; ++c) va[c] += st.vhc[c * nhc + o];
+    }
   }
   fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
   if (lane == 0) { st.fs_x[k] += fx; st.fs_y[k] += fy; st.fs_z[k] += fz; }
   if (st.peratom) {
     e = wave_sum(e);
-    if (lane == 0) st.eacc[k] = e;
+    for (int c = 0; c < 6; ++c) va[c] = wave_sum(va[c]);
+    if (lane == 0) {
+      st.eacc[k] = e;
+      for (int c = 0; c < 6; ++c) st.vref[(size_t)c * st.n + k] += va[c];
+    }
   }
 }
 
@@ -1122,6 +1217,7 @@ __global__ void __launch_bounds__(128) k_dbond(State st) {
   if (i >= st.n) return;
   const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
   double S = 0, Dx = 0, Dy = 0, Dz = 0;
+  double Ex = 0, Ey = 0, Ez = 0;   // per-atom calls: sum of the slots' coefficients of dDeltap_self times their bond vectors
   double w[6] = {0, 0, 0, 0, 0, 0};
   const double cdi = st.CdD[i];
   for (int p = 0; p < nb; ++p) {
@@ -1132,7 +1228,9 @@ __global__ void __launch_bounds__(128) k_dbond(State st) {
     const double cdel = cdi + st.CdD[j];
     const double C1dbo = st.C[0][q], C2pi = st.C[1][q], C2pi2 = st.C[2][q], C1pi = st.C[3][q], C1pi2 = st.C[4][q];
     const double T1 = (C1dbo * cbo + C1dbo * cdel) + C2pi * cpi + C2pi2 * cpi2;
-    S += (st.C[5][q] * cbo + st.C[5][q] * cdel) + st.C[6][q] * cpi + st.C[7][q] * cpi2;
+    const double Sq = (st.C[5][q] * cbo + st.C[5][q] * cdel) + st.C[6][q] * cpi + st.C[7][q] * cpi2;
+    S += Sq;
+    Ex += Sq * st.b_dx[q]; Ey += Sq * st.b_dy[q]; Ez += Sq * st.b_dz[q];
     const double sc = T1 * st.dbop_c[q] + (C1pi * cpi) * st.dlnpi_c[q] + (C1pi2 * cpi2) * st.dlnpi2_c[q];
     Dx += sc * st.b_dx[q]; Dy += sc * st.b_dy[q]; Dz += sc * st.b_dz[q];
     // a central pair force: gradient c d on this end, virial d (x) c d, half at each end
@@ -1141,6 +1239,41 @@ __global__ void __launch_bounds__(128) k_dbond(State st) {
   for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + i] += w[c];
   st.S[i] = S;
   st.Dx[i] = Dx; st.Dy[i] = Dy; st.Dz[i] = Dz;
+  if (st.peratom) { st.vE[i] = Ex; st.vE[n + i] = Ey; st.vE[2 * (size_t)n + i] = Ez; }
+}
+
+// Per-atom virial in the reference's split, completed (per-atom calls; before k_gather, while vatom holds the non-bonded half
+// pairs). The bond-order derivative pass tallies per bond (i, j) [reference bonded.cpp:717-766]: (x_i - x_j) (x) F_i at i with
+// F_i = -(t_i)/2, t_i the pair gradient of the bond plus its dDeltap_self coefficient times dDeltap_self(i); and at every
+// partner k of i, ((x_k - x_i) + (x_k - x_j)) (x) F_k with F_k = -(c_k dBO'_ik)/2, the same for j's side. Summed over the bonds
+// of an atom a: the own part is sum_q (sc_q / 2) d_q (x) d_q + E_a (x) dDeltap_self(a) / 2, and a partner k of a (slot r of a)
+// receives (2 S_a d_r - E_a) (x) (dbop_c_r d_r) / 2, with S_a = sum_q S_q and E_a = sum_q S_q d_q.
+__global__ void __launch_bounds__(128) k_vatom_ref(State st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= st.n) return;
+  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
+  const size_t nsl = (size_t)st.maxb * n;
+  double v[6];
+  for (int c = 0; c < 6; ++c) v[c] = st.vatom[(size_t)c * n + i] + st.vref[(size_t)c * n + i];
+  const double cdi = st.CdD[i];
+  for (int p = 0; p < nb; ++p) {
+    const size_t q = sl(p, i, n);
+    const int j = st.b_j[q];
+    const size_t m = sl(st.b_sym[q], j, n);
+    for (int c = 0; c < 6; ++c) v[c] += st.vslot[c * nsl + m];
+    // this end of the bond: the pair part of t_i (the coefficient of k_dbond)
+    const double cbo = st.Cdbo[q] + st.Cdbo[m], cpi = st.Cdbopi[q] + st.Cdbopi[m], cpi2 = st.Cdbopi2[q] + st.Cdbopi2[m];
+    const double cdel = cdi + st.CdD[j];
+    const double T1 = (st.C[0][q] * cbo + st.C[0][q] * cdel) + st.C[1][q] * cpi + st.C[2][q] * cpi2;
+    const double sc = T1 * st.dbop_c[q] + (st.C[3][q] * cpi) * st.dlnpi_c[q] + (st.C[4][q] * cpi2) * st.dlnpi2_c[q];
+    vir_add(v, 0.5 * sc, st.b_dx[q], st.b_dy[q], st.b_dz[q], st.b_dx[q], st.b_dy[q], st.b_dz[q]);
+    // i as a partner of j (slot m of j)
+    const double dmx = st.b_dx[m], dmy = st.b_dy[m], dmz = st.b_dz[m], Sj2 = 2.0 * st.S[j];
+    vir_add(v, 0.5 * st.dbop_c[m], Sj2 * dmx - st.vE[j], Sj2 * dmy - st.vE[n + j], Sj2 * dmz - st.vE[2 * (size_t)n + j], dmx, dmy,
+            dmz);
+  }
+  vir_add(v, 0.5, st.vE[i], st.vE[n + i], st.vE[2 * (size_t)n + i], st.dDps_x[i], st.dDps_y[i], st.dDps_z[i]);
+  for (int c = 0; c < 6; ++c) st.vref[(size_t)c * n + i] = v[c];
 }
 
 // final gradient: bonded explicit-geometry terms (own + partner slots), BO-derivative pair forces, non-bonded
@@ -1169,28 +1302,32 @@ void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tab
   else hipLaunchKernelGGL(k_bo_correct<double>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
 }
 
-template <class TR, class TS, int G>
+// PA: the per-atom variants (reference-split virial shares), built for G = 64 only: a group of 64 lanes gives the same sums as
+// a smaller group (the extra lanes add exact zeros ahead of the same butterfly), so a per-atom call leaves the energies and the
+// gradient bitwise unchanged
+template <class TR, class TS, int G, bool PA>
 void launch_terms_g(const State& st, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * G, 256)), blk(256), wgrid(nblk(st.n, 4));
-  hipLaunchKernelGGL((k_valence<TR, TS, G>), grid, blk, 0, s, st, g, t);
+  hipLaunchKernelGGL((k_valence<TR, TS, G, PA>), grid, blk, 0, s, st, g, t);
   if (sizeof(TR) == 4) {
-    if (st.census) hipLaunchKernelGGL((k_torsion_f<TS, G, true>), grid, blk, 0, s, st, g, t);
-    else hipLaunchKernelGGL((k_torsion_f<TS, G, false>), grid, blk, 0, s, st, g, t);
+    if (st.census) hipLaunchKernelGGL((k_torsion_f<TS, G, true, PA>), grid, blk, 0, s, st, g, t);
+    else hipLaunchKernelGGL((k_torsion_f<TS, G, false, PA>), grid, blk, 0, s, st, g, t);
   } else {
-    if (st.census) hipLaunchKernelGGL((k_torsion_d<G, true>), grid, blk, 0, s, st, g, t);
-    else hipLaunchKernelGGL((k_torsion_d<G, false>), grid, blk, 0, s, st, g, t);
+    if (st.census) hipLaunchKernelGGL((k_torsion_d<G, true, PA>), grid, blk, 0, s, st, g, t);
+    else hipLaunchKernelGGL((k_torsion_d<G, false, PA>), grid, blk, 0, s, st, g, t);
   }
-  hipLaunchKernelGGL((k_hbond_donor<TR, TS>), dim3(nblk(st.n, kHbWaves)), dim3(64 * kHbWaves), 0, s, st, g, t);
+  hipLaunchKernelGGL((k_hbond_donor<TR, TS, PA>), dim3(nblk(st.n, kHbWaves)), dim3(64 * kHbWaves), 0, s, st, g, t);
   hipLaunchKernelGGL(k_hbond_acceptor<TR>, wgrid, blk, 0, s, st, g, t);
 }
 
 template <class TR, class TS>
 void launch_terms_t(const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
-  if (bmax <= 4) launch_terms_g<TR, TS, 4>(st, g, t, s);
-  else if (bmax <= 8) launch_terms_g<TR, TS, 8>(st, g, t, s);
-  else if (bmax <= 16) launch_terms_g<TR, TS, 16>(st, g, t, s);
-  else if (bmax <= 32) launch_terms_g<TR, TS, 32>(st, g, t, s);
-  else launch_terms_g<TR, TS, 64>(st, g, t, s);
+  if (st.peratom) launch_terms_g<TR, TS, 64, true>(st, g, t, s);
+  else if (bmax <= 4) launch_terms_g<TR, TS, 4, false>(st, g, t, s);
+  else if (bmax <= 8) launch_terms_g<TR, TS, 8, false>(st, g, t, s);
+  else if (bmax <= 16) launch_terms_g<TR, TS, 16, false>(st, g, t, s);
+  else if (bmax <= 32) launch_terms_g<TR, TS, 32, false>(st, g, t, s);
+  else launch_terms_g<TR, TS, 64, false>(st, g, t, s);
 }
 
 void launch_atom_terms(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
@@ -1213,6 +1350,10 @@ void launch_bond_derivatives(const State& st, hipStream_t s) {
 
 void launch_eatom_ref(const State& st, hipStream_t s) {
   hipLaunchKernelGGL(k_eatom_ref, dim3(nblk(st.n, 128)), dim3(128), 0, s, st);
+}
+
+void launch_vatom_ref(const State& st, hipStream_t s) {
+  hipLaunchKernelGGL(k_vatom_ref, dim3(nblk(st.n, 128)), dim3(128), 0, s, st);
 }
 
 void launch_gather(const State& st, hipStream_t s) {

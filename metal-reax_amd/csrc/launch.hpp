@@ -50,6 +50,7 @@ void launch_angle_terms(Precision p, const State& st, const Globals& g, const Ta
 void launch_bond_derivatives(const State& st, hipStream_t s);
 void launch_gather(const State& st, hipStream_t s);
 void launch_eatom_ref(const State& st, hipStream_t s);   // per-atom energies of the reference's attribution (peratom calls)
+void launch_vatom_ref(const State& st, hipStream_t s);   // per-atom virials of the reference's split (peratom calls, before the gather)
 
 // ---- non-bonded terms and the charge-equilibration matrix (k_nonbonded.hip)
 void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s);
