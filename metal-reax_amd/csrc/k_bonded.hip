@@ -228,93 +228,127 @@ __global__ void __attribute__((amdgpu_waves_per_eu(4))) __launch_bounds__(64 * k
   }
 }
 
+// ---- atom groups: G lanes per atom (G a power of two >= the largest bond count), lane p owns bond slot p of its atom ----------
+// Sums over a group use xor shuffles with offsets below G, so they stay inside the group and run in a fixed order.
+template <int G>
+__device__ inline int gsum_int(int v) {
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+template <int G>
+__device__ inline double gsum(double v) {
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline int wave_max(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
 // ================================================================================================================
 // corrected bond orders (always from the canonical end) and atom quantities
 // ================================================================================================================
 // Also finds each slot's mirror on the other atom (b_sym) and the step's largest bond count (flags[7]); a slot without a mirror
-// sets flags[3].
+// sets flags[3]. kBoG lanes per atom, lane p taking slots p, p + kBoG, ...: the slots are independent (the correction reads the
+// uncorrected sums of both ends), so a lane's chain of dependent loads is one slot's instead of the atom's; the total bond order
+// is summed in slot order through the group (the order of a serial loop, so the sums are bitwise those of one thread per atom).
+constexpr int kBoG = 4;
+
+// the mirror slot, the corrected bond orders and the derivative coefficients of slot p of atom i; returns the corrected BO
 template <class TB>
-__global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables t) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline TB bo_correct_slot(const State& st, const Globals& g, const Tables& t, int i, int p) {
+  const int n = st.n;
+  const size_t q = sl(p, i, n);
+  const int j = st.b_j[q];
+  {   // mirror slot of this bond on atom j
+    const int want = shift_neg(st.b_s[q]), nbj = min(st.b_cnt[j], st.maxb);
+    int found = -1;
+    for (int r = 0; r < nbj; ++r)
+      if (st.b_j[sl(r, j, n)] == i && st.b_s[sl(r, j, n)] == want) { found = r; break; }
+    st.b_sym[q] = found < 0 ? 0 : found;
+    if (found < 0) st.flags[3] = 1;
+  }
+  const bool first = canonical_first(i, j, st.b_s[q], st.tag);
+  const int a = first ? i : j, b = first ? j : i;
+  const int ta = st.type[a], tb = st.type[b];
+  const DevPair& P = t.pair[ta * g.ntypes + tb];
+  const TB vi = (TB)t.sp[ta].valency, vj = (TB)t.sp[tb].valency;
+  const TB Dpi = (TB)st.Dp[a], Dpj = (TB)st.Dp[b], Dbi = (TB)st.Dpb[a], Dbj = (TB)st.Dpb[b];
+  const TB BO = (TB)st.bo[q], BOs = (TB)st.bop_s[q], BOpi = (TB)st.bop_pi[q], BOpi2 = (TB)st.bop_pi2[q];
+  TB oBO, oBOs, oBOpi, oBOpi2, C1dbo = 0, C2dbo = 0, C3dbo = 0, C1pi = 0, C2pi = 0, C3pi = 0, C4pi = 0, C1pi2 = 0, C2pi2 = 0, C3pi2 = 0, C4pi2 = 0;
+  if ((TB)P.ovc < TB(0.001) && (TB)P.v13cor < TB(0.001)) {
+    oBO = BO; oBOs = BOs; oBOpi = BOpi; oBOpi2 = BOpi2;
+    C1dbo = 1; C1pi = 1; C1pi2 = 1;
+  } else {
+    const TB boc1 = (TB)g.p_boc1, boc2 = (TB)g.p_boc2;
+    TB f1 = 1, Cf1ij = 0, Cf1ji = 0;
+    if ((TB)P.ovc >= TB(0.001)) {
+      const TB e1i = safe_exp<TB>(-boc1 * Dpi), e2i = safe_exp<TB>(-boc2 * Dpi), e1j = safe_exp<TB>(-boc1 * Dpj), e2j = safe_exp<TB>(-boc2 * Dpj);
+      const TB f2 = e1i + e1j, f3 = -TB(1) / boc2 * rlog<TB>(TB(0.5) * (e2i + e2j));
+      f1 = TB(0.5) * ((vi + f2) / (vi + f2 + f3) + (vj + f2) / (vj + f2 + f3));
+      const TB ui = vi + (f2 + f3), uj = vj + (f2 + f3), S2 = e2i + e2j;
+      Cf1ij = TB(0.5) * (-boc1 * e1i / ui - ((vi + f2) / (ui * ui)) * (-boc1 * e1i + e2i / S2) + -boc1 * e1i / uj -
+                         ((vj + f2) / (uj * uj)) * (-boc1 * e1i + e2i / S2));
+      const TB A = TB(0.5) * f3 * (TB(1) / (ui * ui) + TB(1) / (uj * uj));
+      const TB B = -TB(0.5) * ((ui - f3) / (ui * ui) + (uj - f3) / (uj * uj));
+      Cf1ji = -A * boc1 * e1j + B * e2j / S2;
+    }
+    TB f4 = 1, f5 = 1, Cf45ij = 0, Cf45ji = 0;
+    const TB boc3 = (TB)P.p_boc3, boc4 = (TB)P.p_boc4, boc5 = (TB)P.p_boc5;
+    if ((TB)P.v13cor >= TB(0.001)) {
+      const TB ef4 = safe_exp<TB>(-(boc4 * (BO * BO) - Dbi) * boc3 + boc5);
+      const TB ef5 = safe_exp<TB>(-(boc4 * (BO * BO) - Dbj) * boc3 + boc5);
+      f4 = TB(1) / (TB(1) + ef4);
+      f5 = TB(1) / (TB(1) + ef5);
+      Cf45ij = -f4 * ef4;
+      Cf45ji = -f5 * ef5;
+    }
+    const TB A0 = f1 * (f4 * f5);
+    const TB A1 = -2 * boc3 * boc4 * BO * (Cf45ij + Cf45ji);
+    const TB A2ij = Cf1ij / f1 + boc3 * Cf45ij, A2ji = Cf1ji / f1 + boc3 * Cf45ji;
+    const TB A3ij = A2ij + Cf1ij / f1, A3ji = A2ji + Cf1ji / f1;
+    oBO = BO * A0;
+    oBOpi = BOpi * A0 * f1;
+    oBOpi2 = BOpi2 * A0 * f1;
+    oBOs = oBO - (oBOpi + oBOpi2);
+    C1dbo = A0 + oBO * A1; C2dbo = oBO * A2ij; C3dbo = oBO * A2ji;
+    C1pi = f1 * f1 * f4 * f5; C2pi = oBOpi * A1; C3pi = oBOpi * A3ij; C4pi = oBOpi * A3ji;
+    C1pi2 = f1 * f1 * f4 * f5; C2pi2 = oBOpi2 * A1; C3pi2 = oBOpi2 * A3ij; C4pi2 = oBOpi2 * A3ji;
+  }
+  if (oBO < TB(1e-10)) oBO = 0;
+  if (oBOs < TB(1e-10)) oBOs = 0;
+  if (oBOpi < TB(1e-10)) oBOpi = 0;
+  if (oBOpi2 < TB(1e-10)) oBOpi2 = 0;
+  st.bo[q] = (double)oBO; st.bo_s[q] = (double)oBOs; st.bo_pi[q] = (double)oBOpi; st.bo_pi2[q] = (double)oBOpi2;
+  st.C[0][q] = (double)C1dbo; st.C[1][q] = (double)C2pi; st.C[2][q] = (double)C2pi2; st.C[3][q] = (double)C1pi; st.C[4][q] = (double)C1pi2;
+  st.C[5][q] = (double)(first ? C2dbo : C3dbo);
+  st.C[6][q] = (double)(first ? C3pi : C4pi);
+  st.C[7][q] = (double)(first ? C3pi2 : C4pi2);
+  st.Cdbo[q] = 0; st.Cdbopi[q] = 0; st.Cdbopi2[q] = 0; st.fpx[q] = 0; st.fpy[q] = 0; st.fpz[q] = 0; st.cdp[q] = 0;
+  return oBO;
+}
+
+template <class TB>
+__global__ void __launch_bounds__(256) k_bo_correct(State st, Globals g, Tables t) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kBoG, p0 = gid % kBoG, n = st.n;
+  const bool live = i < n;
+  const int nb = live ? min(st.b_cnt[i], st.maxb) : 0;
   {   // the largest bond count of this step (flags[7]), one atomic per wave
-    int c = i < st.n ? st.b_cnt[i] : 0;
+    int c = live && p0 == 0 ? st.b_cnt[i] : 0;
     for (int off = 32; off > 0; off >>= 1) c = max(c, __shfl_xor(c, off, 64));
     if (lane_id() == 0 && c > 0) atomicMax(&st.flags[7], c);
   }
-  if (i >= st.n) return;
-  const int n = st.n, ti = st.type[i];
-  const int nb = min(st.b_cnt[i], st.maxb);
+  const int nbw = wave_max(nb);
+  const int base = lane_id() - p0;   // the group's first lane
   TB tbo = 0;
-  for (int p = 0; p < nb; ++p) {
-    const size_t q = sl(p, i, n);
-    const int j = st.b_j[q];
-    {   // mirror slot of this bond on atom j
-      const int want = shift_neg(st.b_s[q]), nbj = min(st.b_cnt[j], st.maxb);
-      int found = -1;
-      for (int r = 0; r < nbj; ++r)
-        if (st.b_j[sl(r, j, n)] == i && st.b_s[sl(r, j, n)] == want) { found = r; break; }
-      st.b_sym[q] = found < 0 ? 0 : found;
-      if (found < 0) st.flags[3] = 1;
-    }
-    const bool first = canonical_first(i, j, st.b_s[q], st.tag);
-    const int a = first ? i : j, b = first ? j : i;
-    const int ta = st.type[a], tb = st.type[b];
-    const DevPair& P = t.pair[ta * g.ntypes + tb];
-    const TB vi = (TB)t.sp[ta].valency, vj = (TB)t.sp[tb].valency;
-    const TB Dpi = (TB)st.Dp[a], Dpj = (TB)st.Dp[b], Dbi = (TB)st.Dpb[a], Dbj = (TB)st.Dpb[b];
-    const TB BO = (TB)st.bo[q], BOs = (TB)st.bop_s[q], BOpi = (TB)st.bop_pi[q], BOpi2 = (TB)st.bop_pi2[q];
-    TB oBO, oBOs, oBOpi, oBOpi2, C1dbo = 0, C2dbo = 0, C3dbo = 0, C1pi = 0, C2pi = 0, C3pi = 0, C4pi = 0, C1pi2 = 0, C2pi2 = 0, C3pi2 = 0, C4pi2 = 0;
-    if ((TB)P.ovc < TB(0.001) && (TB)P.v13cor < TB(0.001)) {
-      oBO = BO; oBOs = BOs; oBOpi = BOpi; oBOpi2 = BOpi2;
-      C1dbo = 1; C1pi = 1; C1pi2 = 1;
-    } else {
-      const TB boc1 = (TB)g.p_boc1, boc2 = (TB)g.p_boc2;
-      TB f1 = 1, Cf1ij = 0, Cf1ji = 0;
-      if ((TB)P.ovc >= TB(0.001)) {
-        const TB e1i = safe_exp<TB>(-boc1 * Dpi), e2i = safe_exp<TB>(-boc2 * Dpi), e1j = safe_exp<TB>(-boc1 * Dpj), e2j = safe_exp<TB>(-boc2 * Dpj);
-        const TB f2 = e1i + e1j, f3 = -TB(1) / boc2 * rlog<TB>(TB(0.5) * (e2i + e2j));
-        f1 = TB(0.5) * ((vi + f2) / (vi + f2 + f3) + (vj + f2) / (vj + f2 + f3));
-        const TB ui = vi + (f2 + f3), uj = vj + (f2 + f3), S2 = e2i + e2j;
-        Cf1ij = TB(0.5) * (-boc1 * e1i / ui - ((vi + f2) / (ui * ui)) * (-boc1 * e1i + e2i / S2) + -boc1 * e1i / uj -
-                           ((vj + f2) / (uj * uj)) * (-boc1 * e1i + e2i / S2));
-        const TB A = TB(0.5) * f3 * (TB(1) / (ui * ui) + TB(1) / (uj * uj));
-        const TB B = -TB(0.5) * ((ui - f3) / (ui * ui) + (uj - f3) / (uj * uj));
-        Cf1ji = -A * boc1 * e1j + B * e2j / S2;
-      }
-      TB f4 = 1, f5 = 1, Cf45ij = 0, Cf45ji = 0;
-      const TB boc3 = (TB)P.p_boc3, boc4 = (TB)P.p_boc4, boc5 = (TB)P.p_boc5;
-      if ((TB)P.v13cor >= TB(0.001)) {
-        const TB ef4 = safe_exp<TB>(-(boc4 * (BO * BO) - Dbi) * boc3 + boc5);
-        const TB ef5 = safe_exp<TB>(-(boc4 * (BO * BO) - Dbj) * boc3 + boc5);
-        f4 = TB(1) / (TB(1) + ef4);
-        f5 = TB(1) / (TB(1) + ef5);
-        Cf45ij = -f4 * ef4;
-        Cf45ji = -f5 * ef5;
-      }
-      const TB A0 = f1 * (f4 * f5);
-      const TB A1 = -2 * boc3 * boc4 * BO * (Cf45ij + Cf45ji);
-      const TB A2ij = Cf1ij / f1 + boc3 * Cf45ij, A2ji = Cf1ji / f1 + boc3 * Cf45ji;
-      const TB A3ij = A2ij + Cf1ij / f1, A3ji = A2ji + Cf1ji / f1;
-      oBO = BO * A0;
-      oBOpi = BOpi * A0 * f1;
-      oBOpi2 = BOpi2 * A0 * f1;
-      oBOs = oBO - (oBOpi + oBOpi2);
-      C1dbo = A0 + oBO * A1; C2dbo = oBO * A2ij; C3dbo = oBO * A2ji;
-      C1pi = f1 * f1 * f4 * f5; C2pi = oBOpi * A1; C3pi = oBOpi * A3ij; C4pi = oBOpi * A3ji;
-      C1pi2 = f1 * f1 * f4 * f5; C2pi2 = oBOpi2 * A1; C3pi2 = oBOpi2 * A3ij; C4pi2 = oBOpi2 * A3ji;
-    }
-    if (oBO < TB(1e-10)) oBO = 0;
-    if (oBOs < TB(1e-10)) oBOs = 0;
-    if (oBOpi < TB(1e-10)) oBOpi = 0;
-    if (oBOpi2 < TB(1e-10)) oBOpi2 = 0;
-    st.bo[q] = (double)oBO; st.bo_s[q] = (double)oBOs; st.bo_pi[q] = (double)oBOpi; st.bo_pi2[q] = (double)oBOpi2;
-    st.C[0][q] = (double)C1dbo; st.C[1][q] = (double)C2pi; st.C[2][q] = (double)C2pi2; st.C[3][q] = (double)C1pi; st.C[4][q] = (double)C1pi2;
-    st.C[5][q] = (double)(first ? C2dbo : C3dbo);
-    st.C[6][q] = (double)(first ? C3pi : C4pi);
-    st.C[7][q] = (double)(first ? C3pi2 : C4pi2);
-    st.Cdbo[q] = 0; st.Cdbopi[q] = 0; st.Cdbopi2[q] = 0; st.fpx[q] = 0; st.fpy[q] = 0; st.fpz[q] = 0; st.cdp[q] = 0;
-    tbo += oBO;
+  for (int k0 = 0; k0 < nbw; k0 += kBoG) {   // wave-uniform rounds of kBoG slots
+    const int p = k0 + p0;
+    const TB o = p < nb ? bo_correct_slot<TB>(st, g, t, i, p) : TB(0);
+    for (int l = 0; l < kBoG; ++l) tbo += __shfl(o, base + l, 64);   // slot order; absent slots add exact zeros
   }
+  if (!live || p0 != 0) return;
+  const int ti = st.type[i];
   st.tbo[i] = (double)tbo;
   st.CdD_self[i] = 0; st.fs_x[i] = 0; st.fs_y[i] = 0; st.fs_z[i] = 0;
   for (int k = 0; k < 6; ++k) st.vb[(size_t)k * n + i] = 0;
@@ -341,93 +375,125 @@ __global__ void __launch_bounds__(128) k_bo_correct(State st, Globals g, Tables 
 // ================================================================================================================
 // bond energy, triple-bond stabilisation, lone pair (+ C2), over/under coordination
 // ================================================================================================================
+// kAtG lanes per atom, lane p taking slots p, p + kAtG, ...; the per-atom sums run in slot order through the group (each lane
+// of the group forms the same sums, in the order of a serial loop over the slots, so they are bitwise those of one thread per
+// atom), and the atom's own terms are evaluated by every lane of the group alike.
+constexpr int kAtG = 4;
+
 template <class TR, class TS>
-__global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables t) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= st.n) return;
-  const int n = st.n, ti = st.type[i];
-  double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, cdself = 0, e_c2 = 0;
-  if (st.peratom)   // the reference-split virial shares, accumulated by the angle kernels that follow
+__global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables t) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kAtG, p0 = gid % kAtG, n = st.n;
+  const bool live = i < n;
+  const int ti = live ? st.type[i] : -1;
+  const int nb = ti >= 0 ? min(st.b_cnt[i], st.maxb) : 0;
+  const int nbw = wave_max(nb);
+  const int base = lane_id() - p0;
+  const size_t nsl = (size_t)st.maxb * n;
+  if (live && p0 == 0 && st.peratom)   // the reference-split virial shares, accumulated by the angle kernels that follow
     for (int c = 0; c < 6; ++c) st.vref[(size_t)c * n + i] = 0.0;
-  if (ti >= 0) {
-    const DevSpecies& si = t.sp[ti];
-    const int nb = min(st.b_cnt[i], st.maxb);
-    if (st.peratom) {
-      const size_t nsl = (size_t)st.maxb * n;
-      for (int p = 0; p < nb; ++p) {
-        st.eslot[sl(p, i, n)] = 0.0;
-        for (int c = 0; c < 6; ++c) st.vslot[c * nsl + sl(p, i, n)] = 0.0;
-      }
-    }
-    // bond energy of the bonds whose canonical end is this atom (the reference's tag order, bonded.cpp:222-252)
-    for (int p = 0; p < nb; ++p) {
+  double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, cdself = 0, e_c2 = 0;
+  // bond energy of the bonds whose canonical end is this atom (the reference's tag order, bonded.cpp:222-252)
+  for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+    const int p = k0 + p0;
+    double eb = 0, et = 0, cdt = 0;
+    if (p < nb) {
       const size_t q = sl(p, i, n);
-      const int j = st.b_j[q];
-      if (!canonical_first(i, j, st.b_s[q], st.tag)) continue;
-      const int tj = st.type[j];
-      const DevPair& P = t.pair[ti * g.ntypes + tj];
-      const TR BOs = (TR)st.bo_s[q], BOpi = (TR)st.bo_pi[q], BOpi2 = (TR)st.bo_pi2[q];
-      const TR pw = BOs == TR(0) ? TR(0) : rpow<TR>(BOs, (TR)P.p_be2);
-      const TR ex = rexp<TR>((TR)P.p_be1 * (TR(1) - pw));
-      const TR CEbo = -(TR)P.De_s * ex * (TR(1) - (TR)P.p_be1 * (TR)P.p_be2 * pw);
-      const double e_b = (double)(-(TR)P.De_s * BOs * ex - (TR)P.De_p * BOpi - (TR)P.De_pp * BOpi2);
-      e_bond += e_b;
-      if (st.peratom) st.eslot[q] += 0.5 * e_b;
-      double cdbo = (double)CEbo;
-      st.Cdbopi[q] += -(double)(CEbo + (TR)P.De_p);
-      st.Cdbopi2[q] += -(double)(CEbo + (TR)P.De_pp);
-      const TR BO = (TR)st.bo[q];
-      if ((TS)st.bo[q] >= TS(1.00) && P.tbs) {
-        const TR gp3 = (TR)g.gp3, gp4 = (TR)g.gp4, gp7 = (TR)g.gp7, gp10 = (TR)g.gp10;
-        const TR exphu = rexp<TR>(-gp7 * ((BO - TR(2.50)) * (BO - TR(2.50))));
-        const TR ea = rexp<TR>(-gp3 * ((TR)st.tbo[i] - BO)), eb = rexp<TR>(-gp3 * ((TR)st.tbo[j] - BO));
-        const TR eov = safe_exp<TR>(gp4 * ((TR)st.Delta[i] + (TR)st.Delta[j]));
-        const TR hov = TR(1) / (TR(1) + TR(25.0) * eov);
-        const double e_t = (double)(gp10 * exphu * hov * (ea + eb));
-        e_bond += e_t;
-        if (st.peratom) st.eslot[q] += 0.5 * e_t;
-        cdbo += (double)(gp10 * exphu * hov * (ea + eb) * (gp3 - TR(2.0) * gp7 * (BO - TR(2.50))));
-        cdself += (double)(-gp10 * exphu * hov * (gp3 * ea + TR(25.0) * gp4 * eov * hov * (ea + eb)));
-        st.cdp[q] += (double)(-gp10 * exphu * hov * (gp3 * eb + TR(25.0) * gp4 * eov * hov * (ea + eb)));
+      if (st.peratom) {
+        st.eslot[q] = 0.0;
+        for (int c = 0; c < 6; ++c) st.vslot[c * nsl + q] = 0.0;
       }
-      st.Cdbo[q] += cdbo;
+      const int j = st.b_j[q];
+      if (canonical_first(i, j, st.b_s[q], st.tag)) {
+        const int tj = st.type[j];
+        const DevPair& P = t.pair[ti * g.ntypes + tj];
+        const TR BOs = (TR)st.bo_s[q], BOpi = (TR)st.bo_pi[q], BOpi2 = (TR)st.bo_pi2[q];
+        const TR pw = BOs == TR(0) ? TR(0) : rpow<TR>(BOs, (TR)P.p_be2);
+        const TR ex = rexp<TR>((TR)P.p_be1 * (TR(1) - pw));
+        const TR CEbo = -(TR)P.De_s * ex * (TR(1) - (TR)P.p_be1 * (TR)P.p_be2 * pw);
+        eb = (double)(-(TR)P.De_s * BOs * ex - (TR)P.De_p * BOpi - (TR)P.De_pp * BOpi2);
+        if (st.peratom) st.eslot[q] += 0.5 * eb;
+        double cdbo = (double)CEbo;
+        st.Cdbopi[q] += -(double)(CEbo + (TR)P.De_p);
+        st.Cdbopi2[q] += -(double)(CEbo + (TR)P.De_pp);
+        const TR BO = (TR)st.bo[q];
+        if ((TS)st.bo[q] >= TS(1.00) && P.tbs) {
+          const TR gp3 = (TR)g.gp3, gp4 = (TR)g.gp4, gp7 = (TR)g.gp7, gp10 = (TR)g.gp10;
+          const TR exphu = rexp<TR>(-gp7 * ((BO - TR(2.50)) * (BO - TR(2.50))));
+          const TR ea = rexp<TR>(-gp3 * ((TR)st.tbo[i] - BO)), ebb = rexp<TR>(-gp3 * ((TR)st.tbo[j] - BO));
+          const TR eov = safe_exp<TR>(gp4 * ((TR)st.Delta[i] + (TR)st.Delta[j]));
+          const TR hov = TR(1) / (TR(1) + TR(25.0) * eov);
+          et = (double)(gp10 * exphu * hov * (ea + ebb));
+          if (st.peratom) st.eslot[q] += 0.5 * et;
+          cdbo += (double)(gp10 * exphu * hov * (ea + ebb) * (gp3 - TR(2.0) * gp7 * (BO - TR(2.50))));
+          cdt = (double)(-gp10 * exphu * hov * (gp3 * ea + TR(25.0) * gp4 * eov * hov * (ea + ebb)));
+          st.cdp[q] += (double)(-gp10 * exphu * hov * (gp3 * ebb + TR(25.0) * gp4 * eov * hov * (ea + ebb)));
+        }
+        st.Cdbo[q] += cdbo;
+      }
     }
-    // lone pair
-    if (nb > 0 || g.enobonds) {
-      const TR Dlp = (TR)st.Delta_lp[i], p_lp2 = (TR)si.p_lp2;
-      const TR e2 = safe_exp<TR>(TR(-75) * Dlp), inv = TR(1) / (TR(1) + e2);
-      e_lp += (double)(p_lp2 * Dlp * inv);
-      cdself += (double)((p_lp2 * inv + TR(75) * p_lp2 * Dlp * e2 * (inv * inv)) * (TR)st.dDelta_lp[i]);
+    for (int l = 0; l < kAtG; ++l) {   // slot order; slots without these terms add exact zeros
+      e_bond += __shfl(eb, base + l, 64);
+      e_bond += __shfl(et, base + l, 64);
+      cdself += __shfl(cdt, base + l, 64);
     }
-    // C2 correction, every directed C-C bond (reference bonded.cpp:268-280)
-    if (g.c2_active && si.c2) {
-      const TR p_lp3 = (TR)g.p_lp3, Di = (TR)st.Delta[i];
-      for (int p = 0; p < nb; ++p) {
+  }
+  const DevSpecies& si = t.sp[ti >= 0 ? ti : 0];
+  // lone pair
+  if (ti >= 0 && (nb > 0 || g.enobonds)) {
+    const TR Dlp = (TR)st.Delta_lp[i], p_lp2 = (TR)si.p_lp2;
+    const TR e2 = safe_exp<TR>(TR(-75) * Dlp), inv = TR(1) / (TR(1) + e2);
+    e_lp += (double)(p_lp2 * Dlp * inv);
+    cdself += (double)((p_lp2 * inv + TR(75) * p_lp2 * Dlp * e2 * (inv * inv)) * (TR)st.dDelta_lp[i]);
+  }
+  // C2 correction, every directed C-C bond (reference bonded.cpp:268-280)
+  const bool c2 = ti >= 0 && g.c2_active && si.c2;
+  if (__any(c2)) {
+    const TR p_lp3 = (TR)g.p_lp3, Di = (TR)(c2 ? st.Delta[i] : 0.0);
+    for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+      const int p = k0 + p0;
+      double ec = 0, cdc = 0;
+      if (c2 && p < nb) {
         const size_t q = sl(p, i, n);
-        if (!t.sp[st.type[st.b_j[q]]].c2) continue;
-        const TR vov3 = (TR)st.bo[q] - Di - TR(0.040) * rpow<TR>(Di, TR(4.));
-        const TS Ds = (TS)st.Delta[i];
-        if ((TS)st.bo[q] - Ds - TS(0.040) * rpow<TS>(Ds, TS(4.)) > TS(3.)) {
-          const double e_c = (double)(p_lp3 * ((vov3 - TR(3.0)) * (vov3 - TR(3.0))));
-          e_lp += e_c;
-          e_c2 += e_c;
-          if (st.peratom) st.eslot[q] += 0.5 * e_c;
-          st.Cdbo[q] += (double)(TR(2.) * p_lp3 * (vov3 - TR(3.)));
-          cdself += (double)(TR(2.) * p_lp3 * (vov3 - TR(3.)) * (-TR(1.) - TR(0.16) * rpow<TR>(Di, TR(3.))));
+        if (t.sp[st.type[st.b_j[q]]].c2) {
+          const TR vov3 = (TR)st.bo[q] - Di - TR(0.040) * rpow<TR>(Di, TR(4.));
+          const TS Ds = (TS)st.Delta[i];
+          if ((TS)st.bo[q] - Ds - TS(0.040) * rpow<TS>(Ds, TS(4.)) > TS(3.)) {
+            ec = (double)(p_lp3 * ((vov3 - TR(3.0)) * (vov3 - TR(3.0))));
+            if (st.peratom) st.eslot[q] += 0.5 * ec;
+            st.Cdbo[q] += (double)(TR(2.) * p_lp3 * (vov3 - TR(3.)));
+            cdc = (double)(TR(2.) * p_lp3 * (vov3 - TR(3.)) * (-TR(1.) - TR(0.16) * rpow<TR>(Di, TR(3.))));
+          }
         }
       }
+      for (int l = 0; l < kAtG; ++l) {
+        const double e = __shfl(ec, base + l, 64);
+        e_lp += e;
+        e_c2 += e;
+        cdself += __shfl(cdc, base + l, 64);
+      }
     }
-    // over / under coordination (reference bonded.cpp:282-318, terms.hpp:231-260)
-    {
-      const TR dfvl = si.heavy ? TR(0) : TR(1);
-      TR sum1 = 0, sum2 = 0;
-      for (int p = 0; p < nb; ++p) {
+  }
+  // over / under coordination (reference bonded.cpp:282-318, terms.hpp:231-260)
+  {
+    const TR dfvl = si.heavy ? TR(0) : TR(1);
+    TR sum1 = 0, sum2 = 0;
+    for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+      const int p = k0 + p0;
+      TR a1 = 0, a2 = 0;
+      if (p < nb) {
         const size_t q = sl(p, i, n);
         const int j = st.b_j[q];
         const DevPair& P = t.pair[ti * g.ntypes + st.type[j]];
-        sum1 += (TR)P.p_ovun1 * (TR)P.De_s * (TR)st.bo[q];
-        sum2 += ((TR)st.Delta[j] - dfvl * (TR)st.Delta_lp_temp[j]) * ((TR)st.bo_pi[q] + (TR)st.bo_pi2[q]);
+        a1 = (TR)P.p_ovun1 * (TR)P.De_s * (TR)st.bo[q];
+        a2 = ((TR)st.Delta[j] - dfvl * (TR)st.Delta_lp_temp[j]) * ((TR)st.bo_pi[q] + (TR)st.bo_pi2[q]);
       }
+      for (int l = 0; l < kAtG; ++l) {
+        sum1 += __shfl(a1, base + l, 64);
+        sum2 += __shfl(a2, base + l, 64);
+      }
+    }
+    if (ti >= 0) {
       const bool under = nb > 0 || g.enobonds;
       const TR ovun3 = (TR)g.p_ovun3, ovun4 = (TR)g.p_ovun4, ovun6 = (TR)g.p_ovun6, ovun7 = (TR)g.p_ovun7, ovun8 = (TR)g.p_ovun8;
       const TR p_ovun2 = (TR)si.p_ovun2, p_ovun5 = (TR)si.p_ovun5;
@@ -451,7 +517,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
       if (under) e_un += (double)eun;
       cdself += (double)CEover3;
       if (under) cdself += (double)CEunder3;
-      for (int p = 0; p < nb; ++p) {
+      for (int p = p0; p < nb; p += kAtG) {
         const size_t q = sl(p, i, n);
         const int j = st.b_j[q];
         const DevPair& P = t.pair[ti * g.ntypes + st.type[j]];
@@ -465,6 +531,7 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
       }
     }
   }
+  if (!live || p0 != 0) return;
   st.CdD_self[i] += cdself;
   st.eatom[T_BOND * n + i] = e_bond;
   st.eatom[T_LP * n + i] = e_lp;
@@ -476,22 +543,6 @@ __global__ void __launch_bounds__(128) k_atom_terms(State st, Globals g, Tables 
 // ================================================================================================================
 // valence angle, penalty, 3-body conjugation (centre j)
 // ================================================================================================================
-// ---- atom groups: G lanes per atom (G a power of two >= the largest bond count), lane p owns bond slot p of its atom ----------
-// Sums over a group use xor shuffles with offsets below G, so they stay inside the group and run in a fixed order.
-template <int G>
-__device__ inline int gsum_int(int v) {
-  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-template <int G>
-__device__ inline double gsum(double v) {
-  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-__device__ inline int wave_max(int v) {
-  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
-}
 
 // Valence angle, penalty and three-body conjugation at the centre j [reference bonded.cpp:321-474]. Lane p of j's group evaluates
 // the pairs (p, k) with p < k of j's bonds, k running over a wave-uniform loop: the contributions to slot p stay in the lane, the
@@ -1200,42 +1251,62 @@ __global__ void __launch_bounds__(128) k_eatom_ref(State st) {
 // ================================================================================================================
 // bond-order derivative pass as gathers
 // ================================================================================================================
-__global__ void __launch_bounds__(128) k_cdelta(State st) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= st.n) return;
-  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
-  double c = st.CdD_self[i];
-  for (int p = 0; p < nb; ++p) {
-    const size_t q = sl(p, i, n);
-    c += st.cdp[sl(st.b_sym[q], st.b_j[q], n)];
+// kAtG lanes per atom as in k_atom_terms: a lane takes every kAtG-th slot, the atom sums run in slot order through the group
+__global__ void __launch_bounds__(256) k_cdelta(State st) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kAtG, p0 = gid % kAtG, n = st.n;
+  const bool live = i < n;
+  const int nb = live ? min(st.b_cnt[i], st.maxb) : 0;
+  const int nbw = wave_max(nb), base = lane_id() - p0;
+  double c = live ? st.CdD_self[i] : 0.0;
+  for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+    const int p = k0 + p0;
+    double v = 0;
+    if (p < nb) {
+      const size_t q = sl(p, i, n);
+      v = st.cdp[sl(st.b_sym[q], st.b_j[q], n)];
+    }
+    for (int l = 0; l < kAtG; ++l) c += __shfl(v, base + l, 64);
   }
-  st.CdD[i] = c;
+  if (live && p0 == 0) st.CdD[i] = c;
 }
 
-__global__ void __launch_bounds__(128) k_dbond(State st) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= st.n) return;
-  const int n = st.n, nb = min(st.b_cnt[i], st.maxb);
+__global__ void __launch_bounds__(256) k_dbond(State st) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kAtG, p0 = gid % kAtG, n = st.n;
+  const bool live = i < n;
+  const int nb = live ? min(st.b_cnt[i], st.maxb) : 0;
+  const int nbw = wave_max(nb), base = lane_id() - p0;
   double S = 0, Dx = 0, Dy = 0, Dz = 0;
   double Ex = 0, Ey = 0, Ez = 0;   // per-atom calls: sum of the slots' coefficients of dDeltap_self times their bond vectors
   double w[6] = {0, 0, 0, 0, 0, 0};
-  const double cdi = st.CdD[i];
-  for (int p = 0; p < nb; ++p) {
-    const size_t q = sl(p, i, n);
-    const int j = st.b_j[q];
-    const size_t m = sl(st.b_sym[q], j, n);
-    const double cbo = st.Cdbo[q] + st.Cdbo[m], cpi = st.Cdbopi[q] + st.Cdbopi[m], cpi2 = st.Cdbopi2[q] + st.Cdbopi2[m];
-    const double cdel = cdi + st.CdD[j];
-    const double C1dbo = st.C[0][q], C2pi = st.C[1][q], C2pi2 = st.C[2][q], C1pi = st.C[3][q], C1pi2 = st.C[4][q];
-    const double T1 = (C1dbo * cbo + C1dbo * cdel) + C2pi * cpi + C2pi2 * cpi2;
-    const double Sq = (st.C[5][q] * cbo + st.C[5][q] * cdel) + st.C[6][q] * cpi + st.C[7][q] * cpi2;
-    S += Sq;
-    Ex += Sq * st.b_dx[q]; Ey += Sq * st.b_dy[q]; Ez += Sq * st.b_dz[q];
-    const double sc = T1 * st.dbop_c[q] + (C1pi * cpi) * st.dlnpi_c[q] + (C1pi2 * cpi2) * st.dlnpi2_c[q];
-    Dx += sc * st.b_dx[q]; Dy += sc * st.b_dy[q]; Dz += sc * st.b_dz[q];
-    // a central pair force: gradient c d on this end, virial d (x) c d, half at each end
-    vir_add(w, 0.5 * sc, st.b_dx[q], st.b_dy[q], st.b_dz[q], st.b_dx[q], st.b_dy[q], st.b_dz[q]);
+  const double cdi = live ? st.CdD[i] : 0.0;
+  for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+    const int p = k0 + p0;
+    double Sq = 0, sc = 0, dx = 0, dy = 0, dz = 0;
+    if (p < nb) {
+      const size_t q = sl(p, i, n);
+      const int j = st.b_j[q];
+      const size_t m = sl(st.b_sym[q], j, n);
+      const double cbo = st.Cdbo[q] + st.Cdbo[m], cpi = st.Cdbopi[q] + st.Cdbopi[m], cpi2 = st.Cdbopi2[q] + st.Cdbopi2[m];
+      const double cdel = cdi + st.CdD[j];
+      const double C1dbo = st.C[0][q], C2pi = st.C[1][q], C2pi2 = st.C[2][q], C1pi = st.C[3][q], C1pi2 = st.C[4][q];
+      const double T1 = (C1dbo * cbo + C1dbo * cdel) + C2pi * cpi + C2pi2 * cpi2;
+      Sq = (st.C[5][q] * cbo + st.C[5][q] * cdel) + st.C[6][q] * cpi + st.C[7][q] * cpi2;
+      sc = T1 * st.dbop_c[q] + (C1pi * cpi) * st.dlnpi_c[q] + (C1pi2 * cpi2) * st.dlnpi2_c[q];
+      dx = st.b_dx[q]; dy = st.b_dy[q]; dz = st.b_dz[q];
+    }
+    for (int l = 0; l < kAtG; ++l) {   // slot order; absent slots add exact zeros
+      const double sq = __shfl(Sq, base + l, 64), s = __shfl(sc, base + l, 64);
+      const double ax = __shfl(dx, base + l, 64), ay = __shfl(dy, base + l, 64), az = __shfl(dz, base + l, 64);
+      S += sq;
+      Ex += sq * ax; Ey += sq * ay; Ez += sq * az;
+      Dx += s * ax; Dy += s * ay; Dz += s * az;
+      // a central pair force: gradient c d on this end, virial d (x) c d, half at each end
+      vir_add(w, 0.5 * s, ax, ay, az, ax, ay, az);
+    }
   }
+  if (!live || p0 != 0) return;
   for (int c = 0; c < 6; ++c) st.vb[(size_t)c * n + i] += w[c];
   st.S[i] = S;
   st.Dx[i] = Dx; st.Dy[i] = Dy; st.Dz[i] = Dz;
@@ -1298,8 +1369,9 @@ void launch_bond_build(Precision p, const State& st, const Cell& c, const Global
 
 
 void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
-  if (p == Precision::FP32) hipLaunchKernelGGL(k_bo_correct<float>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
-  else hipLaunchKernelGGL(k_bo_correct<double>, dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+  const dim3 grid(nblk((long long)st.n * kBoG, 256)), blk(256);
+  if (p == Precision::FP32) hipLaunchKernelGGL(k_bo_correct<float>, grid, blk, 0, s, st, g, t);
+  else hipLaunchKernelGGL(k_bo_correct<double>, grid, blk, 0, s, st, g, t);
 }
 
 // PA: the per-atom variants (reference-split virial shares), built for G = 64 only: a group of 64 lanes gives the same sums as
@@ -1331,9 +1403,10 @@ void launch_terms_t(const State& st, const Globals& g, const Tables& t, int bmax
 }
 
 void launch_atom_terms(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
-  if (p == Precision::FP64) hipLaunchKernelGGL((k_atom_terms<double, double>), dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
-  else if (p == Precision::MIXED) hipLaunchKernelGGL((k_atom_terms<float, double>), dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
-  else hipLaunchKernelGGL((k_atom_terms<float, float>), dim3(nblk(st.n, 128)), dim3(128), 0, s, st, g, t);
+  const dim3 grid(nblk((long long)st.n * kAtG, 256)), blk(256);
+  if (p == Precision::FP64) hipLaunchKernelGGL((k_atom_terms<double, double>), grid, blk, 0, s, st, g, t);
+  else if (p == Precision::MIXED) hipLaunchKernelGGL((k_atom_terms<float, double>), grid, blk, 0, s, st, g, t);
+  else hipLaunchKernelGGL((k_atom_terms<float, float>), grid, blk, 0, s, st, g, t);
 }
 
 void launch_angle_terms(Precision p, const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
@@ -1343,7 +1416,7 @@ void launch_angle_terms(Precision p, const State& st, const Globals& g, const Ta
 }
 
 void launch_bond_derivatives(const State& st, hipStream_t s) {
-  const dim3 grid(nblk(st.n, 128)), blk(128);
+  const dim3 grid(nblk((long long)st.n * kAtG, 256)), blk(256);
   hipLaunchKernelGGL(k_cdelta, grid, blk, 0, s, st);
   hipLaunchKernelGGL(k_dbond, grid, blk, 0, s, st);
 }
