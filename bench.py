@@ -49,8 +49,9 @@ def read_traffic():
 def batch_and_serving(ffield_text, precision, slots=16):
     """The batched workload (512 periodic systems, N log-uniform in [100, 3000], half water / half CH4:O2 gas, 50 NVT steps at
     2500 K, dt 0.1 fs) run one at a time, through the slot executor (one context per slot) and packed in one context. Then the serving workload (32 requests, N log-uniform
-    in [200, 20000], 10-1000 steps): one at a time, all queued at once, and as a Poisson trace whose offered load equals the
-    one-at-a-time capacity (so one-at-a-time serving could not keep up), with p50/p99 latency."""
+    in [200, 20000], 10-1000 steps): one at a time, all queued at once, and as Poisson traces at offered loads of 0.25x, 0.5x,
+    1x and 2x the one-at-a-time capacity, with p50/p99 latency and throughput, through the slot executor and through the
+    continuous-batching server (arrivals join running packs at chunk boundaries)."""
     from reaxamd import serving
 
     def timed(reqs, k, realtime=False, packed=False):
@@ -81,30 +82,37 @@ def batch_and_serving(ffield_text, precision, slots=16):
     work = sum(r.atom_steps for r in serving.serving_requests(32, seed=2))
     seq = work / timed(serving.serving_requests(32, seed=2), 1)[1]
     queued = work / timed(serving.serving_requests(32, seed=2), slots)[1]
-    reqs = serving.serving_requests(32, seed=2)
-    rate = seq / (work / len(reqs))   # requests per second = the one-at-a-time capacity
-    rng = np.random.default_rng(3)
-    t = 0.0
-    for r in reqs:
-        t += rng.exponential(1.0 / rate)
-        r.arrival = t
-    arrivals = [r.arrival for r in reqs]
-    done, span = timed(reqs, slots, realtime=True)
-    lat = serving.latency_stats(done)
     # dynamic batching: a free slot runs a large request alone or packs every queued small one into one context; 8 slots (the
     # large requests run as single-system contexts through the 4 hardware queues: 8 concurrent ones served 15% more than 16)
     pslots = 8
     pq = work / timed(serving.serving_requests(32, seed=2), pslots, packed=True)[1]
-    preqs = serving.serving_requests(32, seed=2)
-    for r, a in zip(preqs, arrivals):
-        r.arrival = a
-    pdone, pspan = timed(preqs, pslots, realtime=True, packed=True)
-    plat = serving.latency_stats(pdone)
-    out["serving"] = {"requests": len(reqs), "atom_steps": work, "poisson_rate_per_s": rate, "sequential_atom_steps_per_s": seq,
-                      "packed": {"slots": pslots, "pack_below_atoms": 3000, "all_queued_atom_steps_per_s": pq, "poisson_atom_steps_per_s": work / pspan,
-                                 "p50_latency_s": plat["p50_s"], "p99_latency_s": plat["p99_s"]},
-                      "slot_executor": {"slots": slots, "all_queued_atom_steps_per_s": queued, "poisson_atom_steps_per_s": work / span,
-                                        "p50_latency_s": lat["p50_s"], "p99_latency_s": lat["p99_s"]}}
+    rate = seq / (work / 32)   # requests per second = the one-at-a-time capacity
+    # Poisson arrivals at 0.25x, 0.5x, 1x and 2x that capacity: one sequence of exponential gaps, scaled per rate, served by the
+    # slot executor (one context per request) and by the continuous-batching server (small requests packed; while every slot
+    # is busy, arrivals join running packs at 25-step chunk boundaries)
+    gaps = np.random.default_rng(3).exponential(1.0, size=32)
+    sweeps = {}
+    for label, kw in (("slot_executor", {"slots": slots}), ("continuous", {"slots": pslots, "continuous": True})):
+        sweeps[label] = []
+        for rho in (0.25, 0.5, 1.0, 2.0):
+            creqs = serving.serving_requests(32, seed=2)
+            for r, a in zip(creqs, np.cumsum(gaps) / (rho * rate)):
+                r.arrival = float(a)
+            srv = serving.Server(ffield_text, precision=precision, chunk=25, **kw)
+            t0 = time.perf_counter()
+            cdone = srv.run(creqs, realtime=True)
+            cspan = time.perf_counter() - t0
+            srv.close()
+            clat = serving.latency_stats(cdone)
+            sweeps[label].append({"offered_load": rho, "rate_per_s": rho * rate, "atom_steps_per_s": work / cspan, "span_s": cspan,
+                                  "p50_latency_s": clat["p50_s"], "p99_latency_s": clat["p99_s"]})
+            print("bench: %s %s" % (label, sweeps[label][-1]), file=sys.stderr, flush=True)
+    out["serving"] = {"requests": len(creqs), "atom_steps": work, "capacity_rate_per_s": rate, "sequential_atom_steps_per_s": seq,
+                      "poisson_sweep": {"slot_executor": {"slots": slots, "points": sweeps["slot_executor"]},
+                                        "continuous": {"slots": pslots, "pack_below_atoms": 3000, "chunk_steps": 25,
+                                                       "points": sweeps["continuous"]}},
+                      "all_queued": {"slot_executor": {"slots": slots, "atom_steps_per_s": queued},
+                                     "packed": {"slots": pslots, "pack_below_atoms": 3000, "atom_steps_per_s": pq}}}
     print("bench: serving %s" % out["serving"], file=sys.stderr, flush=True)
     return out
 

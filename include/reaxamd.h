@@ -172,6 +172,18 @@ int rxa_system_results(rxa_context* ctx, double* energies, double* virial, doubl
 /* per-system thermostat target (K) and damping (fs; <= 0: NVE) after rxa_md_setup (either may be NULL) */
 int rxa_md_system_targets(rxa_context* ctx, const double* t_target, const double* t_damp);
 int rxa_get_state(rxa_context* ctx, double* x, double* v, double* q);
+/* ---- continuing a run in a new pack (no counterpart in the reference; for the continuous-batching server, which re-packs
+ * the running systems with newly admitted ones at chunk boundaries)
+ * rxa_md_state: the state a run continues from beyond x, v and q: the charge-solve history of s and t (s_hist[4n], t_hist[4n],
+ * [k n + atom], k = 0 newest, the caller's order) and the thermostat chain of each system (nh[32 nsys]; 32 for one system).
+ * rxa_md_resume: rxa_md_setup for a context whose systems (rxa_set_systems) continue runs where keep[k] != 0: those take the
+ * given charges q[n], history and chain (values of the other systems are ignored) instead of a first charge solve, so their
+ * next step is the one their previous context would have taken (the neighbour lists are rebuilt, which changes only the order
+ * of sums); the other systems start as rxa_md_setup starts them. Charge history is required when qeq_every > 0. */
+int rxa_md_state(rxa_context* ctx, double* s_hist, double* t_hist, double* nh);
+int rxa_md_resume(rxa_context* ctx, const double* v, const double* mass_by_type, double dt, double t_target, double t_damp, double skin,
+                  int qeq_every, int qeq_imax, double qeq_tol, const int* keep, const double* q, const double* s_hist,
+                  const double* t_hist, const double* nh);
 
 /* capacities and counters: out[0] n, [1] skin-list capacity per atom, [2] bond-slot capacity, [3] H-bond candidate capacity,
  * [4] skin-list entries, [5] directed bond slots in use, [6] acceptors within hbond_cut summed over the donors (last evaluation),

@@ -307,6 +307,24 @@ struct Engine {
     HIPCHK(hipStreamSynchronize(stream));
     timer.resolve();
   }
+  // Host <-> device transfers and fills ordered on the engine's stream. The engine's streams are non-blocking, so they do not
+  // wait for the null stream: a plain hipMemset, or a hipMemcpy from pageable memory, may still be in flight when a kernel on
+  // them runs. h2d returns with the copy done (the host buffer may go out of scope); zero is ordered before later work on the
+  // stream (and on the bonded stream, which forks from it).
+  void h2d(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void d2h(void* dst, const void* src, size_t bytes) {
+    sync();
+    if (!bytes) return;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void zero(void* dst, size_t bytes) {
+    if (bytes) HIPCHK(hipMemsetAsync(dst, 0, bytes, stream));
+  }
   // the stream of the bonded chain
   hipStream_t bstream() const {
     const unsigned bonded_phases = 1u << PH_BONDS | 1u << PH_BOCORR | 1u << PH_TERMS | 1u << PH_DBOND;
@@ -360,7 +378,7 @@ struct Engine {
     for (const auto& kv : ff.hbonds) hb[((size_t)kv.first[0] * nt + kv.first[1]) * nt + kv.first[2]] = DevHBond{kv.second.r0, kv.second.p1, kv.second.p2, kv.second.p3};
     auto up = [&](auto& buf, const auto& vec) {
       buf.alloc(vec.size());
-      if (!vec.empty()) HIPCHK(hipMemcpy(buf.p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice));
+      if (!vec.empty()) h2d(buf.p, vec.data(), vec.size() * sizeof(vec[0]));
     };
     std::vector<NbPairF> nbf((size_t)nt * nt);
     for (size_t k = 0; k < nbf.size(); ++k) {
@@ -451,13 +469,13 @@ struct Engine {
   void alloc_atoms() {
     a_x.alloc(n); a_y.alloc(n); a_z.alloc(n); a_q.alloc(n); a_type.alloc(n); a_tag.alloc(n); a_pos.alloc(n);
     d_perm.alloc(n); d_src.alloc(n); d_psd.alloc(24 * (size_t)n); d_psi.alloc(4 * (size_t)n); d_psl.alloc(2 * (size_t)n);
-    HIPCHK(hipMemset(a_q.p, 0, n * sizeof(double)));
+    zero(a_q.p, n * sizeof(double));
     const int NA = 38 + kTerms + 9;   // number of per-atom double arrays below
     a_atom.alloc((size_t)NA * n);
-    HIPCHK(hipMemset(a_atom.p, 0, (size_t)NA * n * sizeof(double)));
+    zero(a_atom.p, (size_t)NA * n * sizeof(double));
     const size_t ni = (6 + kCenCount) * (size_t)n + 8;   // 6 count arrays, 8 flags, the census
     a_iatom.alloc(ni);
-    HIPCHK(hipMemset(a_iatom.p, 0, ni * sizeof(int)));
+    zero(a_iatom.p, ni * sizeof(int));
     State& s = st;
     s.n = n;
     s.x = a_x.p; s.y = a_y.p; s.z = a_z.p; s.q = a_q.p; s.type = a_type.p; s.tag = a_tag.p; s.pos = a_pos.p;
@@ -656,10 +674,10 @@ struct Engine {
     d_sys_start.alloc(S + 1); d_cells.alloc(S); d_ext.alloc(S); d_reb.alloc(S); d_rebany.alloc(1);
     d_pk.alloc(10 * (size_t)S); d_pki.alloc(4 * (size_t)S); d_rowv.alloc(4 * (size_t)n); d_nh_pk.alloc(32 * (size_t)S);
     d_disp_pk.alloc(S); d_sys_out.alloc((size_t)S * (kTerms + 6));
-    HIPCHK(hipMemset(d_pki.p, 0, 4 * (size_t)S * sizeof(int)));
-    HIPCHK(hipMemcpy(d_sys_start.p, start.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_cells.p, cells.data(), S * sizeof(Cell), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(st.sysid, sid.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    zero(d_pki.p, 4 * (size_t)S * sizeof(int));
+    h2d(d_sys_start.p, start.data(), (S + 1) * sizeof(int));
+    h2d(d_cells.p, cells.data(), S * sizeof(Cell));
+    h2d(st.sysid, sid.data(), n * sizeof(int));
     if ((size_t)S > pk_cap_h) {
       if (pki_h) HIPCHK(hipHostFree(pki_h));
       if (pk_h) HIPCHK(hipHostFree(pk_h));
@@ -674,9 +692,9 @@ struct Engine {
     shv_valid = false;
     perm_h.resize(n);
     for (int i = 0; i < n; ++i) perm_h[i] = i;
-    HIPCHK(hipMemcpy(d_perm.p, perm_h.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(a_type.p, ft.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(a_tag.p, tg.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+    h2d(d_perm.p, perm_h.data(), n * sizeof(int));
+    h2d(a_type.p, ft.data(), n * sizeof(int));
+    h2d(a_tag.p, tg.data(), n * sizeof(long long));
     host_types_mass.assign(n, 0.0);
     set_positions(x);
     sync();
@@ -808,7 +826,7 @@ struct Engine {
     // the same atoms in a new order (e.g. LAMMPS re-sorted or re-neighboured): carry charges, charge history and velocities by tag
     if (!realloc && !perm_h.empty()) {
       std::vector<long long> old_tag(n);
-      HIPCHK(hipMemcpy(old_tag.data(), a_tag.p, n * sizeof(long long), hipMemcpyDeviceToHost));
+      d2h(old_tag.data(), a_tag.p, n * sizeof(long long));
       std::unordered_map<long long, int> where;
       where.reserve(2 * (size_t)n);
       for (int i = 0; i < n; ++i) where[old_tag[i]] = i;
@@ -819,7 +837,7 @@ struct Engine {
         if (it == where.end()) same = false; else src[i] = it->second;
       }
       if (same) {
-        HIPCHK(hipMemcpy(d_src.p, src.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        h2d(d_src.p, src.data(), n * sizeof(int));
         permute(d_src.p);
       } else {
         reset_history();
@@ -1104,12 +1122,12 @@ struct Engine {
     const bool had_hist = q_dbl.p && hist_count > 0 && q_dbl_n == n;   // history of the same atoms only
     if (had_hist) {
       keep.resize(8 * (size_t)n);
-      HIPCHK(hipMemcpy(keep.data(), qb.s_hist, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(keep.data() + 4 * (size_t)n, qb.t_hist, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+      d2h(keep.data(), qb.s_hist, 4 * (size_t)n * sizeof(double));
+      d2h(keep.data() + 4 * (size_t)n, qb.t_hist, 4 * (size_t)n * sizeof(double));
     }
     q_dbl.alloc(sz);
     q_dbl_n = n;
-    HIPCHK(hipMemset(q_dbl.p, 0, sz * sizeof(double)));
+    zero(q_dbl.p, sz * sizeof(double));
     double* p = q_dbl.p;
     auto take = [&](size_t k) { double* r = p; p += rnd(k); return r; };
     qb.val = nval ? take(nval) : nullptr;
@@ -1132,8 +1150,8 @@ struct Engine {
     qb.partial_gate = take(3 * kCgBlocks);
     static_assert(2 * sizeof(int) <= sizeof(double), "");
     if (had_hist) {
-      HIPCHK(hipMemcpy(qb.s_hist, keep.data(), 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(qb.t_hist, keep.data() + 4 * (size_t)n, 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+      h2d(qb.s_hist, keep.data(), 4 * (size_t)n * sizeof(double));
+      h2d(qb.t_hist, keep.data() + 4 * (size_t)n, 4 * (size_t)n * sizeof(double));
     } else {
       hist_count = 0;
     }
@@ -1168,9 +1186,9 @@ struct Engine {
     }
     std::vector<double> shld((size_t)nt * nt);
     for (int a = 0; a < nt; ++a) for (int b = 0; b < nt; ++b) shld[(size_t)a * nt + b] = std::pow(q_gam_t[a] * q_gam_t[b], -1.5);
-    HIPCHK(hipMemcpy(d_qchi.p, q_chi_t.data(), nt * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_qeta.p, q_eta_t.data(), nt * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_qshld.p, shld.data(), shld.size() * sizeof(double), hipMemcpyHostToDevice));
+    h2d(d_qchi.p, q_chi_t.data(), nt * sizeof(double));
+    h2d(d_qeta.p, q_eta_t.data(), nt * sizeof(double));
+    h2d(d_qshld.p, shld.data(), shld.size() * sizeof(double));
     const bool first = !qeq_ready;
     q_swa = swa;
     q_swb = swb;
@@ -1208,7 +1226,7 @@ struct Engine {
       sync();
       const int cap = std::max(imax + 2, 2048);
       q_scal.alloc(qeq_scal_doubles(cap));
-      HIPCHK(hipMemset(q_scal.p, 0, q_scal.n * sizeof(double)));
+      zero(q_scal.p, q_scal.n * sizeof(double));
       d_cg_any.alloc((size_t)cap + 2);
       if (cgf_h) HIPCHK(hipHostFree(cgf_h));
       if (cgs_h) HIPCHK(hipHostFree(cgs_h));
@@ -1396,12 +1414,23 @@ struct Engine {
   void charges_out(double* q) {
     std::vector<double> qi(n);
     sync();
-    HIPCHK(hipMemcpy(qi.data(), st.q, n * sizeof(double), hipMemcpyDeviceToHost));
+    d2h(qi.data(), st.q, n * sizeof(double));
     for (int i = 0; i < n; ++i) q[perm_h[i]] = qi[i];
   }
 
   // ---------------------------------------------------------------------------------------------------------------
   void md_setup(const double* v, const double* mass_by_type, double dt, double tt, double tdamp, double sk, int qeq_every, int qimax, double qtol) {
+    md_setup_buffers(v, mass_by_type, dt, tt, tdamp, sk, qeq_every, qimax, qtol);
+    // initial forces (and charges)
+    build_neighbors();
+    launch_md_ref_positions(st, md, stream);
+    if (md_qeq_every > 0) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
+    forces(false);
+    sync();
+  }
+  // velocities, masses, thermostat chains and the run's parameters (the common part of rxa_md_setup and rxa_md_resume)
+  void md_setup_buffers(const double* v, const double* mass_by_type, double dt, double tt, double tdamp, double sk, int qeq_every,
+                        int qimax, double qtol) {
     if (!n) throw ApiError(RXA_ESTATE, "no atoms");
     sync();
     md_buf.alloc(8 * (size_t)n + 64 + 256);
@@ -1414,7 +1443,7 @@ struct Engine {
     md.disp = p + 250;
     std::vector<double> vx(n), vy(n), vz(n), m(n);
     std::vector<int> ty(n);
-    HIPCHK(hipMemcpy(ty.data(), st.type, n * sizeof(int), hipMemcpyDeviceToHost));
+    d2h(ty.data(), st.type, n * sizeof(int));
     // masses by LAMMPS type: map through the host copy of the LAMMPS types kept at set_atoms
     for (int i = 0; i < n; ++i) {
       const int u = perm_h[i];
@@ -1422,16 +1451,16 @@ struct Engine {
       m[i] = mass_by_type[lmp_types[u]];
       if (!(m[i] > 0)) throw ApiError(RXA_EINVAL, "non-positive mass");
     }
-    HIPCHK(hipMemcpy(md.vx, vx.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(md.vy, vy.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(md.vz, vz.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(md.mass, m.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    h2d(md.vx, vx.data(), n * sizeof(double));
+    h2d(md.vy, vy.data(), n * sizeof(double));
+    h2d(md.vz, vz.data(), n * sizeof(double));
+    h2d(md.mass, m.data(), n * sizeof(double));
     std::vector<double> nh(64, 0.0);
     nh[13] = tt; nh[14] = 1.0 / tdamp; nh[15] = dt; nh[16] = 3.0 * n - 3.0; nh[17] = 0.0019872067; nh[18] = 48.88821291 * 48.88821291;
     nh[19] = 1.0;
     md_nve = !(tdamp > 0);   // t_damp <= 0: no thermostat (NVE)
     nh[22] = md_nve ? 1.0 : 0.0;
-    HIPCHK(hipMemcpy(md.nh, nh.data(), nh.size() * sizeof(double), hipMemcpyHostToDevice));
+    h2d(md.nh, nh.data(), nh.size() * sizeof(double));
     if (packed) {   // one chain per system, the same layout
       std::vector<double> nhp(32 * (size_t)S, 0.0);
       for (int k = 0; k < S; ++k) {
@@ -1439,7 +1468,7 @@ struct Engine {
         for (int q = 13; q < 23; ++q) h[q] = nh[q];
         h[16] = 3.0 * (sys_start_h[k + 1] - sys_start_h[k]) - 3.0;
       }
-      HIPCHK(hipMemcpy(d_nh_pk.p, nhp.data(), nhp.size() * sizeof(double), hipMemcpyHostToDevice));
+      h2d(d_nh_pk.p, nhp.data(), nhp.size() * sizeof(double));
     }
     md_dt = dt;
     skin = sk;
@@ -1447,14 +1476,72 @@ struct Engine {
     md_ready = true;
     md_steps = 0;
     v_scale_pending = false;
-    // initial forces (and charges)
+  }
+  std::vector<int> lmp_types;
+
+  // The MD state a run continues from, beyond positions, velocities and charges: the charge-solve history of s and t (4 deep,
+  // newest first, [k n + atom] in the caller's order) and each system's thermostat chain (32 doubles per system).
+  void md_state_out(double* s_hist, double* t_hist, double* nh) {
+    if (!md_ready) throw ApiError(RXA_ESTATE, "rxa_md_setup first");
+    sync();
+    std::vector<double> h(4 * (size_t)n);
+    for (int w = 0; w < 2; ++w) {
+      double* out = w ? t_hist : s_hist;
+      if (!out) continue;
+      if (!qb.s_hist || q_dbl_n != n) throw ApiError(RXA_ESTATE, "no charge solve set up");
+      d2h(h.data(), w ? qb.t_hist : qb.s_hist, h.size() * sizeof(double));
+      for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < n; ++i) out[(size_t)k * n + perm_h[i]] = h[(size_t)k * n + i];
+    }
+    if (nh) d2h(nh, packed ? d_nh_pk.p : md.nh, 32 * (size_t)(packed ? S : 1) * sizeof(double));
+  }
+
+  // rxa_md_setup for a run that continues some systems (keep[k] != 0) from a state read with rxa_get_state and
+  // rxa_md_state: the other systems start as rxa_md_setup starts them (their first charge solve from no history), the kept ones
+  // take the given charges, charge history and thermostat chain instead, so their next step is the one they would have taken
+  // (the lists are rebuilt, which changes only the order of sums). v: velocities of all atoms, the kept systems' as read.
+  void md_resume(const double* v, const double* mass_by_type, double dt, double tt, double tdamp, double sk, int qeq_every, int qimax,
+                 double qtol, const int* keep, const double* q, const double* s_hist, const double* t_hist, const double* nh) {
+    md_setup_buffers(v, mass_by_type, dt, tt, tdamp, sk, qeq_every, qimax, qtol);
     build_neighbors();
     launch_md_ref_positions(st, md, stream);
     if (md_qeq_every > 0) qeq_charges(md_qeq_imax, md_qeq_tol, 0);
+    sync();
+    if (packed) pk_collect();
+    const int ns = packed ? S : 1;
+    std::vector<char> kept_atom(n, 0);   // engine order
+    for (int k = 0; k < ns; ++k) {
+      if (!keep[k]) continue;
+      const int a = packed ? sys_start_h[k] : 0, b = packed ? sys_start_h[k + 1] : n;
+      for (int i = a; i < b; ++i) kept_atom[i] = 1;
+    }
+    auto merge = [&](double* dev, const double* src, int depth) {   // kept atoms' values from src (caller order)
+      std::vector<double> h((size_t)depth * n);
+      d2h(h.data(), dev, h.size() * sizeof(double));
+      for (int k = 0; k < depth; ++k)
+        for (int i = 0; i < n; ++i)
+          if (kept_atom[i]) h[(size_t)k * n + i] = src[(size_t)k * n + perm_h[i]];
+      h2d(dev, h.data(), h.size() * sizeof(double));
+    };
+    merge(st.q, q, 1);
+    if (md_qeq_every > 0) {
+      if (!qb.s_hist || q_dbl_n != n) throw ApiError(RXA_ESTATE, "no charge solve set up");
+      merge(qb.s_hist, s_hist, 4);
+      merge(qb.t_hist, t_hist, 4);
+      hist_count = 4;   // systems without history (zero arrays) extrapolate to zero, the start of a solve without history
+    }
+    {
+      double* dnh = packed ? d_nh_pk.p : md.nh;
+      std::vector<double> h(32 * (size_t)ns);
+      d2h(h.data(), dnh, h.size() * sizeof(double));
+      for (int k = 0; k < ns; ++k)
+        if (keep[k]) std::memcpy(h.data() + 32 * (size_t)k, nh + 32 * (size_t)k, 32 * sizeof(double));
+      h2d(dnh, h.data(), h.size() * sizeof(double));
+    }
+    qeq_h_valid = false;
     forces(false);
     sync();
   }
-  std::vector<int> lmp_types;
 
   void md_run(int steps, double* thermo) {
     if (!md_ready) throw ApiError(RXA_ESTATE, "rxa_md_setup first");
@@ -1534,12 +1621,12 @@ struct Engine {
     if (packed) pk_collect();
     if (thermo) {
       double e[kTerms];
-      HIPCHK(hipMemcpy(e, red.p, sizeof(e), hipMemcpyDeviceToHost));
+      d2h(e, red.p, sizeof(e));
       double nh[22];
-      HIPCHK(hipMemcpy(nh, md.nh, sizeof(nh), hipMemcpyDeviceToHost));
+      d2h(nh, md.nh, sizeof(nh));
       if (packed) {   // totals over the systems: kinetic energy, temperature from the summed degrees of freedom
         std::vector<double> nhp(32 * (size_t)S);
-        HIPCHK(hipMemcpy(nhp.data(), d_nh_pk.p, nhp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        d2h(nhp.data(), d_nh_pk.p, nhp.size() * sizeof(double));
         double ke = 0, dof = 0;
         for (int k = 0; k < S; ++k) { ke += nhp[32 * (size_t)k + 21]; dof += nhp[32 * (size_t)k + 16]; }
         nh[21] = ke;
@@ -1772,6 +1859,21 @@ int rxa_set_systems(rxa_context* c, int nsys, const int64_t* counts, const doubl
   });
 }
 
+int rxa_md_state(rxa_context* c, double* s_hist, double* t_hist, double* nh) {
+  return guard(c, [&](Engine& e) { e.md_state_out(s_hist, t_hist, nh); });
+}
+
+int rxa_md_resume(rxa_context* c, const double* v, const double* mass_by_type, double dt, double t_target, double t_damp, double skin,
+                  int qeq_every, int qeq_imax, double qeq_tol, const int* keep, const double* q, const double* s_hist,
+                  const double* t_hist, const double* nh) {
+  return guard(c, [&](Engine& e) {
+    if (!v || !mass_by_type || !keep || !q || !nh) throw ApiError(RXA_EINVAL, "rxa_md_resume: null argument");
+    if (qeq_every > 0 && (!s_hist || !t_hist)) throw ApiError(RXA_EINVAL, "rxa_md_resume: the charge history is required");
+    if (!(dt > 0) || !(skin >= 0)) throw ApiError(RXA_EINVAL, "invalid MD parameters");
+    e.md_resume(v, mass_by_type, dt, t_target, t_damp, skin, qeq_every, qeq_imax, qeq_tol, keep, q, s_hist, t_hist, nh);
+  });
+}
+
 int rxa_system_results(rxa_context* c, double* energies, double* virial, double* thermo) {
   return guard(c, [&](Engine& e) { e.system_results(energies, virial, thermo); });
 }
@@ -1781,13 +1883,13 @@ int rxa_md_system_targets(rxa_context* c, const double* t_target, const double* 
     if (!e.packed) throw ApiError(RXA_ESTATE, "no packed systems (rxa_set_systems)");
     if (!e.md_ready) throw ApiError(RXA_ESTATE, "rxa_md_setup first");
     std::vector<double> nh(32 * (size_t)e.S);
-    HIPCHK(hipMemcpy(nh.data(), e.d_nh_pk.p, nh.size() * sizeof(double), hipMemcpyDeviceToHost));
+    e.d2h(nh.data(), e.d_nh_pk.p, nh.size() * sizeof(double));
     for (int k = 0; k < e.S; ++k) {
       double* h = nh.data() + 32 * (size_t)k;
       if (t_target) h[13] = t_target[k];
       if (t_damp) { h[14] = t_damp[k] > 0 ? 1.0 / t_damp[k] : 0.0; h[22] = t_damp[k] > 0 ? 0.0 : 1.0; }
     }
-    HIPCHK(hipMemcpy(e.d_nh_pk.p, nh.data(), nh.size() * sizeof(double), hipMemcpyHostToDevice));
+    e.h2d(e.d_nh_pk.p, nh.data(), nh.size() * sizeof(double));
   });
 }
 
@@ -1829,7 +1931,7 @@ int rxa_set_charges(rxa_context* c, const double* q) {
     std::vector<double> qi(e.n);
     for (int i = 0; i < e.n; ++i) qi[i] = q[e.perm_h[i]];
     e.sync();
-    HIPCHK(hipMemcpy(e.st.q, qi.data(), e.n * sizeof(double), hipMemcpyHostToDevice));
+    e.h2d(e.st.q, qi.data(), e.n * sizeof(double));
   });
 }
 
@@ -1861,9 +1963,9 @@ int rxa_census(rxa_context* c, int* out) {
     std::vector<int> b(n), cen((size_t)rxa::kCenCount * n);
     std::vector<double> de(n);
     e.sync();
-    HIPCHK(hipMemcpy(b.data(), e.st.b_cnt, n * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(de.data(), e.st.Delta_e, n * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(cen.data(), e.st.cen, cen.size() * sizeof(int), hipMemcpyDeviceToHost));
+    e.d2h(b.data(), e.st.b_cnt, n * sizeof(int));
+    e.d2h(de.data(), e.st.Delta_e, n * sizeof(double));
+    e.d2h(cen.data(), e.st.cen, cen.size() * sizeof(int));
     // reference order (census.hpp): bonds, angle_sets, torsions, hbonds, nonbonded, sbo_region, lp_trunc
     const int from[7] = {-1, rxa::kCenAngles, rxa::kCenTorsions, rxa::kCenHbonds, rxa::kCenNonbonded, rxa::kCenSbo, -1};
     for (int i = 0; i < n; ++i) {
@@ -1883,14 +1985,14 @@ long rxa_bond_table(rxa_context* c, int32_t* start, int32_t* nbr, double* bo, do
     e.sync();
     std::vector<int> cnt(n), bj((size_t)n * mb);
     std::vector<double> bv((size_t)n * mb), tb(n), np(n);
-    HIPCHK(hipMemcpy(cnt.data(), e.st.b_cnt, n * sizeof(int), hipMemcpyDeviceToHost));
+    e.d2h(cnt.data(), e.st.b_cnt, n * sizeof(int));
     total = 0;
     for (int i = 0; i < n; ++i) total += std::min(cnt[i], mb);
     if (!nbr) return;
-    HIPCHK(hipMemcpy(bj.data(), e.st.b_j, bj.size() * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(bv.data(), e.st.bo, bv.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(tb.data(), e.st.tbo, n * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(np.data(), e.st.nlp, n * sizeof(double), hipMemcpyDeviceToHost));
+    e.d2h(bj.data(), e.st.b_j, bj.size() * sizeof(int));
+    e.d2h(bv.data(), e.st.bo, bv.size() * sizeof(double));
+    e.d2h(tb.data(), e.st.tbo, n * sizeof(double));
+    e.d2h(np.data(), e.st.nlp, n * sizeof(double));
     std::vector<int> inv(n);
     for (int i = 0; i < n; ++i) inv[e.perm_h[i]] = i;
     long k = 0;
@@ -1964,7 +2066,7 @@ int rxa_qeq_solve(rxa_context* c, const double* b, const double* x0, int imax, d
     e.qeq_iterate(imax, tol);
     e.sync();
     std::vector<double> xo(2 * (size_t)n);
-    HIPCHK(hipMemcpy(xo.data(), e.qb.x, xo.size() * sizeof(double), hipMemcpyDeviceToHost));
+    e.d2h(xo.data(), e.qb.x, xo.size() * sizeof(double));
     for (int i = 0; i < n; ++i) { x[e.perm_h[i]] = xo[2 * i]; x[n + e.perm_h[i]] = xo[2 * i + 1]; }
     for (int s = 0; s < 2; ++s) {
       iters[s] = e.last_iters[s];
@@ -2013,16 +2115,16 @@ int rxa_get_state(rxa_context* c, double* x, double* v, double* q) {
     std::vector<double> a(n), b(n), d(n);
     const std::vector<int>& P = e.perm_h;
     if (x) {
-      HIPCHK(hipMemcpy(a.data(), e.st.x, n * sizeof(double), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(b.data(), e.st.y, n * sizeof(double), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(d.data(), e.st.z, n * sizeof(double), hipMemcpyDeviceToHost));
+      e.d2h(a.data(), e.st.x, n * sizeof(double));
+      e.d2h(b.data(), e.st.y, n * sizeof(double));
+      e.d2h(d.data(), e.st.z, n * sizeof(double));
       for (int i = 0; i < n; ++i) { x[3 * P[i]] = a[i]; x[3 * P[i] + 1] = b[i]; x[3 * P[i] + 2] = d[i]; }
     }
     if (v) {
       if (!e.md_ready) throw ApiError(RXA_ESTATE, "no velocities (rxa_md_setup)");
-      HIPCHK(hipMemcpy(a.data(), e.md.vx, n * sizeof(double), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(b.data(), e.md.vy, n * sizeof(double), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(d.data(), e.md.vz, n * sizeof(double), hipMemcpyDeviceToHost));
+      e.d2h(a.data(), e.md.vx, n * sizeof(double));
+      e.d2h(b.data(), e.md.vy, n * sizeof(double));
+      e.d2h(d.data(), e.md.vz, n * sizeof(double));
       for (int i = 0; i < n; ++i) { v[3 * P[i]] = a[i]; v[3 * P[i] + 1] = b[i]; v[3 * P[i] + 2] = d[i]; }
     }
     if (q) e.charges_out(q);
@@ -2122,14 +2224,14 @@ int rxa_stats(rxa_context* c, int64_t* out) {
     e.sync();
     const int n = e.n;
     std::vector<int> cnt(5 * (size_t)n);
-    HIPCHK(hipMemcpy(cnt.data(), e.st.nb_cnt, 5 * (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    e.d2h(cnt.data(), e.st.nb_cnt, 5 * (size_t)n * sizeof(int));
     for (int i = 0; i < n; ++i) {
       out[4] += cnt[i]; out[5] += cnt[n + i]; out[6] += cnt[2 * (size_t)n + i];
       out[10] += cnt[3 * (size_t)n + i]; out[11] += cnt[4 * (size_t)n + i];
     }
     if (e.qeq_ready && e.qb.cnt && e.qeq_h_valid) {
       std::vector<int> qc(n);
-      HIPCHK(hipMemcpy(qc.data(), e.qb.cnt, n * sizeof(int), hipMemcpyDeviceToHost));
+      e.d2h(qc.data(), e.qb.cnt, n * sizeof(int));
       for (int i = 0; i < n; ++i) out[7] += qc[i];
     }
   });

@@ -71,6 +71,8 @@ def lib():
             "rxa_md_setup": (I, [P, P, P, D, D, D, D, I, I, D]),
             "rxa_md_run": (I, [P, I, P]),
             "rxa_get_state": (I, [P, P, P, P]),
+            "rxa_md_state": (I, [P, P, P, P]),
+            "rxa_md_resume": (I, [P, P, P, D, D, D, D, I, I, D, P, P, P, P, P]),
             "rxa_stats": (I, [P, P]),
             "rxa_qeq_stats": (I, [P, P]),
             "rxa_set_positions_device": (I, [P, P]),
@@ -216,6 +218,7 @@ class Engine:
         xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
         self._chk(lib().rxa_set_atoms(self.h, t.size, _p(t), _p(g), _p(xx)))
         self.n = t.size
+        self.nsys = 1
 
     def set_atoms_ghosted(self, nlocal, types, tags, x, nranks=1):
         """LAMMPS' owned + ghost arrays (length nlocal + nghost); the ghosts are checked and the owned atoms loaded."""
@@ -224,6 +227,7 @@ class Engine:
         xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
         self._chk(lib().rxa_set_atoms_ghosted(self.h, int(nlocal), t.size - int(nlocal), _p(t), _p(g), _p(xx), int(nranks)))
         self.n = int(nlocal)
+        self.nsys = 1
 
     def set_systems(self, systems):
         """Packs independent systems (objects with types, tags, x, lo, hi and optionally periodic) into this context."""
@@ -356,6 +360,31 @@ class Engine:
         th = np.zeros(8)
         self._chk(lib().rxa_md_run(self.h, int(steps), _p(th)))
         return dict(zip(("pe", "ke", "temp", "rebuilds", "qeq_iters_mean", "iters_s", "iters_t", "max_disp"), th.tolist()))
+
+    def md_state(self):
+        """Charge history (s and t, each (4, n), newest first) and thermostat chains ((systems, 32)): with state(), what a run
+        continues from (rxa_md_state)."""
+        ns = getattr(self, "nsys", 1)
+        sh, th, nh = np.zeros(4 * self.n), np.zeros(4 * self.n), np.zeros(32 * ns)
+        self._chk(lib().rxa_md_state(self.h, _p(sh), _p(th), _p(nh)))
+        return {"s_hist": sh.reshape(4, -1), "t_hist": th.reshape(4, -1), "nh": nh.reshape(ns, 32)}
+
+    def md_resume(self, v, mass_by_type, keep, q, mdstate, dt=0.25, t_target=300.0, t_damp=25.0, skin=2.0, qeq_every=1, qeq_imax=200,
+                  qeq_tol=1e-6):
+        """rxa_md_setup where the systems with keep[k] continue from charges q, and the history and chains of mdstate (the
+        arrays of md_state(), laid out for this context's systems); the others start fresh (rxa_md_resume)."""
+        ns = getattr(self, "nsys", 1)
+        vv = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        m = np.ascontiguousarray(mass_by_type, dtype=np.float64)
+        kp = np.ascontiguousarray(keep, dtype=np.int32).reshape(-1)
+        qq = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        sh = np.ascontiguousarray(mdstate["s_hist"], dtype=np.float64).reshape(-1)
+        th = np.ascontiguousarray(mdstate["t_hist"], dtype=np.float64).reshape(-1)
+        nh = np.ascontiguousarray(mdstate["nh"], dtype=np.float64).reshape(-1)
+        if len(kp) != ns or len(qq) != self.n or len(sh) != 4 * self.n or len(th) != 4 * self.n or len(nh) != 32 * ns:
+            raise ValueError("md_resume: arrays do not match the context's systems")
+        self._chk(lib().rxa_md_resume(self.h, _p(vv), _p(m), float(dt), float(t_target), float(t_damp), float(skin), int(qeq_every),
+                                       int(qeq_imax), float(qeq_tol), _p(kp), _p(qq), _p(sh), _p(th), _p(nh)))
 
     def state(self):
         x = np.zeros(3 * self.n)

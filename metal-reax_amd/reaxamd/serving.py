@@ -136,6 +136,124 @@ def run_pack_chunks(engine, ffield_text, requests, t0=None):
     return requests
 
 
+class _Moved:
+    """A system at its current positions (the continuing member of a re-pack)."""
+
+    def __init__(self, system, x):
+        self.x = np.asarray(x, dtype=np.float64)
+        self.types, self.tags, self.lo, self.hi = system.types, system.tags, system.lo, system.hi
+        self.periodic = getattr(system, "periodic", (True, True, True))
+        self.name = system.name
+
+    @property
+    def n(self):
+        return len(self.types)
+
+
+def _result(per, g, x, q, k, a, b, thermo):
+    return {"energy": dict(zip(TERMS, per["energy"][k].tolist())), "grad": g[a:b].copy(), "charges": q[a:b].copy(),
+            "positions": x[a:b].copy(),
+            "thermo": {"ke": float(per["ke"][k]), "temp": float(per["temp"][k]), "iters_s": int(per["iters_s"][k]),
+                       "iters_t": int(per["iters_t"][k]), "pack": thermo}}
+
+
+def run_continuous(engine, ffield_text, admit, chunk=50, t0=None, repack_waste=0.25):
+    """Continuous batching in one context: the pack advances in chunks of at most `chunk` steps (a chunk also ends where a
+    member reaches its step count, whose result is taken there); at a chunk boundary `admit(capacity_atoms, block)` hands over
+    newly admitted requests (a list; None once no more will come; block=True when the pack is empty), and the pack is rebuilt
+    with them and its unfinished members, which continue from their state (positions, velocities, charges, charge history and
+    thermostat chain: rxa_md_state / rxa_md_resume). A finished member rides along until the next re-pack (its later steps are
+    not used) unless finished members hold more than `repack_waste` of the pack's atoms. All members share dt and the
+    charge-solve tolerance. A continuing member's result equals that of its run without re-packs up to the order of sums (a
+    re-pack rebuilds its lists); a pack that does not change keeps the bitwise isolation of packed runs."""
+    now = (lambda: time.perf_counter() - t0) if t0 is not None else (lambda: 0.0)
+    e = engine
+    masses = np.array([0.0] + [systems.MASS[el] for el in systems.ELEMENTS])
+    members = []    # [request, steps done, finished] in pack order
+    live = False    # the context holds `members`
+    closed = False
+    cap_atoms = getattr(admit, "max_atoms", 4_000_000)
+    while True:
+        active = [m for m in members if not m[2]]
+        fresh = []
+        if not closed:
+            got = admit(cap_atoms - sum(m[0].system.n for m in active), not active)
+            if got is None:
+                closed = True
+            else:
+                fresh = got
+        if not active and not fresh:
+            if closed:
+                return
+            continue
+        waste = sum(m[0].system.n for m in members if m[2])
+        total = sum(m[0].system.n for m in members) or 1
+        if fresh or not live or waste > repack_waste * total:
+            carried = []
+            if live and active:   # the continuing members' state from the live context
+                x, v, q = e.state()
+                ms = e.md_state()
+                offs = np.concatenate([[0], np.cumsum([m[0].system.n for m in members])])
+                for k, m in enumerate(members):
+                    if m[2]:
+                        continue
+                    a, b = offs[k], offs[k + 1]
+                    carried.append((m, _Moved(m[0].system, x[a:b]), v[a:b], q[a:b], ms["s_hist"][:, a:b], ms["t_hist"][:, a:b],
+                                    ms["nh"][k]))
+            else:
+                carried = [(m, m[0].system, None, None, None, None, None) for m in active]   # not started yet
+            for r in fresh:
+                r.start = now()
+            new_members = [c[0] for c in carried] + [[r, 0, False] for r in fresh]
+            sysl = [c[1] for c in carried] + [r.system for r in fresh]
+            e.set_systems(sysl)
+            e.qeq_setup(0.0, 10.0, e.extract("chi"), e.extract("eta"), e.extract("gamma"))
+            reqs = [m[0] for m in new_members]
+            v = np.concatenate([c[2] if c[2] is not None else systems.maxwell_boltzmann(c[0][0].system.masses(), c[0][0].t_target,
+                                                                                           seed=c[0][0].seed) for c in carried] +
+                               [systems.maxwell_boltzmann(r.system.masses(), r.t_target, seed=r.seed) for r in fresh])
+            n_all = sum(sy.n for sy in sysl)
+            q = np.zeros(n_all)
+            sh, th = np.zeros((4, n_all)), np.zeros((4, n_all))
+            nh = np.zeros((len(sysl), 32))
+            keep = np.zeros(len(sysl), dtype=np.int32)
+            a = 0
+            for k, c in enumerate(carried):
+                b = a + c[1].n
+                if c[3] is not None:
+                    keep[k] = 1
+                    q[a:b] = c[3]
+                    sh[:, a:b] = c[4]
+                    th[:, a:b] = c[5]
+                    nh[k] = c[6]
+                a = b
+            r0 = reqs[0]
+            e.md_resume(v, masses, keep, q, {"s_hist": sh, "t_hist": th, "nh": nh}, dt=r0.dt, t_target=r0.t_target, t_damp=r0.t_damp,
+                        skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=r0.qeq_tol)
+            if any(r.t_target != r0.t_target or r.t_damp != r0.t_damp for r in reqs):
+                e.md_system_targets([r.t_target for r in reqs], [r.t_damp for r in reqs])
+            members = new_members
+            live = True
+        active = [m for m in members if not m[2]]
+        k = min([chunk] + [m[0].steps - m[1] for m in active])
+        thermo = e.md_run(k) if k > 0 else None
+        for m in members:
+            if not m[2]:
+                m[1] += k
+        done = [i for i, m in enumerate(members) if not m[2] and m[1] >= m[0].steps]
+        if done:
+            g = e.compute()["grad"]
+            x, vel, q = e.state()
+            per = e.system_results()
+            offs = np.concatenate([[0], np.cumsum([m[0].system.n for m in members])])
+            t = now()
+            for i in done:
+                r = members[i][0]
+                r.result = _result(per, g, x, q, i, offs[i], offs[i + 1], thermo)
+                r.end = t
+                members[i][2] = True
+
+
 def new_engine(ffield_text, precision):
     e = Engine(0)
     e.pair_style(["NULL", "precision", precision])
@@ -158,14 +276,19 @@ class Server:
     """Concurrent executor of MD requests (see the module docstring). packed=False: each slot runs one request at a time in its
     own context. packed=True: a free slot takes the oldest queued request alone if it has at least pack_below atoms (the
     single-system path, with cell lists), else every queued request below that size that fits the memory budget (at most
-    max_pack_atoms atoms), packed in one context (run_pack_chunks)."""
+    max_pack_atoms atoms), packed in one context (run_pack_chunks). continuous=True: as packed, but while every slot is busy a
+    running pack of small requests also admits the small requests that arrive, at its chunk boundaries (every `chunk` steps;
+    run_continuous), instead of leaving them queued until a slot frees (an idle slot starts a pack of its own instead, which
+    costs no re-pack)."""
 
     def __init__(self, ffield_text, slots=4, precision="mixed", mem_budget_bytes=200 << 30, packed=False, max_pack_atoms=4_000_000,
-                 pack_below=3000):
+                 pack_below=3000, continuous=False, chunk=50):
         self.ffield_text = ffield_text
         self.precision = precision
         self.slots = slots
-        self.packed = packed
+        self.packed = packed or continuous
+        self.continuous = continuous   # packed mode with admission into running packs at chunk boundaries (run_continuous)
+        self.chunk = chunk
         self.max_pack_atoms = max_pack_atoms
         self.pack_below = pack_below   # packed mode: requests of at least this many atoms run alone on the single-system path
         self.budget = mem_budget_bytes
@@ -238,7 +361,7 @@ class Server:
         reqs = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
         waiting = []
-        state = {"submitted": 0}
+        state = {"submitted": 0, "idle": 0}
         cv = threading.Condition()
         errors = []
 
@@ -260,21 +383,49 @@ class Server:
                 atoms += r.system.n
             return pack
 
+        def admit_small(capacity, block):   # continuous packs: the queued small requests that fit (arrival order)
+            with cv:
+                while block and not waiting and state["submitted"] < len(reqs):
+                    cv.wait()
+                if not block and state["idle"] > 0:
+                    return []   # an idle slot starts its own pack: a running one re-packs only when every slot is busy
+                out, atoms, k = [], 0, 0
+                while k < len(waiting):
+                    r = waiting[k]
+                    if r.system.n >= self.pack_below or (out and atoms + r.system.n > capacity):
+                        k += 1
+                        continue
+                    if r.system.n * self.bytes_per_atom > self.budget:
+                        raise ValueError("request %d needs more device memory than the budget" % r.rid)
+                    out.append(waiting.pop(k))
+                    atoms += r.system.n
+                if block and not out:
+                    return None   # nothing small queued: the worker returns to the queue
+                return out
+        admit_small.max_atoms = self.max_pack_atoms
+
         def worker(slot):
             while True:
                 with cv:
+                    state["idle"] += 1
                     while not waiting and state["submitted"] < len(reqs):
                         cv.wait()
+                    state["idle"] -= 1
                     if not waiting:
                         return
                     try:
-                        pack = take()
+                        if self.continuous and waiting[0].system.n < self.pack_below:
+                            pack = None   # a continuous pack admits the queued small requests itself
+                        else:
+                            pack = take()
                     except Exception as ex:
                         errors.append((waiting[0].rid, ex))
                         waiting.clear()
                         return
                 try:
-                    if len(pack) == 1 and pack[0].system.n >= self.pack_below:
+                    if pack is None:
+                        run_continuous(self._engines[slot], self.ffield_text, admit_small, chunk=self.chunk, t0=t0)
+                    elif len(pack) == 1 and pack[0].system.n >= self.pack_below:
                         r = pack[0]
                         r.start = time.perf_counter() - t0
                         r.result = run_one(self._engines[slot], self.ffield_text, r, self.precision)
@@ -283,7 +434,7 @@ class Server:
                         run_pack_chunks(self._engines[slot], self.ffield_text, pack, t0)
                     self.bytes_per_atom = max(self.bytes_per_atom, engine_bytes_per_atom(self._engines[slot]))
                 except Exception as ex:   # surfaced after the run
-                    errors.append((pack[0].rid, ex))
+                    errors.append((pack[0].rid if pack else -1, ex))
 
         threads = [threading.Thread(target=worker, args=(k,), daemon=True) for k in range(self.slots)]
         for th in threads:
@@ -296,7 +447,7 @@ class Server:
             with cv:
                 waiting.append(r)
                 state["submitted"] += 1
-                cv.notify()
+                cv.notify_all()
         with cv:
             cv.notify_all()
         for th in threads:
