@@ -175,6 +175,7 @@ def main():
     ap.add_argument("--long-window", type=int, default=400, help="untimed steps after the timed region that measure R")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (512 systems) and served (Poisson trace) runs")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="also run a streaming kernel (MIB MiB in, MIB MiB out, 10 times) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
@@ -203,6 +204,9 @@ def main():
     e.md_setup(v_eq, masses, dt=0.25, t_target=300.0, t_damp=25.0, skin=2.0, qeq_every=1, qeq_imax=200, qeq_tol=1e-6)
     print("bench: %d atoms, equilibrated %d steps (%s), warmup %d steps" % (w.n, args.equil, th_eq, args.warmup), file=sys.stderr,
           flush=True)
+    # the engine's autotuner picks its launch configuration on this run (untimed; every configuration gives the same results)
+    tuned = e.autotune(steps=20) if not args.no_autotune else e.launch_config()
+    print("bench: launch configuration %s" % tuned, file=sys.stderr, flush=True)
     if args.warmup:
         e.md_run(args.warmup)
 
@@ -301,6 +305,7 @@ def main():
         "long_window": {"steps": args.long_window, "rebuilds": th_long["rebuilds"], "R": R,
                         "ms_per_step": 1e3 * long_s / max(1, args.long_window),
                         "atom_steps_per_s": n * args.long_window / long_s if args.long_window else None},
+        "launch_config": tuned,
         "phase_ms_per_step": phase_ms,
         "thermo": thermo,
     }

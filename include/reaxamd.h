@@ -180,6 +180,16 @@ int rxa_get_state(rxa_context* ctx, double* x, double* v, double* q);
  * given charges q[n], history and chain (values of the other systems are ignored) instead of a first charge solve, so their
  * next step is the one their previous context would have taken (the neighbour lists are rebuilt, which changes only the order
  * of sums); the other systems start as rxa_md_setup starts them. Charge history is required when qeq_every > 0. */
+/* ---- autotuner (no counterpart in the reference): times launch configurations on the running MD and keeps the fastest of
+ * each in turn: lanes per atom of the slot-parallel per-atom kernels (1, 2, 4, 8), the angle kernels' group size (the largest
+ * bond count's power of two, or twice it), the bonded chain on a second stream beside the charge solve or not, and the
+ * non-bonded pass over the charge-matrix rows or over the skin list. Every configuration gives the same results; each trial
+ * advances the run by 2 + 2 x steps MD steps (after rxa_md_setup). choice[5] = lanes, group multiplier, overlap, rows, trials;
+ * trial_ms (may be NULL) receives up to trial_cap trials' ms per step in the order tried. */
+int rxa_autotune(rxa_context* ctx, int steps, int64_t* choice, double* trial_ms, int trial_cap);
+/* the launch configuration the autotuner chooses, set and read directly: [4] = lanes per atom (1, 2, 4, 8), group multiplier
+ * (1, 2), overlap (0, 1), non-bonded over the rows (0, 1); either pointer may be NULL */
+int rxa_launch_config(rxa_context* ctx, const int* set, int* get);
 int rxa_md_state(rxa_context* ctx, double* s_hist, double* t_hist, double* nh);
 int rxa_md_resume(rxa_context* ctx, const double* v, const double* mass_by_type, double dt, double t_target, double t_damp, double skin,
                   int qeq_every, int qeq_imax, double qeq_tol, const int* keep, const double* q, const double* s_hist,

@@ -144,6 +144,7 @@ struct State {
   // decision census (filled only when census != 0): kCenCount * n ints, [k * n + i] for the categories below
   int census;
   int* cen;
+  int lanes;   // lanes per atom of the slot-parallel per-atom kernels (1, 2, 4 or 8; the same results for each)
 };
 
 // census categories recorded by the kernels that take the decisions (the bond count and the lone-pair truncation are read from

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -256,6 +257,7 @@ struct Engine {
   // pass do not depend on it: it runs on a second stream (default priority) between a fork and a join event, overlapping the
   // latency-bound CG kernels. RXA_OVERLAP=0, or timing a bonded phase, keeps everything on one stream.
   bool overlap = true;
+  int g_mult = 1;   // lanes per atom of the angle kernels: the group size for the largest bond count times this (1 or 2)
   hipStream_t sb = nullptr;
   hipEvent_t ev_fork = nullptr, ev_bonded = nullptr, ev_cg = nullptr;
   int* cgf_h = nullptr;      // pinned: packed systems' any-active flags read back per batch
@@ -273,6 +275,7 @@ struct Engine {
     if (bcap_env && std::atoi(bcap_env) > 0) maxb = std::atoi(bcap_env);
     HIPCHK(hipSetDevice(dev));
     st.shv_stride = 0;
+    st.lanes = 4;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&sb, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
@@ -1000,7 +1003,7 @@ struct Engine {
     }
     const hipStream_t b = bstream();
     hipEvent_t t0 = timer.begin(PH_TERMS);
-    launch_angle_terms(prec, st, g, tab, bmax, b);
+    launch_angle_terms(prec, st, g, tab, std::min(64, bmax * g_mult), b);
     timer.end(PH_TERMS, t0);
     t0 = timer.begin(PH_DBOND);
     launch_bond_derivatives(st, b);
@@ -1479,6 +1482,44 @@ struct Engine {
   }
   std::vector<int> lmp_types;
 
+  // Autotuner: launch configurations timed on the running MD (each trial advances the run by 2 + 2 x `steps` steps; the
+  // configurations give the same results, so only the speed changes). Coordinate descent over: lanes per atom of the
+  // slot-parallel per-atom kernels (1, 2, 4, 8), the angle kernels' group size (the bond count's power of two, or twice it),
+  // the two-stream overlap of the bonded chain with the charge solve, and the non-bonded pass over the charge-matrix rows
+  // versus the skin list. Keeps the fastest setting of each in turn; returns the choices and every trial's ms per step.
+  void autotune(int steps, int64_t* choice, double* trial_ms, int trial_cap) {
+    if (!md_ready) throw ApiError(RXA_ESTATE, "rxa_md_setup first");
+    if (steps < 1) throw ApiError(RXA_EINVAL, "autotune: steps must be >= 1");
+    int ntrial = 0;
+    auto timed = [&]() {
+      md_run(2, nullptr);   // first-call effects (a rebuild, capacities) outside the timing
+      double best = 1e300;
+      for (int r = 0; r < 2; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+        md_run(steps, nullptr);
+        best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps);
+      }
+      if (trial_ms && ntrial < trial_cap) trial_ms[ntrial] = best;
+      ++ntrial;
+      return best;
+    };
+    auto pick = [&](auto set, std::initializer_list<int> values) {
+      int best_v = *values.begin();
+      double best_t = 1e300;
+      for (int v : values) {
+        set(v);
+        const double t = timed();
+        if (t < best_t) { best_t = t; best_v = v; }
+      }
+      set(best_v);
+    };
+    pick([&](int v) { st.lanes = v; }, {4, 1, 2, 8});
+    pick([&](int v) { g_mult = v; }, {1, 2});
+    pick([&](int v) { overlap = v != 0; }, {1, 0});
+    pick([&](int v) { nb_rows = v != 0; }, {1, 0});
+    choice[0] = st.lanes; choice[1] = g_mult; choice[2] = overlap ? 1 : 0; choice[3] = nb_rows ? 1 : 0; choice[4] = ntrial;
+  }
+
   // The MD state a run continues from, beyond positions, velocities and charges: the charge-solve history of s and t (4 deep,
   // newest first, [k n + atom] in the caller's order) and each system's thermostat chain (32 doubles per system).
   void md_state_out(double* s_hist, double* t_hist, double* nh) {
@@ -1856,6 +1897,25 @@ int rxa_set_systems(rxa_context* c, int nsys, const int64_t* counts, const doubl
     for (int k = 0; k < nsys; ++k) nn += counts[k];
     e.set_systems(nsys, counts, boxes, periodic, types, tags, x);
     e.lmp_types.assign(types, types + nn);
+  });
+}
+
+int rxa_launch_config(rxa_context* c, const int* set, int* get) {
+  return guard(c, [&](Engine& e) {
+    if (set) {
+      if (set[0] != 1 && set[0] != 2 && set[0] != 4 && set[0] != 8) throw ApiError(RXA_EINVAL, "lanes per atom must be 1, 2, 4 or 8");
+      if (set[1] != 1 && set[1] != 2) throw ApiError(RXA_EINVAL, "the group multiplier must be 1 or 2");
+      e.sync();
+      e.st.lanes = set[0]; e.g_mult = set[1]; e.overlap = set[2] != 0; e.nb_rows = set[3] != 0;
+    }
+    if (get) { get[0] = e.st.lanes; get[1] = e.g_mult; get[2] = e.overlap ? 1 : 0; get[3] = e.nb_rows ? 1 : 0; }
+  });
+}
+
+int rxa_autotune(rxa_context* c, int steps, int64_t* choice, double* trial_ms, int trial_cap) {
+  return guard(c, [&](Engine& e) {
+    if (!choice) throw ApiError(RXA_EINVAL, "rxa_autotune: null argument");
+    e.autotune(steps, choice, trial_ms, trial_cap);
   });
 }
 

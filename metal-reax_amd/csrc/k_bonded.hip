@@ -26,6 +26,8 @@
 // chain whose values they test: double in FP64 and mixed modes, so they match the FP64 engine outside its rounding, float in
 // FP32 mode. Energies, CdDelta and forces are
 // accumulated in double in every mode.
+#include <type_traits>
+
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -249,10 +251,9 @@ __device__ inline int wave_max(int v) {
 // corrected bond orders (always from the canonical end) and atom quantities
 // ================================================================================================================
 // Also finds each slot's mirror on the other atom (b_sym) and the step's largest bond count (flags[7]); a slot without a mirror
-// sets flags[3]. kBoG lanes per atom, lane p taking slots p, p + kBoG, ...: the slots are independent (the correction reads the
+// sets flags[3]. L lanes per atom, lane p taking slots p, p + L, ...: the slots are independent (the correction reads the
 // uncorrected sums of both ends), so a lane's chain of dependent loads is one slot's instead of the atom's; the total bond order
 // is summed in slot order through the group (the order of a serial loop, so the sums are bitwise those of one thread per atom).
-constexpr int kBoG = 4;
 
 // the mirror slot, the corrected bond orders and the derivative coefficients of slot p of atom i; returns the corrected BO
 template <class TB>
@@ -328,10 +329,10 @@ __device__ inline TB bo_correct_slot(const State& st, const Globals& g, const Ta
   return oBO;
 }
 
-template <class TB>
+template <class TB, int L>
 __global__ void __launch_bounds__(256) k_bo_correct(State st, Globals g, Tables t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gid / kBoG, p0 = gid % kBoG, n = st.n;
+  const int i = gid / L, p0 = gid % L, n = st.n;
   const bool live = i < n;
   const int nb = live ? min(st.b_cnt[i], st.maxb) : 0;
   {   // the largest bond count of this step (flags[7]), one atomic per wave
@@ -342,10 +343,10 @@ __global__ void __launch_bounds__(256) k_bo_correct(State st, Globals g, Tables 
   const int nbw = wave_max(nb);
   const int base = lane_id() - p0;   // the group's first lane
   TB tbo = 0;
-  for (int k0 = 0; k0 < nbw; k0 += kBoG) {   // wave-uniform rounds of kBoG slots
+  for (int k0 = 0; k0 < nbw; k0 += L) {   // wave-uniform rounds of L slots
     const int p = k0 + p0;
     const TB o = p < nb ? bo_correct_slot<TB>(st, g, t, i, p) : TB(0);
-    for (int l = 0; l < kBoG; ++l) tbo += __shfl(o, base + l, 64);   // slot order; absent slots add exact zeros
+    for (int l = 0; l < L; ++l) tbo += __shfl(o, base + l, 64);   // slot order; absent slots add exact zeros
   }
   if (!live || p0 != 0) return;
   const int ti = st.type[i];
@@ -375,15 +376,13 @@ __global__ void __launch_bounds__(256) k_bo_correct(State st, Globals g, Tables 
 // ================================================================================================================
 // bond energy, triple-bond stabilisation, lone pair (+ C2), over/under coordination
 // ================================================================================================================
-// kAtG lanes per atom, lane p taking slots p, p + kAtG, ...; the per-atom sums run in slot order through the group (each lane
-// of the group forms the same sums, in the order of a serial loop over the slots, so they are bitwise those of one thread per
-// atom), and the atom's own terms are evaluated by every lane of the group alike.
-constexpr int kAtG = 4;
-
-template <class TR, class TS>
+// L lanes per atom, lane p taking slots p, p + L, ...; the per-atom sums run in slot order through the group (each lane of the
+// group forms the same sums, in the order of a serial loop over the slots, so they are bitwise those of one thread per atom,
+// whatever L), and the atom's own terms are evaluated by every lane of the group alike.
+template <class TR, class TS, int L>
 __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gid / kAtG, p0 = gid % kAtG, n = st.n;
+  const int i = gid / L, p0 = gid % L, n = st.n;
   const bool live = i < n;
   const int ti = live ? st.type[i] : -1;
   const int nb = ti >= 0 ? min(st.b_cnt[i], st.maxb) : 0;
@@ -394,7 +393,7 @@ __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables 
     for (int c = 0; c < 6; ++c) st.vref[(size_t)c * n + i] = 0.0;
   double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, cdself = 0, e_c2 = 0;
   // bond energy of the bonds whose canonical end is this atom (the reference's tag order, bonded.cpp:222-252)
-  for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+  for (int k0 = 0; k0 < nbw; k0 += L) {
     const int p = k0 + p0;
     double eb = 0, et = 0, cdt = 0;
     if (p < nb) {
@@ -432,7 +431,7 @@ __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables 
         st.Cdbo[q] += cdbo;
       }
     }
-    for (int l = 0; l < kAtG; ++l) {   // slot order; slots without these terms add exact zeros
+    for (int l = 0; l < L; ++l) {   // slot order; slots without these terms add exact zeros
       e_bond += __shfl(eb, base + l, 64);
       e_bond += __shfl(et, base + l, 64);
       cdself += __shfl(cdt, base + l, 64);
@@ -450,7 +449,7 @@ __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables 
   const bool c2 = ti >= 0 && g.c2_active && si.c2;
   if (__any(c2)) {
     const TR p_lp3 = (TR)g.p_lp3, Di = (TR)(c2 ? st.Delta[i] : 0.0);
-    for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+    for (int k0 = 0; k0 < nbw; k0 += L) {
       const int p = k0 + p0;
       double ec = 0, cdc = 0;
       if (c2 && p < nb) {
@@ -466,7 +465,7 @@ __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables 
           }
         }
       }
-      for (int l = 0; l < kAtG; ++l) {
+      for (int l = 0; l < L; ++l) {
         const double e = __shfl(ec, base + l, 64);
         e_lp += e;
         e_c2 += e;
@@ -478,7 +477,7 @@ __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables 
   {
     const TR dfvl = si.heavy ? TR(0) : TR(1);
     TR sum1 = 0, sum2 = 0;
-    for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+    for (int k0 = 0; k0 < nbw; k0 += L) {
       const int p = k0 + p0;
       TR a1 = 0, a2 = 0;
       if (p < nb) {
@@ -488,7 +487,7 @@ __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables 
         a1 = (TR)P.p_ovun1 * (TR)P.De_s * (TR)st.bo[q];
         a2 = ((TR)st.Delta[j] - dfvl * (TR)st.Delta_lp_temp[j]) * ((TR)st.bo_pi[q] + (TR)st.bo_pi2[q]);
       }
-      for (int l = 0; l < kAtG; ++l) {
+      for (int l = 0; l < L; ++l) {
         sum1 += __shfl(a1, base + l, 64);
         sum2 += __shfl(a2, base + l, 64);
       }
@@ -517,7 +516,7 @@ __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables 
       if (under) e_un += (double)eun;
       cdself += (double)CEover3;
       if (under) cdself += (double)CEunder3;
-      for (int p = p0; p < nb; p += kAtG) {
+      for (int p = p0; p < nb; p += L) {
         const size_t q = sl(p, i, n);
         const int j = st.b_j[q];
         const DevPair& P = t.pair[ti * g.ntypes + st.type[j]];
@@ -1251,29 +1250,31 @@ __global__ void __launch_bounds__(128) k_eatom_ref(State st) {
 // ================================================================================================================
 // bond-order derivative pass as gathers
 // ================================================================================================================
-// kAtG lanes per atom as in k_atom_terms: a lane takes every kAtG-th slot, the atom sums run in slot order through the group
+// L lanes per atom as in k_atom_terms: a lane takes every L-th slot, the atom sums run in slot order through the group
+template <int L>
 __global__ void __launch_bounds__(256) k_cdelta(State st) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gid / kAtG, p0 = gid % kAtG, n = st.n;
+  const int i = gid / L, p0 = gid % L, n = st.n;
   const bool live = i < n;
   const int nb = live ? min(st.b_cnt[i], st.maxb) : 0;
   const int nbw = wave_max(nb), base = lane_id() - p0;
   double c = live ? st.CdD_self[i] : 0.0;
-  for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+  for (int k0 = 0; k0 < nbw; k0 += L) {
     const int p = k0 + p0;
     double v = 0;
     if (p < nb) {
       const size_t q = sl(p, i, n);
       v = st.cdp[sl(st.b_sym[q], st.b_j[q], n)];
     }
-    for (int l = 0; l < kAtG; ++l) c += __shfl(v, base + l, 64);
+    for (int l = 0; l < L; ++l) c += __shfl(v, base + l, 64);
   }
   if (live && p0 == 0) st.CdD[i] = c;
 }
 
+template <int L>
 __global__ void __launch_bounds__(256) k_dbond(State st) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gid / kAtG, p0 = gid % kAtG, n = st.n;
+  const int i = gid / L, p0 = gid % L, n = st.n;
   const bool live = i < n;
   const int nb = live ? min(st.b_cnt[i], st.maxb) : 0;
   const int nbw = wave_max(nb), base = lane_id() - p0;
@@ -1281,7 +1282,7 @@ __global__ void __launch_bounds__(256) k_dbond(State st) {
   double Ex = 0, Ey = 0, Ez = 0;   // per-atom calls: sum of the slots' coefficients of dDeltap_self times their bond vectors
   double w[6] = {0, 0, 0, 0, 0, 0};
   const double cdi = live ? st.CdD[i] : 0.0;
-  for (int k0 = 0; k0 < nbw; k0 += kAtG) {
+  for (int k0 = 0; k0 < nbw; k0 += L) {
     const int p = k0 + p0;
     double Sq = 0, sc = 0, dx = 0, dy = 0, dz = 0;
     if (p < nb) {
@@ -1296,7 +1297,7 @@ __global__ void __launch_bounds__(256) k_dbond(State st) {
       sc = T1 * st.dbop_c[q] + (C1pi * cpi) * st.dlnpi_c[q] + (C1pi2 * cpi2) * st.dlnpi2_c[q];
       dx = st.b_dx[q]; dy = st.b_dy[q]; dz = st.b_dz[q];
     }
-    for (int l = 0; l < kAtG; ++l) {   // slot order; absent slots add exact zeros
+    for (int l = 0; l < L; ++l) {   // slot order; absent slots add exact zeros
       const double sq = __shfl(Sq, base + l, 64), s = __shfl(sc, base + l, 64);
       const double ax = __shfl(dx, base + l, 64), ay = __shfl(dy, base + l, 64), az = __shfl(dz, base + l, 64);
       S += sq;
@@ -1355,6 +1356,18 @@ __global__ void __launch_bounds__(128) k_gather(State st) {
 
 inline int nblk(long long items, int per) { return (int)((items + per - 1) / per); }
 
+// lanes per atom of the slot-parallel per-atom kernels (bond-order correction, atom terms, derivative passes): 1, 2, 4 or 8,
+// a run-time choice (the autotuner's); every choice gives the same sums bit for bit
+template <class F>
+void with_lanes(int lanes, F&& f) {
+  switch (lanes) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: f(std::integral_constant<int, 4>{}); break;
+  }
+}
+
 template <class TB>
 void bond_build_t(const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
   hipLaunchKernelGGL(k_bond_build<TB>, dim3(nblk(st.n, kBuildWaves)), dim3(64 * kBuildWaves), 0, s, st, c, g, t);
@@ -1369,9 +1382,12 @@ void launch_bond_build(Precision p, const State& st, const Cell& c, const Global
 
 
 void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
-  const dim3 grid(nblk((long long)st.n * kBoG, 256)), blk(256);
-  if (p == Precision::FP32) hipLaunchKernelGGL(k_bo_correct<float>, grid, blk, 0, s, st, g, t);
-  else hipLaunchKernelGGL(k_bo_correct<double>, grid, blk, 0, s, st, g, t);
+  with_lanes(st.lanes, [&](auto lc) {
+    constexpr int L = decltype(lc)::value;
+    const dim3 grid(nblk((long long)st.n * L, 256)), blk(256);
+    if (p == Precision::FP32) hipLaunchKernelGGL((k_bo_correct<float, L>), grid, blk, 0, s, st, g, t);
+    else hipLaunchKernelGGL((k_bo_correct<double, L>), grid, blk, 0, s, st, g, t);
+  });
 }
 
 // PA: the per-atom variants (reference-split virial shares), built for G = 64 only: a group of 64 lanes gives the same sums as
@@ -1403,10 +1419,13 @@ void launch_terms_t(const State& st, const Globals& g, const Tables& t, int bmax
 }
 
 void launch_atom_terms(Precision p, const State& st, const Globals& g, const Tables& t, hipStream_t s) {
-  const dim3 grid(nblk((long long)st.n * kAtG, 256)), blk(256);
-  if (p == Precision::FP64) hipLaunchKernelGGL((k_atom_terms<double, double>), grid, blk, 0, s, st, g, t);
-  else if (p == Precision::MIXED) hipLaunchKernelGGL((k_atom_terms<float, double>), grid, blk, 0, s, st, g, t);
-  else hipLaunchKernelGGL((k_atom_terms<float, float>), grid, blk, 0, s, st, g, t);
+  with_lanes(st.lanes, [&](auto lc) {
+    constexpr int L = decltype(lc)::value;
+    const dim3 grid(nblk((long long)st.n * L, 256)), blk(256);
+    if (p == Precision::FP64) hipLaunchKernelGGL((k_atom_terms<double, double, L>), grid, blk, 0, s, st, g, t);
+    else if (p == Precision::MIXED) hipLaunchKernelGGL((k_atom_terms<float, double, L>), grid, blk, 0, s, st, g, t);
+    else hipLaunchKernelGGL((k_atom_terms<float, float, L>), grid, blk, 0, s, st, g, t);
+  });
 }
 
 void launch_angle_terms(Precision p, const State& st, const Globals& g, const Tables& t, int bmax, hipStream_t s) {
@@ -1416,9 +1435,12 @@ void launch_angle_terms(Precision p, const State& st, const Globals& g, const Ta
 }
 
 void launch_bond_derivatives(const State& st, hipStream_t s) {
-  const dim3 grid(nblk((long long)st.n * kAtG, 256)), blk(256);
-  hipLaunchKernelGGL(k_cdelta, grid, blk, 0, s, st);
-  hipLaunchKernelGGL(k_dbond, grid, blk, 0, s, st);
+  with_lanes(st.lanes, [&](auto lc) {
+    constexpr int L = decltype(lc)::value;
+    const dim3 grid(nblk((long long)st.n * L, 256)), blk(256);
+    hipLaunchKernelGGL(k_cdelta<L>, grid, blk, 0, s, st);
+    hipLaunchKernelGGL(k_dbond<L>, grid, blk, 0, s, st);
+  });
 }
 
 void launch_eatom_ref(const State& st, hipStream_t s) {

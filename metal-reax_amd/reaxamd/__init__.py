@@ -72,6 +72,8 @@ def lib():
             "rxa_md_run": (I, [P, I, P]),
             "rxa_get_state": (I, [P, P, P, P]),
             "rxa_md_state": (I, [P, P, P, P]),
+            "rxa_autotune": (I, [P, I, P, P, I]),
+            "rxa_launch_config": (I, [P, P, P]),
             "rxa_md_resume": (I, [P, P, P, D, D, D, D, I, I, D, P, P, P, P, P]),
             "rxa_stats": (I, [P, P]),
             "rxa_qeq_stats": (I, [P, P]),
@@ -360,6 +362,28 @@ class Engine:
         th = np.zeros(8)
         self._chk(lib().rxa_md_run(self.h, int(steps), _p(th)))
         return dict(zip(("pe", "ke", "temp", "rebuilds", "qeq_iters_mean", "iters_s", "iters_t", "max_disp"), th.tolist()))
+
+    def autotune(self, steps=20):
+        """Times launch configurations on the running MD and keeps the fastest (rxa_autotune); each trial advances the run by
+        2 + 2 x steps steps. Returns the choices and the trials' ms per step."""
+        ch = np.zeros(5, dtype=np.int64)
+        ms = np.zeros(16)
+        self._chk(lib().rxa_autotune(self.h, int(steps), _p(ch), _p(ms), 16))
+        return {"lanes_per_atom": int(ch[0]), "angle_group_multiplier": int(ch[1]), "overlap": bool(ch[2]),
+                "nonbonded_rows": bool(ch[3]), "trial_ms_per_step": ms[:int(ch[4])].tolist()}
+
+    def launch_config(self, lanes=None, group_multiplier=None, overlap=None, rows=None):
+        """Reads, and with arguments sets, the launch configuration (rxa_launch_config)."""
+        cur = np.zeros(4, dtype=np.int32)
+        self._chk(lib().rxa_launch_config(self.h, None, _p(cur)))
+        new = cur.copy()
+        for k, v in enumerate((lanes, group_multiplier, overlap, rows)):
+            if v is not None:
+                new[k] = int(v)
+        if not np.array_equal(new, cur):
+            self._chk(lib().rxa_launch_config(self.h, _p(new), None))
+        return {"lanes_per_atom": int(new[0]), "angle_group_multiplier": int(new[1]), "overlap": bool(new[2]),
+                "nonbonded_rows": bool(new[3])}
 
     def md_state(self):
         """Charge history (s and t, each (4, n), newest first) and thermostat chains ((systems, 32)): with state(), what a run
