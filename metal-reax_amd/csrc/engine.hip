@@ -1117,9 +1117,9 @@ struct Engine {
   void alloc_qeq_doubles() {
     // sub-arrays start on 256-byte boundaries (the CG reads (s, t) pairs as 16-byte loads)
     auto rnd = [](size_t k) { return (k + 31) & ~(size_t)31; };
-    const size_t nmv = 4 * (size_t)qeq_mv_partials(n);
+    const size_t nmv = 4 * (size_t)qeq_mv_partials(n), nqb = 4 * (size_t)qeq_build_blocks(n);
     const size_t nval = qb.fp32 ? 0 : (size_t)n * maxnb;   // FP64 values (FP64 mode only)
-    const size_t sz = (nval ? rnd(nval) : 0) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + 2 * rnd(nmv) +
+    const size_t sz = (nval ? rnd(nval) : 0) + 3 * rnd(n) + 6 * rnd(2 * (size_t)n) + rnd(4 * kCgBlocks) + 2 * rnd(nmv) + rnd(nqb) +
                       2 * rnd(4 * (size_t)n) + rnd(n) + rnd(2 * (size_t)n) + rnd(3 * kCgBlocks);
     std::vector<double> keep;
     const bool had_hist = q_dbl.p && hist_count > 0 && q_dbl_n == n;   // history of the same atoms only
@@ -1146,6 +1146,7 @@ struct Engine {
     qb.partial = take(4 * kCgBlocks);
     qb.partial_mv = take(nmv);
     qb.partial_rs = take(nmv);
+    qb.partial_qb = take(nqb);
     qb.s_hist = take(4 * (size_t)n);
     qb.t_hist = take(4 * (size_t)n);
     qb.lorow = take(n);
@@ -1165,6 +1166,7 @@ struct Engine {
   // extrapolate over the solutions so far and zeros, exactly as for atoms the stock fix has just created.
   void reset_history() {
     hist_count = 0;
+    qeq_started = false;
     if (qb.s_hist && q_dbl_n == n) {
       HIPCHK(hipMemsetAsync(qb.s_hist, 0, 4 * (size_t)n * sizeof(double), stream));
       HIPCHK(hipMemsetAsync(qb.t_hist, 0, 4 * (size_t)n * sizeof(double), stream));
@@ -1202,7 +1204,10 @@ struct Engine {
     qeq_h_valid = false;
   }
 
-  void qeq_matrix() {
+  // The charge matrix of the current positions. start (one system): the build also forms the solve's start from the stock
+  // guess (the history extrapolation, queued before it), so the next qeq_begin skips its own start pass (qeq_started).
+  bool qeq_started = false;
+  void qeq_matrix(bool start = false) {
     if (!qeq_ready) throw ApiError(RXA_ESTATE, "charge equilibration is not set up");
     if (!nbr_valid) build_neighbors();
     if (q_fp32_alloc != qb.fp32) {   // the precision changed since the arrays were sized
@@ -1213,9 +1218,12 @@ struct Engine {
     if (!qeq_h_valid) {
       hipEvent_t t0 = timer.begin(PH_QEQ_H);
       qb.keyed = (2 * smx[0] + 1) * (2 * smx[1] + 1) * (2 * smx[2] + 1) <= 32768 ? 1 : 0;
-      launch_qeq_build(st, cell, g, tab, qb, stream);
+      const bool fuse = start && !packed;
+      if (fuse) launch_qeq_guess(st, qb, hist_count > 0 ? 1 : 0, stream);
+      launch_qeq_build(st, cell, g, tab, qb, fuse ? 1 : 0, stream);
       timer.end(PH_QEQ_H, t0);
       qeq_h_valid = true;
+      qeq_started = fuse;
     }
   }
 
@@ -1305,7 +1313,7 @@ struct Engine {
     if (!n) throw ApiError(RXA_ESTATE, "no atoms");
     if (imax < 1) throw ApiError(RXA_EINVAL, "charge equilibration: maxiter must be >= 1");
     ensure_cg_cap(imax);
-    qeq_matrix();
+    qeq_matrix(true);
     if (packed) {
       launch_fill_int(d_cg_any.p, (int)d_cg_any.n, 0, stream);
       launch_pk_qeq_init(st, qb, pq, imax, tol, hist_count > 0 ? 1 : 0, stream);
@@ -1315,7 +1323,9 @@ struct Engine {
       cg_launch_batch(imax);
       return;
     }
-    launch_qeq_init(st, qb, imax, tol, hist_count > 0 ? 1 : 0, stream);
+    if (qeq_started) launch_qeq_init_fused(st, qb, imax, tol, stream);   // the build formed the start
+    else launch_qeq_init(st, qb, imax, tol, hist_count > 0 ? 1 : 0, stream);
+    qeq_started = false;
     cg_start(imax, tol);
   }
   void cg_start(int imax, double tol) {
@@ -1612,7 +1622,7 @@ struct Engine {
       qeq_h_valid = false;
       ++md_steps;
       const bool do_qeq = md_qeq_every > 0 && md_steps % md_qeq_every == 0;
-      if (do_qeq && nbr_valid) qeq_matrix();   // speculative: the common step keeps its lists; redone after a rebuild
+      if (do_qeq && nbr_valid) qeq_matrix(true);   // speculative: the common step keeps its lists; redone after a rebuild
       HIPCHK(hipEventSynchronize(ev_disp));
       fork_after = ev_disp;   // the bonded chain need not wait for the charge matrix queued above
       if (packed) {
@@ -2078,6 +2088,7 @@ int rxa_qeq_setup(rxa_context* c, double swa, double swb, const double* chi, con
 int rxa_qeq_matvec(rxa_context* c, const double* x, double* y) {
   return guard(c, [&](Engine& e) {
     e.qeq_matrix();
+    e.qeq_started = false;   // the product stages its vectors in the solve's r and p
     const int n = e.n;
     std::vector<double> xi(n), yo(n);
     for (int i = 0; i < n; ++i) xi[i] = x[e.perm_h[i]];
@@ -2114,6 +2125,7 @@ int rxa_qeq_solve(rxa_context* c, const double* b, const double* x0, int imax, d
     if (imax < 1) throw ApiError(RXA_EINVAL, "maxiter must be >= 1");
     e.ensure_cg_cap(imax);
     e.qeq_matrix();
+    e.qeq_started = false;   // the caller's b and x0 replace the stock start
     const int n = e.n;
     std::vector<double> bi(2 * (size_t)n), xi(2 * (size_t)n);
     for (int i = 0; i < n; ++i) {

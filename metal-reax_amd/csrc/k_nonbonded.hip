@@ -358,17 +358,12 @@ constexpr int kQbWaves = 4, kQbRing = 128;
 // finished together (taper, shielded 1/r, stores), so the costly part runs on full waves. Row entries are kept in skin-list
 // order: slot m of the row is the m-th pair of the list within the cutoff. The 16-bit key is the canonical flag and the
 // shift-table index the distance used.
-__global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t, QeqBuffers qb) {
-  __shared__ int ring_j[kQbWaves][kQbRing];
-  __shared__ int ring_k[kQbWaves][kQbRing];   // key | neighbour type << 16
-  __shared__ double ring_r2[kQbWaves][kQbRing];
-  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (i >= st.n) return;
+// fuse: the row also forms the charge solve's start from the guess in x (k_cg_guess ran before): y = H x with the values as
+// built (FP64, or the FP32 values a non-refined FP32 solve iterates with), r = b - (y + eta x), p = r / eta, and the row's
+// b.b and r.p for the block partials; this replaces a separate product pass over the rows.
+__device__ inline void qeq_build_row(const State& st, const Globals& g, const Tables& t, const QeqBuffers& qb, int i, int fuse,
+                                     int* rj, int* rk, double* rr, double (&acc)[4]) {
   const int lane = lane_id();
-  const int wv = threadIdx.x >> 6;
-  int* rj = ring_j[wv];
-  int* rk = ring_k[wv];
-  double* rr = ring_r2[wv];
   const int ti = st.type[i];
   const size_t base = (size_t)i * st.maxnb;
   const int cnt = ti >= 0 ? min(st.nb_cnt[i], st.maxnb) : 0;
@@ -379,6 +374,8 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
   const double* __restrict__ shld = t.qeq_shld + max(ti, 0) * g.ntypes;
   int m = 0, head = 0, fill = 0;
   double lo_sum = 0.0;   // this lane's part of sum_j |lo_ij| (refinement)
+  double y0 = 0.0, y1 = 0.0;   // fuse: this lane's part of (H x) of both systems
+  const double2* __restrict__ x2 = reinterpret_cast<const double2*>(qb.x);
   auto batch = [&](int count) {
     if (lane < count) {
       const int e = (head + lane) & (kQbRing - 1);
@@ -389,6 +386,7 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
       const double h = (Tap * 14.4) * rcbrt_f64(r * r * r + shld[kt >> 16]);
       const size_t o = base + m + lane;
       if (qb.narrow) qb.col16[o] = (unsigned short)j; else qb.col[o] = j;
+      double hv = h;   // the value the start's product uses
       if (qb.fp32) {
         const float hf = (float)h;
         qb.val32[o] = hf;
@@ -397,9 +395,16 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
           const float lo = (float)(h - (double)hf);
           qb.lo32[o] = lo;
           lo_sum += fabs((double)lo);
+        } else {
+          hv = (double)hf;
         }
       } else {
         qb.val[o] = h;
+      }
+      if (fuse) {
+        const double2 xj = x2[j];
+        y0 += hv * xj.x;
+        y1 += hv * xj.y;
       }
       if (qb.keyed) qb.key[o] = (unsigned short)(kt & 0xFFFF);
     }
@@ -447,13 +452,52 @@ __global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t
   }
   if (fill > 0) batch(fill);
   if (qb.refine) lo_sum = wave_sum(lo_sum);
+  if (fuse) { y0 = wave_sum(y0); y1 = wave_sum(y1); }
   if (lane == 0) {
     qb.cnt[i] = m;
     if (qb.refine) qb.lorow[i] = lo_sum;
     const double eta = ti >= 0 ? t.qeq_eta[ti] : 1.0;
+    const double hinv = 1.0 / eta;
+    const double chi = ti >= 0 ? t.qeq_chi[ti] : 0.0;
     qb.eta[i] = eta;
-    qb.hinv[i] = 1.0 / eta;
-    qb.chi[i] = ti >= 0 ? t.qeq_chi[ti] : 0.0;
+    qb.hinv[i] = hinv;
+    qb.chi[i] = chi;
+    if (fuse) {   // the start of k_spmv2's kMode 1 for this row (b from this build's chi: the guess kernel ran before it)
+      const double2 xi = x2[i];
+      const double wv[2] = {y0 + eta * xi.x, y1 + eta * xi.y};
+      for (int s = 0; s < 2; ++s) {
+        const double b = s == 0 ? -chi : -1.0;
+        qb.b[2 * i + s] = b;
+        const double r = b - wv[s];
+        const double d = r * hinv;
+        qb.r[2 * i + s] = r;
+        qb.p[2 * i + s] = d;
+        acc[s] = b * b;
+        acc[2 + s] = r * d;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_qeq_build(State st, Globals g, Tables t, QeqBuffers qb, int fuse) {
+  __shared__ int ring_j[kQbWaves][kQbRing];
+  __shared__ int ring_k[kQbWaves][kQbRing];   // key | neighbour type << 16
+  __shared__ double ring_r2[kQbWaves][kQbRing];
+  __shared__ double sh[4 * kQbWaves];
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int wv = threadIdx.x >> 6;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < st.n) qeq_build_row(st, g, t, qb, i, fuse, ring_j[wv], ring_k[wv], ring_r2[wv], acc);
+  if (fuse) {   // the block's partials of b.b and r.p, its rows in wave order (a fixed order)
+    if (lane_id() == 0)
+      for (int k = 0; k < 4; ++k) sh[k * kQbWaves + wv] = acc[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 4; ++k) {
+        double v = sh[k * kQbWaves];
+        for (int w = 1; w < kQbWaves; ++w) v += sh[k * kQbWaves + w];
+        qb.partial_qb[blockIdx.x * 4 + k] = v;
+      }
   }
 }
 
@@ -504,9 +548,9 @@ void launch_nonbonded_rows(const State& st, const Globals& g, const Tables& t, c
   else hipLaunchKernelGGL(k_nonbonded_rows<int>, grid, blk, 0, s, st, g, t, qb.col, qb.key, qb.cnt);
 }
 
-void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s) {
+void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, int fuse, hipStream_t s) {
   (void)c;
-  hipLaunchKernelGGL(k_qeq_build, dim3(nblk((long long)st.n * 64, 256)), dim3(256), 0, s, st, g, t, qb);
+  hipLaunchKernelGGL(k_qeq_build, dim3(qeq_build_blocks(st.n)), dim3(256), 0, s, st, g, t, qb, fuse);
 }
 
 }  // namespace rxa

@@ -82,6 +82,7 @@ struct QeqBuffers {
   double* partial;           // 4 * kCgBlocks: per-block partials of the element-wise CG kernels
   double* partial_mv;        // 4 * qeq_mv_partials(n): per-block partials of the matrix-vector product
   double* partial_rs;        // 4 * qeq_mv_partials(n): per-block partials of the refinement's verification residual
+  double* partial_qb;        // 4 * qeq_build_blocks(n): per-block partials of the solve's start fused into the matrix build
   // refinement's bound (k_qeq.hip k_check_gate): the FP32 remainders' row sums sum_j |lo_ij| (n), the CG start x0 (2 n, same
   // layout as x) and the gate's per-block partials (3 * kCgBlocks)
   double* lorow = nullptr;
@@ -117,7 +118,11 @@ void launch_pk_qeq_finish(const State& st, QeqBuffers& qb, const PackedQeq& pq, 
 void launch_pk_shift_table(const Cell* cells, int S, double4* shv, const int smx[3], hipStream_t s);
 void launch_pk_wrap(const State& st, const Cell* cells, const int* reb, hipStream_t s);
 void launch_pk_nbr(const State& st, const Cell* cells, const int* sys_start, const int4* ext, const int* reb, double rs, hipStream_t s);
-void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, hipStream_t s);
+// fuse = 1: also the charge solve's start from the guess in x (launch_qeq_guess before it), finished by launch_qeq_init_fused
+void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, int fuse, hipStream_t s);
+inline int qeq_build_blocks(int n) { return (int)(((long long)n * 64 + 255) / 256); }
+void launch_qeq_guess(const State& st, QeqBuffers& qb, int use_hist, hipStream_t s);
+void launch_qeq_init_fused(const State& st, QeqBuffers& qb, int imax, double tol, hipStream_t s);
 // census of the non-bonded cut-off decision (st.cen, category kCenNonbonded)
 void launch_census_nb(const State& st, const Globals& g, hipStream_t s);
 // Non-bonded terms over the charge-matrix rows instead of the skin list (mixed / FP32 modes): valid while the rows belong to the
