@@ -338,10 +338,11 @@ __global__ void __launch_bounds__(kThreads) k_spmv1(int n, int maxnb, const int*
   if (lane == 0) w[i] = y + eta[i] * v[i];
 }
 
-__global__ void __launch_bounds__(kThreads) k_cg_init_fin(QeqBuffers qb, const double* part, int nparts, int imax, double tol) {
-  __shared__ double sh[4 * kThreads / 64];
+template <int NT>
+__global__ void __launch_bounds__(NT) k_cg_init_fin(QeqBuffers qb, const double* part, int nparts, int imax, double tol) {
+  __shared__ double sh[4 * NT / 64];
   double v[4];
-  reduce_partials<4>(part, v, sh, nparts);
+  reduce_partials<4, NT>(part, v, sh, nparts);
   if (threadIdx.x == 0) {
     for (int s = 0; s < 2; ++s) {
       const double bn = sqrt(v[s]);
@@ -647,7 +648,7 @@ void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int 
   const int n = st.n;
   hipLaunchKernelGGL(k_cg_guess, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, use_hist);
   spmv2<1>(n, st.maxnb, qb, qb.fp32 && qb.refine, qb.x, qb.q, 0, s);
-  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kThreads), 0, s, qb, (const double*)qb.partial_mv, mv_blocks(n), imax, tol);
+  hipLaunchKernelGGL(k_cg_init_fin<kThreads>, dim3(1), dim3(kThreads), 0, s, qb, (const double*)qb.partial_mv, mv_blocks(n), imax, tol);
 }
 
 void launch_qeq_guess(const State& st, QeqBuffers& qb, int use_hist, hipStream_t s) {
@@ -655,7 +656,9 @@ void launch_qeq_guess(const State& st, QeqBuffers& qb, int use_hist, hipStream_t
 }
 
 void launch_qeq_init_fused(const State& st, QeqBuffers& qb, int imax, double tol, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(kThreads), 0, s, qb, (const double*)qb.partial_qb, qeq_build_blocks(st.n), imax, tol);
+  // the build leaves a partial per 4 rows: one block of 1024 threads reduces them (4 times the threads of the product's case)
+  hipLaunchKernelGGL(k_cg_init_fin<1024>, dim3(1), dim3(1024), 0, s, qb, (const double*)qb.partial_qb, qeq_build_blocks(st.n), imax,
+                     tol);
 }
 
 namespace {
