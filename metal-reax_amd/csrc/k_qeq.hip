@@ -656,8 +656,8 @@ void launch_qeq_guess(const State& st, QeqBuffers& qb, int use_hist, hipStream_t
 }
 
 void launch_qeq_init_fused(const State& st, QeqBuffers& qb, int imax, double tol, hipStream_t s) {
-  // the build leaves a partial per 4 rows: one block of 1024 threads reduces them (4 times the threads of the product's case)
-  hipLaunchKernelGGL(k_cg_init_fin<1024>, dim3(1), dim3(1024), 0, s, qb, (const double*)qb.partial_qb, qeq_build_blocks(st.n), imax,
+  // the build leaves a partial per 4 rows: one block of 512 threads reduces them (twice the threads of the product's case)
+  hipLaunchKernelGGL(k_cg_init_fin<512>, dim3(1), dim3(512), 0, s, qb, (const double*)qb.partial_qb, qeq_build_blocks(st.n), imax,
                      tol);
 }
 
