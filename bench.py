@@ -117,6 +117,39 @@ def batch_and_serving(ffield_text, precision, slots=16):
     return out
 
 
+def carbon_workload(precision, steps=100, equil=300, warmup=20):
+    """The random-carbon workload (30,000 C at 1.0 g/cm^3, 84.265 A box, 1.2 A minimum distance, seed 1; the carbon force field
+    of the reference's SI, tests/golden/ffield.C2013; charges fixed, checkqeq no): NVT 4000 K, dt 0.25 fs, skin 2 A. Untimed
+    equilibration of the random packing and warm-up, then `steps` timed steps. Reported beside the water line (a parity config
+    of its own, with heavy bond-list churn)."""
+    import reaxamd
+    from reaxamd import systems
+    ff = open(os.path.join(REPO, "tests", "golden", "ffield.C2013")).read()
+    w = systems.random_carbon(30000, density=1.0, dmin=1.2, seed=1)
+    e = reaxamd.Engine(0)
+    e.pair_style(["NULL", "precision", precision, "checkqeq", "no"])
+    e.pair_coeff(ff, ["C"])
+    e.set_box(w.lo, w.hi)
+    e.set_atoms(w.types, w.tags, w.x)
+    masses = np.array([0.0, systems.MASS["C"]])
+    v = systems.maxwell_boltzmann(w.masses(), 4000.0, seed=2)
+    e.md_setup(v, masses, dt=0.25, t_target=4000.0, t_damp=25.0, skin=2.0, qeq_every=0, qeq_imax=200, qeq_tol=1e-6)
+    e.md_run(equil + warmup)
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    th = e.md_run(steps)
+    hip.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    st = e.stats()
+    e.close()
+    return {"workload": "random_carbon_30000_nvt_4000K", "atoms": w.n, "box_A": float(w.hi[0] - w.lo[0]), "precision": precision,
+            "steps": steps, "atom_steps_per_s": w.n * steps / dt, "ms_per_step": 1e3 * dt / steps, "rebuilds": th["rebuilds"],
+            "temp_K": th["temp"], "bond_slots_per_atom": st["bond_slots"] / w.n,
+            "data": "synthetic: seeded random packing, Maxwell-Boltzmann 4000 K; carbon field of the reference's SI"}
+
+
 def cpu_baseline(ffield_text, sysm, traj, seed_steps=4):
     """The FP64 oracle on consecutive positions of the engine's own equilibrated trajectory: for each, the stock fix's charge
     update (history extrapolation s = 4(s0 + s2) - (6 s1 + s3), t = t2 + 3(t0 - t1) from the oracle's own previous solutions, both
@@ -176,6 +209,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (512 systems) and served (Poisson trace) runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
+    ap.add_argument("--no-carbon", action="store_true", help="skip the 30,000-atom random-carbon run reported beside the water line")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="also run a streaming kernel (MIB MiB in, MIB MiB out, 10 times) to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
@@ -311,6 +345,12 @@ def main():
     }
     if args.calib_copy:   # a plain streaming kernel (read MIB MiB, write MIB MiB) to calibrate the PMC byte counters
         e.stream_copy(args.calib_copy << 20, reps=10)
+    if not args.no_carbon:
+        try:
+            out["carbon"] = carbon_workload(args.precision)
+            print("bench: carbon %s" % out["carbon"], file=sys.stderr, flush=True)
+        except Exception as ex:   # the 24k line must still print
+            print("bench: carbon run failed: %s" % ex, file=sys.stderr, flush=True)
     if not args.no_batch:
         try:
             out.update(batch_and_serving(ffield_text, args.precision))
