@@ -239,7 +239,7 @@ def main():
     print("bench: %d atoms, equilibrated %d steps (%s), warmup %d steps" % (w.n, args.equil, th_eq, args.warmup), file=sys.stderr,
           flush=True)
     # the engine's autotuner picks its launch configuration on this run (untimed; every configuration gives the same results)
-    tuned = e.autotune(steps=20) if not args.no_autotune else e.launch_config()
+    tuned = e.autotune(steps=30) if not args.no_autotune else e.launch_config()
     print("bench: launch configuration %s" % tuned, file=sys.stderr, flush=True)
     if args.warmup:
         e.md_run(args.warmup)

@@ -30,6 +30,7 @@ struct DevPair {
   double p_bo1, p_bo2, p_bo3, p_bo4, p_bo5, p_bo6, r_s, r_p, r_pp, p_boc3, p_boc4, p_boc5;
   double p_be1, p_be2, De_s, De_p, De_pp, p_ovun1;
   double D, alpha, r_vdW, gamma_w, rcore, ecore, acore, lgcij, lgre, gamma, v13cor, ovc;
+  double ln_rs, ln_rp, ln_rpp;   // log of r_s, r_p, r_pp (0 where the bond kind is absent)
   int tbs, s_ok, pi_ok, pp_ok;
 };
 

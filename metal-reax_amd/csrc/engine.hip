@@ -361,6 +361,9 @@ struct Engine {
         d.s_ok = ff.sp[i].r_s > 0.0 && ff.sp[j].r_s > 0.0;
         d.pi_ok = ff.sp[i].r_pi > 0.0 && ff.sp[j].r_pi > 0.0;
         d.pp_ok = ff.sp[i].r_pi_pi > 0.0 && ff.sp[j].r_pi_pi > 0.0;
+        d.ln_rs = d.s_ok && p.r_s > 0.0 ? std::log(p.r_s) : 0.0;
+        d.ln_rp = d.pi_ok && p.r_p > 0.0 ? std::log(p.r_p) : 0.0;
+        d.ln_rpp = d.pp_ok && p.r_pp > 0.0 ? std::log(p.r_pp) : 0.0;
         pr[(size_t)i * nt + j] = d;
       }
     std::vector<int2> aidx((size_t)nt * nt * nt, int2{0, 0});
@@ -1513,15 +1516,22 @@ struct Engine {
       ++ntrial;
       return best;
     };
+    // each value is timed twice, the values in order and then in reverse, and keeps the faster of its two trials: a clock or
+    // trajectory that drifts during the search then favours no value by its place in the order
     auto pick = [&](auto set, std::initializer_list<int> values) {
-      int best_v = *values.begin();
-      double best_t = 1e300;
-      for (int v : values) {
-        set(v);
-        const double t = timed();
-        if (t < best_t) { best_t = t; best_v = v; }
-      }
-      set(best_v);
+      const std::vector<int> vals(values);
+      const int nv = (int)vals.size();
+      std::vector<double> t(nv, 1e300);
+      for (int pass = 0; pass < 2; ++pass)
+        for (int k = 0; k < nv; ++k) {
+          const int i = pass ? nv - 1 - k : k;
+          set(vals[i]);
+          t[i] = std::min(t[i], timed());
+        }
+      int best = 0;
+      for (int i = 1; i < nv; ++i)
+        if (t[i] < t[best]) best = i;
+      set(vals[best]);
     };
     pick([&](int v) { st.lanes = v; }, {4, 1, 2, 8});
     pick([&](int v) { g_mult = v; }, {1, 2});

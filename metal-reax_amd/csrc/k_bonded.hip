@@ -134,9 +134,18 @@ __device__ inline void bo_prime_batch(const State& st, const Globals& g, const T
     const DevPair& P = t.pair[ti * g.ntypes + st.type[j]];
     const TB dd = (TB)d;
     TB C12 = 0, C34 = 0, C56 = 0;
-    if (P.s_ok) { C12 = (TB)P.p_bo1 * rpow<TB>(dd / (TB)P.r_s, (TB)P.p_bo2); bs = (TB(1) + bo_cut) * rexp<TB>(C12); }
-    if (P.pi_ok) { C34 = (TB)P.p_bo3 * rpow<TB>(dd / (TB)P.r_p, (TB)P.p_bo4); bp = rexp<TB>(C34); }
-    if (P.pp_ok) { C56 = (TB)P.p_bo5 * rpow<TB>(dd / (TB)P.r_pp, (TB)P.p_bo6); bpp = rexp<TB>(C56); }
+    if constexpr (std::is_same<TB, double>::value) {
+      // (d / r)^p as exp(p (log d - log r)): one logarithm for the three kinds (log r from the pair table) instead of three
+      // general powers; the exponent's rounding error is a few 1e-16, so BO' moves by about 1e-15 relative
+      const double ld = log(dd);
+      if (P.s_ok) { C12 = P.p_bo1 * exp(P.p_bo2 * (ld - P.ln_rs)); bs = (1.0 + bo_cut) * exp(C12); }
+      if (P.pi_ok) { C34 = P.p_bo3 * exp(P.p_bo4 * (ld - P.ln_rp)); bp = exp(C34); }
+      if (P.pp_ok) { C56 = P.p_bo5 * exp(P.p_bo6 * (ld - P.ln_rpp)); bpp = exp(C56); }
+    } else {
+      if (P.s_ok) { C12 = (TB)P.p_bo1 * rpow<TB>(dd / (TB)P.r_s, (TB)P.p_bo2); bs = (TB(1) + bo_cut) * rexp<TB>(C12); }
+      if (P.pi_ok) { C34 = (TB)P.p_bo3 * rpow<TB>(dd / (TB)P.r_p, (TB)P.p_bo4); bp = rexp<TB>(C34); }
+      if (P.pp_ok) { C56 = (TB)P.p_bo5 * rpow<TB>(dd / (TB)P.r_pp, (TB)P.p_bo6); bpp = rexp<TB>(C56); }
+    }
     bkeep = bs + bp + bpp >= bo_cut;
     const TB rr2 = TB(1) / (dd * dd);
     cln = (TB)P.p_bo2 * C12 * rr2;

@@ -130,10 +130,21 @@ __device__ inline double2 row_product(int row, int n, int maxnb, const int* __re
 #pragma unroll
       for (int u = 0; u < kInFlight; ++u) { y0 += h[u] * x[u].x; y1 += h[u] * x[u].y; }
     }
-    for (; k < c; k += kRowLanes) {
-      const double h = val(base + k);
-      const double2 x = v2[col[base + k]];
-      y0 += h * x.x; y1 += h * x.y;
+    if (k < c) {   // at most kInFlight - 1 entries left: one more pass with their loads in flight together, same sum order
+      int j[kInFlight];
+      double h[kInFlight];
+      double2 x[kInFlight];
+#pragma unroll
+      for (int u = 0; u < kInFlight; ++u) {
+        const int kk = k + u * kRowLanes;
+        j[u] = kk < c ? (int)col[base + kk] : -1;
+        h[u] = kk < c ? val(base + kk) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kInFlight; ++u) x[u] = j[u] >= 0 ? v2[j[u]] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int u = 0; u < kInFlight; ++u)
+        if (j[u] >= 0) { y0 += h[u] * x[u].x; y1 += h[u] * x[u].y; }
     }
   }
   for (int off = kRowLanes / 2; off > 0; off >>= 1) {

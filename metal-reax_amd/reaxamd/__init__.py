@@ -365,10 +365,11 @@ class Engine:
 
     def autotune(self, steps=20):
         """Times launch configurations on the running MD and keeps the fastest (rxa_autotune); each trial advances the run by
-        2 + 2 x steps steps. Returns the choices and the trials' ms per step."""
+        2 + 2 x steps steps; each value is timed twice (in order, then in reverse). Returns the choices and the trials' ms per
+        step."""
         ch = np.zeros(5, dtype=np.int64)
-        ms = np.zeros(16)
-        self._chk(lib().rxa_autotune(self.h, int(steps), _p(ch), _p(ms), 16))
+        ms = np.zeros(64)
+        self._chk(lib().rxa_autotune(self.h, int(steps), _p(ch), _p(ms), 64))
         return {"lanes_per_atom": int(ch[0]), "angle_group_multiplier": int(ch[1]), "overlap": bool(ch[2]),
                 "nonbonded_rows": bool(ch[3]), "trial_ms_per_step": ms[:int(ch[4])].tolist()}
 
