@@ -50,8 +50,9 @@ def batch_and_serving(ffield_text, precision, slots=16):
     """The batched workload (512 periodic systems, N log-uniform in [100, 3000], half water / half CH4:O2 gas, 50 NVT steps at
     2500 K, dt 0.1 fs) run one at a time, through the slot executor (one context per slot) and packed in one context. Then the serving workload (32 requests, N log-uniform
     in [200, 20000], 10-1000 steps): one at a time, all queued at once, and as Poisson traces at offered loads of 0.25x, 0.5x,
-    1x and 2x the one-at-a-time capacity, with p50/p99 latency and throughput, through the slot executor and through the
-    continuous-batching server (arrivals join running packs at chunk boundaries)."""
+    1x, 2x and 4x the one-at-a-time capacity, with p50/p99 latency and throughput, through the slot executor and through the
+    continuous-batching server (small requests packed when capacity is tight; arrivals join running packs at chunk
+    boundaries)."""
     from reaxamd import serving
 
     def timed(reqs, k, realtime=False, packed=False):
@@ -87,14 +88,15 @@ def batch_and_serving(ffield_text, precision, slots=16):
     pslots = 8
     pq = work / timed(serving.serving_requests(32, seed=2), pslots, packed=True)[1]
     rate = seq / (work / 32)   # requests per second = the one-at-a-time capacity
-    # Poisson arrivals at 0.25x, 0.5x, 1x and 2x that capacity: one sequence of exponential gaps, scaled per rate, served by the
-    # slot executor (one context per request) and by the continuous-batching server (small requests packed; while every slot
-    # is busy, arrivals join running packs at 25-step chunk boundaries)
+    # Poisson arrivals at 0.25x, 0.5x, 1x, 2x and 4x that capacity: one sequence of exponential gaps, scaled per rate, served by
+    # the slot executor (one context per request) and by the continuous-batching server (a lone small request runs alone while
+    # other slots are idle; otherwise small requests are packed, and while every slot is busy arrivals join running packs at
+    # 25-step chunk boundaries)
     gaps = np.random.default_rng(3).exponential(1.0, size=32)
     sweeps = {}
     for label, kw in (("slot_executor", {"slots": slots}), ("continuous", {"slots": pslots, "continuous": True})):
         sweeps[label] = []
-        for rho in (0.25, 0.5, 1.0, 2.0):
+        for rho in (0.25, 0.5, 1.0, 2.0, 4.0):
             creqs = serving.serving_requests(32, seed=2)
             for r, a in zip(creqs, np.cumsum(gaps) / (rho * rate)):
                 r.arrival = float(a)

@@ -276,10 +276,11 @@ class Server:
     """Concurrent executor of MD requests (see the module docstring). packed=False: each slot runs one request at a time in its
     own context. packed=True: a free slot takes the oldest queued request alone if it has at least pack_below atoms (the
     single-system path, with cell lists), else every queued request below that size that fits the memory budget (at most
-    max_pack_atoms atoms), packed in one context (run_pack_chunks). continuous=True: as packed, but while every slot is busy a
-    running pack of small requests also admits the small requests that arrive, at its chunk boundaries (every `chunk` steps;
-    run_continuous), instead of leaving them queued until a slot frees (an idle slot starts a pack of its own instead, which
-    costs no re-pack)."""
+    max_pack_atoms atoms), packed in one context (run_pack_chunks). continuous=True: small requests are packed only when
+    capacity is tight: while another slot is idle, a lone small request runs alone on the single-system path; otherwise a free
+    slot starts a continuous pack (run_continuous) of the queued small requests, which, while every slot is busy, also admits
+    the small requests that arrive, at its chunk boundaries (every `chunk` steps), instead of leaving them queued until a slot
+    frees (with an idle slot, the arrival takes that slot instead, which costs no re-pack)."""
 
     def __init__(self, ffield_text, slots=4, precision="mixed", mem_budget_bytes=200 << 30, packed=False, max_pack_atoms=4_000_000,
                  pack_below=3000, continuous=False, chunk=50):
@@ -415,7 +416,11 @@ class Server:
                         return
                     try:
                         if self.continuous and waiting[0].system.n < self.pack_below:
-                            pack = None   # a continuous pack admits the queued small requests itself
+                            # while other slots are idle a lone small request runs alone on the single-system path; with
+                            # capacity tight (no other idle slot, or several small requests queued) a continuous pack admits
+                            # the queued small requests itself, and later arrivals join it at chunk boundaries
+                            nsmall = sum(1 for r in waiting if r.system.n < self.pack_below)
+                            pack = [waiting.pop(0)] if nsmall == 1 and state["idle"] > 0 else None
                         else:
                             pack = take()
                     except Exception as ex:
@@ -425,7 +430,7 @@ class Server:
                 try:
                     if pack is None:
                         run_continuous(self._engines[slot], self.ffield_text, admit_small, chunk=self.chunk, t0=t0)
-                    elif len(pack) == 1 and pack[0].system.n >= self.pack_below:
+                    elif len(pack) == 1 and (pack[0].system.n >= self.pack_below or self.continuous):
                         r = pack[0]
                         r.start = time.perf_counter() - t0
                         r.result = run_one(self._engines[slot], self.ffield_text, r, self.precision)
