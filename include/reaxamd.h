@@ -184,7 +184,7 @@ int rxa_get_state(rxa_context* ctx, double* x, double* v, double* q);
  * each in turn: lanes per atom of the slot-parallel per-atom kernels (1, 2, 4, 8), the angle kernels' group size (the largest
  * bond count's power of two, or twice it), the bonded chain on a second stream beside the charge solve or not, and the
  * non-bonded pass over the charge-matrix rows or over the skin list. Every configuration gives the same results. Each value
- * is timed twice, the values in order and then in reverse, and keeps the faster of its two trials (20 trials); each trial
+ * is timed twice, the values in order and then in reverse, and scored by the sum of its two trials (20 trials); each trial
  * advances the run by 2 + 2 x steps MD steps (after rxa_md_setup). choice[5] = lanes, group multiplier, overlap, rows, trials;
  * trial_ms (may be NULL) receives up to trial_cap trials' ms per step in the order tried. */
 int rxa_autotune(rxa_context* ctx, int steps, int64_t* choice, double* trial_ms, int trial_cap);

@@ -1516,17 +1516,18 @@ struct Engine {
       ++ntrial;
       return best;
     };
-    // each value is timed twice, the values in order and then in reverse, and keeps the faster of its two trials: a clock or
-    // trajectory that drifts during the search then favours no value by its place in the order
+    // each value is timed twice, the values in order and then in reverse, and scored by the sum of its two trials: a clock or
+    // trajectory that drifts linearly during the search then adds the same to every value's score (the better of the two
+    // trials would favour whichever value drew one lucky trial)
     auto pick = [&](auto set, std::initializer_list<int> values) {
       const std::vector<int> vals(values);
       const int nv = (int)vals.size();
-      std::vector<double> t(nv, 1e300);
+      std::vector<double> t(nv, 0.0);
       for (int pass = 0; pass < 2; ++pass)
         for (int k = 0; k < nv; ++k) {
           const int i = pass ? nv - 1 - k : k;
           set(vals[i]);
-          t[i] = std::min(t[i], timed());
+          t[i] += timed();
         }
       int best = 0;
       for (int i = 1; i < nv; ++i)

@@ -365,8 +365,8 @@ class Engine:
 
     def autotune(self, steps=20):
         """Times launch configurations on the running MD and keeps the fastest (rxa_autotune); each trial advances the run by
-        2 + 2 x steps steps; each value is timed twice (in order, then in reverse). Returns the choices and the trials' ms per
-        step."""
+        2 + 2 x steps steps; each value is timed twice (in order, then in reverse) and scored by the sum. Returns the choices
+        and the trials' ms per step."""
         ch = np.zeros(5, dtype=np.int64)
         ms = np.zeros(64)
         self._chk(lib().rxa_autotune(self.h, int(steps), _p(ch), _p(ms), 64))

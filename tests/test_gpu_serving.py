@@ -216,3 +216,29 @@ def test_continuous_server_admits_into_a_running_pack(ffield_text):
               np.abs(alone["charges"] - r.result["charges"]).max()), flush=True)
         _close(alone, r.result, r.rid, box=syst[r.rid].hi - syst[r.rid].lo, tol_e=1e-5, tol_x=1e-4, tol_q=1e-4, tol_g=1e-2)
     e.close()
+
+
+@pytest.mark.gpu
+def test_continuous_server_runs_a_lone_request_alone(ffield_text):
+    """Continuous batching packs small requests only when capacity is tight: a small request that arrives while other slots
+    are idle runs alone on the single-system path, so its result is bit for bit the plain executor's."""
+    from reaxamd.serving import Server
+    syst = small_batch(4, seed=6)
+
+    def reqs():
+        out = requests(syst, steps=10)
+        for k, r in enumerate(out):
+            r.arrival = 0.3 * k   # each request finishes long before the next arrives
+        return out
+    plain = Server(ffield_text, slots=1)
+    ra = plain.run(reqs())
+    plain.close()
+    cont = Server(ffield_text, slots=3, continuous=True, chunk=4)
+    rb = cont.run(reqs(), realtime=True)
+    cont.close()
+    for a, b in zip(sorted(ra, key=lambda r: r.rid), sorted(rb, key=lambda r: r.rid)):
+        for k in a.result["energy"]:
+            assert a.result["energy"][k] == b.result["energy"][k], (a.rid, k)
+        assert np.array_equal(a.result["grad"], b.result["grad"]), a.rid
+        assert np.array_equal(a.result["charges"], b.result["charges"]), a.rid
+        assert np.array_equal(a.result["positions"], b.result["positions"]), a.rid
