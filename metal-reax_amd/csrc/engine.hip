@@ -1456,7 +1456,6 @@ struct Engine {
     p += n;
     md.nh = p; p += 64;
     md.partial = p;
-    md.disp = p + 250;
     std::vector<double> vx(n), vy(n), vz(n), m(n);
     std::vector<int> ty(n);
     d2h(ty.data(), st.type, n * sizeof(int));

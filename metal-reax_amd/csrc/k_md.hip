@@ -69,15 +69,6 @@ __global__ void k_drift(State st, MdBuffers md, double dt) {
   st.pos[i] = make_double4(x, y, z, pos_w(st, i));
 }
 
-__global__ void __launch_bounds__(kRT) k_ke_partial(State st, MdBuffers md) {
-  __shared__ double sh[kRT];
-  double v = 0;
-  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT)
-    v += md.mass[i] * (md.vx[i] * md.vx[i] + md.vy[i] * md.vy[i] + md.vz[i] * md.vz[i]);
-  block_sum(&v, 1, sh);
-  if (threadIdx.x == 0) md.partial[blockIdx.x] = v;
-}
-
 // one Nose-Hoover chain half step (LAMMPS FixNH::nhc_temp_integrate, mtchain 3, nc_tchain 1, no drag); writes the velocity factor
 __global__ void __launch_bounds__(kRT) k_nhc(MdBuffers md) {
   __shared__ double sh[kRT];
@@ -152,38 +143,9 @@ __global__ void k_ref_positions(State st, MdBuffers md) {
   md.z0[i] = st.z[i];
 }
 
-__global__ void __launch_bounds__(kRT) k_disp_partial(State st, MdBuffers md) {
-  __shared__ double sh[kRT];
-  double m = 0;
-  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT) {
-    const double dx = st.x[i] - md.x0[i], dy = st.y[i] - md.y0[i], dz = st.z[i] - md.z0[i];
-    m = fmax(m, dx * dx + dy * dy + dz * dz);
-  }
-  sh[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = kRT / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) md.partial[blockIdx.x] = sh[0];
-}
-
-__global__ void __launch_bounds__(kRT) k_disp_final(MdBuffers md) {
-  __shared__ double sh[kRT];
-  sh[threadIdx.x] = threadIdx.x < kRBlocks ? md.partial[threadIdx.x] : 0.0;
-  __syncthreads();
-  for (int s = kRT / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) md.disp[0] = sqrt(sh[0]);
-}
-
-// Fused forms of the integrator's element-wise kernels for the step loop, with the same arithmetic and the same partial-sum layout
-// (kRBlocks blocks, grid-stride) as the separate kernels above:
+// Fused forms of the integrator's element-wise kernels for the step loop (kRBlocks blocks, grid-stride partials):
 //   k_scale_ke:   v *= nh[19] (the end-of-step thermostat factor still to apply, if `apply`), then the partials of sum m v^2;
-//   k_kick_drift: v *= nh[19] (if `scale`), v -= dtf g / m, x += dt v, packed positions, partials of the largest displacement;
-//   k_kick_ke:    v -= dtf g / m, then the partials of sum m v^2.
+//   k_kick_drift: v *= nh[19] (if `scale`), v -= dtf g / m, x += dt v, packed positions, partials of the largest displacement.
 __global__ void __launch_bounds__(kRT) k_scale_ke(State st, MdBuffers md, int apply) {
   __shared__ double sh[kRT];
   const double f = md.nh[19];
@@ -229,20 +191,7 @@ __global__ void __launch_bounds__(kRT) k_kick_drift(State st, MdBuffers md, doub
   if (threadIdx.x == 0) md.partial[blockIdx.x] = sh[0];
 }
 
-__global__ void __launch_bounds__(kRT) k_kick_ke(State st, MdBuffers md, double dtf) {
-  __shared__ double sh[kRT];
-  double v = 0;
-  for (int i = blockIdx.x * kRT + threadIdx.x; i < st.n; i += gridDim.x * kRT) {
-    const double f = dtf / md.mass[i];
-    const double vx = md.vx[i] - f * st.gx[i], vy = md.vy[i] - f * st.gy[i], vz = md.vz[i] - f * st.gz[i];
-    md.vx[i] = vx; md.vy[i] = vy; md.vz[i] = vz;
-    v += md.mass[i] * (vx * vx + vy * vy + vz * vz);
-  }
-  block_sum(&v, 1, sh);
-  if (threadIdx.x == 0) md.partial[blockIdx.x] = v;
-}
-
-// the gather (gather_atom) and k_kick_ke in one pass: the same arithmetic and partial-sum layout as the two kernels in sequence
+// the gather (gather_atom) and the velocity half step in one pass: v -= dtf g / m, then the partials of sum m v^2
 __global__ void __launch_bounds__(kRT) k_gather_kick_ke(State st, MdBuffers md, double dtf) {
   __shared__ double sh[kRT];
   double v = 0;
@@ -280,15 +229,6 @@ void launch_md_gather_step_end(const State& st, MdBuffers& md, double dtf, bool 
   hipLaunchKernelGGL(k_gather_kick_ke, dim3(kRBlocks), dim3(kRT), 0, s, st, md, dtf);
   if (thermostat) hipLaunchKernelGGL(k_nhc, dim3(1), dim3(kRT), 0, s, md);
 }
-// step end: velocity half step, then (if `thermostat`) the thermostat half step, whose velocity factor stays pending
-void launch_md_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s) {
-  if (thermostat) {
-    hipLaunchKernelGGL(k_kick_ke, dim3(kRBlocks), dim3(kRT), 0, s, st, md, dtf);
-    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(kRT), 0, s, md);
-  } else {
-    hipLaunchKernelGGL(k_half_kick, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md, dtf);
-  }
-}
 void launch_md_scale(const State& st, MdBuffers& md, hipStream_t s) {
   hipLaunchKernelGGL(k_scale_v, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md);
 }
@@ -306,17 +246,8 @@ void launch_md_half_kick(const State& st, MdBuffers& md, double dtf, hipStream_t
 void launch_md_drift(const State& st, MdBuffers& md, double dt, hipStream_t s) {
   hipLaunchKernelGGL(k_drift, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md, dt);
 }
-void launch_md_thermostat(const State& st, MdBuffers& md, hipStream_t s) {
-  hipLaunchKernelGGL(k_ke_partial, dim3(kRBlocks), dim3(kRT), 0, s, st, md);
-  hipLaunchKernelGGL(k_nhc, dim3(1), dim3(kRT), 0, s, md);
-  hipLaunchKernelGGL(k_scale_v, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md);
-}
 void launch_md_ref_positions(const State& st, MdBuffers& md, hipStream_t s) {
   hipLaunchKernelGGL(k_ref_positions, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md);
-}
-void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s) {
-  hipLaunchKernelGGL(k_disp_partial, dim3(kRBlocks), dim3(kRT), 0, s, st, md);
-  hipLaunchKernelGGL(k_disp_final, dim3(1), dim3(kRT), 0, s, md);
 }
 
 // plain streaming copy (16-byte elements, grid-stride): a kernel of known traffic for calibrating memory counters

@@ -145,20 +145,16 @@ void launch_energy_reduce(const State& st, double* out /*kTerms + 6 + 1*/, hipSt
 struct MdBuffers {
   double *vx, *vy, *vz, *mass;   // per atom (mass from type)
   double* nh;                    // thermostat state (see k_md.hip)
-  double* disp;                  // max displacement since the last rebuild
   double *x0, *y0, *z0;          // positions at the last rebuild
   double* partial;               // reduction scratch
 };
 void launch_md_half_kick(const State& st, MdBuffers& md, double dtf, hipStream_t s);
 void launch_md_drift(const State& st, MdBuffers& md, double dt, hipStream_t s);
-void launch_md_thermostat(const State& st, MdBuffers& md, hipStream_t s);
 void launch_md_ref_positions(const State& st, MdBuffers& md, hipStream_t s);
-void launch_md_displacement(const State& st, MdBuffers& md, hipStream_t s);
-// fused step kernels (the separate ones above, same arithmetic): see k_md.hip
+// fused step kernels: see k_md.hip
 void launch_md_step_start(const State& st, MdBuffers& md, double dtf, double dt, bool thermostat, bool pending, hipStream_t s);
 int md_disp_partials();   // per-block partials of the kick-drift kernel's largest squared displacement (in md.partial)
 void launch_md_gather_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s);
-void launch_md_step_end(const State& st, MdBuffers& md, double dtf, bool thermostat, hipStream_t s);
 void launch_md_scale(const State& st, MdBuffers& md, hipStream_t s);
 void launch_pk_ref(const State& st, MdBuffers& md, const int* reb, hipStream_t s);
 void launch_pk_disp(const State& st, MdBuffers& md, const int* sys_start, int S, double lim, int* reb, int* any, double* disp,
