@@ -116,22 +116,42 @@ __global__ void k_pk_ref(State st, MdBuffers md, const int* reb) {
   md.z0[i] = st.z[i];
 }
 
-// ---- one wave per system ----------------------------------------------------------------------------------------------
-__device__ inline int sys_wave() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); }
+// ---- one block per system (grid = the number of systems) -----------------------------------------------------------------
+// Thread t takes the system's atoms t, t + kSysThreads, ... in order; the block combines its values by a xor butterfly per
+// wave and then the waves in order, so a system's sums depend only on its own atoms (not on the systems packed with it).
+constexpr int kSysThreads = 256;
+__device__ inline double sys_block_sum(double v, double* sh) {   // every thread gets the total
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane_id() == 0) sh[threadIdx.x / 64] = v;
+  __syncthreads();
+  double s = sh[0];
+  for (int w = 1; w < kSysThreads / 64; ++w) s += sh[w];
+  return s;
+}
+__device__ inline double sys_block_max(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  __syncthreads();
+  if (lane_id() == 0) sh[threadIdx.x / 64] = v;
+  __syncthreads();
+  double m = sh[0];
+  for (int w = 1; w < kSysThreads / 64; ++w) m = fmax(m, sh[w]);
+  return m;
+}
 
 // largest displacement since the system's last rebuild; reb[s] = 1 where it exceeds half the skin (any[0] |= 1)
-__global__ void __launch_bounds__(256) k_pk_disp(State st, MdBuffers md, const int* sys_start, int S, double lim2, int* reb, int* any,
-                                                double* disp) {
-  const int s = sys_wave();
+__global__ void __launch_bounds__(kSysThreads) k_pk_disp(State st, MdBuffers md, const int* sys_start, int S, double lim2, int* reb,
+                                                        int* any, double* disp) {
+  __shared__ double sh[kSysThreads / 64];
+  const int s = blockIdx.x;
   if (s >= S) return;
-  const int lane = lane_id();
   double m = 0;
-  for (int i = sys_start[s] + lane; i < sys_start[s + 1]; i += 64) {
+  for (int i = sys_start[s] + threadIdx.x; i < sys_start[s + 1]; i += kSysThreads) {
     const double dx = st.x[i] - md.x0[i], dy = st.y[i] - md.y0[i], dz = st.z[i] - md.z0[i];
     m = fmax(m, dx * dx + dy * dy + dz * dz);
   }
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-  if (lane == 0) {
+  m = sys_block_max(m, sh);
+  if (threadIdx.x == 0) {
     const int r = m > lim2 ? 1 : 0;
     reb[s] = r;
     disp[s] = sqrt(m);
@@ -141,15 +161,15 @@ __global__ void __launch_bounds__(256) k_pk_disp(State st, MdBuffers md, const i
 
 // One Nose-Hoover chain half step per system (the single-system k_nhc, state nh + 32 s, same layout), from the system's
 // sum of m v^2; NVE systems only record temperature and kinetic energy.
-__global__ void __launch_bounds__(256) k_pk_nhc(State st, MdBuffers md, const int* sys_start, int S, double* nh_all) {
-  const int s = sys_wave();
+__global__ void __launch_bounds__(kSysThreads) k_pk_nhc(State st, MdBuffers md, const int* sys_start, int S, double* nh_all) {
+  __shared__ double sh[kSysThreads / 64];
+  const int s = blockIdx.x;
   if (s >= S) return;
-  const int lane = lane_id();
   double v = 0;
-  for (int i = sys_start[s] + lane; i < sys_start[s + 1]; i += 64)
+  for (int i = sys_start[s] + threadIdx.x; i < sys_start[s + 1]; i += kSysThreads)
     v += md.mass[i] * (md.vx[i] * md.vx[i] + md.vy[i] * md.vy[i] + md.vz[i] * md.vz[i]);
-  v = wave_sum(v);
-  if (lane != 0) return;
+  v = sys_block_sum(v, sh);
+  if (threadIdx.x != 0) return;
   double* nh = nh_all + 32 * (size_t)s;
   const double t_target = nh[13], t_freq = nh[14], dt = nh[15], dof = nh[16], boltz = nh[17], mvv2e = nh[18];
   const double ke2 = v * mvv2e;
@@ -211,16 +231,17 @@ __global__ void k_pk_scale_v(State st, MdBuffers md, const double* nh_all) {
 }
 
 // per-system energy terms and virial: out[s * (kTerms + 6) + k]
-__global__ void __launch_bounds__(256) k_pk_energy(State st, const int* sys_start, int S, double* out) {
-  const int s = sys_wave();
+__global__ void __launch_bounds__(kSysThreads) k_pk_energy(State st, const int* sys_start, int S, double* out) {
+  __shared__ double sh[kSysThreads / 64];
+  const int s = blockIdx.x;
   if (s >= S) return;
-  const int lane = lane_id(), n = st.n;
+  const int n = st.n;
   for (int k = 0; k < kTerms + 6; ++k) {
     const double* a = k < kTerms ? st.eatom + (size_t)k * n : st.vatom + (size_t)(k - kTerms) * n;
     double v = 0;
-    for (int i = sys_start[s] + lane; i < sys_start[s + 1]; i += 64) v += a[i];
-    v = wave_sum(v);
-    if (lane == 0) out[(size_t)s * (kTerms + 6) + k] = v;
+    for (int i = sys_start[s] + threadIdx.x; i < sys_start[s + 1]; i += kSysThreads) v += a[i];
+    v = sys_block_sum(v, sh);
+    if (threadIdx.x == 0) out[(size_t)s * (kTerms + 6) + k] = v;
   }
 }
 
@@ -241,14 +262,14 @@ void launch_pk_ref(const State& st, MdBuffers& md, const int* reb, hipStream_t s
 }
 void launch_pk_disp(const State& st, MdBuffers& md, const int* sys_start, int S, double lim, int* reb, int* any, double* disp,
                     hipStream_t s) {
-  hipLaunchKernelGGL(k_pk_disp, dim3(nblk((long long)S * 64, 256)), dim3(256), 0, s, st, md, sys_start, S, lim * lim, reb, any, disp);
+  hipLaunchKernelGGL(k_pk_disp, dim3(S), dim3(kSysThreads), 0, s, st, md, sys_start, S, lim * lim, reb, any, disp);
 }
 void launch_pk_thermostat(const State& st, MdBuffers& md, const int* sys_start, int S, double* nh_all, hipStream_t s) {
-  hipLaunchKernelGGL(k_pk_nhc, dim3(nblk((long long)S * 64, 256)), dim3(256), 0, s, st, md, sys_start, S, nh_all);
+  hipLaunchKernelGGL(k_pk_nhc, dim3(S), dim3(kSysThreads), 0, s, st, md, sys_start, S, nh_all);
   hipLaunchKernelGGL(k_pk_scale_v, dim3(nblk(st.n, 256)), dim3(256), 0, s, st, md, (const double*)nh_all);
 }
 void launch_pk_energy(const State& st, const int* sys_start, int S, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_pk_energy, dim3(nblk((long long)S * 64, 256)), dim3(256), 0, s, st, sys_start, S, out);
+  hipLaunchKernelGGL(k_pk_energy, dim3(S), dim3(kSysThreads), 0, s, st, sys_start, S, out);
 }
 
 }  // namespace rxa

@@ -453,8 +453,9 @@ __global__ void __launch_bounds__(kThreads) k_cg_charges(int n, QeqBuffers qb, d
 }
 
 // ---- several independent systems packed in one engine ---------------------------------------------------------------
-// The same two CG systems (s, t) per packed system, with that system's own scalars: one wave per system sums its rows' values
-// (lane = row offset within the system, then a fixed butterfly), so a system's solve does not depend on the systems beside it.
+// The same two CG systems (s, t) per packed system, with that system's own scalars: one block per system sums its rows' values
+// (thread t takes the rows t, t + kSysThreads, ... of the system in order, then block_reduce's fixed order), so a system's solve
+// does not depend on the systems beside it.
 // pk[10 s + 4 k + {0, 1, 2, 3}] = |b|, r.p, alpha, beta of solve k; pk[10 s + 8, 9] = the sums of s and t for the charges;
 // pki[4 s + 2 k + {0, 1}] = state, the stock loop index. State 1: active; 0: stopped; 2: stopped and its true residual verified
 // by the refinement check (only with refinement). rowv[4 i + {0, 1}] = d.q (b.b at the start), rowv[4 i + {2, 3}] = r.p.
@@ -475,18 +476,19 @@ __global__ void k_pk_cg_init(int n, QeqBuffers qb, double* rowv) {
 // kind 0: start (|b|, r.p, state for iteration 1); 1: alpha of iteration it; 2: r.p, beta and state after iteration it;
 // 3: sums of s and t; 4: refinement check after iteration it (stopped solves: the true residual's r.p restarts the solve with
 // beta 0 while it exceeds tol, else the solve is verified)
-__global__ void __launch_bounds__(256) k_pk_cg_seg(int kind, int S, const int* sys_start, const double* rowv, const double* x, double* pk,
-                                                  int* pki, int it, int imax, double tol, int* any) {
-  const int sy = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+constexpr int kSysThreads = 256;   // threads per system of the packed reductions
+__global__ void __launch_bounds__(kSysThreads) k_pk_cg_seg(int kind, int S, const int* sys_start, const double* rowv, const double* x,
+                                                          double* pk, int* pki, int it, int imax, double tol, int* any) {
+  const int sy = blockIdx.x;   // one block per system (grid = S)
   if (sy >= S) return;
-  const int lane = lane_id();
+  __shared__ double sh[4 * kSysThreads / 64];
   double v[4] = {0, 0, 0, 0};
-  for (int i = sys_start[sy] + lane; i < sys_start[sy + 1]; i += 64) {
+  for (int i = sys_start[sy] + threadIdx.x; i < sys_start[sy + 1]; i += kSysThreads) {
     if (kind == 3) { v[0] += x[2 * i]; v[1] += x[2 * i + 1]; }
     else for (int k = 0; k < 4; ++k) v[k] += rowv[4 * i + k];
   }
-  for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
-  if (lane != 0) return;
+  block_reduce<4, kSysThreads>(v, sh);
+  if (threadIdx.x != 0) return;
   double* P = pk + 10 * (size_t)sy;
   int* A = pki + 4 * (size_t)sy;
   if (kind == 3) { P[8] = v[0]; P[9] = v[1]; return; }
@@ -709,8 +711,8 @@ void launch_pk_qeq_init(const State& st, QeqBuffers& qb, const PackedQeq& pq, in
   hipLaunchKernelGGL(k_cg_guess, dim3(kCgBlocks), dim3(kThreads), 0, s, n, qb, use_hist);
   spmv2<0>(n, st.maxnb, qb, qb.fp32 && qb.refine, qb.x, qb.q, 0, s);
   hipLaunchKernelGGL(k_pk_cg_init, dim3(pk_blocks(n, 256)), dim3(256), 0, s, n, qb, pq.rowv);
-  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pk_blocks((long long)pq.S * 64, 256)), dim3(256), 0, s, 0, pq.S, pq.sys_start, pq.rowv,
-                     (const double*)qb.x, pq.pk, pq.pki, 0, imax, tol, pq.any);
+  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pq.S), dim3(kSysThreads), 0, s, 0, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk,
+                     pq.pki, 0, imax, tol, pq.any);
 }
 
 void launch_pk_qeq_iteration(const State& st, QeqBuffers& qb, const PackedQeq& pq, int it, int imax, double tol, hipStream_t s) {
@@ -719,10 +721,10 @@ void launch_pk_qeq_iteration(const State& st, QeqBuffers& qb, const PackedQeq& p
     hipLaunchKernelGGL((k_pk_cg_mvdir<decltype(v), col_t<decltype(col)>>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, st.maxnb, col, v, qb,
                        pq.sysid, (const double*)pq.pk, (const int*)pq.pki, pq.rowv, it);
   });
-  const dim3 sg(pk_blocks((long long)pq.S * 64, 256));
-  hipLaunchKernelGGL(k_pk_cg_seg, sg, dim3(256), 0, s, 1, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
+  const dim3 sg(pq.S);
+  hipLaunchKernelGGL(k_pk_cg_seg, sg, dim3(kSysThreads), 0, s, 1, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
   hipLaunchKernelGGL(k_pk_cg_update, dim3(pk_blocks(n, 256)), dim3(256), 0, s, n, qb, pq.sysid, (const double*)pq.pk, (const int*)pq.pki, pq.rowv);
-  hipLaunchKernelGGL(k_pk_cg_seg, sg, dim3(256), 0, s, 2, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
+  hipLaunchKernelGGL(k_pk_cg_seg, sg, dim3(kSysThreads), 0, s, 2, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
 }
 
 void launch_pk_qeq_restart(const State& st, QeqBuffers& qb, const PackedQeq& pq, int it, int imax, double tol, hipStream_t s) {
@@ -734,13 +736,13 @@ void launch_pk_qeq_restart(const State& st, QeqBuffers& qb, const PackedQeq& pq,
   else
     hipLaunchKernelGGL((k_pk_cg_check<int>), dim3(mv_grid(n)), dim3(kMvThreads), 0, s, n, st.maxnb, (const int*)qb.col, v, qb, pq.sysid,
                        (const int*)pq.pki, pq.rowv);
-  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pk_blocks((long long)pq.S * 64, 256)), dim3(256), 0, s, 4, pq.S, pq.sys_start, pq.rowv,
-                     (const double*)qb.x, pq.pk, pq.pki, it, imax, tol, pq.any);
+  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pq.S), dim3(kSysThreads), 0, s, 4, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk,
+                     pq.pki, it, imax, tol, pq.any);
 }
 
 void launch_pk_qeq_finish(const State& st, QeqBuffers& qb, const PackedQeq& pq, hipStream_t s) {
-  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pk_blocks((long long)pq.S * 64, 256)), dim3(256), 0, s, 3, pq.S, pq.sys_start, pq.rowv,
-                     (const double*)qb.x, pq.pk, pq.pki, 0, 0, 0.0, pq.any);
+  hipLaunchKernelGGL(k_pk_cg_seg, dim3(pq.S), dim3(kSysThreads), 0, s, 3, pq.S, pq.sys_start, pq.rowv, (const double*)qb.x, pq.pk,
+                     pq.pki, 0, 0, 0.0, pq.any);
   hipLaunchKernelGGL(k_pk_cg_charges, dim3(pk_blocks(st.n, 256)), dim3(256), 0, s, st.n, qb, pq.sysid, (const double*)pq.pk, st.q);
 }
 
