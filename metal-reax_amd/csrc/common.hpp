@@ -31,6 +31,7 @@ struct DevPair {
   double p_be1, p_be2, De_s, De_p, De_pp, p_ovun1;
   double D, alpha, r_vdW, gamma_w, rcore, ecore, acore, lgcij, lgre, gamma, v13cor, ovc;
   double ln_rs, ln_rp, ln_rpp;   // log of r_s, r_p, r_pp (0 where the bond kind is absent)
+  double pgw, irvdw, ircore, re6;   // (1 / gamma_w)^p_vdW1, 1 / r_vdW, 1 / rcore, lgre^6
   int tbs, s_ok, pi_ok, pp_ok;
 };
 

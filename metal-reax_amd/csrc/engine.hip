@@ -364,6 +364,10 @@ struct Engine {
         d.ln_rs = d.s_ok && p.r_s > 0.0 ? std::log(p.r_s) : 0.0;
         d.ln_rp = d.pi_ok && p.r_p > 0.0 ? std::log(p.r_p) : 0.0;
         d.ln_rpp = d.pp_ok && p.r_pp > 0.0 ? std::log(p.r_pp) : 0.0;
+        d.pgw = std::pow(1.0 / p.gamma_w, ff.gp[28]);
+        d.irvdw = 1.0 / p.r_vdW;
+        d.ircore = 1.0 / p.rcore;
+        d.re6 = (p.lgre * p.lgre * p.lgre) * (p.lgre * p.lgre * p.lgre);
         pr[(size_t)i * nt + j] = d;
       }
     std::vector<int2> aidx((size_t)nt * nt * nt, int2{0, 0});
@@ -1017,8 +1021,8 @@ struct Engine {
   void nonbonded_launch() {
     hipEvent_t t0 = timer.begin(PH_NONB);
     // the charge-matrix rows of these positions already list the non-bonded pairs in skin-list order
-    const bool rows = prec != Precision::FP64 && qeq_h_valid && qb.keyed && qb.key && g.q_swb >= g.nonb_cut && nb_rows;
-    if (rows) launch_nonbonded_rows(st, g, tab, qb, stream);
+    const bool rows = qeq_h_valid && qb.keyed && qb.key && g.q_swb >= g.nonb_cut && nb_rows;
+    if (rows) launch_nonbonded_rows(prec, st, g, tab, qb, stream);
     else launch_nonbonded(prec, st, cell, g, tab, stream);
     timer.end(PH_NONB, t0);
   }

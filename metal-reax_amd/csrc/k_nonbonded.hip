@@ -5,8 +5,10 @@
 // (atomic-free, bitwise repeatable: fixed lane assignment + fixed butterfly reduction). The distance is the canonical one, so both
 // ends agree on the cut-off decision and on r. Reference: src/cpu/nonbonded.cpp:18-82 (pair loop, owner-computes counting: a
 // physical pair counted once = half from each end here), include/reaxmetal/terms.hpp:262-325 (formulas), terms_host.hpp:17-39
-// (taper). FP64 mode uses the reference's Horner taper and pow(x, 0.33333333333333); FP32 terms use the algebraically equal
-// t-form of the taper (no cancellation of O(10) coefficients in float) and the float cube root.
+// (taper). FP64 mode uses the reference's Horner taper and its exponents (pow(x, 0.33333333333333) for the cube root), each
+// pow evaluated as exp(y log x); FP32 terms use the algebraically equal t-form of the taper (no cancellation of O(10)
+// coefficients in float) and the float cube root. Both precisions run the same two passes: over the skin list with the pairs
+// within the cutoff compacted into full waves, and over the charge-matrix rows when those are current.
 //
 // H (stock fix qeq/reaxff compute_H / calculate_H, LAMMPS fix_qeq_reaxff.cpp): H_ij = 14.4 Tap_q(r) / (r^3 + (g_i g_j)^-1.5)^(1/3) for
 // r^2 <= swb^2, stored as full rows (both directions, every image): FP64 values in FP64 mode, FP32 values otherwise; 16-bit
@@ -17,119 +19,65 @@
 namespace rxa {
 namespace {
 
-template <class TR>
-__device__ inline void taper_eval(const Globals& g, TR r, TR& Tap, TR& dTap) {
-  if (sizeof(TR) == 8) {
-    const double* c = g.tap;
-    const double rr = (double)r;
-    double T = c[7] * rr + c[6];
-    T = T * rr + c[5]; T = T * rr + c[4]; T = T * rr + c[3]; T = T * rr + c[2]; T = T * rr + c[1]; T = T * rr + c[0];
-    double dT = 7 * c[7] * rr + 6 * c[6];
-    dT = dT * rr + 5 * c[5]; dT = dT * rr + 4 * c[4]; dT = dT * rr + 3 * c[3]; dT = dT * rr + 2 * c[2];
-    dT += c[1] / rr;
-    Tap = (TR)T;
-    dTap = (TR)dT;
-  } else {
-    const TR swa = (TR)g.nonb_low, d = (TR)g.nonb_cut - (TR)g.nonb_low;
-    const TR x = (r - swa) / d, x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
-    Tap = TR(1) - x4 * (TR(35) - TR(84) * x + TR(70) * x2 - TR(20) * x3);
-    dTap = (x3 * (TR(-140) + TR(420) * x - TR(420) * x2 + TR(140) * x3)) / (d * r);
-  }
+struct NbAcc { double gx = 0, gy = 0, gz = 0, ev = 0, ee = 0, v[6] = {0, 0, 0, 0, 0, 0}; };
+
+// this end's gradient -CE dvec, half of the pair energies and half of the pair virial r_ij (x) f_j = -CE dvec (x) dvec
+__device__ inline void nb_acc(NbAcc& a, double ce, double evdw, double eel, double dx, double dy, double dz) {
+  a.gx += -ce * dx; a.gy += -ce * dy; a.gz += -ce * dz;
+  a.ev += 0.5 * evdw;
+  a.ee += 0.5 * eel;
+  const double hv = -0.5 * ce;
+  a.v[0] += hv * dx * dx; a.v[1] += hv * dy * dy; a.v[2] += hv * dz * dz; a.v[3] += hv * dx * dy; a.v[4] += hv * dx * dz; a.v[5] += hv * dy * dz;
 }
 
-template <class TR> __device__ inline TR cbrt_ref(TR x);
-template <> __device__ inline double cbrt_ref(double x) { return pow(x, 0.33333333333333); }
-template <class TR> __device__ inline TR tpow(TR a, TR b);
-template <> __device__ inline double tpow(double a, double b) { return pow(a, b); }
-template <class TR> __device__ inline TR texp(TR a);
-template <> __device__ inline double texp(double a) { return exp(a); }
-
-template <class TR>
-__global__ void __launch_bounds__(256) k_nonbonded(State st, Globals g, Tables t) {
-  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (i >= st.n) return;
-  const int lane = lane_id();
-  const int ti = st.type[i];
-  double gx = 0, gy = 0, gz = 0, ev = 0, ee = 0;
-  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-  if (ti >= 0) {
-    const size_t base = (size_t)i * st.maxnb;
-    const int cnt = min(st.nb_cnt[i], st.maxnb);
-    const TR qi = (TR)st.q[i];
-    const TR p_vdW1 = (TR)g.p_vdW1, p1i = TR(1) / p_vdW1;
-    const int vt = g.vdw_type;
-    const double4 pi = st.pos[i];
-    for (int k = lane; k < cnt; k += 64) {
-      const int j = st.nb_j[base + k];
-      double dx, dy, dz;
-      int tj;
-      const double r2 = pair_dvec_list(st, pi, j, st.nb_s[base + k], dx, dy, dz, tj);
-      if (tj < 0) continue;
-      const double rd = sqrt(r2);
-      if (!(rd <= g.nonb_cut)) continue;
-      const DevPair& P = t.pair[ti * g.ntypes + tj];
-      const TR r = (TR)rd;
-      TR Tap, dTap;
-      taper_eval<TR>(g, r, Tap, dTap);
-      TR evdw = 0, CEvd;
-      const TR alpha = (TR)P.alpha, rvdw = (TR)P.r_vdW, D = (TR)P.D;
-      if (vt == 1 || vt == 3) {
-        const TR pr = tpow<TR>(r, p_vdW1), pg = tpow<TR>(TR(1) / (TR)P.gamma_w, p_vdW1), S = pr + pg;
-        const TR fn13 = tpow<TR>(S, p1i);
-        const TR e1 = texp<TR>(alpha * (TR(1) - fn13 / rvdw)), e2 = texp<TR>(TR(0.5) * alpha * (TR(1) - fn13 / rvdw));
-        const TR e0 = D * (e1 - TR(2) * e2);
-        evdw = Tap * e0;
-        const TR dfn13 = fn13 / S * pr / (r * r);
-        CEvd = dTap * e0 - Tap * D * (alpha / rvdw) * (e1 - e2) * dfn13;
-      } else {
-        const TR e1 = texp<TR>(alpha * (TR(1) - r / rvdw)), e2 = texp<TR>(TR(0.5) * alpha * (TR(1) - r / rvdw));
-        const TR e0 = D * (e1 - TR(2) * e2);
-        evdw = Tap * e0;
-        CEvd = dTap * e0 - Tap * D * (alpha / rvdw) * (e1 - e2) / r;
-      }
-      if (vt == 2 || vt == 3) {
-        const TR acore = (TR)P.acore, rcore = (TR)P.rcore;
-        const TR ec = (TR)P.ecore * texp<TR>(acore * (TR(1) - (r / rcore)));
-        evdw += Tap * ec;
-        CEvd += dTap * ec + Tap * (-(acore / rcore) * ec) / r;
-        if (g.lg) {
-          const TR r6 = (r * r * r) * (r * r * r), r5 = r6 / r, lre = (TR)P.lgre, re6 = (lre * lre * lre) * (lre * lre * lre);
-          const TR elg = -((TR)P.lgcij / (r6 + re6));
-          evdw += Tap * elg;
-          CEvd += dTap * elg + Tap * (-TR(6) * elg * r5 / (r6 + re6)) / r;
-        }
-      }
-      const TR qj = (TR)st.q[j];
-      const TR d1 = r * r * r + (TR)P.gamma, d3 = cbrt_ref<TR>(d1);
-      const TR qq = TR(332.06371) * qi * qj;
-      const TR eel = qq * Tap / d3;
-      const TR CE = CEvd + qq * (dTap - Tap * r / d1) / d3;
-      const double ce = (double)CE;
-      gx += -ce * dx; gy += -ce * dy; gz += -ce * dz;
-      ev += 0.5 * (double)evdw;
-      ee += 0.5 * (double)eel;
-      // half of the pair virial  r_ij (x) f_j = -CE dvec (x) dvec
-      const double hv = -0.5 * ce;
-      v0 += hv * dx * dx; v1 += hv * dy * dy; v2 += hv * dz * dz; v3 += hv * dx * dy; v4 += hv * dx * dz; v5 += hv * dy * dz;
+// FP64 terms of one pair (i, j) within nonb_cut at distance r, dvec = x_j - x_i: the reference's formulas in double with its
+// Horner taper in r and its exponents (0.33333333333333 for the shielded Coulomb's cube root as written), pow(x, y) evaluated
+// as exp(y log x) (a few ulp), the type pair's (1 / gamma_w)^p_vdW1 and the reciprocals of r_vdW and rcore from the table,
+// e1 = e2^2 for the two Morse exponentials, and one division for 1 / r
+__device__ inline void nb_pair_d(const Globals& g, const DevPair* __restrict__ row, double qi, double qj, int tj, double r,
+                                 double dx, double dy, double dz, NbAcc& a) {
+  const DevPair& P = row[tj];
+  const double p_vdW1 = g.p_vdW1, p1i = 1.0 / g.p_vdW1;
+  const int vt = g.vdw_type;
+  const double ir = 1.0 / r;
+  const double* c = g.tap;
+  double Tap = c[7] * r + c[6];
+  Tap = Tap * r + c[5]; Tap = Tap * r + c[4]; Tap = Tap * r + c[3]; Tap = Tap * r + c[2]; Tap = Tap * r + c[1]; Tap = Tap * r + c[0];
+  double dTap = 7 * c[7] * r + 6 * c[6];
+  dTap = dTap * r + 5 * c[5]; dTap = dTap * r + 4 * c[4]; dTap = dTap * r + 3 * c[3]; dTap = dTap * r + 2 * c[2];
+  dTap += c[1] * ir;
+  const double alpha = P.alpha, D = P.D, a_r = alpha * P.irvdw;
+  double evdw, CEvd;
+  if (vt == 1 || vt == 3) {
+    const double pr = exp(p_vdW1 * log(r)), S = pr + P.pgw;
+    const double fn13 = exp(p1i * log(S));
+    const double e2 = exp(0.5 * alpha * (1.0 - fn13 * P.irvdw)), e1 = e2 * e2;
+    const double e0 = D * (e1 - 2.0 * e2);
+    evdw = Tap * e0;
+    const double dfn13 = fn13 / S * pr * (ir * ir);
+    CEvd = dTap * e0 - Tap * D * a_r * (e1 - e2) * dfn13;
+  } else {
+    const double e2 = exp(0.5 * alpha * (1.0 - r * P.irvdw)), e1 = e2 * e2;
+    const double e0 = D * (e1 - 2.0 * e2);
+    evdw = Tap * e0;
+    CEvd = dTap * e0 - Tap * D * a_r * (e1 - e2) * ir;
+  }
+  if (vt == 2 || vt == 3) {
+    const double ec = P.ecore * exp(P.acore * (1.0 - r * P.ircore));
+    evdw += Tap * ec;
+    CEvd += dTap * ec + Tap * (-(P.acore * P.ircore) * ec) * ir;
+    if (g.lg) {
+      const double r6 = (r * r * r) * (r * r * r), idn = 1.0 / (r6 + P.re6);
+      const double elg = -P.lgcij * idn;
+      evdw += Tap * elg;
+      CEvd += dTap * elg + Tap * (-6.0 * elg * (r6 * ir) * idn) * ir;
     }
   }
-  gx = wave_sum(gx); gy = wave_sum(gy); gz = wave_sum(gz); ev = wave_sum(ev); ee = wave_sum(ee);
-  v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2); v3 = wave_sum(v3); v4 = wave_sum(v4); v5 = wave_sum(v5);
-  if (lane == 0) {
-    const int n = st.n;
-    st.gnb_x[i] = gx; st.gnb_y[i] = gy; st.gnb_z[i] = gz;
-    st.eatom[T_VDW * n + i] = ev;
-    st.eatom[T_ELE * n + i] = ee;
-    double ep = 0;
-    if (ti >= 0) {
-      const DevSpecies& s = t.sp[ti];
-      const TR q = (TR)st.q[i];
-      ep = (double)(TR(23.02) * ((TR)s.chi * q + ((TR)s.eta / TR(2.)) * (q * q)));
-    }
-    st.eatom[T_POL * n + i] = ep;
-    st.vatom[0 * n + i] = v0; st.vatom[1 * n + i] = v1; st.vatom[2 * n + i] = v2;
-    st.vatom[3 * n + i] = v3; st.vatom[4 * n + i] = v4; st.vatom[5 * n + i] = v5;
-  }
+  const double d1 = r * r * r + P.gamma, id3 = 1.0 / exp(0.33333333333333 * log(d1));
+  const double qq = 332.06371 * qi * qj;
+  const double eel = qq * Tap * id3;
+  const double CE = CEvd + qq * (dTap - Tap * r / d1) * id3;
+  nb_acc(a, CE, evdw, eel, dx, dy, dz);
 }
 
 // Single-precision terms (mixed and FP32 modes): the same pair loop with the hardware base-2 exponential and logarithm
@@ -137,8 +85,6 @@ __global__ void __launch_bounds__(256) k_nonbonded(State st, Globals g, Tables t
 // in float with (1/gamma_w)^p_vdW1 precomputed, and a squared-distance pre-test before the FP64 square root of the cut-off test.
 __device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
-
-struct NbAcc { double gx = 0, gy = 0, gz = 0, ev = 0, ee = 0, v[6] = {0, 0, 0, 0, 0, 0}; };
 
 // one pair (i, j) within nonb_cut at distance rd, dvec = x_j - x_i: this end's gradient, half of the pair energy and virial
 __device__ inline void nb_pair_f(const Globals& g, const NbPairF* __restrict__ row, float qi, float qj, int tj, double rd, double dx,
@@ -192,35 +138,75 @@ __device__ inline void nb_pair_f(const Globals& g, const NbPairF* __restrict__ r
   a.v[0] += hv * dx * dx; a.v[1] += hv * dy * dy; a.v[2] += hv * dz * dz; a.v[3] += hv * dx * dy; a.v[4] += hv * dx * dz; a.v[5] += hv * dy * dz;
 }
 
+// the per-precision parts of the non-bonded kernels below: the charges' type, the pair table row, the pair evaluation and the
+// polarization energy
+template <bool D> struct NbPrec;
+template <> struct NbPrec<false> {
+  using Q = float;
+  using Row = NbPairF;
+  static __device__ const NbPairF* row(const Globals& g, const Tables& t, int ti) { return t.nbf + ti * g.ntypes; }
+  static __device__ void pair(const Globals& g, const NbPairF* row, float qi, float qj, int tj, double rd, double dx, double dy,
+                              double dz, NbAcc& a) {
+    nb_pair_f(g, row, qi, qj, tj, rd, dx, dy, dz, a);
+  }
+  static __device__ double pol(const DevSpecies& sp, double qd) {
+    const float q = (float)qd;
+    return (double)(23.02f * ((float)sp.chi * q + ((float)sp.eta / 2.0f) * (q * q)));
+  }
+};
+template <> struct NbPrec<true> {
+  using Q = double;
+  using Row = DevPair;
+  static __device__ const DevPair* row(const Globals& g, const Tables& t, int ti) { return t.pair + ti * g.ntypes; }
+  static __device__ void pair(const Globals& g, const DevPair* row, double qi, double qj, int tj, double rd, double dx, double dy,
+                              double dz, NbAcc& a) {
+    nb_pair_d(g, row, qi, qj, tj, rd, dx, dy, dz, a);
+  }
+  static __device__ double pol(const DevSpecies& sp, double q) { return 23.02 * (sp.chi * q + (sp.eta / 2.) * (q * q)); }
+};
+
+// wave sums in a fixed butterfly order, then lane 0 stores the atom's gradient, energies and virial
+template <bool D>
+__device__ inline void nb_finish(const State& st, const Tables& t, int i, int ti, int lane, const NbAcc& a) {
+  double gx = wave_sum(a.gx), gy = wave_sum(a.gy), gz = wave_sum(a.gz), ev = wave_sum(a.ev), ee = wave_sum(a.ee);
+  double v0 = wave_sum(a.v[0]), v1 = wave_sum(a.v[1]), v2 = wave_sum(a.v[2]), v3 = wave_sum(a.v[3]), v4 = wave_sum(a.v[4]),
+         v5 = wave_sum(a.v[5]);
+  if (lane == 0) {
+    const int n = st.n;
+    st.gnb_x[i] = gx; st.gnb_y[i] = gy; st.gnb_z[i] = gz;
+    st.eatom[T_VDW * n + i] = ev;
+    st.eatom[T_ELE * n + i] = ee;
+    st.eatom[T_POL * n + i] = ti >= 0 ? NbPrec<D>::pol(t.sp[ti], st.q[i]) : 0.0;
+    st.vatom[0 * n + i] = v0; st.vatom[1 * n + i] = v1; st.vatom[2 * n + i] = v2;
+    st.vatom[3 * n + i] = v3; st.vatom[4 * n + i] = v4; st.vatom[5 * n + i] = v5;
+  }
+}
+
 // Wave per atom over the skin list: the distance test runs on all lanes, pairs within nonb_cut are compacted (list order) into
 // a per-wave LDS ring, and the term arithmetic runs on full waves of 64 pairs.
 constexpr int kNbWaves = 4;
 constexpr int kNbRing = 128;
 
-__global__ void __launch_bounds__(64 * kNbWaves) k_nonbonded_f(State st, Globals g, Tables t) {
-  __shared__ int rj_[kNbWaves][kNbRing], rt_[kNbWaves][kNbRing];
-  __shared__ double rx_[kNbWaves][kNbRing], ry_[kNbWaves][kNbRing], rz_[kNbWaves][kNbRing], rr_[kNbWaves][kNbRing];
-  const int w = threadIdx.x >> 6;
-  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (i >= st.n) return;
-  int* rj = rj_[w]; int* rt = rt_[w];
-  double *rx = rx_[w], *ry = ry_[w], *rz = rz_[w], *rr = rr_[w];
+template <bool D>
+__device__ inline void nb_list_wave(const State& st, const Globals& g, const Tables& t, int i, int* rj, int* rt, double* rx,
+                                    double* ry, double* rz, double* rr) {
+  using Q = typename NbPrec<D>::Q;
   const int lane = lane_id();
   const int ti = st.type[i];
   NbAcc a;
   if (ti >= 0) {
     const size_t base = (size_t)i * st.maxnb;
     const int cnt = min(st.nb_cnt[i], st.maxnb);
-    const float qi = (float)st.q[i];
+    const Q qi = (Q)st.q[i];
     const double4 pi = st.pos[i];
     const double cut = g.nonb_cut, cut2 = cut * cut * (1.0 + 1e-12);
-    const NbPairF* __restrict__ row = t.nbf + ti * g.ntypes;
+    const typename NbPrec<D>::Row* __restrict__ row = NbPrec<D>::row(g, t, ti);
     int head = 0, fill = 0;
     auto batch = [&](int m) {
       if (lane < m) {
         const int e = (head + lane) & (kNbRing - 1);
         const int j = rj[e];
-        nb_pair_f(g, row, qi, (float)st.q[j], rt[e], rr[e], rx[e], ry[e], rz[e], a);
+        NbPrec<D>::pair(g, row, qi, (Q)st.q[j], rt[e], rr[e], rx[e], ry[e], rz[e], a);
       }
     };
     for (int k0 = 0; k0 < cnt; k0 += 64) {
@@ -252,24 +238,25 @@ __global__ void __launch_bounds__(64 * kNbWaves) k_nonbonded_f(State st, Globals
     }
     if (fill > 0) batch(fill);
   }
-  double gx = wave_sum(a.gx), gy = wave_sum(a.gy), gz = wave_sum(a.gz), ev = wave_sum(a.ev), ee = wave_sum(a.ee);
-  double v0 = wave_sum(a.v[0]), v1 = wave_sum(a.v[1]), v2 = wave_sum(a.v[2]), v3 = wave_sum(a.v[3]), v4 = wave_sum(a.v[4]),
-         v5 = wave_sum(a.v[5]);
-  if (lane == 0) {
-    const int n = st.n;
-    st.gnb_x[i] = gx; st.gnb_y[i] = gy; st.gnb_z[i] = gz;
-    st.eatom[T_VDW * n + i] = ev;
-    st.eatom[T_ELE * n + i] = ee;
-    double ep = 0;
-    if (ti >= 0) {
-      const DevSpecies& sp = t.sp[ti];
-      const float q = (float)st.q[i];
-      ep = (double)(23.02f * ((float)sp.chi * q + ((float)sp.eta / 2.0f) * (q * q)));
-    }
-    st.eatom[T_POL * n + i] = ep;
-    st.vatom[0 * n + i] = v0; st.vatom[1 * n + i] = v1; st.vatom[2 * n + i] = v2;
-    st.vatom[3 * n + i] = v3; st.vatom[4 * n + i] = v4; st.vatom[5 * n + i] = v5;
-  }
+  nb_finish<D>(st, t, i, ti, lane, a);
+}
+
+__global__ void __launch_bounds__(64 * kNbWaves) k_nonbonded_f(State st, Globals g, Tables t) {
+  __shared__ int rj_[kNbWaves][kNbRing], rt_[kNbWaves][kNbRing];
+  __shared__ double rx_[kNbWaves][kNbRing], ry_[kNbWaves][kNbRing], rz_[kNbWaves][kNbRing], rr_[kNbWaves][kNbRing];
+  const int w = threadIdx.x >> 6;
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  nb_list_wave<false>(st, g, t, i, rj_[w], rt_[w], rx_[w], ry_[w], rz_[w], rr_[w]);
+}
+
+__global__ void __launch_bounds__(64 * kNbWaves) k_nonbonded_d(State st, Globals g, Tables t) {
+  __shared__ int rj_[kNbWaves][kNbRing], rt_[kNbWaves][kNbRing];
+  __shared__ double rx_[kNbWaves][kNbRing], ry_[kNbWaves][kNbRing], rz_[kNbWaves][kNbRing], rr_[kNbWaves][kNbRing];
+  const int w = threadIdx.x >> 6;
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  nb_list_wave<true>(st, g, t, i, rj_[w], rt_[w], rx_[w], ry_[w], rz_[w], rr_[w]);
 }
 
 // pair_dvec_list for a charge-matrix entry: the same operands in the same order, the shift vector found through the entry's key
@@ -287,68 +274,66 @@ __device__ inline double pair_dvec_key(const State& st, const double4 pi, const 
 }
 #pragma clang fp contract(on)
 
-// The single-precision terms over the charge-matrix rows: the rows hold, in skin-list order, exactly the pairs within the matrix
+// The non-bonded terms over the charge-matrix rows: the rows hold, in skin-list order, exactly the pairs within the matrix
 // cutoff, so every lane carries a pair (no distance pre-pass, no compaction) and a row costs its column and key (4 bytes per
 // pair with 16-bit columns) instead of the 8-byte skin entries of all pairs out to the list radius. Lane l takes entries l,
 // l + 64, ..., the assignment of the skin-list kernel's compacted batches, so the sums are the same.
-// (5 waves per SIMD: 96 VGPRs without spills; the compiler's choice, 106 VGPRs and 4 waves, is slower)
-template <class TC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_nonbonded_rows(State st, Globals g, Tables t, const TC* __restrict__ col,
-                                                        const unsigned short* __restrict__ key, const int* __restrict__ rcnt) {
-  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (i >= st.n) return;
+template <bool D, class TC>
+__device__ inline void nb_rows_wave(const State& st, const Globals& g, const Tables& t, const TC* __restrict__ col,
+                                    const unsigned short* __restrict__ key, const int* __restrict__ rcnt, int i) {
+  using Q = typename NbPrec<D>::Q;
   const int lane = lane_id();
   const int ti = st.type[i];
   NbAcc a;
   if (ti >= 0) {
     const size_t base = (size_t)i * st.maxnb;
     const int cnt = rcnt[i];
-    const float qi = (float)st.q[i];
+    const Q qi = (Q)st.q[i];
     const double4 pi = st.pos[i];
     const double cut = g.nonb_cut, cut2 = cut * cut * (1.0 + 1e-12);
-    const NbPairF* __restrict__ row = t.nbf + ti * g.ntypes;
+    const typename NbPrec<D>::Row* __restrict__ row = NbPrec<D>::row(g, t, ti);
     // two-deep software pipeline as in the matrix build: columns/keys two strides ahead, positions and charges one ahead
     // (lanes past the row's end carry the row atom itself and key 0, valid indices never used)
     int jB = i, kyB = 0, jC = i, kyC = 0;
     if (lane < cnt) { jC = (int)col[base + lane]; kyC = key[base + lane]; }
     double4 pC = st.pos[jC];
-    float qC = (float)st.q[jC];
+    Q qC = (Q)st.q[jC];
     if (lane + 64 < cnt) { jB = (int)col[base + lane + 64]; kyB = key[base + lane + 64]; }
     for (int k = lane; k < cnt; k += 64) {
       const double4 pj = pC;
-      const float qj = qC;
+      const Q qj = qC;
       const int ky = kyC;
       jC = jB; kyC = kyB;
       pC = st.pos[jC];
-      qC = (float)st.q[jC];
+      qC = (Q)st.q[jC];
       jB = i; kyB = 0;
       if (k + 128 < cnt) { jB = (int)col[base + k + 128]; kyB = key[base + k + 128]; }
       double dx, dy, dz;
       const double r2 = pair_dvec_key(st, pi, pj, ky, dx, dy, dz);
       if (r2 <= cut2) {
         const double rd = sqrt(r2);
-        if (rd <= cut) nb_pair_f(g, row, qi, qj, pos_type(pj.w), rd, dx, dy, dz, a);
+        if (rd <= cut) NbPrec<D>::pair(g, row, qi, qj, pos_type(pj.w), rd, dx, dy, dz, a);
       }
     }
   }
-  double gx = wave_sum(a.gx), gy = wave_sum(a.gy), gz = wave_sum(a.gz), ev = wave_sum(a.ev), ee = wave_sum(a.ee);
-  double v0 = wave_sum(a.v[0]), v1 = wave_sum(a.v[1]), v2 = wave_sum(a.v[2]), v3 = wave_sum(a.v[3]), v4 = wave_sum(a.v[4]),
-         v5 = wave_sum(a.v[5]);
-  if (lane == 0) {
-    const int n = st.n;
-    st.gnb_x[i] = gx; st.gnb_y[i] = gy; st.gnb_z[i] = gz;
-    st.eatom[T_VDW * n + i] = ev;
-    st.eatom[T_ELE * n + i] = ee;
-    double ep = 0;
-    if (ti >= 0) {
-      const DevSpecies& sp = t.sp[ti];
-      const float q = (float)st.q[i];
-      ep = (double)(23.02f * ((float)sp.chi * q + ((float)sp.eta / 2.0f) * (q * q)));
-    }
-    st.eatom[T_POL * n + i] = ep;
-    st.vatom[0 * n + i] = v0; st.vatom[1 * n + i] = v1; st.vatom[2 * n + i] = v2;
-    st.vatom[3 * n + i] = v3; st.vatom[4 * n + i] = v4; st.vatom[5 * n + i] = v5;
-  }
+  nb_finish<D>(st, t, i, ti, lane, a);
+}
+
+// (5 waves per SIMD: 96 VGPRs without spills; the compiler's choice, 106 VGPRs and 4 waves, is slower)
+template <class TC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_nonbonded_rows(State st, Globals g, Tables t, const TC* __restrict__ col,
+                                                        const unsigned short* __restrict__ key, const int* __restrict__ rcnt) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  nb_rows_wave<false, TC>(st, g, t, col, key, rcnt, i);
+}
+
+template <class TC>
+__global__ void __launch_bounds__(256) k_nonbonded_rows_d(State st, Globals g, Tables t, const TC* __restrict__ col,
+                                                          const unsigned short* __restrict__ key, const int* __restrict__ rcnt) {
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= st.n) return;
+  nb_rows_wave<true, TC>(st, g, t, col, key, rcnt, i);
 }
 
 constexpr int kQbWaves = 4, kQbRing = 128;
@@ -538,14 +523,19 @@ void launch_census_nb(const State& st, const Globals& g, hipStream_t s) {
 void launch_nonbonded(Precision p, const State& st, const Cell& c, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * 64, 256)), blk(256);
   (void)c;
-  if (p == Precision::FP64) hipLaunchKernelGGL(k_nonbonded<double>, grid, blk, 0, s, st, g, t);
+  if (p == Precision::FP64) hipLaunchKernelGGL(k_nonbonded_d, grid, blk, 0, s, st, g, t);
   else hipLaunchKernelGGL(k_nonbonded_f, grid, blk, 0, s, st, g, t);
 }
 
-void launch_nonbonded_rows(const State& st, const Globals& g, const Tables& t, const QeqBuffers& qb, hipStream_t s) {
+void launch_nonbonded_rows(Precision p, const State& st, const Globals& g, const Tables& t, const QeqBuffers& qb, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * 64, 256)), blk(256);
-  if (qb.narrow) hipLaunchKernelGGL(k_nonbonded_rows<unsigned short>, grid, blk, 0, s, st, g, t, qb.col16, qb.key, qb.cnt);
-  else hipLaunchKernelGGL(k_nonbonded_rows<int>, grid, blk, 0, s, st, g, t, qb.col, qb.key, qb.cnt);
+  if (p == Precision::FP64) {
+    if (qb.narrow) hipLaunchKernelGGL(k_nonbonded_rows_d<unsigned short>, grid, blk, 0, s, st, g, t, qb.col16, qb.key, qb.cnt);
+    else hipLaunchKernelGGL(k_nonbonded_rows_d<int>, grid, blk, 0, s, st, g, t, qb.col, qb.key, qb.cnt);
+  } else {
+    if (qb.narrow) hipLaunchKernelGGL(k_nonbonded_rows<unsigned short>, grid, blk, 0, s, st, g, t, qb.col16, qb.key, qb.cnt);
+    else hipLaunchKernelGGL(k_nonbonded_rows<int>, grid, blk, 0, s, st, g, t, qb.col, qb.key, qb.cnt);
+  }
 }
 
 void launch_qeq_build(const State& st, const Cell& c, const Globals& g, const Tables& t, QeqBuffers& qb, int fuse, hipStream_t s) {

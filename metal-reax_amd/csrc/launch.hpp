@@ -125,9 +125,9 @@ void launch_qeq_guess(const State& st, QeqBuffers& qb, int use_hist, hipStream_t
 void launch_qeq_init_fused(const State& st, QeqBuffers& qb, int imax, double tol, hipStream_t s);
 // census of the non-bonded cut-off decision (st.cen, category kCenNonbonded)
 void launch_census_nb(const State& st, const Globals& g, hipStream_t s);
-// Non-bonded terms over the charge-matrix rows instead of the skin list (mixed / FP32 modes): valid while the rows belong to the
-// current positions, carry keys, and the matrix cutoff covers nonb_cut.
-void launch_nonbonded_rows(const State& st, const Globals& g, const Tables& t, const QeqBuffers& qb, hipStream_t s);
+// Non-bonded terms over the charge-matrix rows instead of the skin list: valid while the rows belong to the current positions,
+// carry keys, and the matrix cutoff covers nonb_cut.
+void launch_nonbonded_rows(Precision p, const State& st, const Globals& g, const Tables& t, const QeqBuffers& qb, hipStream_t s);
 void launch_qeq_matvec1(const State& st, const QeqBuffers& qb, const double* x, double* y, hipStream_t s);
 void launch_qeq_init(const State& st, QeqBuffers& qb, int imax, double tol, int use_hist, hipStream_t s);
 void launch_qeq_mvdir(const State& st, QeqBuffers& qb, int it, int imax, double tol, hipStream_t s);   // beta, d, q = A d, d.q partials

@@ -3,7 +3,8 @@
 A scratch-resident local turns every access into a memory round trip; it happened once silently when a helper selected between a
 reference into global memory and a reference to a local (k_qeq_build went from 117 to 185 us per 24k-atom step). The device
 assembly of every source is generated here (hipcc, gfx950, the build's flags) and each kernel's private segment size read
-from its descriptor. Allowed: a reserved but unused frame (FP32-term per-atom terms), spills of the FP64-mode H-bond donor, and
+from its descriptor; a frame that no scratch or buffer instruction of the kernel touches is reserved, not used, and passes.
+Allowed: a reserved frame (FP32-term per-atom terms), spills of the FP64-mode H-bond donor, and
 the FP32-term torsion kept at 3 waves per SIMD, which spills one 8-byte value that is reloaded once per outer partner, outside
 the inner loop (24 us per step faster than the spill-free 2-wave build), its census-recording variant, and the per-atom
 variants (only a force call that asks for per-atom tallies launches them).
@@ -38,6 +39,13 @@ def _asm(src, out):
     return open(out).read()
 
 
+def _touches_scratch(asm, name):
+    """Whether the kernel's code (from its label to its end marker) has a scratch or buffer load or store."""
+    i = asm.index("\n" + name + ":") + 1
+    j = asm.index(".Lfunc_end", i)
+    return re.search(r"^\s*(scratch|buffer)_(load|store)", asm[i:j], re.M) is not None
+
+
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
 def test_no_scratch_in_engine_kernels(tmp_path):
     srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
@@ -52,6 +60,8 @@ def test_no_scratch_in_engine_kernels(tmp_path):
                 continue   # library kernels (hipCUB radix sort)
             seen += 1
             size = int(re.search(r"amdhsa_private_segment_fixed_size (\d+)", body).group(1))
+            if size and not _touches_scratch(asm, name):
+                size = 0   # a frame reserved by the compiler that no instruction of the kernel reads or writes
             short = re.sub(r"(ENS_|EvNS_).*", "", name.replace("_ZN3rxa12_GLOBAL__N_1", ""))
             short = re.sub(r"^\d+", "", short)
             if size and not PERATOM.match(short) and (short not in ALLOWED or size > MAX_ALLOWED_BYTES):

@@ -1,4 +1,4 @@
-"""The single-precision non-bonded pass over the charge-matrix rows against the same pass over the skin list.
+"""The non-bonded pass over the charge-matrix rows against the same pass over the skin list, in every precision.
 
 Within a step whose charge matrix belongs to the current positions, the rows list, in skin-list order, exactly the pairs within
 the matrix cutoff (10 A = nonb_cut here), with a 16-bit key that reproduces each pair's displacement (canonical end, shift
@@ -29,7 +29,7 @@ def _engine(ff, w, prec, rows):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "fp32"])
 @pytest.mark.parametrize("which", ["water648", "gas1053"])
 def test_rows_pass_equals_skin_pass(ffield_text, prec, which):
     w = systems.water_box(6, seed=1) if which == "water648" else systems.ch4_o2_gas(1053, density=0.3, seed=5)
