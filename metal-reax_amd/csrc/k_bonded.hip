@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(256) k_atom_terms(State st, Globals g, Tables 
 // PA: also the reference's per-atom virial split (bonded.cpp:462-466): each angle's (x_i - x_j) (x) F_i + (x_k - x_j) (x) F_k,
 // a third to each of i, j and k (the ends through their slots).
 template <class TR, class TS, int G, bool PA>
-__global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) {
+__device__ inline void valence_body(const State& st, const Globals& g, const Tables& t) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = gid / G, p = gid % G, n = st.n;
   const bool live = j < n;
@@ -754,6 +754,18 @@ __global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) 
     n_ang = gsum_int<G>(n_ang);
     if (live && p == 0) { st.cen[kCenAngles * n + j] = n_ang; st.cen[kCenSbo * n + j] = sbo_region; }
   }
+}
+
+// FP32 terms at the compiler's choice of registers; FP64 terms held to 256 VGPRs, 2 waves per SIMD (the compiler's choice, 260
+// VGPRs, leaves one): the bound spills two 8-byte values before the partner loop, reloaded once per partner k outside the
+// angle arithmetic, and gains 2-4% of the FP64 step
+template <class TS, int G, bool PA>
+__global__ void __launch_bounds__(256) k_valence(State st, Globals g, Tables t) {
+  valence_body<float, TS, G, PA>(st, g, t);
+}
+template <int G, bool PA>
+__global__ void __attribute__((amdgpu_waves_per_eu(2))) __launch_bounds__(256) k_valence_d(State st, Globals g, Tables t) {
+  valence_body<double, double, G, PA>(st, g, t);
 }
 
 template <class TR>
@@ -1405,7 +1417,8 @@ void launch_bo_correct(Precision p, const State& st, const Globals& g, const Tab
 template <class TR, class TS, int G, bool PA>
 void launch_terms_g(const State& st, const Globals& g, const Tables& t, hipStream_t s) {
   const dim3 grid(nblk((long long)st.n * G, 256)), blk(256), wgrid(nblk(st.n, 4));
-  hipLaunchKernelGGL((k_valence<TR, TS, G, PA>), grid, blk, 0, s, st, g, t);
+  if (sizeof(TR) == 4) hipLaunchKernelGGL((k_valence<TS, G, PA>), grid, blk, 0, s, st, g, t);
+  else hipLaunchKernelGGL((k_valence_d<G, PA>), grid, blk, 0, s, st, g, t);
   if (sizeof(TR) == 4) {
     if (st.census) hipLaunchKernelGGL((k_torsion_f<TS, G, true, PA>), grid, blk, 0, s, st, g, t);
     else hipLaunchKernelGGL((k_torsion_f<TS, G, false, PA>), grid, blk, 0, s, st, g, t);
