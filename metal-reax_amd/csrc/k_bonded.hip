@@ -87,13 +87,19 @@ template <class T> __device__ inline V3<T> cross3(const V3<T>& a, const V3<T>& b
 // theta = atan2(|a x b|, a.b) and sin = |a x b| / (|a||b|) instead: acosf of a cosine within an ulp of -1 or 1 loses half the
 // digits, and sinf(acosf(-1)) is -8.7e-8 (the float nearest pi lies above pi), a negative sine that flips the sign of the
 // torsion's 1/tan guard. The cosine stays the clamped dot product in both.
-template <class T>
+// kTheta = false (the torsion, which needs only the cosine and the sine): FP64 leaves theta unset and takes
+// sin = sqrt((1 - c)(1 + c)), the reference's sin(acos(c)) to within a few ulp without the two trigonometric calls (1 - c and
+// 1 + c are exact where they are small, so the sine keeps its relative accuracy near 0 and pi); FP32 is unchanged.
+template <class T, bool kTheta = true>
 __device__ inline void angle_and_dcos(const V3<T>& a, T da, const V3<T>& b, T db, T& theta, T& cost, T& sint, V3<T>& dca, V3<T>& dcj,
                                       V3<T>& dcb) {
   cost = dot3(a, b) / (da * db);
   if (cost > T(1)) cost = T(1);
   if (cost < T(-1)) cost = T(-1);
-  if (sizeof(T) == 8) {
+  if (sizeof(T) == 8 && !kTheta) {
+    theta = T(0);
+    sint = rsqrt_<T>((T(1) - cost) * (T(1) + cost));
+  } else if (sizeof(T) == 8) {
     theta = racos<T>(cost);
     sint = rsin<T>(theta);
   } else {
@@ -846,8 +852,8 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
         const V3<TR> dji{(TR)st.b_dx[qi], (TR)st.b_dy[qi], (TR)st.b_dz[qi]};
         TR thijk, cijk_, sijk;
         V3<TR> dci, dcj, dck;   // d cos(theta_ijk) / d x_i, x_j, x_k
-        angle_and_dcos<TR>(dji, rij, djk, rjk, thijk, cijk_, sijk, dci, dcj, dck);
-        const TR cijk = sizeof(TR) == 8 ? rcos<TR>(thijk) : cijk_, tijk = tan_inv<TR>(sijk, cijk);
+        angle_and_dcos<TR, false>(dji, rij, djk, rjk, thijk, cijk_, sijk, dci, dcj, dck);
+        const TR cijk = cijk_, tijk = tan_inv<TR>(sijk, cijk);
         const TR et2ij = rexp<TR>(-p_tor2 * BOA_ij), ec2ij = rexp<TR>(-p_cot2 * ((BOA_ij - TR(1.5)) * (BOA_ij - TR(1.5))));
         for (int pl = 0; pl < nbk; ++pl) {
           if (pl == symk) continue;
@@ -866,8 +872,8 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
           const V3<TR> dkl{(TR)st.b_dx[ql], (TR)st.b_dy[ql], (TR)st.b_dz[ql]};
           TR thjkl, cjkl_, sjkl;
           V3<TR> ecj, eck, ecl;   // d cos(theta_jkl) / d x_j, x_k, x_l
-          angle_and_dcos<TR>(dkj, rjk, dkl, rkl, thjkl, cjkl_, sjkl, ecj, eck, ecl);
-          const TR cjkl = sizeof(TR) == 8 ? rcos<TR>(thjkl) : cjkl_, tjkl = tan_inv<TR>(sjkl, cjkl);
+          angle_and_dcos<TR, false>(dkj, rjk, dkl, rkl, thjkl, cjkl_, sjkl, ecj, eck, ecl);
+          const TR cjkl = cjkl_, tjkl = tan_inv<TR>(sjkl, cjkl);
           // x_i - x_l in j's frame
           const V3<TR> dli{dji.x - djk.x - dkl.x, dji.y - djk.y - dkl.y, dji.z - djk.z - dkl.z};
           const TR rli = rsqrt_<TR>(dot3(dli, dli));
@@ -878,7 +884,6 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
           const V3<TR>& bij = dji;
           const TR ucos = -dot3(bij, djk) * dot3(djk, dkl) + (rjk * rjk) * dot3(bij, dkl);
           const TR usin = -rjk * dot3(bij, cross3(djk, dkl));
-          const TR omega = ratan2<TR>(usin, ucos);
           const TR htra = rij + c1 * (rkl * c2 - rjk), htrb = rjk - rij * c1 - rkl * c2;
           const TR hthd = rij * s1 * (rjk - rkl * c2), hthe = rkl * s2 * (rjk - rij * c1);
           const TR hnra = rkl * s1 * s2, hnhd = rij * rkl * c1 * s2, hnhe = rij * rkl * s1 * c2;
@@ -893,7 +898,19 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
           const V3<TR> doi{sc * (ga * bij.x - dli.x + gd * dci.x), sc * (ga * bij.y - dli.y + gd * dci.y), sc * (ga * bij.z - dli.z + gd * dci.z)};
           const V3<TR> doj{sc * (-ga * bij.x - htrb / rjk * djk.x + gd * dcj.x + ge * ecj.x), sc * (-ga * bij.y - htrb / rjk * djk.y + gd * dcj.y + ge * ecj.y),
                            sc * (-ga * bij.z - htrb / rjk * djk.z + gd * dcj.z + ge * ecj.z)};
-          const TR cw = rcos<TR>(omega), c2w = rcos<TR>(TR(2) * omega), c3w = rcos<TR>(TR(3) * omega);
+          TR cw, c2w, c3w;
+          if (sizeof(TR) == 8) {
+            // cos(omega) for omega = atan2(usin, ucos), and cos(2 omega), cos(3 omega) by the multiple-angle identities (the
+            // reference's cos(atan2(..)), cos(2 omega), cos(3 omega) to within a few ulp, without the four trigonometric calls;
+            // atan2(0, 0) = 0 gives cos = 1)
+            const TR h2 = ucos * ucos + usin * usin;
+            cw = h2 > TR(0) ? ucos / rsqrt_<TR>(h2) : TR(1);
+            c2w = TR(2) * (cw * cw) - TR(1);
+            c3w = (TR(4) * (cw * cw) - TR(3)) * cw;
+          } else {
+            const TR omega = ratan2<TR>(usin, ucos);
+            cw = rcos<TR>(omega); c2w = rcos<TR>(TR(2) * omega); c3w = rcos<TR>(TR(3) * omega);
+          }
           const TR x = TR(2) - BOpijk - f11;
           const TR et1 = rexp<TR>((TR)P.p_tor1 * (x * x));
           const TR et2kl = rexp<TR>(-p_tor2 * BOA_kl), ec2kl = rexp<TR>(-p_cot2 * ((BOA_kl - TR(1.5)) * (BOA_kl - TR(1.5))));
@@ -998,14 +1015,15 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   }
 }
 
-// FP32 terms at 3 waves per SIMD (168 VGPRs); FP64 at the compiler's choice. kCen: the variant that also records the torsion
+// FP32 terms at 3 waves per SIMD (168 VGPRs); FP64 at 2 (the compiler's choice, 258 VGPRs, leaves one; the bound spills two
+// 8-byte values, reloaded once per outer partner i, outside the loop over l). kCen: the variant that also records the torsion
 // census (launched only for a force call that asks for it, so the counter costs the ordinary step nothing)
 template <class TS, int G, bool kCen, bool PA>
 __global__ void __attribute__((amdgpu_waves_per_eu(3))) __launch_bounds__(256) k_torsion_f(State st, Globals g, Tables t) {
   torsion_body<float, TS, G, kCen, PA>(st, g, t);
 }
 template <int G, bool kCen, bool PA>
-__global__ void __launch_bounds__(256) k_torsion_d(State st, Globals g, Tables t) {
+__global__ void __attribute__((amdgpu_waves_per_eu(2))) __launch_bounds__(256) k_torsion_d(State st, Globals g, Tables t) {
   torsion_body<double, double, G, kCen, PA>(st, g, t);
 }
 

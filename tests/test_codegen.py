@@ -4,8 +4,8 @@ A scratch-resident local turns every access into a memory round trip; it happene
 reference into global memory and a reference to a local (k_qeq_build went from 117 to 185 us per 24k-atom step). The device
 assembly of every source is generated here (hipcc, gfx950, the build's flags) and each kernel's private segment size read
 from its descriptor; a frame that no scratch or buffer instruction of the kernel touches is reserved, not used, and passes.
-Allowed: a reserved frame (FP32-term per-atom terms), spills of the FP64-mode H-bond donor, the FP64 valence kernel held to 2
-waves per SIMD (two 8-byte values reloaded once per partner, outside the angle arithmetic), and
+Allowed: a reserved frame (FP32-term per-atom terms), spills of the FP64-mode H-bond donor, the FP64 valence and torsion
+kernels held to 2 waves per SIMD (two 8-byte values each, reloaded once per outer partner, outside the innermost loop), and
 the FP32-term torsion kept at 3 waves per SIMD, which spills one 8-byte value that is reloaded once per outer partner, outside
 the inner loop (24 us per step faster than the spill-free 2-wave build), its census-recording variant, and the per-atom
 variants (only a force call that asks for per-atom tallies launches them).
@@ -27,7 +27,8 @@ HIPCC = "/opt/rocm/bin/hipcc"
 ALLOWED = {"k_atom_termsIfdE", "k_atom_termsIffE", "k_hbond_donorIddLb0EE"} | \
     {"k_torsion_fIfLi%dELb0ELb0EE" % g for g in (8, 16, 32, 64)} | \
     {"k_torsion_fI%sLi%dELb%dELb0EE" % (ts, g, c) for ts in "df" for g in (4, 8, 16, 32, 64) for c in (0, 1) if ts == "d" or c == 1} | \
-    {"k_valence_dILi%dELb0EE" % g for g in (4, 8, 16, 32, 64)}
+    {"k_valence_dILi%dELb0EE" % g for g in (4, 8, 16, 32, 64)} | \
+    {"k_torsion_dILi%dELb%dELb0EE" % (g, c) for g in (4, 8, 16, 32, 64) for c in (0, 1)}
 MAX_ALLOWED_BYTES = 72
 # the per-atom variants (last template flag set: the reference-split virial shares), launched only by a force call that asks
 # for per-atom tallies, may use scratch
