@@ -898,19 +898,12 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
           const V3<TR> doi{sc * (ga * bij.x - dli.x + gd * dci.x), sc * (ga * bij.y - dli.y + gd * dci.y), sc * (ga * bij.z - dli.z + gd * dci.z)};
           const V3<TR> doj{sc * (-ga * bij.x - htrb / rjk * djk.x + gd * dcj.x + ge * ecj.x), sc * (-ga * bij.y - htrb / rjk * djk.y + gd * dcj.y + ge * ecj.y),
                            sc * (-ga * bij.z - htrb / rjk * djk.z + gd * dcj.z + ge * ecj.z)};
-          TR cw, c2w, c3w;
-          if (sizeof(TR) == 8) {
-            // cos(omega) for omega = atan2(usin, ucos), and cos(2 omega), cos(3 omega) by the multiple-angle identities (the
-            // reference's cos(atan2(..)), cos(2 omega), cos(3 omega) to within a few ulp, without the four trigonometric calls;
-            // atan2(0, 0) = 0 gives cos = 1)
-            const TR h2 = ucos * ucos + usin * usin;
-            cw = h2 > TR(0) ? ucos / rsqrt_<TR>(h2) : TR(1);
-            c2w = TR(2) * (cw * cw) - TR(1);
-            c3w = (TR(4) * (cw * cw) - TR(3)) * cw;
-          } else {
-            const TR omega = ratan2<TR>(usin, ucos);
-            cw = rcos<TR>(omega); c2w = rcos<TR>(TR(2) * omega); c3w = rcos<TR>(TR(3) * omega);
-          }
+          // cos(omega) for omega = atan2(usin, ucos), and cos(2 omega), cos(3 omega) by the multiple-angle identities (the
+          // reference's cos(atan2(..)), cos(2 omega), cos(3 omega) to within a few ulp, without the four trigonometric calls;
+          // atan2(0, 0) = 0 gives cos = 1)
+          const TR h2 = ucos * ucos + usin * usin;
+          const TR cw = h2 > TR(0) ? ucos / rsqrt_<TR>(h2) : TR(1);
+          const TR c2w = TR(2) * (cw * cw) - TR(1), c3w = (TR(4) * (cw * cw) - TR(3)) * cw;
           const TR x = TR(2) - BOpijk - f11;
           const TR et1 = rexp<TR>((TR)P.p_tor1 * (x * x));
           const TR et2kl = rexp<TR>(-p_tor2 * BOA_kl), ec2kl = rexp<TR>(-p_cot2 * ((BOA_kl - TR(1.5)) * (BOA_kl - TR(1.5))));
@@ -1015,7 +1008,7 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   }
 }
 
-// FP32 terms at 3 waves per SIMD (168 VGPRs); FP64 at 2 (the compiler's choice, 258 VGPRs, leaves one; the bound spills two
+// FP32 terms at 3 waves per SIMD (at most 168 VGPRs, no spill); FP64 at 2 (the compiler's choice, 258 VGPRs, leaves one; the bound spills two
 // 8-byte values, reloaded once per outer partner i, outside the loop over l). kCen: the variant that also records the torsion
 // census (launched only for a force call that asks for it, so the counter costs the ordinary step nothing)
 template <class TS, int G, bool kCen, bool PA>

@@ -6,9 +6,7 @@ assembly of every source is generated here (hipcc, gfx950, the build's flags) an
 from its descriptor; a frame that no scratch or buffer instruction of the kernel touches is reserved, not used, and passes.
 Allowed: a reserved frame (FP32-term per-atom terms), spills of the FP64-mode H-bond donor, the FP64 valence and torsion
 kernels held to 2 waves per SIMD (two 8-byte values each, reloaded once per outer partner, outside the innermost loop), and
-the FP32-term torsion kept at 3 waves per SIMD, which spills one 8-byte value that is reloaded once per outer partner, outside
-the inner loop (24 us per step faster than the spill-free 2-wave build), its census-recording variant, and the per-atom
-variants (only a force call that asks for per-atom tallies launches them).
+the per-atom variants (only a force call that asks for per-atom tallies launches them).
 """
 import concurrent.futures
 import os
@@ -21,12 +19,8 @@ import pytest
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(REPO, "metal-reax_amd", "csrc")
 HIPCC = "/opt/rocm/bin/hipcc"
-# FP32-term torsion (template arguments: the gate type, d in mixed mode and f in FP32 mode, the group size, the census flag): the
-# ordinary FP32-mode variants (Lb0) of groups of 8..64 lanes, every ordinary mixed-mode variant (its FP64 gate product takes the
-# registers), and every census-recording variant (Lb1, launched only by a force call that asks for the census)
+# FP64 valence and torsion at 2 waves per SIMD (the torsion's census-recording variant Lb1 too)
 ALLOWED = {"k_atom_termsIfdE", "k_atom_termsIffE", "k_hbond_donorIddLb0EE"} | \
-    {"k_torsion_fIfLi%dELb0ELb0EE" % g for g in (8, 16, 32, 64)} | \
-    {"k_torsion_fI%sLi%dELb%dELb0EE" % (ts, g, c) for ts in "df" for g in (4, 8, 16, 32, 64) for c in (0, 1) if ts == "d" or c == 1} | \
     {"k_valence_dILi%dELb0EE" % g for g in (4, 8, 16, 32, 64)} | \
     {"k_torsion_dILi%dELb%dELb0EE" % (g, c) for g in (4, 8, 16, 32, 64) for c in (0, 1)}
 MAX_ALLOWED_BYTES = 72
