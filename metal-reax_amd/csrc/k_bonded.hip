@@ -1008,9 +1008,9 @@ __device__ __forceinline__ void torsion_body(State st, Globals g, Tables t) {
   }
 }
 
-// FP32 terms at 3 waves per SIMD (at most 168 VGPRs, no spill); FP64 at 2 (the compiler's choice, 258 VGPRs, leaves one; the bound spills two
-// 8-byte values, reloaded once per outer partner i, outside the loop over l). kCen: the variant that also records the torsion
-// census (launched only for a force call that asks for it, so the counter costs the ordinary step nothing)
+// FP32 terms at 3 waves per SIMD (at most 168 VGPRs, no spill); FP64 at 2 (the compiler's choice, 258 VGPRs, leaves one; the
+// bound spills two 8-byte values, reloaded once per outer partner i, outside the loop over l). kCen: the variant that also
+// records the torsion census (launched only for a force call that asks for it, so the counter costs the ordinary step nothing)
 template <class TS, int G, bool kCen, bool PA>
 __global__ void __attribute__((amdgpu_waves_per_eu(3))) __launch_bounds__(256) k_torsion_f(State st, Globals g, Tables t) {
   torsion_body<float, TS, G, kCen, PA>(st, g, t);
