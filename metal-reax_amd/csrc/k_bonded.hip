@@ -1023,10 +1023,11 @@ __global__ void __attribute__((amdgpu_waves_per_eu(2))) __launch_bounds__(256) k
 // ================================================================================================================
 template <class TR>
 __device__ inline void cos_and_dcos(const V3<TR>& a, TR da, const V3<TR>& b, TR db, TR& cost, V3<TR>& dca, V3<TR>& dcj, V3<TR>& dcb) {
-  cost = dot3(a, b) / (da * db);
+  const TR inv = TR(1) / (da * db), ab = dot3(a, b);   // one division for the cosine and its derivatives
+  cost = ab * inv;
   if (cost > TR(1)) cost = TR(1);
   if (cost < TR(-1)) cost = TR(-1);
-  const TR da2 = da * da, db2 = db * db, inv = TR(1) / (da * db), inv3 = inv * inv * inv, cd = dot3(a, b) * inv3;
+  const TR da2 = da * da, db2 = db * db, inv3 = inv * inv * inv, cd = ab * inv3;
   dca = {b.x * inv - cd * db2 * a.x, b.y * inv - cd * db2 * a.y, b.z * inv - cd * db2 * a.z};
   dcj = {-(b.x + a.x) * inv + cd * (db2 * a.x + da2 * b.x), -(b.y + a.y) * inv + cd * (db2 * a.y + da2 * b.y),
          -(b.z + a.z) * inv + cd * (db2 * a.z + da2 * b.z)};
@@ -1034,19 +1035,21 @@ __device__ inline void cos_and_dcos(const V3<TR>& a, TR da, const V3<TR>& b, TR 
 }
 
 // H-bond triple (i, j, k) [reference terms of bonded.cpp:654-715]: sin^4(theta/2) = ((1 - cos theta) / 2)^2, so no acos is
-// needed; eh2 = exp(-p2 BO_ij) is the caller's (it depends on the bond only)
+// needed; eh2 = exp(-p2 BO_ij) is the caller's (it depends on the bond only). C3 is the reference's CH3 / r_jk. Divisions by
+// r_jk and r0 are products with their reciprocals (two divisions per triple instead of five)
 template <class TR>
 __device__ inline void hbond_term(const DevHBond& P, TR eh2, const V3<TR>& dji, TR rij, const V3<TR>& djk, TR rjk, TR& e, TR& CH1, TR& CH2,
-                                  TR& CH3, V3<TR>& dti, V3<TR>& dtj, V3<TR>& dtk) {
+                                  TR& C3, V3<TR>& dti, V3<TR>& dtj, V3<TR>& dtk) {
   TR ct;
   cos_and_dcos<TR>(dji, rij, djk, rjk, ct, dti, dtj, dtk);
   const TR cx = TR(1) - ct, hs = TR(0.5) * cx, s4 = hs * hs;
   const TR r0 = (TR)P.r0, p1 = (TR)P.p1, p2 = (TR)P.p2, p3 = (TR)P.p3;
-  const TR eh3 = rexp<TR>(-p3 * (r0 / rjk + rjk / r0 - TR(2)));
+  const TR ir = TR(1) / rjk, ir0 = TR(1) / r0;
+  const TR eh3 = rexp<TR>(-p3 * (r0 * ir + rjk * ir0 - TR(2)));
   e = p1 * (TR(1) - eh2) * eh3 * s4;
   CH1 = p1 * p2 * eh2 * eh3 * s4;
   CH2 = -p1 / TR(2) * (TR(1) - eh2) * eh3 * cx;
-  CH3 = -p3 * (-r0 / (rjk * rjk) + TR(1) / r0) * e;
+  C3 = (-p3 * (-r0 * (ir * ir) + ir0) * e) * ir;
 }
 
 // Donor side: one wave per donor hydrogen j. A first pass over j's H-bond candidates (chunks of 64) computes each distance once,
@@ -1128,14 +1131,13 @@ __global__ void __launch_bounds__(64 * kHbWaves) k_hbond_donor(State st, Globals
         if (P.r0 > 0.0) {
           ++n_hb;
           const TR e2 = rexp<TR>(-(TR)P.p2 * (TR)st.bo[q]);
-          TR e, CH1, CH2, CH3;
+          TR e, CH1, CH2, c3;
           V3<TR> dti, dtj, dtk;
-          hbond_term<TR>(P, e2, dji, rij, djk, rjk, e, CH1, CH2, CH3, dti, dtj, dtk);
+          hbond_term<TR>(P, e2, dji, rij, djk, rjk, e, CH1, CH2, c3, dti, dtj, dtk);
           e_hb += (double)e;
           et = (double)e;
           c1 = (double)CH1;
           fx = (double)(CH2 * dti.x); fy = (double)(CH2 * dti.y); fz = (double)(CH2 * dti.z);
-          const TR c3 = CH3 / rjk;
           fjx += (double)(CH2 * dtj.x - c3 * djk.x);
           fjy += (double)(CH2 * dtj.y - c3 * djk.y);
           fjz += (double)(CH2 * dtj.z - c3 * djk.z);
